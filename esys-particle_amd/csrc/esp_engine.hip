@@ -1,0 +1,1444 @@
+// esp_engine.hip - host side of the engine: the C-ABI of include/esp.h and the
+// per-step orchestration of CLatticeMaster::runOneStep / TSubLattice
+// (LatticeMaster.cpp:2602-2725, SubLattice.hpp:2224-2536) over the kernels.
+//
+// Stepping.  A step needs host bookkeeping only when the neighbour table must
+// be rebuilt (a particle moved more than verletDist/2 since the last rebuild),
+// when bonds broke in the previous step, or on the first step.  All other steps
+// are pure device work, so the host queues `steps_per_batch` of them at once;
+// each queued kernel checks the previous step's status on the device and stops
+// the queue when bookkeeping is due (the halt is recorded, later steps return).
+// After the batch the host synchronises once, performs the bookkeeping for the
+// halted step and continues.  The decisions are the reference's, made at the
+// same steps.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <functional>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/esp.h"
+#include "esp_internal.h"
+#include "esp_math.h"
+
+namespace {
+
+struct EspError : std::runtime_error {
+  int code;
+  EspError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHK(expr)                                                                     \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      throw EspError(ESP_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <class T>
+struct DBuf {
+  T* p = nullptr;
+  size_t cap = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  DBuf(DBuf&& o) noexcept : p(o.p), cap(o.cap) { o.p = nullptr; o.cap = 0; }
+  DBuf& operator=(DBuf&& o) noexcept
+  {
+    std::swap(p, o.p);
+    std::swap(cap, o.cap);
+    return *this;
+  }
+  ~DBuf()
+  {
+    if (p) (void)hipFree(p);
+  }
+  // capacity for n elements; contents are not kept
+  void reserve(size_t n)
+  {
+    if (n <= cap && p) return;
+    if (p) HIPCHK(hipFree(p));
+    p = nullptr;
+    size_t c = std::max<size_t>(n + n / 4, 64);
+    HIPCHK(hipMalloc(&p, c * sizeof(T)));
+    cap = c;
+  }
+  void swap(DBuf& o)
+  {
+    std::swap(p, o.p);
+    std::swap(cap, o.cap);
+  }
+};
+
+struct DynGroup {
+  std::string name;
+  int kind;  // 1 NRotElastic, 2 RotElastic, 3 RotFriction
+  double k = 0, ks = 0, mu_d = 0, mu_s = 0;
+  int scaling = 1, meanR = 1;
+  int n = 0;
+  DBuf<int2> ij;
+  DBuf<long long> key;
+  DBuf<double4> ff;
+  long long update_ts = 0;
+  std::vector<int> ex;
+  int rec_off = 0;
+};
+
+struct BondGroup {
+  std::string name;
+  esp_rot_bond_prms prm;
+  int n = 0;
+  DBuf<int2> ij;
+  DBuf<double4> k4, mx4, D4;
+  DBuf<long long> skeys;  // sorted keys of live bonds (exclusion lookups)
+  int nkeys = 0;
+  std::vector<int> hid1, hid2;  // ids in storage order
+  std::vector<double4> h_k4, h_mx4, h_D4;  // staged until the first step
+  int rec_off = 0, bond_off = 0;
+};
+
+struct Wall {
+  std::string name;
+  double o[3], n[3], step[3];
+  bool has_motion = false;
+  std::vector<double> hist;
+  double force[3] = {0, 0, 0};
+};
+
+struct WallGroup {
+  std::string name;
+  int wall;
+  double k;
+};
+
+struct Gravity {
+  std::string name;
+  double g[3];
+};
+
+inline long long pkey(int a, int b)
+{
+  return (long long)(((unsigned long long)(unsigned)a << 32) | (unsigned long long)(unsigned)b);
+}
+
+thread_local std::string g_create_err;
+
+}  // namespace
+
+struct esp_engine {
+  esp_config cfg;
+  int rot = 0, do2d = 0;
+  double dt = 0, alpha = 0;
+  GridParams grid;
+  int ncell = 0;
+  hipStream_t stream = nullptr;
+  int K = 32;
+
+  // host staging (until the first step) and host mirrors
+  std::vector<int> h_id, h_tag;
+  std::vector<double> h_pos, h_rad, h_mass, h_vel;
+  std::vector<unsigned> h_flags;
+  std::vector<int> h_conn;
+  std::unordered_map<int, int> id2slot;
+  bool started = false;
+  int n = 0;
+
+  DBuf<double4> pos, vel, ang, quat, old, frc, mom;
+  DBuf<unsigned> flags;
+  DBuf<int> id, by_id, cell;
+  DBuf<int> cell_count, cell_start, cell_fill, cell_items;
+  DBuf<int> cand_cnt, cand_off, cand_p, cand_q;
+  DBuf<long long> cand_key;
+  int ncand = 0;
+  DBuf<int> keep, keep_off, src, scan_tmp;
+  DBuf<int> np_p, np_q;  // flagged (new) pair list
+  DBuf<long long> np_key;
+  int nnew = 0;
+  DBuf<int2> tmp_ij, tmp_ij2;
+  DBuf<long long> tmp_key, tmp_key2;
+  DBuf<double4> tmp_ff, tmp_a, tmp_b, tmp_c;
+  DBuf<int> inc_cnt, inc_off, inc_fill, inc_ent;
+  int nrec = 0;
+  DBuf<double> contrib;
+  DBuf<StepStatus> ring;
+  int ring_size = 0;
+  DBuf<int> halted, broken_list, dev_int;
+  int broken_cap = 0;
+  DBuf<double> wall_part, wall_hist;
+
+  std::vector<DynGroup> dg;
+  std::vector<BondGroup> bg;
+  std::vector<Wall> walls;
+  std::vector<WallGroup> wg;
+  std::vector<Gravity> grav;
+  std::vector<int> dord, bord, word, gord;
+
+  long long t = 0, timestamp = 0, last_ns = -1;
+  bool first_time = true, pending_host = false;
+  long long num_host_phases = 0;
+  std::vector<int> rebuild_steps;
+  std::vector<int> breaks;  // step, id1, id2
+  std::string err;
+
+  PView view()
+  {
+    PView v;
+    v.pos = pos.p; v.vel = vel.p; v.ang = ang.p; v.quat = quat.p; v.old = old.p;
+    v.frc = frc.p; v.mom = mom.p; v.flags = flags.p; v.id = id.p; v.n = n;
+    return v;
+  }
+};
+
+namespace {
+
+void sync(esp_engine* e) { HIPCHK(hipStreamSynchronize(e->stream)); }
+
+template <class T>
+void h2d(esp_engine* e, T* d, const T* h, size_t cnt)
+{
+  if (cnt) HIPCHK(hipMemcpyAsync(d, h, cnt * sizeof(T), hipMemcpyHostToDevice, e->stream));
+}
+template <class T>
+void d2h(esp_engine* e, T* h, const T* d, size_t cnt)
+{
+  if (cnt) HIPCHK(hipMemcpyAsync(h, d, cnt * sizeof(T), hipMemcpyDeviceToHost, e->stream));
+}
+template <class T>
+void zero(esp_engine* e, T* d, size_t cnt)
+{
+  if (cnt) HIPCHK(hipMemsetAsync(d, 0, cnt * sizeof(T), e->stream));
+}
+
+// exclusive scan of `cnt` ints, returns the total (synchronises)
+int scan_total(esp_engine* e, const int* in, int* out, int cnt)
+{
+  e->scan_tmp.reserve(scan_tmp_size(cnt));
+  launch_exclusive_scan(e->stream, in, out, cnt, e->scan_tmp.p);
+  HIPCHK(hipGetLastError());
+  int total = 0;
+  d2h(e, &total, out + cnt, 1);
+  sync(e);
+  return total;
+}
+
+std::vector<int> name_order(size_t n, const std::function<const std::string&(int)>& nm)
+{
+  std::vector<int> o(n);
+  for (size_t i = 0; i < n; i++) o[i] = (int)i;
+  std::sort(o.begin(), o.end(), [&](int a, int b) { return nm(a) < nm(b); });
+  return o;
+}
+
+void refresh_orders(esp_engine* e)
+{
+  e->dord = name_order(e->dg.size(), [&](int i) -> const std::string& { return e->dg[i].name; });
+  e->bord = name_order(e->bg.size(), [&](int i) -> const std::string& { return e->bg[i].name; });
+  e->word = name_order(e->wg.size(), [&](int i) -> const std::string& { return e->wg[i].name; });
+  e->gord = name_order(e->grav.size(), [&](int i) -> const std::string& { return e->grav[i].name; });
+}
+
+bool name_taken(esp_engine* e, const std::string& n)
+{
+  for (auto& g : e->dg) if (g.name == n) return true;
+  for (auto& g : e->bg) if (g.name == n) return true;
+  for (auto& g : e->wg) if (g.name == n) return true;
+  for (auto& g : e->grav) if (g.name == n) return true;
+  return false;
+}
+
+int find_dg(esp_engine* e, const char* n)
+{
+  for (size_t i = 0; i < e->dg.size(); i++) if (e->dg[i].name == n) return (int)i;
+  return -1;
+}
+int find_bg(esp_engine* e, const char* n)
+{
+  for (size_t i = 0; i < e->bg.size(); i++) if (e->bg[i].name == n) return (int)i;
+  return -1;
+}
+int find_wall(esp_engine* e, const char* n)
+{
+  for (size_t i = 0; i < e->walls.size(); i++) if (e->walls[i].name == n) return (int)i;
+  return -1;
+}
+
+void require_setup(esp_engine* e, const char* what)
+{
+  if (e->started)
+    throw EspError(ESP_ERR_STATE, std::string(what) +
+                                      " must be called before the first time step on this engine");
+}
+
+// NeighborTable::index on the host (same arithmetic as the device kernel)
+int host_cell(esp_engine* e, double x, double y, double z)
+{
+  const GridParams& g = e->grid;
+  double f[3] = {std::floor((x - g.x0) / g.dim), std::floor((y - g.y0) / g.dim),
+                 std::floor((z - g.z0) / g.dim)};
+  for (double v : f)
+    if (!(v >= -2147483648.0 && v < 2147483648.0)) return -1;
+  int ix = (int)f[0] - g.gx, iy = (int)f[1] - g.gy, iz = (int)f[2] - g.gz;
+  if (ix >= 0 && ix < g.nx && iy >= 0 && iy < g.ny && iz >= 0 && iz < g.nz)
+    return g.ny * g.nz * ix + g.nz * iy + iz;
+  return -1;
+}
+
+ExclSet excl_set(esp_engine* e, const DynGroup& G)
+{
+  ExclSet s;
+  memset(&s, 0, sizeof s);
+  for (int b : G.ex) {
+    s.keys[s.count] = e->bg[b].skeys.p;
+    s.n[s.count] = e->bg[b].nkeys;
+    s.count++;
+  }
+  return s;
+}
+
+// ---------------------------------------------------------------------------
+// device upload at the first step
+// ---------------------------------------------------------------------------
+void upload(esp_engine* e)
+{
+  const int n = e->n;
+  const size_t cap = std::max(n, 1);
+  e->pos.reserve(cap); e->vel.reserve(cap); e->old.reserve(cap); e->frc.reserve(cap);
+  e->flags.reserve(cap); e->id.reserve(cap); e->by_id.reserve(cap); e->cell.reserve(cap);
+  if (e->rot) { e->ang.reserve(cap); e->quat.reserve(cap); e->mom.reserve(cap); }
+  std::vector<double4> P(n), Vv(n), O(n), A(n), Q(n), Z(n);
+  for (int i = 0; i < n; i++) {
+    const double m = e->h_mass[i], r = e->h_rad[i];
+    P[i] = make_double4(e->h_pos[3 * i], e->h_pos[3 * i + 1], e->h_pos[3 * i + 2], r);
+    Vv[i] = make_double4(e->h_vel[3 * i], e->h_vel[3 * i + 1], e->h_vel[3 * i + 2],
+                         (m != 0.0) ? 1.0 / m : 0.0);
+    O[i] = make_double4(P[i].x, P[i].y, P[i].z, m);
+    Z[i] = make_double4(0, 0, 0, 0);
+    if (e->rot) {
+      // CRotParticle: I = 0.4 m r^2 (3-D) / 0.5 m r^2 (2-D) (RotParticle.cpp:80-124, 160-177)
+      const double I = e->do2d ? 0.5 * m * r * r : 0.4 * m * r * r;
+      A[i] = make_double4(0, 0, 0, 1.0 / I);
+      Q[i] = make_double4(1.0, 0, 0, 0);
+    }
+  }
+  h2d(e, e->pos.p, P.data(), n);
+  h2d(e, e->vel.p, Vv.data(), n);
+  h2d(e, e->old.p, O.data(), n);
+  h2d(e, e->frc.p, Z.data(), n);
+  if (e->rot) {
+    h2d(e, e->ang.p, A.data(), n);
+    h2d(e, e->quat.p, Q.data(), n);
+    h2d(e, e->mom.p, Z.data(), n);
+  }
+  h2d(e, e->flags.p, e->h_flags.data(), n);
+  h2d(e, e->id.p, e->h_id.data(), n);
+  std::vector<int> order(n);
+  for (int i = 0; i < n; i++) order[i] = i;
+  std::sort(order.begin(), order.end(), [&](int a, int b) { return e->h_id[a] < e->h_id[b]; });
+  h2d(e, e->by_id.p, order.data(), n);
+
+  int total_bonds = 0;
+  for (int gi : e->bord) {
+    BondGroup& G = e->bg[gi];
+    G.bond_off = total_bonds;
+    total_bonds += G.n;
+    const size_t c = std::max(G.n, 1);
+    G.ij.reserve(c); G.k4.reserve(c); G.mx4.reserve(c); G.D4.reserve(c); G.skeys.reserve(c);
+    std::vector<int2> ij(G.n);
+    std::vector<long long> keys(G.n);
+    for (int b = 0; b < G.n; b++) {
+      ij[b] = make_int2(e->id2slot[G.hid1[b]], e->id2slot[G.hid2[b]]);
+      keys[b] = pkey(G.hid1[b], G.hid2[b]);
+    }
+    std::sort(keys.begin(), keys.end());
+    G.nkeys = G.n;
+    h2d(e, G.ij.p, ij.data(), G.n);
+    h2d(e, G.k4.p, G.h_k4.data(), G.n);
+    h2d(e, G.mx4.p, G.h_mx4.data(), G.n);
+    h2d(e, G.D4.p, G.h_D4.data(), G.n);
+    h2d(e, G.skeys.p, keys.data(), G.n);
+    sync(e);
+    std::vector<double4>().swap(G.h_k4);
+    std::vector<double4>().swap(G.h_mx4);
+    std::vector<double4>().swap(G.h_D4);
+  }
+  e->broken_cap = std::max(total_bonds, 1);
+  e->broken_list.reserve(e->broken_cap);
+  e->ring_size = e->K + 2;
+  e->ring.reserve(e->ring_size);
+  zero(e, e->ring.p, e->ring_size);
+  e->halted.reserve(1);
+  zero(e, e->halted.p, 1);
+  e->dev_int.reserve(4);
+  e->cell_count.reserve(e->ncell + 1);
+  e->cell_start.reserve(e->ncell + 1);
+  e->cell_fill.reserve(e->ncell + 1);
+  e->cell_items.reserve(cap);
+  e->cand_cnt.reserve(cap + 1);
+  e->cand_off.reserve(cap + 1);
+  e->inc_cnt.reserve(cap + 1);
+  e->inc_off.reserve(cap + 1);
+  e->inc_fill.reserve(cap + 1);
+  const int nb = std::max(gather_blocks(n), 1);
+  const int nwg = (int)e->wg.size();
+  e->wall_part.reserve((size_t)nb * std::max(nwg, 1) * 3);
+  e->wall_hist.reserve((size_t)(e->K + 1) * std::max(nwg, 1) * 3);
+  sync(e);
+  e->started = true;
+}
+
+// ---------------------------------------------------------------------------
+// neighbour search: ParallelParticleArray::rebuild + rebuildInteractions +
+// resetDisplacements (pp_array.hpp:433-554, SubLattice.hpp:2360-2410)
+// ---------------------------------------------------------------------------
+void rebuild(esp_engine* e, long long step)
+{
+  PView p = e->view();
+  zero(e, e->cell_count.p, e->ncell + 1);
+  zero(e, e->dev_int.p, 1);
+  launch_cells_count(e->stream, p, e->grid, e->cell.p, e->cell_count.p, e->dev_int.p);
+  int nout = 0;
+  d2h(e, &nout, e->dev_int.p, 1);
+  sync(e);
+  if (nout > 0)
+    throw EspError(ESP_ERR_STATE,
+                   "particles left the neighbour-table domain; removing particles "
+                   "(NeighborTable::build) is not supported by the device engine");
+  scan_total(e, e->cell_count.p, e->cell_start.p, e->ncell);
+  zero(e, e->cell_fill.p, e->ncell);
+  launch_cells_fill(e->stream, e->n, e->cell.p, e->cell_start.p, e->cell_fill.p, e->cell_items.p);
+  launch_cells_sort(e->stream, e->ncell, e->cell_start.p, e->cell_items.p);
+  // full candidate list (getFullList), sorted by (id1, id2)
+  launch_pairs(e->stream, 0, 0, p, e->grid, e->alpha, e->by_id.p, e->cell.p, e->cell_start.p,
+               e->cell_items.p, e->cand_cnt.p, nullptr, nullptr, nullptr, nullptr);
+  const int total = scan_total(e, e->cand_cnt.p, e->cand_off.p, e->n);
+  e->cand_p.reserve(std::max(total, 1));
+  e->cand_q.reserve(std::max(total, 1));
+  e->cand_key.reserve(std::max(total, 1));
+  launch_pairs(e->stream, 1, 0, p, e->grid, e->alpha, e->by_id.p, e->cell.p, e->cell_start.p,
+               e->cell_items.p, nullptr, e->cand_off.p, e->cand_p.p, e->cand_q.p, e->cand_key.p);
+  HIPCHK(hipGetLastError());
+  e->ncand = total;
+  e->timestamp++;
+  e->rebuild_steps.push_back((int)step);
+}
+
+// flagged pair list (getNewList) and clearing of all flags
+void new_pair_list(esp_engine* e)
+{
+  PView p = e->view();
+  launch_pairs(e->stream, 0, 1, p, e->grid, e->alpha, e->by_id.p, e->cell.p, e->cell_start.p,
+               e->cell_items.p, e->cand_cnt.p, nullptr, nullptr, nullptr, nullptr);
+  const int total = scan_total(e, e->cand_cnt.p, e->cand_off.p, e->n);
+  e->np_p.reserve(std::max(total, 1));
+  e->np_q.reserve(std::max(total, 1));
+  e->np_key.reserve(std::max(total, 1));
+  launch_pairs(e->stream, 1, 1, p, e->grid, e->alpha, e->by_id.p, e->cell.p, e->cell_start.p,
+               e->cell_items.p, nullptr, e->cand_off.p, e->np_p.p, e->np_q.p, e->np_key.p);
+  launch_clear_flag_bit(e->stream, p);
+  HIPCHK(hipGetLastError());
+  e->nnew = total;
+}
+
+// PIS_NE::update / PIS_ED::update (pi_storage_ne.hpp:49-102, pi_storage_ed.hpp:84-161)
+void dyn_update(esp_engine* e, DynGroup& G, bool& new_list_taken)
+{
+  const bool hist = G.kind == 3;
+  PView p = e->view();
+  ExclSet ex = excl_set(e, G);
+  if (G.update_ts != e->timestamp) {
+    const int nc = e->ncand;
+    e->keep.reserve(std::max(nc, 1) + 1);
+    e->keep_off.reserve(std::max(nc, 1) + 1);
+    if (hist) {
+      e->src.reserve(std::max(nc, 1));
+      launch_ed_mark(e->stream, nc, e->cand_p.p, e->cand_q.p, e->cand_key.p, ex, G.key.p, G.n,
+                     G.ij.p, p, e->keep.p, e->src.p);
+    } else {
+      launch_ne_mark(e->stream, nc, e->cand_key.p, ex, e->keep.p);
+    }
+    const int total = scan_total(e, e->keep.p, e->keep_off.p, nc);
+    e->tmp_ij.reserve(std::max(total, 1));
+    e->tmp_key.reserve(std::max(total, 1));
+    if (hist) e->tmp_ff.reserve(std::max(total, 1));
+    launch_pair_emit(e->stream, nc, e->keep.p, e->keep_off.p, e->cand_p.p, e->cand_q.p,
+                     e->cand_key.p, hist ? e->src.p : nullptr, hist ? G.ff.p : nullptr,
+                     e->tmp_ij.p, e->tmp_key.p, hist ? e->tmp_ff.p : nullptr);
+    HIPCHK(hipGetLastError());
+    G.ij.swap(e->tmp_ij);
+    G.key.swap(e->tmp_key);
+    if (hist) G.ff.swap(e->tmp_ff);
+    G.n = total;
+  } else {
+    int nc = 0;
+    if (!new_list_taken) {
+      new_pair_list(e);
+      new_list_taken = true;
+      nc = e->nnew;
+    }
+    if (nc > 0) {
+      e->keep.reserve(nc + 1);
+      e->keep_off.reserve(nc + 1);
+      launch_filter_new(e->stream, nc, e->np_key.p, ex, G.key.p, G.n, hist ? 1 : 0, e->keep.p);
+      const int add = scan_total(e, e->keep.p, e->keep_off.p, nc);
+      if (add > 0) {
+        e->tmp_ij2.reserve(add);
+        e->tmp_key2.reserve(add);
+        launch_pack_pairs(e->stream, nc, e->keep.p, e->keep_off.p, e->np_p.p, e->np_q.p,
+                          e->np_key.p, e->tmp_ij2.p, e->tmp_key2.p);
+        const int tot = G.n + add;
+        e->tmp_ij.reserve(tot);
+        e->tmp_key.reserve(tot);
+        if (hist) e->tmp_ff.reserve(tot);
+        launch_merge(e->stream, G.n, G.ij.p, G.key.p, hist ? G.ff.p : nullptr, add, e->tmp_ij2.p,
+                     e->tmp_key2.p, e->tmp_ij.p, e->tmp_key.p, hist ? e->tmp_ff.p : nullptr);
+        HIPCHK(hipGetLastError());
+        sync(e);
+        G.ij.swap(e->tmp_ij);
+        G.key.swap(e->tmp_key);
+        if (hist) G.ff.swap(e->tmp_ff);
+        G.n = tot;
+      }
+    }
+  }
+  G.update_ts = e->timestamp;
+}
+
+void wall_params(esp_engine* e, WallParam* out)
+{
+  for (int k = 0; k < ESP_MAX_WALL_GROUPS; k++) memset(&out[k], 0, sizeof(WallParam));
+  for (size_t k = 0; k < e->word.size(); k++) {
+    const WallGroup& G = e->wg[e->word[k]];
+    const Wall& W = e->walls[G.wall];
+    for (int c = 0; c < 3; c++) {
+      out[k].o[c] = W.o[c];
+      out[k].nrm[c] = W.n[c];
+    }
+    out[k].k = G.k;
+  }
+}
+
+void rebuild_csr(esp_engine* e)
+{
+  int rec = 0;
+  for (int gi : e->dord) { e->dg[gi].rec_off = rec; rec += e->dg[gi].n; }
+  for (int gi : e->bord) { e->bg[gi].rec_off = rec; rec += e->bg[gi].n; }
+  if ((long long)rec >= (1ll << 29))
+    throw EspError(ESP_ERR_CAPACITY, "more than 2^29 interactions");
+  e->nrec = rec;
+  e->contrib.reserve((size_t)std::max(rec, 1) * 9);
+  zero(e, e->inc_cnt.p, e->n + 1);
+  for (int gi : e->dord) launch_inc_count(e->stream, e->dg[gi].n, e->dg[gi].ij.p, e->inc_cnt.p);
+  for (int gi : e->bord) launch_inc_count(e->stream, e->bg[gi].n, e->bg[gi].ij.p, e->inc_cnt.p);
+  const int total = scan_total(e, e->inc_cnt.p, e->inc_off.p, e->n);
+  e->inc_ent.reserve(std::max(total, 1));
+  zero(e, e->inc_fill.p, e->n + 1);
+  for (int gi : e->dord) {
+    DynGroup& G = e->dg[gi];
+    launch_inc_fill(e->stream, G.n, G.ij.p, G.rec_off, G.kind == 3 ? 1 : 0, e->inc_off.p,
+                    e->inc_fill.p, e->inc_ent.p);
+  }
+  for (int gi : e->bord) {
+    BondGroup& G = e->bg[gi];
+    launch_inc_fill(e->stream, G.n, G.ij.p, G.rec_off, 1, e->inc_off.p, e->inc_fill.p,
+                    e->inc_ent.p);
+  }
+  launch_inc_sort(e->stream, e->n, e->inc_off.p, e->inc_ent.p);
+  HIPCHK(hipGetLastError());
+}
+
+// bonds broken in the previous step leave their groups (PIS_EB::update)
+bool remove_broken_bonds(esp_engine* e, int nbroken)
+{
+  if (nbroken <= 0) return false;
+  std::vector<int> list(nbroken);
+  d2h(e, list.data(), e->broken_list.p, nbroken);
+  sync(e);
+  std::sort(list.begin(), list.end());
+  PView p = e->view();
+  bool any = false;
+  for (int gi : e->bord) {
+    BondGroup& G = e->bg[gi];
+    auto lo = std::lower_bound(list.begin(), list.end(), G.bond_off);
+    auto hi = std::lower_bound(list.begin(), list.end(), G.bond_off + G.n);
+    const int cnt = (int)(hi - lo);
+    if (cnt == 0) continue;
+    any = true;
+    std::vector<long long> rem;
+    std::vector<char> gone(G.n, 0);
+    for (auto it = lo; it != hi; ++it) {
+      const int b = *it - G.bond_off;
+      gone[b] = 1;
+      rem.push_back(pkey(G.hid1[b], G.hid2[b]));
+      e->breaks.push_back((int)e->t);
+      e->breaks.push_back(G.hid1[b]);
+      e->breaks.push_back(G.hid2[b]);
+    }
+    std::sort(rem.begin(), rem.end());
+    // device: flag particles, compact bond arrays
+    e->keep.reserve(G.n + 1);
+    e->keep_off.reserve(G.n + 1);
+    e->dev_int.reserve(4);
+    DBuf<int> dlist;
+    dlist.reserve(cnt);
+    h2d(e, dlist.p, &*lo, cnt);
+    launch_bond_mark_broken(e->stream, cnt, dlist.p, G.bond_off, G.n, e->keep.p, p, G.ij.p);
+    const int nb2 = scan_total(e, e->keep.p, e->keep_off.p, G.n);
+    DBuf<int2> ij2;
+    DBuf<double4> k2, m2, D2;
+    const size_t c2 = std::max(nb2, 1);
+    ij2.reserve(c2); k2.reserve(c2); m2.reserve(c2); D2.reserve(c2);
+    launch_compact_bonds(e->stream, G.n, e->keep.p, e->keep_off.p, G.ij.p, G.k4.p, G.mx4.p,
+                         G.D4.p, ij2.p, k2.p, m2.p, D2.p);
+    // sorted keys
+    DBuf<long long> drem;
+    drem.reserve(rem.size());
+    h2d(e, drem.p, rem.data(), rem.size());
+    launch_keys_mark_removed(e->stream, G.nkeys, G.skeys.p, (int)rem.size(), drem.p, e->keep.p);
+    const int nk2 = scan_total(e, e->keep.p, e->keep_off.p, G.nkeys);
+    DBuf<long long> sk2;
+    sk2.reserve(std::max(nk2, 1));
+    launch_keys_compact(e->stream, G.nkeys, e->keep.p, e->keep_off.p, G.skeys.p, sk2.p);
+    HIPCHK(hipGetLastError());
+    sync(e);
+    G.ij.swap(ij2); G.k4.swap(k2); G.mx4.swap(m2); G.D4.swap(D2); G.skeys.swap(sk2);
+    G.nkeys = nk2;
+    int w = 0;
+    for (int b = 0; b < G.n; b++)
+      if (!gone[b]) { G.hid1[w] = G.hid1[b]; G.hid2[w] = G.hid2[b]; w++; }
+    G.hid1.resize(w);
+    G.hid2.resize(w);
+    G.n = nb2;
+  }
+  // renumber bond offsets
+  int off = 0;
+  for (int gi : e->bord) { e->bg[gi].bond_off = off; off += e->bg[gi].n; }
+  return any;
+}
+
+// TSubLattice::updateInteractions (SubLattice.hpp:2416-2466)
+void update_interactions(esp_engine* e, int nbroken_prev)
+{
+  bool need_update = remove_broken_bonds(e, nbroken_prev);
+  bool changed = need_update;
+  if (e->timestamp > e->last_ns || need_update) {
+    bool new_list_taken = false;
+    for (int gi : e->dord) dyn_update(e, e->dg[gi], new_list_taken);
+    if (!e->wg.empty()) {
+      WallParam w[ESP_MAX_WALL_GROUPS];
+      wall_params(e, w);
+      launch_wall_update(e->stream, e->view(), e->grid, (int)e->wg.size(), w);
+      HIPCHK(hipGetLastError());
+    }
+    e->last_ns = e->timestamp;
+    changed = true;
+  }
+  if (changed) rebuild_csr(e);
+}
+
+// ---------------------------------------------------------------------------
+// one step's kernels
+// ---------------------------------------------------------------------------
+void launch_step(esp_engine* e, const StepCtl& ctl, double* whist_out)
+{
+  PView p = e->view();
+  for (int gi : e->dord) {
+    DynGroup& G = e->dg[gi];
+    if (G.kind == 1)
+      launch_nrot_elastic(e->stream, ctl, p, G.n, G.ij.p, G.k, G.scaling, e->do2d, e->contrib.p,
+                          G.rec_off);
+    else if (G.kind == 2)
+      launch_rot_elastic(e->stream, ctl, p, G.n, G.ij.p, G.k, G.scaling, e->do2d, e->contrib.p,
+                         G.rec_off);
+    else
+      launch_rot_friction(e->stream, ctl, p, G.n, G.ij.p, G.ff.p, G.k, G.ks, G.scaling, G.meanR,
+                          e->do2d, G.mu_d, G.mu_s, e->dt, e->contrib.p, G.rec_off);
+  }
+  for (int gi : e->bord) {
+    BondGroup& G = e->bg[gi];
+    launch_rot_bond(e->stream, ctl, p, G.n, G.ij.p, G.k4.p, G.mx4.p, G.D4.p, G.prm.truncated,
+                    G.prm.beta1, G.prm.beta2, e->contrib.p, G.rec_off, G.bond_off,
+                    e->broken_list.p);
+  }
+  GatherParams gp;
+  memset(&gp, 0, sizeof gp);
+  gp.inc_off = e->inc_off.p;
+  gp.inc_ent = e->inc_ent.p;
+  gp.contrib = e->contrib.p;
+  gp.dt = e->dt;
+  gp.rot = e->rot;
+  gp.do2d = e->do2d;
+  gp.nwg = (int)e->wg.size();
+  wall_params(e, gp.wall);
+  gp.ngrav = (int)e->gord.size();
+  for (size_t k = 0; k < e->gord.size(); k++)
+    for (int c = 0; c < 3; c++) gp.g[k][c] = e->grav[e->gord[k]].g[c];
+  gp.wall_part = e->wall_part.p;
+  launch_gather_integrate(e->stream, ctl, p, gp);
+  launch_wall_finalize(e->stream, ctl, e->wall_part.p, gather_blocks(e->n), gp.nwg, whist_out);
+  HIPCHK(hipGetLastError());
+}
+
+unsigned long long thresh_bits(esp_engine* e)
+{
+  const double a = 0.5 * e->alpha;
+  const double sr = a * a;
+  unsigned long long b;
+  memcpy(&b, &sr, 8);
+  return b;
+}
+
+void record_walls(esp_engine* e, const std::vector<double>& h, int steps)
+{
+  const int nwg = (int)e->wg.size();
+  for (int s = 0; s < steps; s++)
+    for (int w = 0; w < (int)e->walls.size(); w++) {
+      double f[3] = {0, 0, 0};
+      // a wall's force is the sum over its wall groups (each adds to the same CWall)
+      for (int k = 0; k < nwg; k++)
+        if (e->wg[e->word[k]].wall == w)
+          for (int c = 0; c < 3; c++) f[c] = f[c] + h[((size_t)s * nwg + k) * 3 + c];
+      for (int c = 0; c < 3; c++) {
+        e->walls[w].hist.push_back(f[c]);
+        e->walls[w].force[c] = f[c];
+      }
+    }
+}
+
+void apply_wall_motion(esp_engine* e)
+{
+  for (auto& W : e->walls)
+    if (W.has_motion)
+      for (int c = 0; c < 3; c++) W.o[c] = W.o[c] + W.step[c];
+}
+
+// one step with host bookkeeping (first step, rebuild, broken bonds)
+void host_step(esp_engine* e)
+{
+  e->num_host_phases++;
+  const long long step = e->t + 1;
+  int nbroken_prev = 0;
+  bool checked = false;
+  if (e->first_time) {
+    rebuild(e, step);  // forced search of the first runOneStep
+    e->first_time = false;
+    checked = true;
+  }
+  e->t = step;
+  apply_wall_motion(e);
+  if (!checked) {
+    StepStatus st;
+    d2h(e, &st, e->ring.p + ((step - 1) % e->ring_size), 1);
+    sync(e);
+    nbroken_prev = st.nbroken;
+    if (st.maxdisp2 > thresh_bits(e)) rebuild(e, step);
+  }
+  update_interactions(e, nbroken_prev);
+  launch_clear_status(e->stream, e->ring.p, e->ring_size, step, 1, e->halted.p);
+  StepCtl ctl;
+  ctl.prev = e->ring.p + ((step - 1) % e->ring_size);
+  ctl.cur = e->ring.p + (step % e->ring_size);
+  ctl.halted = e->halted.p;
+  ctl.thresh = thresh_bits(e);
+  ctl.step_id = 1;
+  ctl.guarded = 0;
+  ctl.first = 1;
+  launch_step(e, ctl, e->wall_hist.p);
+  const int nwg = (int)e->wg.size();
+  if (nwg) {
+    std::vector<double> h((size_t)nwg * 3);
+    d2h(e, h.data(), e->wall_hist.p, h.size());
+    sync(e);
+    record_walls(e, h, 1);
+  }
+}
+
+// up to K steps queued without host bookkeeping; returns steps completed
+int fast_batch(esp_engine* e, int k)
+{
+  const long long step0 = e->t + 1;
+  const int nwg = (int)e->wg.size();
+  std::vector<double> before;
+  for (auto& W : e->walls) before.insert(before.end(), W.o, W.o + 3);
+  std::vector<std::vector<double>> wall_o;
+  launch_clear_status(e->stream, e->ring.p, e->ring_size, step0, k, e->halted.p);
+  for (int s = 0; s < k; s++) {
+    apply_wall_motion(e);  // pre-time-step runnables of this step
+    std::vector<double> snap;
+    for (auto& W : e->walls) snap.insert(snap.end(), W.o, W.o + 3);
+    wall_o.push_back(snap);
+    const long long step = step0 + s;
+    StepCtl ctl;
+    ctl.prev = e->ring.p + ((step - 1) % e->ring_size);
+    ctl.cur = e->ring.p + (step % e->ring_size);
+    ctl.halted = e->halted.p;
+    ctl.thresh = thresh_bits(e);
+    ctl.step_id = s + 1;
+    ctl.guarded = 1;
+    ctl.first = 1;
+    launch_step(e, ctl, e->wall_hist.p + (size_t)s * std::max(nwg, 1) * 3);
+  }
+  int halted = 0;
+  d2h(e, &halted, e->halted.p, 1);
+  std::vector<double> h((size_t)k * nwg * 3);
+  if (nwg) d2h(e, h.data(), e->wall_hist.p, h.size());
+  sync(e);
+  const int done = halted ? halted - 1 : k;
+  if (done < k) {  // walls stay where the last completed step left them
+    const std::vector<double>& o = done > 0 ? wall_o[done - 1] : before;
+    for (size_t w = 0; w < e->walls.size(); w++)
+      for (int c = 0; c < 3; c++) e->walls[w].o[c] = o[3 * w + c];
+  }
+  if (nwg) record_walls(e, h, done);
+  e->t += done;
+  e->pending_host = done < k;
+  return done;
+}
+
+void run_steps(esp_engine* e, int nsteps)
+{
+  if (!e->started) {
+    refresh_orders(e);
+    if (e->wg.size() > ESP_MAX_WALL_GROUPS)
+      throw EspError(ESP_ERR_CAPACITY, "more than 8 wall interaction groups");
+    if (e->grav.size() > ESP_MAX_GRAVITY)
+      throw EspError(ESP_ERR_CAPACITY, "more than 4 gravity groups");
+    for (auto& G : e->dg)
+      if (G.ex.size() > ESP_MAX_EXCL)
+        throw EspError(ESP_ERR_CAPACITY, "more than 4 exclusions on one group");
+    upload(e);
+  }
+  int done = 0;
+  while (done < nsteps) {
+    if (e->first_time || e->pending_host) {
+      host_step(e);
+      e->pending_host = false;
+      done++;
+      continue;
+    }
+    done += fast_batch(e, std::min(e->K, nsteps - done));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// queries
+// ---------------------------------------------------------------------------
+std::vector<int> id_order(esp_engine* e)
+{
+  std::vector<int> order(e->n);
+  for (int i = 0; i < e->n; i++) order[i] = i;
+  std::sort(order.begin(), order.end(), [&](int a, int b) { return e->h_id[a] < e->h_id[b]; });
+  return order;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C-ABI
+// ===========================================================================
+#define API_BEGIN try {
+#define API_END                                                  \
+  }                                                              \
+  catch (const EspError& ex)                                     \
+  {                                                              \
+    e->err = ex.what();                                          \
+    return ex.code;                                              \
+  }                                                              \
+  catch (const std::exception& ex)                               \
+  {                                                              \
+    e->err = ex.what();                                          \
+    return ESP_ERR_STATE;                                        \
+  }
+
+extern "C" {
+
+const char* esp_create_error(void) { return g_create_err.c_str(); }
+
+const char* esp_last_error(const esp_engine* e) { return e ? e->err.c_str() : g_create_err.c_str(); }
+
+int esp_create(const esp_config* cfg, esp_engine** out)
+{
+  g_create_err.clear();
+  if (!cfg || !out) {
+    g_create_err = "null argument";
+    return ESP_ERR_ARG;
+  }
+  if (cfg->particle_type != ESP_PTYPE_NROT && cfg->particle_type != ESP_PTYPE_ROT) {
+    g_create_err = "unknown particle type";
+    return ESP_ERR_ARG;
+  }
+  if (!(cfg->grid_spacing > 0) || !(cfg->verlet_dist >= 0)) {
+    g_create_err = "gridSpacing must be > 0 and verletDist >= 0";
+    return ESP_ERR_ARG;
+  }
+  // TSubLattice::initNeighborTable (SubLattice.hpp:455-478) and the 1x1x1
+  // ParallelParticleArray table (pp_array.hpp:61-103)
+  const double nr = cfg->grid_spacing;
+  double mn[3], mx[3], nmin[3], nmax[3];
+  for (int c = 0; c < 3; c++) { mn[c] = cfg->domain_min[c]; mx[c] = cfg->domain_max[c]; }
+  double sz[3];
+  for (int c = 0; c < 3; c++) {
+    sz[c] = mx[c] - mn[c];
+    sz[c] = nr * std::ceil(sz[c] / nr);
+  }
+  for (int c = 0; c < 3; c++) {
+    const double grow = sz[c] - (mx[c] - mn[c]);
+    nmin[c] = mn[c] - grow * 0.5;
+    nmax[c] = mx[c] + grow * 0.5;
+  }
+  long ng[3];
+  for (int c = 0; c < 3; c++) {
+    ng[c] = lrint((nmax[c] - nmin[c]) / nr);
+    if (std::fabs((double)ng[c] - (nmax[c] - nmin[c]) / nr) > 1e-6 || ng[c] < 1) {
+      g_create_err = "size doesn't fit range";
+      return ESP_ERR_ARG;
+    }
+  }
+  const double ncells = (double)(ng[0] + 2) * (double)(ng[1] + 2) * (double)(ng[2] + 2);
+  if (ncells > 2.0e8) {
+    g_create_err = "neighbour table too large";
+    return ESP_ERR_CAPACITY;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+    g_create_err = "no HIP device available";
+    return ESP_ERR_DEVICE;
+  }
+  if (cfg->device < 0 || cfg->device >= ndev) {
+    g_create_err = "bad device ordinal";
+    return ESP_ERR_ARG;
+  }
+  if (hipSetDevice(cfg->device) != hipSuccess) {
+    g_create_err = "hipSetDevice failed";
+    return ESP_ERR_DEVICE;
+  }
+  esp_engine* e = new esp_engine();
+  e->cfg = *cfg;
+  e->rot = cfg->particle_type == ESP_PTYPE_ROT;
+  e->do2d = cfg->do_2d;
+  e->dt = cfg->dt;
+  e->alpha = cfg->verlet_dist;
+  e->K = std::max(1, std::min(cfg->steps_per_batch > 0 ? cfg->steps_per_batch : 32, 4096));
+  e->grid.x0 = nmin[0]; e->grid.y0 = nmin[1]; e->grid.z0 = nmin[2];
+  e->grid.dim = nr;
+  e->grid.gx = e->grid.gy = e->grid.gz = -1;
+  e->grid.nx = (int)ng[0] + 2; e->grid.ny = (int)ng[1] + 2; e->grid.nz = (int)ng[2] + 2;
+  e->ncell = e->grid.nx * e->grid.ny * e->grid.nz;
+  if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
+    g_create_err = "hipStreamCreate failed";
+    delete e;
+    return ESP_ERR_DEVICE;
+  }
+  *out = e;
+  return ESP_OK;
+}
+
+void esp_destroy(esp_engine* e)
+{
+  if (!e) return;
+  if (e->stream) {
+    (void)hipStreamSynchronize(e->stream);
+  }
+  hipStream_t s = e->stream;
+  delete e;
+  if (s) (void)hipStreamDestroy(s);
+}
+
+int esp_set_time_step_size(esp_engine* e, double dt)
+{
+  e->dt = dt;
+  return ESP_OK;
+}
+
+int esp_add_particles(esp_engine* e, int n, const int* id, const int* tag, const double* pos,
+                      const double* rad, const double* mass, const double* vel)
+{
+  API_BEGIN
+  require_setup(e, "createParticles");
+  if (n < 0 || (n > 0 && (!id || !pos || !rad))) throw EspError(ESP_ERR_ARG, "bad particle arrays");
+  std::unordered_map<int, int> added;
+  for (int i = 0; i < n; i++) {
+    if (id[i] < 0) throw EspError(ESP_ERR_ARG, "particle ids must be >= 0");
+    if (e->id2slot.count(id[i]) || added.count(id[i]))
+      throw EspError(ESP_ERR_ARG, "duplicate particle id " + std::to_string(id[i]));
+    added[id[i]] = 1;
+  }
+  for (int i = 0; i < n; i++) {
+    // NeighborTable::insert (ntable.hpp:231-255): particles outside are dropped
+    if (host_cell(e, pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]) < 0) continue;
+    const double r = rad[i];
+    const int slot = e->n++;
+    e->id2slot[id[i]] = slot;
+    e->h_id.push_back(id[i]);
+    e->h_tag.push_back(tag ? tag[i] : 0);
+    for (int c = 0; c < 3; c++) e->h_pos.push_back(pos[3 * i + c]);
+    for (int c = 0; c < 3; c++) e->h_vel.push_back(vel ? vel[3 * i + c] : 0.0);
+    e->h_rad.push_back(r);
+    e->h_mass.push_back(mass ? mass[i] : (e->do2d ? r * r : r * r * r));
+    e->h_flags.push_back(PF_DYN | (e->rot ? PF_ROT : 0u));
+  }
+  return ESP_OK;
+  API_END
+}
+
+int esp_set_particle_density(esp_engine* e, int tag, int mask, double rho)
+{
+  API_BEGIN
+  require_setup(e, "setParticleDensity");
+  const double pi = 3.141592654;
+  for (int i = 0; i < e->n; i++) {
+    if ((e->h_tag[i] & mask) != (tag & mask)) continue;
+    const double r = e->h_rad[i];
+    e->h_mass[i] = e->do2d ? rho * pi * r * r : (4.0 / 3.0) * rho * pi * r * r * r;
+  }
+  return ESP_OK;
+  API_END
+}
+
+int esp_set_particle_non_dynamic(esp_engine* e, int tag)
+{
+  API_BEGIN
+  require_setup(e, "setParticleNonDynamic");
+  for (int i = 0; i < e->n; i++)
+    if (e->h_tag[i] == tag) e->h_flags[i] &= ~(PF_DYN | PF_ROT);
+  return ESP_OK;
+  API_END
+}
+
+int esp_set_particle_non_rotational(esp_engine* e, int tag)
+{
+  API_BEGIN
+  require_setup(e, "setParticleNonRotational");
+  for (int i = 0; i < e->n; i++)
+    if (e->h_tag[i] == tag) e->h_flags[i] &= ~PF_ROT;
+  return ESP_OK;
+  API_END
+}
+
+int esp_add_connections(esp_engine* e, int n, const int* id1, const int* id2, const int* tag)
+{
+  API_BEGIN
+  require_setup(e, "createConnections");
+  if (n < 0 || (n > 0 && (!id1 || !id2))) throw EspError(ESP_ERR_ARG, "bad connection arrays");
+  for (int i = 0; i < n; i++) {
+    e->h_conn.push_back(tag ? tag[i] : 0);
+    e->h_conn.push_back(id1[i]);
+    e->h_conn.push_back(id2[i]);
+  }
+  return ESP_OK;
+  API_END
+}
+
+int esp_add_rot_bond_group(esp_engine* e, const char* name, const esp_rot_bond_prms* prm,
+                           const double* per_bond_max, int n_per_bond)
+{
+  API_BEGIN
+  require_setup(e, "createInteractionGroup");
+  if (!e->rot) throw EspError(ESP_ERR_STATE, "rotational bonds need particle type RotSphere");
+  if (!name || !prm) throw EspError(ESP_ERR_ARG, "bad group");
+  if (name_taken(e, name)) throw EspError(ESP_ERR_ARG, std::string("interaction group '") + name + "' exists");
+  BondGroup G;
+  G.name = name;
+  G.prm = *prm;
+  // connections of this tag in arrival order; repeated id pairs are rejected by
+  // isIn at their second occurrence (pi_storage_e.hpp:216-250)
+  std::vector<int> cidx;
+  for (size_t c = 0; c < e->h_conn.size(); c += 3)
+    if (e->h_conn[c] == prm->tag) cidx.push_back((int)c);
+  if (per_bond_max && (int)cidx.size() != n_per_bond)
+    throw EspError(ESP_ERR_ARG, "per_bond_max has " + std::to_string(n_per_bond) + " entries for " +
+                                    std::to_string(cidx.size()) + " connections");
+  std::unordered_map<long long, int> seen;
+  for (size_t j = 0; j < cidx.size(); j++) {
+    const int a = e->h_conn[cidx[j] + 1], b = e->h_conn[cidx[j] + 2];
+    const int id1 = std::min(a, b), id2 = std::max(a, b);
+    const long long key = pkey(id1, id2);
+    if (seen.count(key)) continue;
+    seen[key] = 1;
+    auto i1 = e->id2slot.find(id1), i2 = e->id2slot.find(id2);
+    if (i1 == e->id2slot.end() || i2 == e->id2slot.end()) continue;
+    const int s1 = i1->second, s2 = i2->second;
+    double maxes[4] = {prm->max_n, prm->max_s, prm->max_t, prm->max_b};
+    if (per_bond_max) memcpy(maxes, per_bond_max + 4 * j, sizeof maxes);
+    const BondConst bc = esp_bond_constants(
+        mk3(e->h_pos[3 * s1], e->h_pos[3 * s1 + 1], e->h_pos[3 * s1 + 2]), e->h_rad[s1],
+        mk3(e->h_pos[3 * s2], e->h_pos[3 * s2 + 1], e->h_pos[3 * s2 + 2]), e->h_rad[s2], prm->kr,
+        prm->ks, prm->kt, prm->kb, maxes, prm->scaling, prm->meanR_scaling, e->do2d);
+    G.hid1.push_back(id1);
+    G.hid2.push_back(id2);
+    G.h_k4.push_back(make_double4(bc.kr, bc.ks, bc.kt, bc.kb));
+    G.h_mx4.push_back(make_double4(bc.max_n, bc.max_s, bc.max_t, bc.max_b));
+    G.h_D4.push_back(make_double4(bc.D.x, bc.D.y, bc.D.z, 0.0));
+  }
+  G.n = (int)G.hid1.size();
+  e->bg.push_back(std::move(G));
+  return ESP_OK;
+  API_END
+}
+
+static int add_dyn(esp_engine* e, const char* name, int kind, double k, double ks, double mud,
+                   double mus, int scaling, int meanR)
+{
+  API_BEGIN
+  require_setup(e, "createInteractionGroup");
+  if (!name) throw EspError(ESP_ERR_ARG, "bad group name");
+  if (name_taken(e, name)) throw EspError(ESP_ERR_ARG, std::string("interaction group '") + name + "' exists");
+  DynGroup G;
+  G.name = name;
+  G.kind = kind;
+  G.k = k; G.ks = ks; G.mu_d = mud; G.mu_s = mus;
+  G.scaling = scaling; G.meanR = meanR;
+  e->dg.push_back(std::move(G));
+  return ESP_OK;
+  API_END
+}
+
+int esp_add_nrot_elastic_group(esp_engine* e, const char* name, double k, int scaling)
+{
+  return add_dyn(e, name, 1, k, 0, 0, 0, scaling, 1);
+}
+
+int esp_add_rot_elastic_group(esp_engine* e, const char* name, double kr, int scaling)
+{
+  if (!e->rot) {
+    e->err = "RotElastic needs particle type RotSphere";
+    return ESP_ERR_STATE;
+  }
+  return add_dyn(e, name, 2, kr, 0, 0, 0, scaling, 1);
+}
+
+int esp_add_rot_friction_group(esp_engine* e, const char* name, const esp_rot_friction_prms* p)
+{
+  if (!e->rot) {
+    e->err = "RotFriction needs particle type RotSphere";
+    return ESP_ERR_STATE;
+  }
+  if (!p) {
+    e->err = "null parameters";
+    return ESP_ERR_ARG;
+  }
+  if (p->rigid) {
+    e->err = "RotFriction rigid=True is not supported";
+    return ESP_ERR_ARG;
+  }
+  return add_dyn(e, name, 3, p->k_n, p->k_s, p->mu_d, p->mu_s, p->scaling, p->meanR_scaling);
+}
+
+int esp_add_exclusion(esp_engine* e, const char* excluding, const char* excluded)
+{
+  API_BEGIN
+  require_setup(e, "createExclusion");
+  const int b = find_bg(e, excluding ? excluding : "");
+  const int d = find_dg(e, excluded ? excluded : "");
+  if (b < 0) throw EspError(ESP_ERR_ARG, std::string("excluding group '") + (excluding ? excluding : "") + "' is not a bonded group");
+  if (d < 0) throw EspError(ESP_ERR_ARG, std::string("excluded group '") + (excluded ? excluded : "") + "' is not a dynamic group");
+  if (e->dg[d].ex.size() >= ESP_MAX_EXCL) throw EspError(ESP_ERR_CAPACITY, "too many exclusions");
+  e->dg[d].ex.push_back(b);  // dynamic groups are empty before the first step: nothing to purge
+  return ESP_OK;
+  API_END
+}
+
+int esp_add_wall(esp_engine* e, const char* name, const double o[3], const double n[3])
+{
+  API_BEGIN
+  if (!name || find_wall(e, name) >= 0) throw EspError(ESP_ERR_ARG, "bad or duplicate wall name");
+  require_setup(e, "createWall");
+  Wall W;
+  W.name = name;
+  for (int c = 0; c < 3; c++) { W.o[c] = o[c]; W.n[c] = n[c]; W.step[c] = 0.0; }
+  e->walls.push_back(W);
+  return ESP_OK;
+  API_END
+}
+
+int esp_add_elastic_wall_group(esp_engine* e, const char* name, const char* wall, double k)
+{
+  API_BEGIN
+  require_setup(e, "createInteractionGroup");
+  const int w = find_wall(e, wall ? wall : "");
+  if (w < 0) throw EspError(ESP_ERR_ARG, std::string("unknown wall '") + (wall ? wall : "") + "'");
+  if (!name || name_taken(e, name)) throw EspError(ESP_ERR_ARG, "bad or duplicate group name");
+  if (e->wg.size() >= ESP_MAX_WALL_GROUPS) throw EspError(ESP_ERR_CAPACITY, "more than 8 wall groups");
+  e->wg.push_back(WallGroup{name, w, k});
+  return ESP_OK;
+  API_END
+}
+
+int esp_add_gravity(esp_engine* e, const char* name, const double g[3])
+{
+  API_BEGIN
+  require_setup(e, "createInteractionGroup");
+  if (!name || name_taken(e, name)) throw EspError(ESP_ERR_ARG, "bad or duplicate group name");
+  if (e->grav.size() >= ESP_MAX_GRAVITY) throw EspError(ESP_ERR_CAPACITY, "more than 4 gravity groups");
+  Gravity G;
+  G.name = name;
+  for (int c = 0; c < 3; c++) G.g[c] = g[c];
+  e->grav.push_back(G);
+  return ESP_OK;
+  API_END
+}
+
+int esp_move_wall_by(esp_engine* e, const char* name, const double d[3])
+{
+  API_BEGIN
+  const int w = find_wall(e, name ? name : "");
+  if (w < 0) throw EspError(ESP_ERR_ARG, std::string("unknown wall '") + (name ? name : "") + "'");
+  for (int c = 0; c < 3; c++) e->walls[w].o[c] = e->walls[w].o[c] + d[c];
+  return ESP_OK;
+  API_END
+}
+
+int esp_set_wall_step_motion(esp_engine* e, const char* name, const double d[3])
+{
+  API_BEGIN
+  const int w = find_wall(e, name ? name : "");
+  if (w < 0) throw EspError(ESP_ERR_ARG, std::string("unknown wall '") + (name ? name : "") + "'");
+  for (int c = 0; c < 3; c++) e->walls[w].step[c] = d[c];
+  e->walls[w].has_motion = (d[0] != 0.0 || d[1] != 0.0 || d[2] != 0.0);
+  return ESP_OK;
+  API_END
+}
+
+int esp_run_time_steps(esp_engine* e, int nsteps)
+{
+  API_BEGIN
+  if (nsteps < 0) throw EspError(ESP_ERR_ARG, "negative step count");
+  if (nsteps == 0) return ESP_OK;
+  HIPCHK(hipSetDevice(e->cfg.device));
+  run_steps(e, nsteps);
+  return ESP_OK;
+  API_END
+}
+
+int esp_get_num_particles(esp_engine* e) { return e->n; }
+
+int esp_get_stats(esp_engine* e, esp_stats* o)
+{
+  API_BEGIN
+  memset(o, 0, sizeof *o);
+  o->time_step = e->t;
+  o->num_rebuilds = e->timestamp;
+  o->num_particles = e->n;
+  for (auto& G : e->bg) o->num_bonds += G.n;
+  for (auto& G : e->dg) o->num_dynamic += G.n;
+  o->num_candidates = e->ncand;
+  o->num_broken = (long long)e->breaks.size() / 3;
+  o->num_host_phases = e->num_host_phases;
+  return ESP_OK;
+  API_END
+}
+
+int esp_get_particle_ids(esp_engine* e, int* out)
+{
+  API_BEGIN
+  std::vector<int> order = id_order(e);
+  for (int k = 0; k < e->n; k++) out[k] = e->h_id[order[k]];
+  return ESP_OK;
+  API_END
+}
+
+int esp_get_particle_data(esp_engine* e, int field, double* out)
+{
+  API_BEGIN
+  const int n = e->n;
+  std::vector<int> order = id_order(e);
+  if (!e->started) {
+    for (int k = 0; k < n; k++) {
+      const int i = order[k];
+      const double r = e->h_rad[i];
+      switch (field) {
+        case ESP_FIELD_POS:
+        case ESP_FIELD_OLDPOS:
+          for (int c = 0; c < 3; c++) out[3 * k + c] = e->h_pos[3 * i + c];
+          break;
+        case ESP_FIELD_VEL:
+          for (int c = 0; c < 3; c++) out[3 * k + c] = e->h_vel[3 * i + c];
+          break;
+        case ESP_FIELD_ANGVEL:
+        case ESP_FIELD_FORCE:
+        case ESP_FIELD_MOMENT:
+          for (int c = 0; c < 3; c++) out[3 * k + c] = 0.0;
+          break;
+        case ESP_FIELD_QUAT:
+          out[4 * k] = 1.0;
+          out[4 * k + 1] = out[4 * k + 2] = out[4 * k + 3] = 0.0;
+          break;
+        case ESP_FIELD_RAD: out[k] = r; break;
+        case ESP_FIELD_MASS: out[k] = e->h_mass[i]; break;
+        default: throw EspError(ESP_ERR_ARG, "unknown field");
+      }
+    }
+    return ESP_OK;
+  }
+  HIPCHK(hipSetDevice(e->cfg.device));
+  std::vector<double4> buf(std::max(n, 1));
+  const double4* src = nullptr;
+  int sel = 0;  // 0 xyz, 1 quat, 2 w
+  switch (field) {
+    case ESP_FIELD_POS: src = e->pos.p; break;
+    case ESP_FIELD_VEL: src = e->vel.p; break;
+    case ESP_FIELD_ANGVEL: src = e->rot ? e->ang.p : nullptr; break;
+    case ESP_FIELD_QUAT: src = e->rot ? e->quat.p : nullptr; sel = 1; break;
+    case ESP_FIELD_FORCE: src = e->frc.p; break;
+    case ESP_FIELD_MOMENT: src = e->rot ? e->mom.p : nullptr; break;
+    case ESP_FIELD_RAD: src = e->pos.p; sel = 2; break;
+    case ESP_FIELD_MASS: src = e->old.p; sel = 2; break;
+    case ESP_FIELD_OLDPOS: src = e->old.p; break;
+    default: throw EspError(ESP_ERR_ARG, "unknown field");
+  }
+  if (src) {
+    d2h(e, buf.data(), src, n);
+    sync(e);
+  }
+  for (int k = 0; k < n; k++) {
+    const int i = order[k];
+    if (!src) {
+      if (sel == 1) { out[4 * k] = 1.0; out[4 * k + 1] = out[4 * k + 2] = out[4 * k + 3] = 0.0; }
+      else for (int c = 0; c < 3; c++) out[3 * k + c] = 0.0;
+      continue;
+    }
+    const double4 v = buf[i];
+    if (sel == 0) { out[3 * k] = v.x; out[3 * k + 1] = v.y; out[3 * k + 2] = v.z; }
+    else if (sel == 1) { out[4 * k] = v.x; out[4 * k + 1] = v.y; out[4 * k + 2] = v.z; out[4 * k + 3] = v.w; }
+    else out[k] = v.w;
+  }
+  return ESP_OK;
+  API_END
+}
+
+int esp_get_wall_position(esp_engine* e, const char* name, double out[3])
+{
+  API_BEGIN
+  const int w = find_wall(e, name ? name : "");
+  if (w < 0) throw EspError(ESP_ERR_ARG, "unknown wall");
+  for (int c = 0; c < 3; c++) out[c] = e->walls[w].o[c];
+  return ESP_OK;
+  API_END
+}
+
+int esp_get_wall_force(esp_engine* e, const char* name, double out[3])
+{
+  API_BEGIN
+  const int w = find_wall(e, name ? name : "");
+  if (w < 0) throw EspError(ESP_ERR_ARG, "unknown wall");
+  for (int c = 0; c < 3; c++) out[c] = e->walls[w].force[c];
+  return ESP_OK;
+  API_END
+}
+
+int esp_get_wall_force_history(esp_engine* e, const char* name, int n, double* out)
+{
+  API_BEGIN
+  const int w = find_wall(e, name ? name : "");
+  if (w < 0) throw EspError(ESP_ERR_ARG, "unknown wall");
+  const int have = (int)e->walls[w].hist.size() / 3;
+  const int m = std::min(n, have);
+  if (out && m > 0)
+    memcpy(out, e->walls[w].hist.data() + 3 * (size_t)(have - m), sizeof(double) * 3 * m);
+  return m;
+  API_END
+}
+
+int esp_get_interaction_count(esp_engine* e, const char* name)
+{
+  const int d = find_dg(e, name ? name : "");
+  if (d >= 0) return e->dg[d].n;
+  const int b = find_bg(e, name ? name : "");
+  if (b >= 0) return e->bg[b].n;
+  return -1;
+}
+
+int esp_get_interactions(esp_engine* e, const char* name, int* id1, int* id2)
+{
+  API_BEGIN
+  const int d = find_dg(e, name ? name : "");
+  if (d >= 0) {
+    DynGroup& G = e->dg[d];
+    std::vector<long long> k(std::max(G.n, 1));
+    d2h(e, k.data(), G.key.p, G.n);
+    sync(e);
+    for (int i = 0; i < G.n; i++) {
+      id1[i] = (int)(k[i] >> 32);
+      id2[i] = (int)(k[i] & 0xffffffffLL);
+    }
+    return G.n;
+  }
+  const int b = find_bg(e, name ? name : "");
+  if (b < 0) throw EspError(ESP_ERR_ARG, "unknown interaction group");
+  BondGroup& G = e->bg[b];
+  std::vector<long long> k(G.n);
+  for (int i = 0; i < G.n; i++) k[i] = pkey(G.hid1[i], G.hid2[i]);
+  std::sort(k.begin(), k.end());
+  for (int i = 0; i < G.n; i++) {
+    id1[i] = (int)(k[i] >> 32);
+    id2[i] = (int)(k[i] & 0xffffffffLL);
+  }
+  return G.n;
+  API_END
+}
+
+int esp_get_friction_state(esp_engine* e, const char* name, double* ffric, int* flg)
+{
+  API_BEGIN
+  const int d = find_dg(e, name ? name : "");
+  if (d < 0 || e->dg[d].kind != 3) throw EspError(ESP_ERR_ARG, "not a RotFriction group");
+  DynGroup& G = e->dg[d];
+  std::vector<double4> f(std::max(G.n, 1));
+  d2h(e, f.data(), G.ff.p, G.n);
+  sync(e);
+  for (int i = 0; i < G.n; i++) {
+    ffric[3 * i] = f[i].x;
+    ffric[3 * i + 1] = f[i].y;
+    ffric[3 * i + 2] = f[i].z;
+    flg[i] = (int)f[i].w;
+  }
+  return G.n;
+  API_END
+}
+
+int esp_get_neighbour_pairs(esp_engine* e, int* id1, int* id2)
+{
+  API_BEGIN
+  if (id1 && id2 && e->ncand > 0) {
+    std::vector<long long> k(e->ncand);
+    d2h(e, k.data(), e->cand_key.p, e->ncand);
+    sync(e);
+    for (int i = 0; i < e->ncand; i++) {
+      id1[i] = (int)(k[i] >> 32);
+      id2[i] = (int)(k[i] & 0xffffffffLL);
+    }
+  }
+  return e->ncand;
+  API_END
+}
+
+int esp_get_bond_breaks(esp_engine* e, int* step, int* id1, int* id2)
+{
+  API_BEGIN
+  const int n = (int)e->breaks.size() / 3;
+  if (step && id1 && id2) {
+    std::vector<std::array<int, 3>> v(n);
+    for (int i = 0; i < n; i++) v[i] = {e->breaks[3 * i], e->breaks[3 * i + 1], e->breaks[3 * i + 2]};
+    std::sort(v.begin(), v.end());
+    for (int i = 0; i < n; i++) { step[i] = v[i][0]; id1[i] = v[i][1]; id2[i] = v[i][2]; }
+  }
+  return n;
+  API_END
+}
+
+int esp_get_rebuild_steps(esp_engine* e, int* steps)
+{
+  API_BEGIN
+  if (steps && !e->rebuild_steps.empty())
+    memcpy(steps, e->rebuild_steps.data(), sizeof(int) * e->rebuild_steps.size());
+  return (int)e->rebuild_steps.size();
+  API_END
+}
+
+}  // extern "C"

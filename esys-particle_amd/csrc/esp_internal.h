@@ -1,0 +1,155 @@
+// esp_internal.h - device data layout and kernel launchers of the engine.
+//
+// Layout in HBM (structure of arrays; every per-particle field is one array of
+// double4 so that one 32-byte load fetches a field for a gathered neighbour):
+//   pos[i]    = (x, y, z, radius)
+//   vel[i]    = (vx, vy, vz, 1/mass)
+//   ang[i]    = (wx, wy, wz, 1/inertia)        rotational engines only
+//   quat[i]   = (s, x, y, z)                   rotational engines only
+//   old[i]    = (x, y, z, mass)                position at the last neighbour search
+//   frc[i], mom[i]                             total force / moment of the last step
+//   flags[i]  = dynamic | rotational<<1 | bond-broken flag<<2 |
+//               wall-group membership<<8 (8 bits) | wall-group inner<<16 (8 bits)
+// Interactions are structure-of-arrays per group (pair slots int2, history
+// double4); every interaction writes one 9-double record (F1, M1, M2) per step
+// into a shared buffer, and a per-particle CSR of record references sums them
+// in a fixed order - the deterministic, atomic-free force reduction.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#define ESP_MAX_WALL_GROUPS 8
+#define ESP_MAX_GRAVITY 4
+#define ESP_MAX_EXCL 4
+
+#define PF_DYN 1u
+#define PF_ROT 2u
+#define PF_FLAG 4u
+#define PF_WALL_SHIFT 8
+#define PF_INNER_SHIFT 16
+
+// Status of one time step, written by that step's kernels.
+struct StepStatus {
+  unsigned long long maxdisp2;  // bits of max |x - x_old|^2 (non-negative doubles order as integers)
+  int nbroken;                  // bonds whose break criterion held in this step
+  int pad;
+};
+
+// Per-step guard shared by the kernels of one queued step.
+struct StepCtl {
+  const StepStatus* prev;   // status of the previous step
+  StepStatus* cur;          // status of this step
+  int* halted;              // 0, or 1 + index of the first step that needs the host
+  unsigned long long thresh;  // bits of (verletDist/2)^2
+  int step_id;              // 1 + index of this step in its batch
+  int guarded;              // 0 for steps launched after host bookkeeping
+  int first;                // the kernel that records the halt
+};
+
+struct GridParams {
+  double x0, y0, z0, dim;
+  int gx, gy, gz;  // global index of the table's minimum corner (-1)
+  int nx, ny, nz;
+};
+
+struct PView {
+  double4* pos;
+  double4* vel;
+  double4* ang;
+  double4* quat;
+  double4* old;
+  double4* frc;
+  double4* mom;
+  unsigned* flags;
+  int* id;
+  int n;
+};
+
+struct WallParam {
+  double o[3];
+  double nrm[3];
+  double k;
+};
+
+struct GatherParams {
+  const int* inc_off;
+  const int* inc_ent;
+  const double* contrib;
+  double dt;
+  int rot, do2d;
+  int nwg;
+  WallParam wall[ESP_MAX_WALL_GROUPS];
+  int ngrav;
+  double g[ESP_MAX_GRAVITY][3];
+  double* wall_part;  // [block][nwg][3]
+};
+
+struct ExclSet {
+  const long long* keys[ESP_MAX_EXCL];
+  int n[ESP_MAX_EXCL];
+  int count;
+};
+
+// ---- per-step kernels ----
+void launch_rot_bond(hipStream_t s, StepCtl ctl, PView p, int nb, const int2* ij, const double4* k4,
+                     const double4* mx4, const double4* D4, double truncated, double beta1,
+                     double beta2, double* contrib, int rec_off, int bond_off, int* broken_list);
+void launch_rot_friction(hipStream_t s, StepCtl ctl, PView p, int nc, const int2* ij, double4* ff,
+                         double k, double ks, int scaling, int meanR, int do2d, double mu_d,
+                         double mu_s, double dt, double* contrib, int rec_off);
+void launch_nrot_elastic(hipStream_t s, StepCtl ctl, PView p, int nc, const int2* ij, double k,
+                         int scaling, int do2d, double* contrib, int rec_off);
+void launch_rot_elastic(hipStream_t s, StepCtl ctl, PView p, int nc, const int2* ij, double k,
+                        int scaling, int do2d, double* contrib, int rec_off);
+void launch_gather_integrate(hipStream_t s, StepCtl ctl, PView p, const GatherParams& gp);
+void launch_wall_finalize(hipStream_t s, StepCtl ctl, const double* wall_part, int nblocks,
+                          int nwg, double* out);
+void launch_clear_status(hipStream_t s, StepStatus* ring, int ring_size, long long first_step,
+                         int count, int* halted);
+int gather_blocks(int n);
+
+// ---- rebuild / update kernels ----
+void launch_cells_count(hipStream_t s, PView p, GridParams g, int* cell, int* cell_count,
+                        int* n_out);
+void launch_cells_fill(hipStream_t s, int n, const int* cell, const int* cell_start,
+                       int* cell_fill, int* cell_items);
+void launch_cells_sort(hipStream_t s, int ncell, const int* cell_start, int* cell_items);
+void launch_pairs(hipStream_t s, int mode_emit, int flagged, PView p, GridParams g, double alpha,
+                  const int* by_id, const int* cell, const int* cell_start, const int* cell_items,
+                  int* cnt, const int* off, int* out_p, int* out_q, long long* out_key);
+void launch_clear_flag_bit(hipStream_t s, PView p);
+void launch_ed_mark(hipStream_t s, int nc, const int* cp, const int* cq, const long long* ckey,
+                    ExclSet ex, const long long* okey, int nold, const int2* oij, PView p,
+                    int* keep, int* src);
+void launch_ne_mark(hipStream_t s, int nc, const long long* ckey, ExclSet ex, int* keep);
+void launch_pair_emit(hipStream_t s, int nc, const int* keep, const int* off, const int* cp,
+                      const int* cq, const long long* ckey, const int* src, const double4* off_ff,
+                      int2* nij, long long* nkey, double4* nff);
+void launch_filter_new(hipStream_t s, int nc, const long long* ckey, ExclSet ex,
+                       const long long* okey, int nold, int check_present, int* keep);
+void launch_merge(hipStream_t s, int nold, const int2* oij, const long long* okey,
+                  const double4* off_, int nnew, const int2* nij_in, const long long* nkey_in,
+                  int2* ij_out, long long* key_out, double4* ff_out);
+void launch_pack_pairs(hipStream_t s, int n, const int* keep, const int* off, const int* cp,
+                       const int* cq, const long long* ckey, int2* ij, long long* key);
+void launch_bond_mark_broken(hipStream_t s, int nbroken, const int* list, int bond_off, int nb,
+                             int* removed, PView p, const int2* ij);
+void launch_compact_bonds(hipStream_t s, int nb, const int* removed, const int* off,
+                          const int2* ij, const double4* k4, const double4* mx4,
+                          const double4* D4, int2* ij2, double4* k42, double4* mx42,
+                          double4* D42);
+void launch_keys_mark_removed(hipStream_t s, int nk, const long long* keys, int nrem,
+                              const long long* rem_sorted, int* keep);
+void launch_keys_compact(hipStream_t s, int nk, const int* keep, const int* off,
+                         const long long* keys, long long* out);
+void launch_wall_update(hipStream_t s, PView p, GridParams g, int nwg, const WallParam* walls);
+void launch_inc_count(hipStream_t s, int nc, const int2* ij, int* inc_cnt);
+void launch_inc_fill(hipStream_t s, int nc, const int2* ij, int rec_off, int has_m,
+                     const int* inc_off, int* inc_fill, int* inc_ent);
+void launch_inc_sort(hipStream_t s, int n, const int* inc_off, int* inc_ent);
+
+// exclusive scan of n ints into out (out[n] = total); tmp sized by scan_tmp_size
+size_t scan_tmp_size(int n);
+void launch_exclusive_scan(hipStream_t s, const int* in, int* out, int n, int* tmp);

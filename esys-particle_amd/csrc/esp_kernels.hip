@@ -1,0 +1,990 @@
+// esp_kernels.hip - HIP kernels of the per-timestep DEM engine (gfx950, FP64).
+//
+// Per step (launched back to back on one stream, no host synchronisation):
+//   k_rot_bond / k_rot_friction / k_nrot_elastic / k_rot_elastic
+//       one thread per interaction; each writes its force record (F1, M1, M2)
+//   k_gather_integrate
+//       one thread per particle: wall and body forces, then the records of its
+//       interactions in a fixed order (CSR), then the CParticle/CRotParticle
+//       first-order update and the max-displacement reduction
+//   k_wall_finalize
+//       fixed-order sum of the per-block wall-force partials
+// Every per-step kernel first evaluates the previous step's status; if that
+// step displaced a particle by more than verletDist/2 or broke a bond, the host
+// must do bookkeeping first, so the kernel records the halt and returns, and
+// all later queued steps return too.
+//
+// Rebuild / update kernels (launched from the host's bookkeeping path):
+//   cell binning (counting sort + per-cell ordering), candidate pair search over
+//   the 27-cell neighbourhood emitting pairs sorted by (id1, id2), interaction
+//   storage updates (history carried over by key lookup), bond removal,
+//   wall candidate refresh and the per-particle CSR of force records.
+#include "esp_internal.h"
+#include "esp_math.h"
+
+#define BS 256
+
+static inline int nblk(long long n, int bs = BS) { return (int)((n + bs - 1) / bs); }
+
+__device__ __forceinline__ dv3 xyz(double4 a) { return mk3(a.x, a.y, a.z); }
+
+__device__ __forceinline__ bool step_skip(const StepCtl& c)
+{
+  if (!c.guarded) return false;
+  if (*(volatile int*)c.halted) return true;
+  const unsigned long long md = c.prev->maxdisp2;
+  const int nb = c.prev->nbroken;
+  if (md > c.thresh || nb > 0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *c.halted = c.step_id;
+    return true;
+  }
+  return false;
+}
+
+// ---------------------------------------------------------------------------
+// per-step interaction kernels
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(BS) void k_rot_bond(StepCtl ctl, PView p, int nb, const int2* __restrict__ ij,
+                                                 const double4* __restrict__ k4,
+                                                 const double4* __restrict__ mx4,
+                                                 const double4* __restrict__ D4, double truncated,
+                                                 double beta1, double beta2, double* contrib,
+                                                 int rec_off, int bond_off, int* broken_list)
+{
+  if (step_skip(ctl)) return;
+  const int b = blockIdx.x * BS + threadIdx.x;
+  if (b >= nb) return;
+  const int2 e = ij[b];
+  const double4 P1 = p.pos[e.x], P2 = p.pos[e.y];
+  const double4 Q1 = p.quat[e.x], Q2 = p.quat[e.y];
+  const double4 K = k4[b], D = D4[b];
+  BondResult o;
+  esp_rot_bond_law(xyz(P1), dquat{Q1.x, mk3(Q1.y, Q1.z, Q1.w)}, P1.w, xyz(P2),
+                   dquat{Q2.x, mk3(Q2.y, Q2.z, Q2.w)}, P2.w, K.x, K.y, K.z, K.w, xyz(D), o);
+  double* r = contrib + 9 * (size_t)(rec_off + b);
+  r[0] = o.f1.x; r[1] = o.f1.y; r[2] = o.f1.z;
+  r[3] = o.m1.x; r[4] = o.m1.y; r[5] = o.m1.z;
+  r[6] = o.m2.x; r[7] = o.m2.y; r[8] = o.m2.z;
+  const double4 M = mx4[b];
+  if (esp_bond_broken(o.nForce, o.shForce, o.tMoment, o.bMoment, M.x, M.y, M.z, M.w, truncated,
+                      beta1, beta2)) {
+    const int slot = atomicAdd(&ctl.cur->nbroken, 1);
+    broken_list[slot] = bond_off + b;
+  }
+}
+
+__global__ __launch_bounds__(BS) void k_rot_friction(StepCtl ctl, PView p, int nc, const int2* __restrict__ ij,
+                                                     double4* __restrict__ ff, double k, double ks,
+                                                     int scaling, int meanR, int do2d, double mu_d,
+                                                     double mu_s, double dt, double* contrib,
+                                                     int rec_off)
+{
+  if (step_skip(ctl)) return;
+  const int c = blockIdx.x * BS + threadIdx.x;
+  if (c >= nc) return;
+  const int2 e = ij[c];
+  const double4 P1 = p.pos[e.x], P2 = p.pos[e.y];
+  double* r = contrib + 9 * (size_t)(rec_off + c);
+  const dv3 Dq = sub3(xyz(P1), xyz(P2));
+  const double eq = P1.w + P2.w;
+  if (!(dot3(Dq, Dq) < (eq * eq))) {  // not touching: history cleared, no force
+    ff[c] = make_double4(0.0, 0.0, 0.0, 0.0);
+    for (int t = 0; t < 9; t++) r[t] = 0.0;
+    return;
+  }
+  const double4 st = ff[c];
+  const double4 V1 = p.vel[e.x], V2 = p.vel[e.y], W1 = p.ang[e.x], W2 = p.ang[e.y];
+  double kk, kks;
+  esp_friction_k(k, ks, scaling, meanR, P1.w, P2.w, do2d, kk, kks);
+  dv3 ffr = mk3(st.x, st.y, st.z);
+  bool slipping = (((int)st.w) & 1) != 0;
+  dv3 f1, m1, m2;
+  esp_rot_friction_law(xyz(P1), xyz(V1), xyz(W1), P1.w, xyz(P2), xyz(V2), xyz(W2), P2.w, kk, kks,
+                       mu_d, mu_s, dt, ffr, slipping, f1, m1, m2);
+  ff[c] = make_double4(ffr.x, ffr.y, ffr.z, (double)((slipping ? 1 : 0) | 2));
+  r[0] = f1.x; r[1] = f1.y; r[2] = f1.z;
+  r[3] = m1.x; r[4] = m1.y; r[5] = m1.z;
+  r[6] = m2.x; r[7] = m2.y; r[8] = m2.z;
+}
+
+// CElasticInteraction::calcForces (ElasticInteraction.cpp:54-71)
+__global__ __launch_bounds__(BS) void k_nrot_elastic(StepCtl ctl, PView p, int nc, const int2* __restrict__ ij,
+                                                     double k, int scaling, int do2d,
+                                                     double* contrib, int rec_off)
+{
+  if (step_skip(ctl)) return;
+  const int c = blockIdx.x * BS + threadIdx.x;
+  if (c >= nc) return;
+  const int2 e = ij[c];
+  const double4 P1 = p.pos[e.x], P2 = p.pos[e.y];
+  double* r = contrib + 9 * (size_t)(rec_off + c);
+  const dv3 D = sub3(xyz(P1), xyz(P2));
+  double dist = dot3(D, D);
+  const double eq_dist = P1.w + P2.w;
+  if (dist < (eq_dist * eq_dist)) {
+    const double kk = esp_elastic_k(k, scaling, P1.w, P2.w, do2d);
+    dist = sqrt(dist);
+    const dv3 force = mul3(D, kk * (dist - eq_dist) / dist);
+    r[0] = -force.x; r[1] = -force.y; r[2] = -force.z;
+  } else {
+    r[0] = 0.0; r[1] = 0.0; r[2] = 0.0;
+  }
+}
+
+// CRotElasticInteraction::calcForces (RotElasticInteraction.cpp:69-88)
+__global__ __launch_bounds__(BS) void k_rot_elastic(StepCtl ctl, PView p, int nc, const int2* __restrict__ ij,
+                                                    double k, int scaling, int do2d,
+                                                    double* contrib, int rec_off)
+{
+  if (step_skip(ctl)) return;
+  const int c = blockIdx.x * BS + threadIdx.x;
+  if (c >= nc) return;
+  const int2 e = ij[c];
+  const double4 P1 = p.pos[e.x], P2 = p.pos[e.y];
+  double* r = contrib + 9 * (size_t)(rec_off + c);
+  const dv3 rb = sub3(xyz(P1), xyz(P2));
+  const double rbNorm = norm3(rb);
+  const double r_0Norm = P1.w + P2.w;
+  if (rbNorm < r_0Norm) {
+    const dv3 delta_r = div3(mul3(rb, rbNorm - r_0Norm), rbNorm);
+    const dv3 force = mul3(delta_r, esp_rotelastic_k(k, scaling, P1.w, P2.w, do2d));
+    r[0] = -force.x; r[1] = -force.y; r[2] = -force.z;
+  } else {
+    r[0] = 0.0; r[1] = 0.0; r[2] = 0.0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// gather + integrate
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
+{
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long w = __shfl_xor(v, o, 64);
+    v = w > v ? w : v;
+  }
+  return v;
+}
+
+__global__ __launch_bounds__(BS) void k_gather_integrate(StepCtl ctl, PView p, GatherParams gp)
+{
+  if (step_skip(ctl)) return;
+  __shared__ double red[BS];
+  __shared__ unsigned long long mred[BS / 64];
+  const int i = blockIdx.x * BS + threadIdx.x;
+  double wf[ESP_MAX_WALL_GROUPS][3];
+  for (int w = 0; w < ESP_MAX_WALL_GROUPS; w++) wf[w][0] = wf[w][1] = wf[w][2] = 0.0;
+  unsigned long long md = 0;
+  if (i < p.n) {
+    const double4 P = p.pos[i];
+    const double4 Vv = p.vel[i];
+    const double4 O = p.old[i];
+    const unsigned fl = p.flags[i];
+    dv3 x = xyz(P), v = xyz(Vv);
+    const double r = P.w, invm = Vv.w, mass = O.w;
+    dv3 F = mk3(0.0, 0.0, 0.0), M = mk3(0.0, 0.0, 0.0);
+    // wall interaction groups, name order (EWallInteraction.hpp:42-78)
+    for (int w = 0; w < gp.nwg; w++) {
+      if (!((fl >> (PF_WALL_SHIFT + w)) & 1u)) continue;
+      const WallParam& W = gp.wall[w];
+      const dv3 n = mk3(W.nrm[0], W.nrm[1], W.nrm[2]);
+      const double dist = dot3(sub3(x, mk3(W.o[0], W.o[1], W.o[2])), n);
+      if (dist < r) {
+        double f = 1.0;
+        if (!gp.do2d) f *= 3.141592654 * r;
+        const dv3 force = mul3(n, (f * W.k) * (r - dist));
+        F = add3(F, force);
+        if ((fl >> (PF_INNER_SHIFT + w)) & 1u) {
+          wf[w][0] = force.x; wf[w][1] = force.y; wf[w][2] = force.z;
+        }
+      }
+    }
+    // body forces, name order (BodyForceGroup.hpp:41-65)
+    for (int g = 0; g < gp.ngrav; g++) F = add3(F, mul3(mk3(gp.g[g][0], gp.g[g][1], gp.g[g][2]), mass));
+    // interaction records: dynamic groups then bonded groups, each in storage order
+    const int e0 = gp.inc_off[i], e1 = gp.inc_off[i + 1];
+    for (int t = e0; t < e1; t++) {
+      const int ent = gp.inc_ent[t];
+      const double* rec = gp.contrib + 9 * (size_t)(ent >> 2);
+      dv3 f = mk3(rec[0], rec[1], rec[2]);
+      if (ent & 1) f = neg3(f);
+      F = add3(F, f);
+      if (ent & 2) {
+        const double* m = rec + 3 + 3 * (ent & 1);
+        M = add3(M, mk3(m[0], m[1], m[2]));
+      }
+    }
+    const double dt = gp.dt;
+    if (gp.rot) {
+      const double4 A = p.ang[i];
+      const double4 Q = p.quat[i];
+      dv3 w = xyz(A);
+      dquat q = dquat{Q.x, mk3(Q.y, Q.z, Q.w)};
+      if (fl & PF_ROT) {
+        if (gp.do2d) {
+          F = mk3(F.x, F.y, 0);
+          M = mk3(0, 0, M.z);
+        }
+        w = add3(w, mul3(M, dt * A.w));
+        const dquat dq = qmul4(dquat{0, w}, q);
+        const double h = dt / 2.0;
+        q.s += h * dq.s;
+        q.v = add3(q.v, mul3(dq.v, h));
+      } else {
+        w = mk3(0.0, 0.0, 0.0);
+      }
+      if (fl & PF_DYN) {
+        if (gp.do2d) F = mk3(F.x, F.y, 0);
+        v = add3(v, mul3(mul3(F, invm), dt));
+        x = add3(x, mul3(v, dt));
+      }
+      const double module = sqrt(q.s * q.s + q.v.x * q.v.x + q.v.y * q.v.y + q.v.z * q.v.z);
+      if (module != 0.0) {
+        const double inverse = 1.0 / module;
+        q.s = inverse * q.s;
+        q.v = mul3(q.v, inverse);
+      }
+      p.ang[i] = make_double4(w.x, w.y, w.z, A.w);
+      p.quat[i] = make_double4(q.s, q.v.x, q.v.y, q.v.z);
+      p.mom[i] = make_double4(M.x, M.y, M.z, 0.0);
+    } else if (fl & PF_DYN) {
+      if (gp.do2d) F = mk3(F.x, F.y, 0);
+      v = add3(v, mul3(mul3(F, invm), dt));
+      x = add3(x, mul3(v, dt));
+    }
+    p.pos[i] = make_double4(x.x, x.y, x.z, r);
+    p.vel[i] = make_double4(v.x, v.y, v.z, invm);
+    p.frc[i] = make_double4(F.x, F.y, F.z, 0.0);
+    const dv3 d = sub3(x, xyz(O));
+    md = (unsigned long long)__double_as_longlong(dot3(d, d));
+  }
+  // max displacement: wave, then block, then one atomic per block
+  md = wave_max_u64(md);
+  if ((threadIdx.x & 63) == 0) mred[threadIdx.x >> 6] = md;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long m = mred[0];
+    for (int w = 1; w < BS / 64; w++) m = mred[w] > m ? mred[w] : m;
+    if (m) atomicMax(&ctl.cur->maxdisp2, m);
+  }
+  // wall-force partials: fixed-order tree per block
+  for (int w = 0; w < gp.nwg; w++) {
+    for (int c = 0; c < 3; c++) {
+      __syncthreads();
+      red[threadIdx.x] = wf[w][c];
+      __syncthreads();
+      for (int s = BS / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + s];
+        __syncthreads();
+      }
+      if (threadIdx.x == 0) gp.wall_part[((size_t)blockIdx.x * gp.nwg + w) * 3 + c] = red[0];
+    }
+  }
+}
+
+__global__ __launch_bounds__(BS) void k_wall_finalize(StepCtl ctl, const double* part, int nblocks, int nwg,
+                                                      double* out)
+{
+  if (step_skip(ctl)) return;
+  __shared__ double red[BS];
+  for (int w = 0; w < nwg; w++)
+    for (int c = 0; c < 3; c++) {
+      double s = 0.0;
+      for (int b = threadIdx.x; b < nblocks; b += BS) s = s + part[((size_t)b * nwg + w) * 3 + c];
+      __syncthreads();
+      red[threadIdx.x] = s;
+      __syncthreads();
+      for (int h = BS / 2; h > 0; h >>= 1) {
+        if (threadIdx.x < h) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + h];
+        __syncthreads();
+      }
+      if (threadIdx.x == 0) out[w * 3 + c] = red[0];
+    }
+}
+
+__global__ void k_clear_status(StepStatus* ring, int ring_size, long long first_step, int count,
+                               int* halted)
+{
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < count) {
+    StepStatus& s = ring[(first_step + t) % ring_size];
+    s.maxdisp2 = 0ull;
+    s.nbroken = 0;
+    s.pad = 0;
+  }
+  if (t == 0) *halted = 0;
+}
+
+// ---------------------------------------------------------------------------
+// neighbour table
+// ---------------------------------------------------------------------------
+// NeighborTable::index (ntable.hpp:128-139): floor((x - x0)/dim) - global index,
+// with values outside the int range (and NaN) mapped to "outside" as x86's
+// conversion does.
+__device__ __forceinline__ bool cell_coords(const GridParams& g, double x, double y, double z,
+                                            int& ix, int& iy, int& iz)
+{
+  const double fx = floor((x - g.x0) / g.dim);
+  const double fy = floor((y - g.y0) / g.dim);
+  const double fz = floor((z - g.z0) / g.dim);
+  if (!(fx >= -2147483648.0 && fx < 2147483648.0 && fy >= -2147483648.0 && fy < 2147483648.0 &&
+        fz >= -2147483648.0 && fz < 2147483648.0))
+    return false;
+  ix = (int)fx - g.gx;
+  iy = (int)fy - g.gy;
+  iz = (int)fz - g.gz;
+  return true;
+}
+
+__global__ void k_cells_count(PView p, GridParams g, int* cell, int* cell_count, int* n_out)
+{
+  const int i = blockIdx.x * BS + threadIdx.x;
+  if (i >= p.n) return;
+  const double4 P = p.pos[i];
+  int ix, iy, iz, c = -1;
+  if (cell_coords(g, P.x, P.y, P.z, ix, iy, iz) && ix >= 0 && ix < g.nx && iy >= 0 && iy < g.ny &&
+      iz >= 0 && iz < g.nz)
+    c = g.ny * g.nz * ix + g.nz * iy + iz;
+  cell[i] = c;
+  if (c < 0) atomicAdd(n_out, 1);
+  else atomicAdd(&cell_count[c], 1);
+  // resetDisplacement (Particle.h:124)
+  const double4 O = p.old[i];
+  p.old[i] = make_double4(P.x, P.y, P.z, O.w);
+}
+
+__global__ void k_cells_fill(int n, const int* cell, const int* cell_start, int* cell_fill,
+                             int* cell_items)
+{
+  const int i = blockIdx.x * BS + threadIdx.x;
+  if (i >= n) return;
+  const int c = cell[i];
+  if (c < 0) return;
+  const int k = atomicAdd(&cell_fill[c], 1);
+  cell_items[cell_start[c] + k] = i;
+}
+
+// cell contents in particle-array order
+__global__ void k_cells_sort(int ncell, const int* cell_start, int* items)
+{
+  const int c = blockIdx.x * BS + threadIdx.x;
+  if (c >= ncell) return;
+  const int a = cell_start[c], b = cell_start[c + 1];
+  for (int i = a + 1; i < b; i++) {
+    const int v = items[i];
+    int j = i - 1;
+    while (j >= a && items[j] > v) {
+      items[j + 1] = items[j];
+      j--;
+    }
+    items[j + 1] = v;
+  }
+}
+
+__device__ __forceinline__ long long pair_key(int a, int b)
+{
+  return (long long)(((unsigned long long)(unsigned)a << 32) | (unsigned long long)(unsigned)b);
+}
+
+// Pairs of particle p (taken in ascending id order) with every particle q of
+// higher id in the 27 cells around p's cell: the full list keeps pairs with
+// |x_p - x_q|^2 <= (r_p + r_q + alpha)^2 (ntable.hpp:471-521); the flagged list
+// keeps pairs of two flagged particles without a distance test
+// (ntable.hpp:532-583).  The emit pass sorts each particle's run by the partner
+// id, so the whole list comes out sorted by (id1, id2).
+__global__ void k_pairs(int emit, int flagged, PView p, GridParams g, double alpha,
+                        const int* by_id, const int* cell, const int* cell_start,
+                        const int* cell_items, int* cnt, const int* off, int* out_p, int* out_q,
+                        long long* out_key)
+{
+  const int k = blockIdx.x * BS + threadIdx.x;
+  if (k >= p.n) return;
+  const int pi = by_id[k];
+  if (flagged && !(p.flags[pi] & PF_FLAG)) {
+    if (!emit) cnt[k] = 0;
+    return;
+  }
+  const int c = cell[pi];
+  const int cx = c / (g.ny * g.nz);
+  const int rem = c - cx * (g.ny * g.nz);
+  const int cy = rem / g.nz;
+  const int cz = rem - cy * g.nz;
+  const double4 P = p.pos[pi];
+  const int idp = p.id[pi];
+  const int base = emit ? off[k] : 0;
+  int count = 0;
+  const int x0 = cx > 0 ? cx - 1 : cx, x1 = cx < g.nx - 1 ? cx + 1 : cx;
+  const int y0 = cy > 0 ? cy - 1 : cy, y1 = cy < g.ny - 1 ? cy + 1 : cy;
+  const int z0 = cz > 0 ? cz - 1 : cz, z1 = cz < g.nz - 1 ? cz + 1 : cz;
+  for (int x = x0; x <= x1; x++)
+    for (int y = y0; y <= y1; y++)
+      for (int z = z0; z <= z1; z++) {
+        const int c2 = g.ny * g.nz * x + g.nz * y + z;
+        const int t1 = cell_start[c2 + 1];
+        for (int t = cell_start[c2]; t < t1; t++) {
+          const int q = cell_items[t];
+          const int idq = p.id[q];
+          if (idq <= idp) continue;
+          if (flagged) {
+            if (!(p.flags[q] & PF_FLAG)) continue;
+          } else {
+            const double4 Q = p.pos[q];
+            const dv3 d = sub3(xyz(P), xyz(Q));
+            const double dist2 = dot3(d, d);
+            const double dmax = P.w + Q.w + alpha;
+            if (!(dist2 <= (dmax * dmax))) continue;
+          }
+          if (emit) {
+            out_p[base + count] = pi;
+            out_q[base + count] = q;
+            out_key[base + count] = pair_key(idp, idq);
+          }
+          count++;
+        }
+      }
+  if (!emit) {
+    cnt[k] = count;
+    return;
+  }
+  for (int i = base + 1; i < base + count; i++) {
+    const long long kv = out_key[i];
+    const int qv = out_q[i];
+    int j = i - 1;
+    while (j >= base && out_key[j] > kv) {
+      out_key[j + 1] = out_key[j];
+      out_q[j + 1] = out_q[j];
+      j--;
+    }
+    out_key[j + 1] = kv;
+    out_q[j + 1] = qv;
+  }
+}
+
+__global__ void k_clear_flag_bit(PView p)
+{
+  const int i = blockIdx.x * BS + threadIdx.x;
+  if (i < p.n) p.flags[i] &= ~PF_FLAG;
+}
+
+__device__ __forceinline__ int bsearch_key(const long long* a, int n, long long key)
+{
+  int lo = 0, hi = n - 1;
+  while (lo <= hi) {
+    const int mid = (lo + hi) >> 1;
+    const long long v = a[mid];
+    if (v == key) return mid;
+    if (v < key) lo = mid + 1;
+    else hi = mid - 1;
+  }
+  return -1;
+}
+// number of elements < key (strict) or <= key
+__device__ __forceinline__ int lower_bound_key(const long long* a, int n, long long key)
+{
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (a[mid] < key) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+__device__ __forceinline__ int upper_bound_key(const long long* a, int n, long long key)
+{
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (a[mid] <= key) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+__device__ __forceinline__ bool excluded(const ExclSet& ex, long long key)
+{
+  for (int x = 0; x < ex.count; x++)
+    if (bsearch_key(ex.keys[x], ex.n[x], key) >= 0) return true;
+  return false;
+}
+
+// PIS_ED::update on a rebuilt table (pi_storage_ed.hpp:97-136): a candidate is
+// kept unless excluded; it inherits the history of the stored interaction with
+// the same key if that one is persistent (RotFricInteraction.cpp:345-351).
+__global__ void k_ed_mark(int nc, const int* cp, const int* cq, const long long* ckey, ExclSet ex,
+                          const long long* okey, int nold, const int2* oij, PView p, int* keep,
+                          int* src)
+{
+  const int c = blockIdx.x * BS + threadIdx.x;
+  if (c >= nc) return;
+  const long long key = ckey[c];
+  if (excluded(ex, key)) {
+    keep[c] = 0;
+    src[c] = -1;
+    return;
+  }
+  keep[c] = 1;
+  int s = -1;
+  const int j = bsearch_key(okey, nold, key);
+  if (j >= 0) {
+    const int2 e = oij[j];
+    const double4 P1 = p.pos[e.x], P2 = p.pos[e.y];
+    const dv3 D = sub3(xyz(P1), xyz(P2));
+    const double dist = dot3(D, D);
+    const double eq = P1.w + P2.w;
+    if (dist <= (eq * eq)) s = j;
+  }
+  src[c] = s;
+}
+
+__global__ void k_ne_mark(int nc, const long long* ckey, ExclSet ex, int* keep)
+{
+  const int c = blockIdx.x * BS + threadIdx.x;
+  if (c < nc) keep[c] = excluded(ex, ckey[c]) ? 0 : 1;
+}
+
+__global__ void k_pair_emit(int nc, const int* keep, const int* off, const int* cp, const int* cq,
+                            const long long* ckey, const int* src, const double4* off_ff,
+                            int2* nij, long long* nkey, double4* nff)
+{
+  const int c = blockIdx.x * BS + threadIdx.x;
+  if (c >= nc || !keep[c]) return;
+  const int w = off[c];
+  nij[w] = make_int2(cp[c], cq[c]);
+  nkey[w] = ckey[c];
+  if (nff) {
+    const int s = src ? src[c] : -1;
+    nff[w] = s >= 0 ? off_ff[s] : make_double4(0.0, 0.0, 0.0, 0.0);
+  }
+}
+
+// pairs of the flagged list that a group takes: not excluded and (for history
+// groups) not already stored (pi_storage_ed.hpp:137-156, pi_storage_ne.hpp:79-96)
+__global__ void k_filter_new(int nc, const long long* ckey, ExclSet ex, const long long* okey,
+                             int nold, int check_present, int* keep)
+{
+  const int c = blockIdx.x * BS + threadIdx.x;
+  if (c >= nc) return;
+  const long long key = ckey[c];
+  int k = excluded(ex, key) ? 0 : 1;
+  if (k && check_present && bsearch_key(okey, nold, key) >= 0) k = 0;
+  keep[c] = k;
+}
+
+__global__ void k_pack_pairs(int n, const int* keep, const int* off, const int* cp, const int* cq,
+                             const long long* ckey, int2* ij, long long* key)
+{
+  const int c = blockIdx.x * BS + threadIdx.x;
+  if (c >= n || !keep[c]) return;
+  const int w = off[c];
+  ij[w] = make_int2(cp[c], cq[c]);
+  key[w] = ckey[c];
+}
+
+// merge of two key-sorted lists; equal keys keep stored entries first
+__global__ void k_merge(int nold, const int2* oij, const long long* okey, const double4* off_,
+                        int nnew, const int2* nij_in, const long long* nkey_in, int2* ij_out,
+                        long long* key_out, double4* ff_out)
+{
+  const int t = blockIdx.x * BS + threadIdx.x;
+  if (t < nold) {
+    const long long key = okey[t];
+    const int w = t + lower_bound_key(nkey_in, nnew, key);
+    ij_out[w] = oij[t];
+    key_out[w] = key;
+    if (ff_out) ff_out[w] = off_[t];
+  } else if (t < nold + nnew) {
+    const int j = t - nold;
+    const long long key = nkey_in[j];
+    const int w = j + upper_bound_key(okey, nold, key);
+    ij_out[w] = nij_in[j];
+    key_out[w] = key;
+    if (ff_out) ff_out[w] = make_double4(0.0, 0.0, 0.0, 0.0);
+  }
+}
+
+// PIS_EB::update (pi_storage_eb.hpp:23-49): broken bonds leave; both
+// particles are flagged for the next flagged pair list.
+__global__ void k_bond_mark_broken(int nbroken, const int* list, int bond_off, int nb, int* keep,
+                                   PView p, const int2* ij)
+{
+  const int t = blockIdx.x * BS + threadIdx.x;
+  if (t >= nbroken) return;
+  const int b = list[t] - bond_off;
+  if (b < 0 || b >= nb) return;
+  keep[b] = 0;
+  const int2 e = ij[b];
+  atomicOr(&p.flags[e.x], PF_FLAG);
+  atomicOr(&p.flags[e.y], PF_FLAG);
+}
+
+__global__ void k_fill_int(int n, int* a, int v)
+{
+  const int t = blockIdx.x * BS + threadIdx.x;
+  if (t < n) a[t] = v;
+}
+
+__global__ void k_compact_bonds(int nb, const int* keep, const int* off, const int2* ij,
+                                const double4* k4, const double4* mx4, const double4* D4,
+                                int2* ij2, double4* k42, double4* mx42, double4* D42)
+{
+  const int b = blockIdx.x * BS + threadIdx.x;
+  if (b >= nb || !keep[b]) return;
+  const int w = off[b];
+  ij2[w] = ij[b];
+  k42[w] = k4[b];
+  mx42[w] = mx4[b];
+  D42[w] = D4[b];
+}
+
+__global__ void k_keys_mark_removed(int nk, const long long* keys, int nrem,
+                                    const long long* rem_sorted, int* keep)
+{
+  const int t = blockIdx.x * BS + threadIdx.x;
+  if (t < nk) keep[t] = bsearch_key(rem_sorted, nrem, keys[t]) >= 0 ? 0 : 1;
+}
+
+__global__ void k_keys_compact(int nk, const int* keep, const int* off, const long long* keys,
+                               long long* out)
+{
+  const int t = blockIdx.x * BS + threadIdx.x;
+  if (t < nk && keep[t]) out[off[t]] = keys[t];
+}
+
+// CEWallInteractionGroup::Update (EWallInteractionGroup.hpp:57-82): particles
+// with |(x - o).n| < gridSpacing become candidates; isInInner (ntable.hpp:207-222)
+// decides whether their force counts towards the wall force.
+__global__ void k_wall_update(PView p, GridParams g, int nwg, WallParam w0, WallParam w1,
+                              WallParam w2, WallParam w3, WallParam w4, WallParam w5,
+                              WallParam w6, WallParam w7)
+{
+  const int i = blockIdx.x * BS + threadIdx.x;
+  if (i >= p.n) return;
+  const WallParam W[ESP_MAX_WALL_GROUPS] = {w0, w1, w2, w3, w4, w5, w6, w7};
+  const double4 P = p.pos[i];
+  unsigned fl = p.flags[i] & ~((0xFFu << PF_WALL_SHIFT) | (0xFFu << PF_INNER_SHIFT));
+  int ix = 0, iy = 0, iz = 0;
+  bool inner = false;
+  if (cell_coords(g, P.x, P.y, P.z, ix, iy, iz)) {
+    if (g.nz < 3) inner = (ix > 0) && (ix < g.nx - 1) && (iy > 0) && (iy < g.ny - 1);
+    else
+      inner = (ix > 0) && (ix < g.nx - 1) && (iy > 0) && (iy < g.ny - 1) && (iz > 0) &&
+              (iz < g.nz - 1);
+  }
+  for (int w = 0; w < nwg; w++) {
+    const dv3 n = mk3(W[w].nrm[0], W[w].nrm[1], W[w].nrm[2]);
+    const double dist = dot3(sub3(xyz(P), mk3(W[w].o[0], W[w].o[1], W[w].o[2])), n);
+    if (fabs(dist) < g.dim) {
+      fl |= 1u << (PF_WALL_SHIFT + w);
+      if (inner) fl |= 1u << (PF_INNER_SHIFT + w);
+    }
+  }
+  p.flags[i] = fl;
+}
+
+// ---------------------------------------------------------------------------
+// per-particle CSR of force records
+// ---------------------------------------------------------------------------
+__global__ void k_inc_count(int nc, const int2* ij, int* cnt)
+{
+  const int c = blockIdx.x * BS + threadIdx.x;
+  if (c >= nc) return;
+  const int2 e = ij[c];
+  atomicAdd(&cnt[e.x], 1);
+  atomicAdd(&cnt[e.y], 1);
+}
+
+__global__ void k_inc_fill(int nc, const int2* ij, int rec_off, int has_m, const int* off,
+                           int* fill, int* ent)
+{
+  const int c = blockIdx.x * BS + threadIdx.x;
+  if (c >= nc) return;
+  const int2 e = ij[c];
+  const int v = ((rec_off + c) << 2) | (has_m << 1);
+  ent[off[e.x] + atomicAdd(&fill[e.x], 1)] = v;
+  ent[off[e.y] + atomicAdd(&fill[e.y], 1)] = v | 1;
+}
+
+__global__ void k_inc_sort(int n, const int* off, int* ent)
+{
+  const int i = blockIdx.x * BS + threadIdx.x;
+  if (i >= n) return;
+  const int a = off[i], b = off[i + 1];
+  for (int s = a + 1; s < b; s++) {
+    const int v = ent[s];
+    int j = s - 1;
+    while (j >= a && ent[j] > v) {
+      ent[j + 1] = ent[j];
+      j--;
+    }
+    ent[j + 1] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// exclusive scan (1024 elements per block, recursive over block sums)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(BS) void k_scan_local(const int* in, int nvalid, int* out, int m, int* bsum)
+{
+  __shared__ int s[BS];
+  const int base = blockIdx.x * 4 * BS + threadIdx.x * 4;
+  int v[4];
+  int acc = 0;
+  for (int k = 0; k < 4; k++) {
+    const int idx = base + k;
+    v[k] = (idx < nvalid) ? in[idx] : 0;
+    acc += v[k];
+  }
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = 1; o < BS; o <<= 1) {
+    const int t = threadIdx.x >= o ? s[threadIdx.x - o] : 0;
+    __syncthreads();
+    s[threadIdx.x] += t;
+    __syncthreads();
+  }
+  int run = s[threadIdx.x] - acc;  // exclusive prefix of this thread
+  for (int k = 0; k < 4; k++) {
+    const int idx = base + k;
+    if (idx < m) out[idx] = run;
+    run += v[k];
+  }
+  if (bsum && threadIdx.x == BS - 1) bsum[blockIdx.x] = s[BS - 1];
+}
+
+__global__ void k_scan_add(int* out, int m, const int* boff)
+{
+  const int idx = blockIdx.x * 4 * BS + threadIdx.x * 4;
+  const int add = boff[blockIdx.x];
+  for (int k = 0; k < 4; k++)
+    if (idx + k < m) out[idx + k] += add;
+}
+
+static size_t scan_tmp_rec(long long m)
+{
+  const long long nb = (m + 4 * BS - 1) / (4 * BS);
+  if (nb <= 1) return 0;
+  return (size_t)(2 * nb) + scan_tmp_rec(nb);
+}
+
+size_t scan_tmp_size(int n) { return scan_tmp_rec((long long)n + 1) + 16; }
+
+static void scan_rec(hipStream_t s, const int* in, int nvalid, int* out, int m, int* tmp)
+{
+  const int nb = (m + 4 * BS - 1) / (4 * BS);
+  int* bsum = nb > 1 ? tmp : nullptr;
+  hipLaunchKernelGGL(k_scan_local, dim3(nb), dim3(BS), 0, s, in, nvalid, out, m, bsum);
+  if (nb > 1) {
+    int* boff = tmp + nb;
+    scan_rec(s, bsum, nb, boff, nb, tmp + 2 * nb);
+    hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(BS), 0, s, out, m, boff);
+  }
+}
+
+void launch_exclusive_scan(hipStream_t s, const int* in, int* out, int n, int* tmp)
+{
+  scan_rec(s, in, n, out, n + 1, tmp);
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+int gather_blocks(int n) { return n > 0 ? nblk(n) : 0; }
+
+void launch_rot_bond(hipStream_t s, StepCtl ctl, PView p, int nb, const int2* ij, const double4* k4,
+                     const double4* mx4, const double4* D4, double truncated, double beta1,
+                     double beta2, double* contrib, int rec_off, int bond_off, int* broken_list)
+{
+  if (nb <= 0) return;
+  hipLaunchKernelGGL(k_rot_bond, dim3(nblk(nb)), dim3(BS), 0, s, ctl, p, nb, ij, k4, mx4, D4,
+                     truncated, beta1, beta2, contrib, rec_off, bond_off, broken_list);
+}
+
+void launch_rot_friction(hipStream_t s, StepCtl ctl, PView p, int nc, const int2* ij, double4* ff,
+                         double k, double ks, int scaling, int meanR, int do2d, double mu_d,
+                         double mu_s, double dt, double* contrib, int rec_off)
+{
+  if (nc <= 0) return;
+  hipLaunchKernelGGL(k_rot_friction, dim3(nblk(nc)), dim3(BS), 0, s, ctl, p, nc, ij, ff, k, ks,
+                     scaling, meanR, do2d, mu_d, mu_s, dt, contrib, rec_off);
+}
+
+void launch_nrot_elastic(hipStream_t s, StepCtl ctl, PView p, int nc, const int2* ij, double k,
+                         int scaling, int do2d, double* contrib, int rec_off)
+{
+  if (nc <= 0) return;
+  hipLaunchKernelGGL(k_nrot_elastic, dim3(nblk(nc)), dim3(BS), 0, s, ctl, p, nc, ij, k, scaling,
+                     do2d, contrib, rec_off);
+}
+
+void launch_rot_elastic(hipStream_t s, StepCtl ctl, PView p, int nc, const int2* ij, double k,
+                        int scaling, int do2d, double* contrib, int rec_off)
+{
+  if (nc <= 0) return;
+  hipLaunchKernelGGL(k_rot_elastic, dim3(nblk(nc)), dim3(BS), 0, s, ctl, p, nc, ij, k, scaling,
+                     do2d, contrib, rec_off);
+}
+
+void launch_gather_integrate(hipStream_t s, StepCtl ctl, PView p, const GatherParams& gp)
+{
+  if (p.n <= 0) return;
+  hipLaunchKernelGGL(k_gather_integrate, dim3(nblk(p.n)), dim3(BS), 0, s, ctl, p, gp);
+}
+
+void launch_wall_finalize(hipStream_t s, StepCtl ctl, const double* wall_part, int nblocks,
+                          int nwg, double* out)
+{
+  if (nwg <= 0) return;
+  hipLaunchKernelGGL(k_wall_finalize, dim3(1), dim3(BS), 0, s, ctl, wall_part, nblocks, nwg, out);
+}
+
+void launch_clear_status(hipStream_t s, StepStatus* ring, int ring_size, long long first_step,
+                         int count, int* halted)
+{
+  hipLaunchKernelGGL(k_clear_status, dim3(nblk(count > 0 ? count : 1)), dim3(BS), 0, s, ring,
+                     ring_size, first_step, count, halted);
+}
+
+void launch_cells_count(hipStream_t s, PView p, GridParams g, int* cell, int* cell_count,
+                        int* n_out)
+{
+  if (p.n <= 0) return;
+  hipLaunchKernelGGL(k_cells_count, dim3(nblk(p.n)), dim3(BS), 0, s, p, g, cell, cell_count, n_out);
+}
+
+void launch_cells_fill(hipStream_t s, int n, const int* cell, const int* cell_start,
+                       int* cell_fill, int* cell_items)
+{
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_cells_fill, dim3(nblk(n)), dim3(BS), 0, s, n, cell, cell_start, cell_fill,
+                     cell_items);
+}
+
+void launch_cells_sort(hipStream_t s, int ncell, const int* cell_start, int* cell_items)
+{
+  if (ncell <= 0) return;
+  hipLaunchKernelGGL(k_cells_sort, dim3(nblk(ncell)), dim3(BS), 0, s, ncell, cell_start, cell_items);
+}
+
+void launch_pairs(hipStream_t s, int mode_emit, int flagged, PView p, GridParams g, double alpha,
+                  const int* by_id, const int* cell, const int* cell_start, const int* cell_items,
+                  int* cnt, const int* off, int* out_p, int* out_q, long long* out_key)
+{
+  if (p.n <= 0) return;
+  hipLaunchKernelGGL(k_pairs, dim3(nblk(p.n)), dim3(BS), 0, s, mode_emit, flagged, p, g, alpha,
+                     by_id, cell, cell_start, cell_items, cnt, off, out_p, out_q, out_key);
+}
+
+void launch_clear_flag_bit(hipStream_t s, PView p)
+{
+  if (p.n <= 0) return;
+  hipLaunchKernelGGL(k_clear_flag_bit, dim3(nblk(p.n)), dim3(BS), 0, s, p);
+}
+
+void launch_ed_mark(hipStream_t s, int nc, const int* cp, const int* cq, const long long* ckey,
+                    ExclSet ex, const long long* okey, int nold, const int2* oij, PView p,
+                    int* keep, int* src)
+{
+  if (nc <= 0) return;
+  hipLaunchKernelGGL(k_ed_mark, dim3(nblk(nc)), dim3(BS), 0, s, nc, cp, cq, ckey, ex, okey, nold,
+                     oij, p, keep, src);
+}
+
+void launch_ne_mark(hipStream_t s, int nc, const long long* ckey, ExclSet ex, int* keep)
+{
+  if (nc <= 0) return;
+  hipLaunchKernelGGL(k_ne_mark, dim3(nblk(nc)), dim3(BS), 0, s, nc, ckey, ex, keep);
+}
+
+void launch_pair_emit(hipStream_t s, int nc, const int* keep, const int* off, const int* cp,
+                      const int* cq, const long long* ckey, const int* src, const double4* off_ff,
+                      int2* nij, long long* nkey, double4* nff)
+{
+  if (nc <= 0) return;
+  hipLaunchKernelGGL(k_pair_emit, dim3(nblk(nc)), dim3(BS), 0, s, nc, keep, off, cp, cq, ckey, src,
+                     off_ff, nij, nkey, nff);
+}
+
+void launch_filter_new(hipStream_t s, int nc, const long long* ckey, ExclSet ex,
+                       const long long* okey, int nold, int check_present, int* keep)
+{
+  if (nc <= 0) return;
+  hipLaunchKernelGGL(k_filter_new, dim3(nblk(nc)), dim3(BS), 0, s, nc, ckey, ex, okey, nold,
+                     check_present, keep);
+}
+
+void launch_merge(hipStream_t s, int nold, const int2* oij, const long long* okey,
+                  const double4* off_, int nnew, const int2* nij_in, const long long* nkey_in,
+                  int2* ij_out, long long* key_out, double4* ff_out)
+{
+  if (nold + nnew <= 0) return;
+  hipLaunchKernelGGL(k_merge, dim3(nblk((long long)nold + nnew)), dim3(BS), 0, s, nold, oij, okey,
+                     off_, nnew, nij_in, nkey_in, ij_out, key_out, ff_out);
+}
+
+void launch_pack_pairs(hipStream_t s, int n, const int* keep, const int* off, const int* cp,
+                       const int* cq, const long long* ckey, int2* ij, long long* key)
+{
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_pack_pairs, dim3(nblk(n)), dim3(BS), 0, s, n, keep, off, cp, cq, ckey, ij, key);
+}
+
+void launch_bond_mark_broken(hipStream_t s, int nbroken, const int* list, int bond_off, int nb,
+                             int* keep, PView p, const int2* ij)
+{
+  if (nb <= 0) return;
+  hipLaunchKernelGGL(k_fill_int, dim3(nblk(nb)), dim3(BS), 0, s, nb, keep, 1);
+  if (nbroken <= 0) return;
+  hipLaunchKernelGGL(k_bond_mark_broken, dim3(nblk(nbroken)), dim3(BS), 0, s, nbroken, list,
+                     bond_off, nb, keep, p, ij);
+}
+
+void launch_compact_bonds(hipStream_t s, int nb, const int* keep, const int* off,
+                          const int2* ij, const double4* k4, const double4* mx4,
+                          const double4* D4, int2* ij2, double4* k42, double4* mx42,
+                          double4* D42)
+{
+  if (nb <= 0) return;
+  hipLaunchKernelGGL(k_compact_bonds, dim3(nblk(nb)), dim3(BS), 0, s, nb, keep, off, ij, k4, mx4,
+                     D4, ij2, k42, mx42, D42);
+}
+
+void launch_keys_mark_removed(hipStream_t s, int nk, const long long* keys, int nrem,
+                              const long long* rem_sorted, int* keep)
+{
+  if (nk <= 0) return;
+  hipLaunchKernelGGL(k_keys_mark_removed, dim3(nblk(nk)), dim3(BS), 0, s, nk, keys, nrem,
+                     rem_sorted, keep);
+}
+
+void launch_keys_compact(hipStream_t s, int nk, const int* keep, const int* off,
+                         const long long* keys, long long* out)
+{
+  if (nk <= 0) return;
+  hipLaunchKernelGGL(k_keys_compact, dim3(nblk(nk)), dim3(BS), 0, s, nk, keep, off, keys, out);
+}
+
+void launch_wall_update(hipStream_t s, PView p, GridParams g, int nwg, const WallParam* w)
+{
+  if (p.n <= 0) return;
+  hipLaunchKernelGGL(k_wall_update, dim3(nblk(p.n)), dim3(BS), 0, s, p, g, nwg, w[0], w[1], w[2],
+                     w[3], w[4], w[5], w[6], w[7]);
+}
+
+void launch_inc_count(hipStream_t s, int nc, const int2* ij, int* inc_cnt)
+{
+  if (nc <= 0) return;
+  hipLaunchKernelGGL(k_inc_count, dim3(nblk(nc)), dim3(BS), 0, s, nc, ij, inc_cnt);
+}
+
+void launch_inc_fill(hipStream_t s, int nc, const int2* ij, int rec_off, int has_m,
+                     const int* inc_off, int* inc_fill, int* inc_ent)
+{
+  if (nc <= 0) return;
+  hipLaunchKernelGGL(k_inc_fill, dim3(nblk(nc)), dim3(BS), 0, s, nc, ij, rec_off, has_m, inc_off,
+                     inc_fill, inc_ent);
+}
+
+void launch_inc_sort(hipStream_t s, int n, const int* inc_off, int* inc_ent)
+{
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_inc_sort, dim3(nblk(n)), dim3(BS), 0, s, n, inc_off, inc_ent);
+}

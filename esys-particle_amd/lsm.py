@@ -1,0 +1,475 @@
+"""Python facade mirroring the ESyS-Particle scripting API for one subdomain.
+
+A script written against ``esys.lsm`` (``LsmMpi``, ``NRotElasticPrms``,
+``RotFrictionPrms``, ``RotBondPrms``, ``BrittleBeamPrms``, ``GravityPrms``,
+``NRotElasticWallPrms``, ``createInteractionGroup``, ``runTimeStep`` ...) drives
+the MI355X engine through the C-ABI in ``include/esp.h``.  Method names, argument
+names and defaults follow the reference's Boost.Python bindings
+(Python/esys/lsm/LsmMpiPy.cpp:2140-3380, InteractionParamsPy.cpp:1216-2422).
+
+The whole particle set of the model lives on one GPU: ``numWorkerProcesses`` and
+``mpiDimList`` are accepted for source compatibility and ignored.
+"""
+import math
+
+import numpy as np
+
+from . import _native
+
+
+# --------------------------------------------------------------------------
+# value types (esys.lsm.util)
+# --------------------------------------------------------------------------
+class Vec3:
+    """3-vector (Foundation/vec3.h)."""
+
+    __slots__ = ("x", "y", "z")
+
+    def __init__(self, x=0.0, y=0.0, z=0.0):
+        if hasattr(x, "__len__") and not isinstance(x, str):
+            x, y, z = x
+        self.x, self.y, self.z = float(x), float(y), float(z)
+
+    def __getitem__(self, i):
+        return (self.x, self.y, self.z)[i]
+
+    def __len__(self):
+        return 3
+
+    def __iter__(self):
+        return iter((self.x, self.y, self.z))
+
+    def X(self):
+        return self.x
+
+    def Y(self):
+        return self.y
+
+    def Z(self):
+        return self.z
+
+    def __add__(self, o):
+        return Vec3(self.x + o[0], self.y + o[1], self.z + o[2])
+
+    def __sub__(self, o):
+        return Vec3(self.x - o[0], self.y - o[1], self.z - o[2])
+
+    def __mul__(self, s):
+        return Vec3(self.x * s, self.y * s, self.z * s)
+
+    __rmul__ = __mul__
+
+    def norm(self):
+        return math.sqrt(self.x * self.x + self.y * self.y + self.z * self.z)
+
+    def __repr__(self):
+        return "Vec3(%r, %r, %r)" % (self.x, self.y, self.z)
+
+
+class BoundingBox:
+    def __init__(self, minPt, maxPt):
+        self.minPt = Vec3(minPt)
+        self.maxPt = Vec3(maxPt)
+
+    def getMinPt(self):
+        return self.minPt
+
+    def getMaxPt(self):
+        return self.maxPt
+
+
+class _Sphere:
+    def __init__(self, id, posn, radius, mass=None):
+        self.id = int(id)
+        self.posn = Vec3(posn)
+        self.radius = float(radius)
+        self.mass = None if mass is None else float(mass)
+        self.tag = 0
+
+    def getId(self):
+        return self.id
+
+    def getTag(self):
+        return self.tag
+
+    def setTag(self, tag):
+        self.tag = int(tag)
+
+    def getPosn(self):
+        return self.posn
+
+    def getRadius(self):
+        return self.radius
+
+    def getMass(self):
+        return self.mass
+
+
+class NRotSphere(_Sphere):
+    """NRotSphere(id, posn, radius, mass) (ParticlePy.cpp)."""
+
+
+class RotSphere(_Sphere):
+    """RotSphere(id, posn, radius, mass) (RotParticlePy.cpp:36-39)."""
+
+
+class TaggedIdConnection:
+    """Bond connection (id1, id2, tag) for createConnections."""
+
+    def __init__(self, id1, id2, tag=0):
+        self.id1, self.id2, self.tag = int(id1), int(id2), int(tag)
+
+    def getP1Id(self):
+        return self.id1
+
+    def getP2Id(self):
+        return self.id2
+
+    def getTag(self):
+        return self.tag
+
+
+class Runnable:
+    """Base class of pre/post time-step runnables (Foundation/Runnable.h)."""
+
+    def run(self):
+        raise NotImplementedError
+
+
+# --------------------------------------------------------------------------
+# interaction parameter classes (InteractionParamsPy.cpp)
+# --------------------------------------------------------------------------
+class InteractionPrms:
+    def getName(self):
+        return self.name
+
+
+class NRotElasticPrms(InteractionPrms):
+    """NRotElasticPrms(name, normalK, scaling=True) (InteractionParamsPy.cpp:195-226)."""
+
+    def __init__(self, name, normalK, scaling=True):
+        self.name, self.normalK, self.scaling = name, float(normalK), bool(scaling)
+
+
+class RotElasticPrms(InteractionPrms):
+    """RotElasticPrms(name, normalK, scaling=True) (RotElasticInteraction.cpp:23-30)."""
+
+    def __init__(self, name, normalK, scaling=True):
+        self.name, self.normalK, self.scaling = name, float(normalK), bool(scaling)
+
+
+class RotFrictionPrms(InteractionPrms):
+    """RotFrictionPrms(name, normalK, dynamicMu, staticMu, shearK, scaling=True,
+    rigid=False, meanR_scaling=True) (InteractionParamsPy.cpp:772-833)."""
+
+    def __init__(self, name, normalK, dynamicMu, staticMu, shearK, scaling=True,
+                 rigid=False, meanR_scaling=True):
+        self.name = name
+        self.normalK, self.dynamicMu, self.staticMu, self.shearK = (
+            float(normalK), float(dynamicMu), float(staticMu), float(shearK))
+        self.scaling, self.rigid, self.meanR_scaling = bool(scaling), bool(rigid), bool(meanR_scaling)
+
+
+class RotBondPrms(InteractionPrms):
+    """RotBondPrms(name, normalK, shearK, torsionK, bendingK, normalBrkForce,
+    shearBrkForce, torsionBrkForce, bendingBrkForce, tag, scaling=True,
+    meanR_scaling=True, truncated=1.0) (InteractionParamsPy.cpp:414-548).
+
+    ``perBondStrengths`` (extension): an (n, 4) array of (normal, shear, torsion,
+    bending) break values, one row per connection of ``tag`` in creation order,
+    which the reference can only express through its restart files."""
+
+    def __init__(self, name, normalK, shearK, torsionK, bendingK, normalBrkForce,
+                 shearBrkForce, torsionBrkForce, bendingBrkForce, tag, scaling=True,
+                 meanR_scaling=True, truncated=1.0, perBondStrengths=None):
+        self.name = name
+        self.kr, self.ks, self.kt, self.kb = (float(normalK), float(shearK), float(torsionK),
+                                              float(bendingK))
+        self.max_n, self.max_s, self.max_t, self.max_b = (
+            float(normalBrkForce), float(shearBrkForce), float(torsionBrkForce),
+            float(bendingBrkForce))
+        self.tag = int(tag)
+        self.scaling, self.meanR_scaling = bool(scaling), bool(meanR_scaling)
+        self.truncated = float(truncated)
+        self.beta1 = 1.0
+        self.beta2 = 1.0
+        self.perBondStrengths = perBondStrengths
+
+    def _c(self):
+        return {"kr": self.kr, "ks": self.ks, "kt": self.kt, "kb": self.kb,
+                "max_n": self.max_n, "max_s": self.max_s, "max_t": self.max_t,
+                "max_b": self.max_b, "tag": self.tag, "scaling": int(self.scaling),
+                "meanR_scaling": int(self.meanR_scaling), "truncated": self.truncated,
+                "beta1": self.beta1, "beta2": self.beta2}
+
+
+class BrittleBeamPrms(RotBondPrms):
+    """BrittleBeamPrms(name, youngsModulus, poissonsRatio, cohesion, tanAngle, tag,
+    meanR_scaling=True, truncated=1.0, beta1=1.0, beta2=1.0): stiffnesses and
+    strengths from elastic moduli (ARotBondedInteraction.cpp:91-109,
+    RotBondedInteraction.cpp:73-94).  As in the reference's worker
+    (RotSubLattice.hpp:123-166) beta1/beta2 do not reach the bonds and stay 1."""
+
+    def __init__(self, name, youngsModulus, poissonsRatio, cohesion, tanAngle, tag,
+                 meanR_scaling=True, truncated=1.0, beta1=1.0, beta2=1.0, perBondStrengths=None):
+        shear_modulus = youngsModulus / (2.0 * (1. + poissonsRatio))
+        kr = math.pi * youngsModulus
+        ks = math.pi * shear_modulus
+        kb = math.pi * youngsModulus
+        kt = math.pi * shear_modulus
+        RotBondPrms.__init__(self, name, kr, ks, kt, kb, math.pi * cohesion / tanAngle,
+                             math.pi * cohesion, math.pi * cohesion,
+                             math.pi * cohesion / tanAngle, tag, True, meanR_scaling,
+                             truncated, perBondStrengths)
+
+
+class GravityPrms(InteractionPrms):
+    """GravityPrms(name, acceleration) (InteractionParamsPy.cpp:917-925)."""
+
+    def __init__(self, name, acceleration):
+        self.name, self.acceleration = name, Vec3(acceleration)
+
+
+class NRotElasticWallPrms(InteractionPrms):
+    """NRotElasticWallPrms(name, wallName, normalK) (WallPrmsPy.cpp)."""
+
+    def __init__(self, name, wallName, normalK):
+        self.name, self.wallName, self.normalK = name, wallName, float(normalK)
+
+
+# --------------------------------------------------------------------------
+# LsmMpi
+# --------------------------------------------------------------------------
+_PTYPES = {"NRotSphere": _native.ESP_PTYPE_NROT, "RotSphere": _native.ESP_PTYPE_ROT}
+
+
+class LsmMpi:
+    """Model driver with the reference's ``esys.lsm.LsmMpi`` interface
+    (LsmMpiPy.cpp:196-1300) over one GPU-resident subdomain."""
+
+    def __init__(self, numWorkerProcesses=1, mpiDimList=(1, 1, 1), device=0, stepsPerBatch=32):
+        self._device = int(device)
+        self._steps_per_batch = int(stepsPerBatch)
+        self._ptype = None
+        self._grid = None
+        self._verlet = None
+        self._dt = 0.0
+        self._nts = 0
+        self._do2d = False
+        self._engine = None
+        self._pre = []
+        self._post = []
+
+    # the library this facade binds; the product path is the HIP engine
+    def _open_native(self):
+        return _native.load_hip_library()
+
+    # ---- model setup ----
+    def initNeighbourSearch(self, particleType, gridSpacing, verletDist):
+        if particleType not in _PTYPES:
+            raise ValueError("particle type %r not supported (NRotSphere, RotSphere)" % particleType)
+        self._ptype = _PTYPES[particleType]
+        self._grid = float(gridSpacing)
+        self._verlet = float(verletDist)
+
+    initNeighborSearch = initNeighbourSearch
+    initVerletModel = initNeighbourSearch
+
+    def getParticleType(self):
+        return {v: k for k, v in _PTYPES.items()}[self._ptype]
+
+    def force2dComputations(self, do2d=True):
+        if self._engine is not None:
+            raise RuntimeError("force2dComputations must precede setSpatialDomain")
+        self._do2d = bool(do2d)
+
+    def setSpatialDomain(self, bbox, maxPt=None):
+        if maxPt is not None:
+            bbox = BoundingBox(bbox, maxPt)
+        if self._ptype is None:
+            raise RuntimeError("initNeighbourSearch must be called before setSpatialDomain")
+        if self._engine is not None:
+            raise RuntimeError("spatial domain already set")
+        self._engine = _native.Engine(self._open_native(), self._ptype, self._grid, self._verlet,
+                                      list(bbox.getMinPt()), list(bbox.getMaxPt()), self._dt,
+                                      self._do2d, self._device, self._steps_per_batch)
+
+    def _e(self):
+        if self._engine is None:
+            raise RuntimeError("setSpatialDomain has not been called")
+        return self._engine
+
+    @property
+    def engine(self):
+        return self._e()
+
+    def setTimeStepSize(self, dt):
+        self._dt = float(dt)
+        if self._engine is not None:
+            self._engine.set_time_step_size(dt)
+
+    def getTimeStepSize(self):
+        return self._dt
+
+    def setNumTimeSteps(self, n):
+        self._nts = int(n)
+
+    def getNumTimeSteps(self):
+        return self._nts
+
+    def getTimeStep(self):
+        return self._e().stats()["time_step"]
+
+    # ---- particles ----
+    def createParticles(self, particles):
+        ps = list(particles)
+        if not ps:
+            return
+        ids = np.array([p.getId() for p in ps], np.int32)
+        pos = np.array([list(p.getPosn()) for p in ps], np.float64)
+        rad = np.array([p.getRadius() for p in ps], np.float64)
+        tags = np.array([p.getTag() for p in ps], np.int32)
+        masses = [p.getMass() for p in ps]
+        if any(m is None for m in masses):
+            mass = None if all(m is None for m in masses) else np.array(
+                [(m if m is not None else (r ** 2 if self._do2d else r ** 3))
+                 for m, r in zip(masses, rad)], np.float64)
+        else:
+            mass = np.array(masses, np.float64)
+        self._e().add_particles(ids, pos, rad, mass, tags)
+
+    def createParticle(self, particle):
+        self.createParticles([particle])
+
+    def createParticleArrays(self, ids, posn, radius, mass=None, tag=None, vel=None):
+        """Bulk particle creation from arrays (same semantics as createParticles)."""
+        self._e().add_particles(ids, posn, radius, mass, tag, vel)
+
+    def getNumParticles(self):
+        return self._e().num_particles()
+
+    def setParticleDensity(self, tag, mask, Density):
+        self._e().set_particle_density(tag, mask, Density)
+
+    def setParticleNonDynamic(self, tag):
+        self._e().set_particle_non_dynamic(tag)
+
+    def setParticleNonRotational(self, tag):
+        self._e().set_particle_non_rotational(tag)
+
+    # ---- connections / groups ----
+    def createConnections(self, connections):
+        cs = list(connections)
+        if not cs:
+            return
+        self._e().add_connections([c.getP1Id() for c in cs], [c.getP2Id() for c in cs],
+                                  [c.getTag() for c in cs])
+
+    def createConnectionArrays(self, id1, id2, tag=None):
+        """Bulk connection creation from arrays (same semantics as createConnections)."""
+        self._e().add_connections(id1, id2, tag)
+
+    def createInteractionGroup(self, prms):
+        e = self._e()
+        if isinstance(prms, RotBondPrms):
+            e.add_rot_bond_group(prms.name, prms._c(), prms.perBondStrengths)
+        elif isinstance(prms, NRotElasticPrms):
+            e.add_nrot_elastic_group(prms.name, prms.normalK, prms.scaling)
+        elif isinstance(prms, RotElasticPrms):
+            e.add_rot_elastic_group(prms.name, prms.normalK, prms.scaling)
+        elif isinstance(prms, RotFrictionPrms):
+            e.add_rot_friction_group(prms.name, {
+                "k_n": prms.normalK, "mu_d": prms.dynamicMu, "mu_s": prms.staticMu,
+                "k_s": prms.shearK, "scaling": int(prms.scaling), "rigid": int(prms.rigid),
+                "meanR_scaling": int(prms.meanR_scaling)})
+        elif isinstance(prms, GravityPrms):
+            e.add_gravity(prms.name, list(prms.acceleration))
+        elif isinstance(prms, NRotElasticWallPrms):
+            e.add_elastic_wall_group(prms.name, prms.wallName, prms.normalK)
+        else:
+            raise TypeError("interaction parameters %r are not supported" % type(prms).__name__)
+
+    def createGravity(self, prms):
+        self.createInteractionGroup(prms)
+
+    def createExclusion(self, interactionName1, interactionName2):
+        self._e().add_exclusion(interactionName1, interactionName2)
+
+    # ---- walls ----
+    def createWall(self, name, posn, normal):
+        self._e().add_wall(name, list(Vec3(posn)), list(Vec3(normal)))
+
+    def moveWallBy(self, name, move):
+        self._e().move_wall_by(name, list(Vec3(move)))
+
+    def setWallStepMotion(self, name, move):
+        """Move wall ``name`` by ``move`` at the start of every time step, as a
+        pre-time-step Runnable calling moveWallBy would, without a Python call per step."""
+        self._e().set_wall_step_motion(name, list(Vec3(move)))
+
+    def getWallPosition(self, name):
+        return Vec3(self._e().wall_position(name))
+
+    def getWallForce(self, name):
+        return Vec3(self._e().wall_force(name))
+
+    # ---- runnables / stepping ----
+    def addPreTimeStepRunnable(self, runnable):
+        self._pre.append(runnable)
+
+    def addPostTimeStepRunnable(self, runnable):
+        self._post.append(runnable)
+
+    def runTimeStep(self):
+        for r in self._pre:
+            r.run()
+        self._e().run(1)
+        for r in self._post:
+            r.run()
+
+    def runTimeSteps(self, n):
+        """n x runTimeStep; queued on the device in one call when no Python
+        runnables are registered."""
+        if self._pre or self._post:
+            for _ in range(int(n)):
+                self.runTimeStep()
+        else:
+            self._e().run(int(n))
+
+    def run(self):
+        remaining = self._nts - self.getTimeStep()
+        if remaining > 0:
+            self.runTimeSteps(remaining)
+
+    # ---- state access ----
+    def getParticleIds(self):
+        return self._e().ids()
+
+    def getParticleData(self, field):
+        """Per-particle field ('pos', 'vel', 'angvel', 'quat', 'force', 'moment',
+        'rad', 'mass', 'oldpos') as an array in ascending id order."""
+        return self._e().field(field)
+
+    def getParticlePosn(self, id):
+        ids = self._e().ids()
+        k = np.searchsorted(ids, id)
+        if k >= len(ids) or ids[k] != id:
+            raise KeyError(id)
+        return Vec3(self._e().field("pos")[k])
+
+    def getInteractionPairs(self, name):
+        return self._e().interactions(name)
+
+    def getNumInteractions(self, name):
+        return self._e().interaction_count(name)
+
+    def getBondBreaks(self):
+        return self._e().bond_breaks()
+
+    def getStats(self):
+        return self._e().stats()
+
+    def exit(self):
+        if self._engine is not None:
+            self._engine.close()
+            self._engine = None

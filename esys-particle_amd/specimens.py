@@ -1,0 +1,218 @@
+"""Seeded synthetic specimens for tests and benchmarks.
+
+Each builder returns a plain dict of numpy arrays and parameters; ``apply(sim, spec)``
+feeds it to an ``LsmMpi``-compatible object through the reference's setup calls
+(initNeighbourSearch, setSpatialDomain, createParticles, createConnections,
+createInteractionGroup, createExclusion, createWall ...), so the same specimen
+drives the HIP engine and the CPU oracle.
+
+Packing: random sequential insertion (small specimens) or the grow-and-relax
+packer of ``geometry.py`` (large ones); bonds join pairs with
+|x_i - x_j| <= 1.01 (r_i + r_j).
+"""
+import math
+
+import numpy as np
+
+from . import lsm
+
+
+def _cell_neighbours(pos, cutoff):
+    """All pairs (i<j) with |x_i - x_j| <= cutoff (cell-list brute force)."""
+    lo = pos.min(axis=0)
+    key = np.floor((pos - lo) / cutoff).astype(np.int64)
+    dims = key.max(axis=0) + 3
+    lin = (key[:, 0] + 1) * dims[1] * dims[2] + (key[:, 1] + 1) * dims[2] + (key[:, 2] + 1)
+    order = np.argsort(lin, kind="stable")
+    ls = lin[order]
+    pairs = []
+    for dx in (-1, 0, 1):
+        for dy in (-1, 0, 1):
+            for dz in (-1, 0, 1):
+                off = dx * dims[1] * dims[2] + dy * dims[2] + dz
+                tgt = lin + off
+                a = np.searchsorted(ls, tgt, "left")
+                b = np.searchsorted(ls, tgt, "right")
+                cnt = b - a
+                if cnt.sum() == 0:
+                    continue
+                i = np.repeat(np.arange(len(pos)), cnt)
+                j = order[np.concatenate([np.arange(x, y) for x, y in zip(a, b) if y > x])]
+                m = i < j
+                pairs.append(np.stack([i[m], j[m]], 1))
+    if not pairs:
+        return np.zeros((0, 2), np.int64)
+    pr = np.concatenate(pairs)
+    d = np.linalg.norm(pos[pr[:, 0]] - pos[pr[:, 1]], axis=1)
+    return pr[d <= cutoff]
+
+
+def rsa_packing(n, radii, box_lo, box_hi, seed, max_tries=200000, gap=0.0):
+    """Random sequential non-overlapping insertion inside [box_lo + r, box_hi - r]."""
+    rng = np.random.default_rng(seed)
+    box_lo, box_hi = np.asarray(box_lo, float), np.asarray(box_hi, float)
+    rmax = float(np.max(radii))
+    cs = 2 * rmax + gap
+    dims = np.maximum(((box_hi - box_lo) / cs).astype(int) + 1, 1)
+    grid = {}
+    pos = np.zeros((n, 3))
+    placed = 0
+    tries = 0
+    while placed < n:
+        tries += 1
+        if tries > max_tries:
+            raise RuntimeError("random packing too dense: placed %d of %d" % (placed, n))
+        r = radii[placed]
+        p = box_lo + r + rng.random(3) * (box_hi - box_lo - 2 * r)
+        c = tuple(((p - box_lo) / cs).astype(int))
+        ok = True
+        for dx in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dz in (-1, 0, 1):
+                    for j in grid.get((c[0] + dx, c[1] + dy, c[2] + dz), ()):
+                        if np.sum((pos[j] - p) ** 2) < (r + radii[j] + gap) ** 2:
+                            ok = False
+                            break
+                    if not ok:
+                        break
+                if not ok:
+                    break
+            if not ok:
+                break
+        if not ok:
+            continue
+        pos[placed] = p
+        grid.setdefault(c, []).append(placed)
+        placed += 1
+    return pos
+
+
+def bond_pairs(pos, rad, factor=1.01):
+    """Pairs with |x_i - x_j| <= factor (r_i + r_j), i < j."""
+    cand = _cell_neighbours(pos, 2 * factor * float(rad.max()))
+    d = np.linalg.norm(pos[cand[:, 0]] - pos[cand[:, 1]], axis=1)
+    keep = d <= factor * (rad[cand[:, 0]] + rad[cand[:, 1]])
+    return cand[keep]
+
+
+def gravity_settling(n=10000, seed=1, box=(20.0, 40.0, 20.0), rmin=0.2, rmax=0.5, dt=1e-4,
+                     k=1.0, wall_k=1.0, alpha=0.2):
+    """Non-rotational gravity settling: NRotSphere, radii U[rmin, rmax], random
+    sequential insertion, NRotElastic contacts, elastic bottom wall, gravity."""
+    rng = np.random.default_rng(seed)
+    rad = rng.uniform(rmin, rmax, n)
+    pos = rsa_packing(n, rad, (0, 0, 0), box, seed + 1)
+    return {
+        "ptype": "NRotSphere", "grid": 2 * rmax + alpha, "alpha": alpha, "dt": dt,
+        "domain": ((-1.0, -1.0, -1.0), (box[0] + 1.0, box[1] + 1.0, box[2] + 1.0)),
+        "ids": np.arange(n, dtype=np.int32), "pos": pos, "rad": rad,
+        "density": 1.0,
+        "elastic": [("elastic", k)],
+        "gravity": (0.0, -9.81, 0.0),
+        "walls": [("bottom", (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), wall_k, (0.0, 0.0, 0.0))],
+    }
+
+
+def bonded_compression(n=1000, seed=3, rmin=0.2, rmax=1.0, box=None, phi=0.3, dt=1e-5,
+                       kr=1e5, ks=3e4, kt=3e4, kb=3e4, coh_mean=50.0, coh_sd=5.0, mu=0.6,
+                       wall_speed=0.05, alpha=0.2, bond_factor=1.01, packing=None):
+    """Bonded rotational uniaxial compression: RotSphere, radii log-uniform in
+    [rmin, rmax], bonds for |x_i - x_j| <= 1.01 (r_i + r_j) with per-bond
+    tensile/shear cohesion ~ N(coh_mean, coh_sd) (max_n = max_b = pi T,
+    max_s = max_t = pi C), RotFriction mu on unbonded contacts, top and bottom
+    walls approaching at wall_speed.
+
+    ``box`` defaults to a 1:2:1 prism sized for solid fraction ``phi``;
+    ``packing`` = (pos, rad) overrides the packing."""
+    rng = np.random.default_rng(seed)
+    if packing is None:
+        rad = np.exp(rng.uniform(math.log(rmin), math.log(rmax), n))
+        rad = np.sort(rad)[::-1].copy()  # largest first: insertion stays feasible
+        if box is None:
+            vs = float(np.sum(4.0 / 3.0 * math.pi * rad ** 3))
+            s = (vs / phi / 2.0) ** (1.0 / 3.0)
+            box = (s, 2 * s, s)
+        pos = rsa_packing(n, rad, (0, 0, 0), box, seed + 1)
+    else:
+        pos, rad = packing
+        n = len(rad)
+        if box is None:
+            box = tuple(np.ceil(pos.max(axis=0) + rad.max()))
+    pr = bond_pairs(pos, rad, bond_factor)
+    brng = np.random.default_rng(seed + 2)
+    T = brng.normal(coh_mean, coh_sd, len(pr))
+    C = brng.normal(coh_mean, coh_sd, len(pr))
+    strengths = np.stack([math.pi * T, math.pi * C, math.pi * C, math.pi * T], 1)
+    ly = box[1]
+    v_step = wall_speed * dt
+    return {
+        "ptype": "RotSphere", "grid": 2 * rmax + alpha, "alpha": alpha, "dt": dt,
+        "domain": ((-1.0, -1.0, -1.0), (box[0] + 1.0, ly + 1.0, box[2] + 1.0)),
+        "ids": np.arange(n, dtype=np.int32), "pos": pos, "rad": rad, "density": 1.0,
+        "bonds": pr.astype(np.int32),
+        "bond_prms": dict(name="bonded", normalK=kr, shearK=ks, torsionK=kt, bendingK=kb,
+                          tag=0, strengths=strengths),
+        "friction": ("friction", kr, mu, mu, ks),
+        "exclusion": ("bonded", "friction"),
+        "walls": [("bottom", (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), kr, (0.0, v_step, 0.0)),
+                  ("top", (0.0, ly, 0.0), (0.0, -1.0, 0.0), kr, (0.0, -v_step, 0.0))],
+    }
+
+
+def lattice_block(nx=8, ny=12, nz=8, r=0.5, jitter=1e-3, rad_spread=0.0, seed=7, dt=1e-4,
+                  kr=1e5, ks=3e4, kt=3e4, kb=3e4, coh_mean=50.0, coh_sd=5.0, mu=0.6,
+                  wall_speed=0.05, vel_sd=0.0, alpha=0.2, bond_factor=1.01):
+    """Bonded rotational block on a jittered simple-cubic lattice (spacing 2r):
+    every lattice neighbour pair is bonded, so bonds start near equilibrium.
+    Same interaction set-up as bonded_compression (walls, friction, exclusion)."""
+    rng = np.random.default_rng(seed)
+    g = np.stack(np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij"), -1)
+    pos = (g.reshape(-1, 3) + 0.5) * (2 * r) + 1.0
+    n = len(pos)
+    pos = pos + rng.normal(0, jitter, pos.shape)
+    rad = r * (1 + rng.uniform(-rad_spread, rad_spread, n))
+    box = (nx * 2 * r + 2.0, ny * 2 * r + 2.0, nz * 2 * r + 2.0)
+    spec = bonded_compression(seed=seed, rmax=float(rad.max()), box=box, dt=dt, kr=kr, ks=ks,
+                              kt=kt, kb=kb, coh_mean=coh_mean, coh_sd=coh_sd, mu=mu,
+                              wall_speed=wall_speed, alpha=alpha, bond_factor=bond_factor,
+                              packing=(pos, rad))
+    if vel_sd > 0:
+        spec["vel"] = rng.normal(0, vel_sd, (n, 3))
+    return spec
+
+
+def apply(sim, spec, steps_per_batch=None):
+    """Set a specimen up on an LsmMpi-compatible simulation object."""
+    sim.initNeighbourSearch(spec["ptype"], spec["grid"], spec["alpha"])
+    sim.setTimeStepSize(spec["dt"])
+    lo, hi = spec["domain"]
+    sim.setSpatialDomain(lsm.BoundingBox(lsm.Vec3(*lo), lsm.Vec3(*hi)))
+    n = len(spec["ids"])
+    sim.createParticleArrays(spec["ids"], spec["pos"], spec["rad"],
+                             mass=spec["rad"] ** 3, tag=np.zeros(n, np.int32),
+                             vel=spec.get("vel"))
+    if spec.get("density") is not None:
+        sim.setParticleDensity(0, 0, spec["density"])
+    if "bonds" in spec:
+        b = spec["bonds"]
+        ids = spec["ids"]
+        sim.createConnectionArrays(ids[b[:, 0]], ids[b[:, 1]], np.zeros(len(b), np.int32))
+        bp = spec["bond_prms"]
+        sim.createInteractionGroup(lsm.RotBondPrms(
+            bp["name"], bp["normalK"], bp["shearK"], bp["torsionK"], bp["bendingK"],
+            1.0, 1.0, 1.0, 1.0, bp["tag"], perBondStrengths=bp.get("strengths")))
+    for name, k in spec.get("elastic", []):
+        sim.createInteractionGroup(lsm.NRotElasticPrms(name, k))
+    if "friction" in spec:
+        name, kn, mud, mus, kss = spec["friction"]
+        sim.createInteractionGroup(lsm.RotFrictionPrms(name, kn, mud, mus, kss))
+    if "exclusion" in spec:
+        sim.createExclusion(*spec["exclusion"])
+    if spec.get("gravity") is not None:
+        sim.createInteractionGroup(lsm.GravityPrms("gravity", lsm.Vec3(*spec["gravity"])))
+    for name, o, nrm, k, step in spec.get("walls", []):
+        sim.createWall(name, lsm.Vec3(*o), lsm.Vec3(*nrm))
+        sim.createInteractionGroup(lsm.NRotElasticWallPrms(name + "_wall", name, k))
+        if any(step):
+            sim.setWallStepMotion(name, lsm.Vec3(*step))
+    return sim

@@ -1,0 +1,214 @@
+/*
+ * esp.h - C-ABI of the MI355X per-timestep DEM particle engine.
+ *
+ * One handle (esp_engine) is one subdomain: one device-resident particle set
+ * with its neighbour table, interaction groups, walls and body forces.  Every
+ * entry point is extern "C", takes plain pointers and sizes, returns an int
+ * status (ESP_OK == 0) and never throws; the message of the last failure is
+ * available from esp_last_error().  Calls on one handle are not re-entrant and
+ * must come from one host thread; all device work runs on the handle's stream.
+ *
+ * Each declaration names the reference interface it replaces (paths relative
+ * to the ESyS-Particle 2.3.5 source tree).  In the reference these calls cross
+ * the Boost.Python -> CLatticeMaster -> MPI -> TSubLattice chain; here the
+ * Python facade (esys-particle_amd/lsm.py) calls them directly through ctypes.
+ *
+ * Particle arrays passed in or out are structure-of-arrays in the caller's
+ * order for inputs; outputs are ordered by ascending particle id.
+ */
+#ifndef ESP_H
+#define ESP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ESP_OK 0
+#define ESP_ERR_ARG -1      /* bad argument / unknown name */
+#define ESP_ERR_STATE -2    /* call not allowed in the current state */
+#define ESP_ERR_DEVICE -3   /* HIP runtime failure */
+#define ESP_ERR_CAPACITY -4 /* size above a documented limit */
+#define ESP_ERR_NUMERIC -5  /* non-finite state detected */
+
+#define ESP_PTYPE_NROT 0 /* "NRotSphere" -> CParticle   (LsmMpiPy.cpp:227-256) */
+#define ESP_PTYPE_ROT 1  /* "RotSphere"  -> CRotParticle */
+
+/* particle fields for esp_get_particle_data */
+#define ESP_FIELD_POS 0     /* 3 doubles per particle */
+#define ESP_FIELD_VEL 1     /* 3 */
+#define ESP_FIELD_ANGVEL 2  /* 3 (zeros for NRot) */
+#define ESP_FIELD_QUAT 3    /* 4: scalar, x, y, z (identity for NRot) */
+#define ESP_FIELD_FORCE 4   /* 3: total force of the last step */
+#define ESP_FIELD_MOMENT 5  /* 3: total moment of the last step */
+#define ESP_FIELD_RAD 6     /* 1 */
+#define ESP_FIELD_MASS 7    /* 1 */
+#define ESP_FIELD_OLDPOS 8  /* 3: position at the last neighbour search */
+
+/* Lattice configuration.  Mirrors LsmMpi.initNeighbourSearch(particleType,
+ * gridSpacing, verletDist) (LsmMpiPy.cpp:227-256, LatticeMaster.cpp:528-553),
+ * setSpatialDomain (TSubLattice::initNeighborTable, SubLattice.hpp:455-478),
+ * setTimeStepSize (LatticeMaster.cpp:574) and the 2-D switch
+ * (CParticle::setDo2dCalculations, Particle.h:185). */
+typedef struct esp_config {
+  int particle_type;     /* ESP_PTYPE_* */
+  double grid_spacing;   /* neighbour-table cell size, >= 2*r_max + verlet_dist */
+  double verlet_dist;    /* alpha: search cutoff added to r_i + r_j */
+  double domain_min[3];  /* spatial domain (grown to a multiple of grid_spacing) */
+  double domain_max[3];
+  double dt;             /* time step size */
+  int do_2d;             /* 1: particles move in the x-y plane only */
+  int device;            /* HIP device ordinal */
+  int steps_per_batch;   /* steps queued per host synchronisation (>=1) */
+} esp_config;
+
+/* RotBondPrms / BrittleBeamPrms after the Python-side mapping
+ * (InteractionParamsPy.cpp:414-667, RotBondedInteraction.cpp:40-94).
+ * beta1/beta2 are fixed at 1 by the reference's worker (RotSubLattice.hpp:123-166)
+ * and are exposed here for completeness. */
+typedef struct esp_rot_bond_prms {
+  double kr, ks, kt, kb;                 /* normalK, shearK, torsionK, bendingK */
+  double max_n, max_s, max_t, max_b;     /* normal/shear/torsion/bending break values */
+  int tag;                               /* connection tag the group binds */
+  int scaling;                           /* scale with particle size (default 1) */
+  int meanR_scaling;                     /* mean radius (1) or min radius (0) */
+  double truncated;                      /* tensile truncation (default 1) */
+  double beta1, beta2;                   /* bending / torsion weights (default 1) */
+} esp_rot_bond_prms;
+
+/* RotFrictionPrms (InteractionParamsPy.cpp:772-833, RotFricInteraction.cpp:34-57). */
+typedef struct esp_rot_friction_prms {
+  double k_n;        /* normalK */
+  double mu_d;       /* dynamicMu */
+  double mu_s;       /* staticMu */
+  double k_s;        /* shearK */
+  int scaling;       /* default 1 */
+  int rigid;         /* default 0; 1 is rejected (rigid-body variant not built) */
+  int meanR_scaling; /* default 1 */
+} esp_rot_friction_prms;
+
+/* Counters of one engine.  Interaction counts are the live totals. */
+typedef struct esp_stats {
+  long long time_step;       /* steps completed (CLatticeMaster::m_t) */
+  long long num_rebuilds;    /* neighbour-table rebuilds (ppa timestamp) */
+  long long num_particles;
+  long long num_bonds;       /* live bonds over all bonded groups */
+  long long num_dynamic;     /* interactions over all dynamic groups */
+  long long num_candidates;  /* pairs of the last full neighbour list */
+  long long num_broken;      /* bond-break events so far */
+  long long num_host_phases; /* steps that needed host-side bookkeeping */
+} esp_stats;
+
+typedef struct esp_engine esp_engine;
+
+/* ---- lifecycle ---------------------------------------------------------- */
+/* LsmMpi(...) + initNeighbourSearch + setSpatialDomain + setTimeStepSize
+ * (LsmMpiPy.cpp:196-256, 2991-3010; SubLattice.hpp:455-478). */
+int esp_create(const esp_config* cfg, esp_engine** out);
+void esp_destroy(esp_engine* e);
+/* Message of the last failure on this handle ("" when none). */
+const char* esp_last_error(const esp_engine* e);
+/* Message of the last esp_create failure (no handle exists then). */
+const char* esp_create_error(void);
+/* setTimeStepSize (LatticeMaster.cpp:574, SubLattice.hpp:2163-2210). */
+int esp_set_time_step_size(esp_engine* e, double dt);
+
+/* ---- particles ---------------------------------------------------------- */
+/* createParticles (LsmMpiPy.cpp:383-426) -> NeighborTable::insert
+ * (ntable.hpp:231-255): particles outside the padded grid are dropped.
+ * mass may be NULL (mass = r^3, SimpleParticleData.hpp:159-162),
+ * vel may be NULL (zero).  tag may be NULL (0). */
+int esp_add_particles(esp_engine* e, int n, const int* id, const int* tag,
+                      const double* pos, const double* rad, const double* mass,
+                      const double* vel);
+/* setParticleDensity (SubLattice.hpp:2990-3001, Particle.cpp:108-122,
+ * RotParticle.cpp:160-177). */
+int esp_set_particle_density(esp_engine* e, int tag, int mask, double rho);
+/* setParticleNonDynamic / setParticleNonRotational (Particle.h:137,
+ * RotParticle.h:204-206): applied to particles with the given tag. */
+int esp_set_particle_non_dynamic(esp_engine* e, int tag);
+int esp_set_particle_non_rotational(esp_engine* e, int tag);
+
+/* ---- bonds -------------------------------------------------------------- */
+/* createConnections (LsmMpiPy.cpp:284-288, SubLattice.hpp:600-620). */
+int esp_add_connections(esp_engine* e, int n, const int* id1, const int* id2,
+                        const int* tag);
+/* createInteractionGroup(RotBondPrms|BrittleBeamPrms): TRotSubLattice::addRotBondedIG
+ * (RotSubLattice.hpp:116-186) + PIS_E::tryInsert (pi_storage_e.hpp:216-250).
+ * per_bond_max (NULL or 4*n_per_bond doubles: max_n, max_s, max_t, max_b before
+ * size scaling) overrides the group strengths bond by bond, in the order of the
+ * group's connections; this carries the per-bond strengths the reference can only
+ * load from its restart format (RotBondedInteraction.cpp:406-428). */
+int esp_add_rot_bond_group(esp_engine* e, const char* name,
+                           const esp_rot_bond_prms* prms,
+                           const double* per_bond_max, int n_per_bond);
+
+/* ---- dynamic pair groups ------------------------------------------------ */
+/* createInteractionGroup(NRotElasticPrms): ParallelInteractionStorage_NE
+ * <T,CElasticInteraction> (SubLattice.hpp:1038-1054, ElasticInteraction.cpp:25-71). */
+int esp_add_nrot_elastic_group(esp_engine* e, const char* name, double k, int scaling);
+/* createInteractionGroup(RotElasticPrms): PIS_NE<CRotParticle,CRotElasticInteraction>
+ * (RotElasticInteraction.cpp:40-88). */
+int esp_add_rot_elastic_group(esp_engine* e, const char* name, double kr, int scaling);
+/* createInteractionGroup(RotFrictionPrms): PIS_ED<CRotParticle,CRotFrictionInteraction>
+ * (SubLattice.hpp:1202-1227, RotFricInteraction.cpp:103-231). */
+int esp_add_rot_friction_group(esp_engine* e, const char* name,
+                               const esp_rot_friction_prms* prms);
+/* createExclusion(bonded, dynamic): TSubLattice::setExIG (SubLattice.hpp:1980-2009)
+ * -> PIS_ED::addExIG / PIS_NE::addExIG (pi_storage_ed.hpp:20-40). */
+int esp_add_exclusion(esp_engine* e, const char* excluding, const char* excluded);
+
+/* ---- walls and body forces ---------------------------------------------- */
+/* createWall (LsmMpiPy.cpp:1145-1151, Wall.cpp:35-43). */
+int esp_add_wall(esp_engine* e, const char* name, const double origin[3],
+                 const double normal[3]);
+/* createInteractionGroup(NRotElasticWallPrms): CEWallInteractionGroup
+ * (EWallInteractionGroup.hpp:42-82, EWallInteraction.hpp:26-78). */
+int esp_add_elastic_wall_group(esp_engine* e, const char* name, const char* wall,
+                               double k);
+/* createInteractionGroup(GravityPrms): BodyForceGroup (BodyForceGroup.hpp:41-65). */
+int esp_add_gravity(esp_engine* e, const char* name, const double g[3]);
+/* moveWallBy (LsmMpiPy.cpp:1161, SubLattice.hpp:2878-2893): immediate. */
+int esp_move_wall_by(esp_engine* e, const char* name, const double d[3]);
+/* A pre-time-step runnable that calls moveWallBy(name, d) at the start of every
+ * step (compbench.py:20-35 pattern, LatticeMaster.cpp:2647).  d = 0 removes it. */
+int esp_set_wall_step_motion(esp_engine* e, const char* name, const double d[3]);
+
+/* ---- stepping ----------------------------------------------------------- */
+/* nsteps x runTimeStep (LsmMpiPy.cpp:1061-1064 -> CLatticeMaster::runOneStep,
+ * LatticeMaster.cpp:2602-2725 -> TSubLattice::checkNeighbors / searchNeighbors /
+ * updateInteractions / oneStep, SubLattice.hpp:2224-2536). */
+int esp_run_time_steps(esp_engine* e, int nsteps);
+
+/* ---- queries ------------------------------------------------------------ */
+int esp_get_num_particles(esp_engine* e);
+int esp_get_stats(esp_engine* e, esp_stats* out);
+/* Particle ids, ascending. */
+int esp_get_particle_ids(esp_engine* e, int* out);
+/* One ESP_FIELD_* for every particle, ordered by ascending id. */
+int esp_get_particle_data(esp_engine* e, int field, double* out);
+/* getWallPosition / getWallForce (LsmMpiPy.cpp:2767-2786, LatticeMaster.cpp:993-1019). */
+int esp_get_wall_position(esp_engine* e, const char* name, double out[3]);
+int esp_get_wall_force(esp_engine* e, const char* name, double out[3]);
+/* Wall force after each of the last `n` steps (oldest first); returns the count. */
+int esp_get_wall_force_history(esp_engine* e, const char* name, int n, double* out);
+/* Number of interactions stored in a group (dynamic or bonded), -1 if unknown. */
+int esp_get_interaction_count(esp_engine* e, const char* name);
+/* Particle-id pairs of a group (id1 < id2), sorted by (id1, id2). */
+int esp_get_interactions(esp_engine* e, const char* name, int* id1, int* id2);
+/* Shear-spring history of a RotFriction group, same order as esp_get_interactions:
+ * ffric 3 doubles per contact, flags bit0 = slipping, bit1 = touching. */
+int esp_get_friction_state(esp_engine* e, const char* name, double* ffric, int* flags);
+/* Pairs of the last full neighbour list (NeighborTable::getFullList,
+ * ntable.hpp:590-621), sorted by (id1, id2); returns the count when ids are NULL. */
+int esp_get_neighbour_pairs(esp_engine* e, int* id1, int* id2);
+/* Bond-break events (step, id1, id2) in (step, id1, id2) order; returns the count
+ * when the pointers are NULL. */
+int esp_get_bond_breaks(esp_engine* e, int* step, int* id1, int* id2);
+/* Steps at which the neighbour table was rebuilt; returns the count when NULL. */
+int esp_get_rebuild_steps(esp_engine* e, int* steps);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ESP_H */
