@@ -1,0 +1,182 @@
+"""Benchmark: bonded rotational uniaxial compression, 100,000 spheres, FP64.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+
+Workload (one step = one runTimeStep of the whole specimen on one GPU):
+  100,000 RotSphere, radii log-uniform in [0.2, 1.0], dense seeded packing in a
+  1:2:1 prism; bonds (RotBondPrms kr=1e5, ks=kt=kb=3e4) for pairs with
+  |x_i - x_j| <= 1.01 (r_i + r_j), per-bond tensile/shear cohesion ~ N(50, 5);
+  RotFriction mu=0.6 on unbonded contacts (excluded while bonded); top and bottom
+  elastic walls approaching at 0.05 per unit time; dt = 1e-5.
+
+Prints one JSON line: throughput in particle-steps/s over the timed steps, the
+roofline of the dominant kernel (algorithmic bytes / HIP-event-measured launch
+time against 8 TB/s), and the oracle (CPU restatement, 1 core) timed on a
+bounded sample of the same specimen.
+"""
+import argparse
+import importlib
+import importlib.util
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+HBM_PEAK_GBS = 8000.0
+
+# algorithmic bytes (compulsory state traffic) per unit, FP64 structure of arrays
+B_PARTICLE_ROT = 256  # read pos3 vel3 w3 q4 r 1/m 1/I old3 (19 doubles), write pos vel w q (13)
+B_BOND = 96           # i, j (2 x int32) + kr ks kt kb + 4 strengths + D0 (3 doubles)
+B_FRICTION = 64       # i, j + shear spring (3) + flags, read and write
+B_ELASTIC = 8         # i, j
+
+
+def _load(name, path, pkgdir=None):
+    if name in sys.modules:
+        return sys.modules[name]
+    spec = importlib.util.spec_from_file_location(
+        name, path, submodule_search_locations=[pkgdir] if pkgdir else None)
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def load_pkg():
+    d = os.path.join(ROOT, "esys-particle_amd")
+    return _load("esys_particle_amd", os.path.join(d, "__init__.py"), d)
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(pkg, spm, spec, budget_s):
+    """The oracle (C restatement of the reference path, one thread) stepping the
+    same specimen from its initial state until ~budget_s seconds have passed."""
+    orc = _load("esp_oracle_driver", os.path.join(ROOT, "oracle", "oracle.py"))
+    sim = orc.oracle_lsm_class(pkg)()
+    spm.apply(sim, spec)
+    n = len(spec["ids"])
+    sim.runTimeSteps(1)  # first step: initial neighbour search, excluded
+    steps, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < budget_s:
+        sim.runTimeSteps(10)
+        steps += 10
+    el = time.perf_counter() - t0
+    return {"value": steps * n / el, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+            "sample": "steps 2-%d of the same 100k bonded specimen (%.1f s)" % (steps + 1, el)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20000)
+    ap.add_argument("--warmup", type=int, default=500)
+    ap.add_argument("--particles", type=int, default=100000)
+    ap.add_argument("--batch", type=int, default=64, help="steps queued per host sync")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the engine holds one subdomain on one device "
+              "(multi-GPU runs are not part of this build)", file=sys.stderr)
+        sys.exit(2)
+
+    pkg = load_pkg()
+    spm = importlib.import_module("esys_particle_amd.specimens")
+    t0 = time.perf_counter()
+    spec = spm.dense_bonded_compression(n=args.particles, seed=3)
+    log("specimen: %d particles, %d bonds, box %s (%.1f s)" % (
+        len(spec["ids"]), len(spec["bonds"]), tuple(round(b, 2) for b in spec["box"]),
+        time.perf_counter() - t0))
+
+    sim = pkg.LsmMpi(device=0, stepsPerBatch=args.batch)
+    spm.apply(sim, spec)
+    eng = sim.engine
+    t0 = time.perf_counter()
+    sim.runTimeSteps(args.warmup)
+    log("warmup %d steps: %.2f s, stats %s" % (args.warmup, time.perf_counter() - t0, eng.stats()))
+
+    s0 = eng.stats()
+    t0 = time.perf_counter()
+    sim.runTimeSteps(args.steps)  # returns after the last step completed on the device
+    el = time.perf_counter() - t0
+    s1 = eng.stats()
+    n = s1["num_particles"]
+    value = n * args.steps / el
+    log("timed %d steps: %.3f s, stats %s" % (args.steps, el, s1))
+
+    # per-kernel device time, HIP events on the engine's stream, same step count
+    eng.set_kernel_timing(True)
+    t1 = time.perf_counter()
+    sim.runTimeSteps(args.steps)
+    el_t = time.perf_counter() - t1
+    kt = eng.kernel_times()
+    eng.set_kernel_timing(False)
+    s2 = eng.stats()
+    log("instrumented %d steps: %.3f s, kernels %s" % (args.steps, el_t, kt))
+
+    nb = 0.5 * (s1["num_bonds"] + s2["num_bonds"])
+    nd = 0.5 * (s1["num_dynamic"] + s2["num_dynamic"])
+    per_launch = {"bond": nb * B_BOND, "friction": nd * B_FRICTION,
+                  "gather_integrate": n * B_PARTICLE_ROT}
+    dom = max((k for k in per_launch if kt[k][1] > 0), key=lambda k: kt[k][0])
+    avg_s = kt[dom][0] / kt[dom][1] * 1e-3
+    achieved = per_launch[dom] / avg_s / 1e9
+    step_bytes = n * B_PARTICLE_ROT + nb * B_BOND + nd * B_FRICTION
+    ms_per_step = el / args.steps * 1e3
+    out = {
+        "metric": "particle-steps/s",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "bonded rotational uniaxial compression, %d RotSphere, radii "
+                        "log-uniform [0.2, 1.0], RotBond kr=1e5 ks=kt=kb=3e4, cohesion "
+                        "N(50,5), RotFriction mu=0.6, walls 0.05/time, dt=1e-5" % n,
+            "particles": n,
+            "bonds": int(s1["num_bonds"]),
+            "friction_pairs": int(s1["num_dynamic"]),
+            "prism": [round(b, 3) for b in spec["box"]],
+            "steps_per_sync": args.batch,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom,
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "algorithmic_bytes_per_launch": per_launch[dom],
+            "avg_launch_us": avg_s * 1e6,
+        },
+        "step_roofline": {
+            "algorithmic_bytes_per_step": step_bytes,
+            "achieved_GBs": step_bytes / (ms_per_step * 1e-3) / 1e9,
+            "frac": step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        },
+        "kernel_ms_per_step": {k: v[0] / max(args.steps, 1) for k, v in kt.items()},
+        "host_phases": int(s1["num_host_phases"] - s0["num_host_phases"]),
+        "bond_breaks_timed": int(s1["num_broken"] - s0["num_broken"]),
+    }
+    if not args.no_cpu_baseline:
+        t0 = time.perf_counter()
+        out["cpu_baseline"] = cpu_baseline(pkg, spm, spec, args.cpu_budget)
+        log("cpu baseline: %.1f s" % (time.perf_counter() - t0))
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -110,7 +110,11 @@ SIGNATURES = {
     "get_neighbour_pairs": (_i, [_vp, _pi, _pi]),
     "get_bond_breaks": (_i, [_vp, _pi, _pi, _pi]),
     "get_rebuild_steps": (_i, [_vp, _pi]),
+    "set_kernel_timing": (_i, [_vp, _i]),
+    "get_kernel_times": (_i, [_vp, _pd, ctypes.POINTER(ctypes.c_longlong)]),
 }
+
+KERNEL_CLASSES = ("bond", "friction", "elastic", "gather_integrate", "wall_reduce")
 
 
 class NativeLib:
@@ -359,6 +363,18 @@ class Engine:
         b = np.zeros(max(n, 1), np.int32)
         self._check(self.lib.get_bond_breaks(self.h, _ip(s), _ip(a), _ip(b)), "getBondBreaks")
         return np.stack([s[:n], a[:n], b[:n]], axis=1)
+
+    def set_kernel_timing(self, enable):
+        self._check(self.lib.set_kernel_timing(self.h, int(bool(enable))), "setKernelTiming")
+
+    def kernel_times(self):
+        """{class: (total_ms, launches)} since timing was enabled (resets)."""
+        ms = np.zeros(len(KERNEL_CLASSES))
+        n = np.zeros(len(KERNEL_CLASSES), np.int64)
+        self._check(self.lib.get_kernel_times(
+            self.h, _dp(ms), n.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong))),
+            "getKernelTimes")
+        return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(KERNEL_CLASSES)}
 
     def rebuild_steps(self):
         n = self._check(self.lib.get_rebuild_steps(self.h, None), "getRebuildSteps")

@@ -21,7 +21,21 @@ FLAGS = [
 ]
 
 
+GEOM_OUT = os.path.join(HERE, "libesp_geometry.so")
+
+
+def build_geometry(verbose=False):
+    cxx = os.environ.get("CXX", "g++")
+    cmd = [cxx, "-O3", "-std=c++17", "-fPIC", "-shared", "-fopenmp", os.path.join(CSRC, "geometry.cpp"),
+           "-o", GEOM_OUT + ".tmp"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(GEOM_OUT + ".tmp", GEOM_OUT)
+
+
 def build(verbose=False):
+    build_geometry(verbose)
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     tmp = OUT + ".tmp"
     cmd = [hipcc] + FLAGS + [os.path.join(CSRC, s) for s in SOURCES] + ["-o", tmp]

@@ -64,7 +64,10 @@ struct DBuf {
   void reserve(size_t n)
   {
     if (n <= cap && p) return;
-    if (p) HIPCHK(hipFree(p));
+    if (p) {
+      HIPCHK(hipDeviceSynchronize());  // queued kernels may still read the old buffer
+      HIPCHK(hipFree(p));
+    }
     p = nullptr;
     size_t c = std::max<size_t>(n + n / 4, 64);
     HIPCHK(hipMalloc(&p, c * sizeof(T)));
@@ -187,6 +190,19 @@ struct esp_engine {
   std::vector<int> breaks;  // step, id1, id2
   std::string err;
 
+  // kernel timing: one (start, stop, class) record per launch of a queued step
+  bool timing = false;
+  struct TimedLaunch {
+    hipEvent_t a, b;
+    int cls;
+    int step_id;
+  };
+  std::vector<TimedLaunch> tl;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  double k_ms[ESP_KCLASS_COUNT] = {0, 0, 0, 0, 0};
+  long long k_n[ESP_KCLASS_COUNT] = {0, 0, 0, 0, 0};
+
   PView view()
   {
     PView v;
@@ -199,6 +215,50 @@ struct esp_engine {
 namespace {
 
 void sync(esp_engine* e) { HIPCHK(hipStreamSynchronize(e->stream)); }
+
+hipEvent_t next_event(esp_engine* e)
+{
+  if (e->ev_used == e->ev_pool.size()) {
+    hipEvent_t ev;
+    HIPCHK(hipEventCreate(&ev));
+    e->ev_pool.push_back(ev);
+  }
+  return e->ev_pool[e->ev_used++];
+}
+
+// event pair around one kernel launch (no-op unless timing is on)
+struct LaunchTimer {
+  esp_engine* e;
+  hipEvent_t b;
+  int cls, step_id;
+  LaunchTimer(esp_engine* e_, int cls_, int step_id_) : e(e_), cls(cls_), step_id(step_id_)
+  {
+    if (!e->timing) return;
+    hipEvent_t a = next_event(e);
+    b = next_event(e);
+    HIPCHK(hipEventRecord(a, e->stream));
+    e->tl.push_back({a, b, cls, step_id});
+  }
+  ~LaunchTimer() noexcept(false)
+  {
+    if (e->timing) HIPCHK(hipEventRecord(b, e->stream));
+  }
+};
+
+// after a synchronisation: add the launches of steps 1..done of the batch
+void collect_timing(esp_engine* e, int done)
+{
+  if (!e->timing) return;
+  for (auto& t : e->tl) {
+    if (t.step_id > done) continue;
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, t.a, t.b));
+    e->k_ms[t.cls] += ms;
+    e->k_n[t.cls]++;
+  }
+  e->tl.clear();
+  e->ev_used = 0;
+}
 
 template <class T>
 void h2d(esp_engine* e, T* d, const T* h, size_t cnt)
@@ -650,6 +710,7 @@ void launch_step(esp_engine* e, const StepCtl& ctl, double* whist_out)
   PView p = e->view();
   for (int gi : e->dord) {
     DynGroup& G = e->dg[gi];
+    LaunchTimer lt(e, G.kind == 3 ? ESP_KCLASS_FRICTION : ESP_KCLASS_ELASTIC, ctl.step_id);
     if (G.kind == 1)
       launch_nrot_elastic(e->stream, ctl, p, G.n, G.ij.p, G.k, G.scaling, e->do2d, e->contrib.p,
                           G.rec_off);
@@ -662,6 +723,7 @@ void launch_step(esp_engine* e, const StepCtl& ctl, double* whist_out)
   }
   for (int gi : e->bord) {
     BondGroup& G = e->bg[gi];
+    LaunchTimer lt(e, ESP_KCLASS_BOND, ctl.step_id);
     launch_rot_bond(e->stream, ctl, p, G.n, G.ij.p, G.k4.p, G.mx4.p, G.D4.p, G.prm.truncated,
                     G.prm.beta1, G.prm.beta2, e->contrib.p, G.rec_off, G.bond_off,
                     e->broken_list.p);
@@ -680,8 +742,14 @@ void launch_step(esp_engine* e, const StepCtl& ctl, double* whist_out)
   for (size_t k = 0; k < e->gord.size(); k++)
     for (int c = 0; c < 3; c++) gp.g[k][c] = e->grav[e->gord[k]].g[c];
   gp.wall_part = e->wall_part.p;
-  launch_gather_integrate(e->stream, ctl, p, gp);
-  launch_wall_finalize(e->stream, ctl, e->wall_part.p, gather_blocks(e->n), gp.nwg, whist_out);
+  {
+    LaunchTimer lt(e, ESP_KCLASS_GATHER, ctl.step_id);
+    launch_gather_integrate(e->stream, ctl, p, gp);
+  }
+  if (gp.nwg) {
+    LaunchTimer lt(e, ESP_KCLASS_WALL, ctl.step_id);
+    launch_wall_finalize(e->stream, ctl, e->wall_part.p, gather_blocks(e->n), gp.nwg, whist_out);
+  }
   HIPCHK(hipGetLastError());
 }
 
@@ -751,12 +819,11 @@ void host_step(esp_engine* e)
   ctl.first = 1;
   launch_step(e, ctl, e->wall_hist.p);
   const int nwg = (int)e->wg.size();
-  if (nwg) {
-    std::vector<double> h((size_t)nwg * 3);
-    d2h(e, h.data(), e->wall_hist.p, h.size());
-    sync(e);
-    record_walls(e, h, 1);
-  }
+  std::vector<double> h((size_t)nwg * 3);
+  if (nwg) d2h(e, h.data(), e->wall_hist.p, h.size());
+  sync(e);
+  collect_timing(e, 1);
+  if (nwg) record_walls(e, h, 1);
 }
 
 // up to K steps queued without host bookkeeping; returns steps completed
@@ -790,6 +857,7 @@ int fast_batch(esp_engine* e, int k)
   if (nwg) d2h(e, h.data(), e->wall_hist.p, h.size());
   sync(e);
   const int done = halted ? halted - 1 : k;
+  collect_timing(e, done);
   if (done < k) {  // walls stay where the last completed step left them
     const std::vector<double>& o = done > 0 ? wall_o[done - 1] : before;
     for (size_t w = 0; w < e->walls.size(); w++)
@@ -945,6 +1013,7 @@ void esp_destroy(esp_engine* e)
   if (e->stream) {
     (void)hipStreamSynchronize(e->stream);
   }
+  for (hipEvent_t ev : e->ev_pool) (void)hipEventDestroy(ev);
   hipStream_t s = e->stream;
   delete e;
   if (s) (void)hipStreamDestroy(s);
@@ -1438,6 +1507,30 @@ int esp_get_rebuild_steps(esp_engine* e, int* steps)
   if (steps && !e->rebuild_steps.empty())
     memcpy(steps, e->rebuild_steps.data(), sizeof(int) * e->rebuild_steps.size());
   return (int)e->rebuild_steps.size();
+  API_END
+}
+
+int esp_set_kernel_timing(esp_engine* e, int enable)
+{
+  API_BEGIN
+  e->timing = enable != 0;
+  e->tl.clear();
+  e->ev_used = 0;
+  for (int k = 0; k < ESP_KCLASS_COUNT; k++) { e->k_ms[k] = 0; e->k_n[k] = 0; }
+  return ESP_OK;
+  API_END
+}
+
+int esp_get_kernel_times(esp_engine* e, double* ms, long long* launches)
+{
+  API_BEGIN
+  for (int k = 0; k < ESP_KCLASS_COUNT; k++) {
+    if (ms) ms[k] = e->k_ms[k];
+    if (launches) launches[k] = e->k_n[k];
+    e->k_ms[k] = 0;
+    e->k_n[k] = 0;
+  }
+  return ESP_OK;
   API_END
 }
 

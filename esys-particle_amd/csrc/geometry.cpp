@@ -1,0 +1,134 @@
+// geometry.cpp - seeded dense packing of polydisperse spheres in a box
+// (setup-time input generation for specimens; not on the per-step path).
+//
+// Grow-and-relax: centres start uniformly random with radii scaled by s < 1;
+// s grows geometrically while overlaps are relaxed by Jacobi sweeps that move
+// each overlapping pair apart along their centre line (half the overlap each,
+// damped) and keep spheres inside the box.  The result is a jammed-like
+// packing whose near-contacts give the bond network of a bonded specimen.
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+namespace {
+
+struct SplitMix {
+  uint64_t s;
+  explicit SplitMix(uint64_t seed) : s(seed) {}
+  uint64_t next()
+  {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  }
+  double uniform() { return (next() >> 11) * (1.0 / 9007199254740992.0); }
+};
+
+struct Grid {
+  double cs;
+  int nx, ny, nz;
+  std::vector<int> start, items;
+  void build(const std::vector<double>& x, int n, const double* box, double cell)
+  {
+    cs = cell;
+    nx = std::max(1, (int)(box[0] / cs));
+    ny = std::max(1, (int)(box[1] / cs));
+    nz = std::max(1, (int)(box[2] / cs));
+    start.assign((size_t)nx * ny * nz + 1, 0);
+    items.resize(n);
+    std::vector<int> c(n);
+    for (int i = 0; i < n; i++) {
+      int ix = std::min(nx - 1, std::max(0, (int)(x[3 * i] / cs)));
+      int iy = std::min(ny - 1, std::max(0, (int)(x[3 * i + 1] / cs)));
+      int iz = std::min(nz - 1, std::max(0, (int)(x[3 * i + 2] / cs)));
+      c[i] = (ix * ny + iy) * nz + iz;
+      start[c[i] + 1]++;
+    }
+    for (size_t k = 1; k < start.size(); k++) start[k] += start[k - 1];
+    std::vector<int> fill(start.begin(), start.end() - 1);
+    for (int i = 0; i < n; i++) items[fill[c[i]]++] = i;
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+// pos: out, 3n doubles.  rad: n radii.  box: extents (origin at 0).
+// Returns the final maximum overlap relative to the smallest radius.
+// Each sweep computes every particle's displacement from its own overlaps (a
+// gather, so the sweep is deterministic under any thread count).
+double esp_geom_pack(int n, const double* rad, const double* box, unsigned long long seed,
+                     int max_sweeps, double* pos)
+{
+  SplitMix rng(seed);
+  std::vector<double> x(3 * (size_t)n), d(3 * (size_t)n);
+  double rmax = 0, rmin = 1e300;
+  for (int i = 0; i < n; i++) {
+    rmax = std::max(rmax, rad[i]);
+    rmin = std::min(rmin, rad[i]);
+  }
+  for (int i = 0; i < n; i++)
+    for (int c = 0; c < 3; c++) x[3 * i + c] = rng.uniform() * box[c];
+  double s = 0.2;
+  Grid g;
+  std::vector<int> cellof(n);
+  double maxov = 0;
+  for (int sweep = 0; sweep < max_sweeps; sweep++) {
+    g.build(x, n, box, 2 * rmax * s * 1.0001);
+    maxov = 0;
+#pragma omp parallel for schedule(static) reduction(max : maxov)
+    for (int i = 0; i < n; i++) {
+      const int cx = std::min(g.nx - 1, std::max(0, (int)(x[3 * i] / g.cs)));
+      const int cy = std::min(g.ny - 1, std::max(0, (int)(x[3 * i + 1] / g.cs)));
+      const int cz = std::min(g.nz - 1, std::max(0, (int)(x[3 * i + 2] / g.cs)));
+      double ddx = 0, ddy = 0, ddz = 0;
+      for (int ox = -1; ox <= 1; ox++)
+        for (int oy = -1; oy <= 1; oy++)
+          for (int oz = -1; oz <= 1; oz++) {
+            const int x2 = cx + ox, y2 = cy + oy, z2 = cz + oz;
+            if (x2 < 0 || y2 < 0 || z2 < 0 || x2 >= g.nx || y2 >= g.ny || z2 >= g.nz) continue;
+            const int c2 = (x2 * g.ny + y2) * g.nz + z2;
+            for (int b = g.start[c2]; b < g.start[c2 + 1]; b++) {
+              const int j = g.items[b];
+              if (j == i) continue;
+              const double dx = x[3 * j] - x[3 * i], dy = x[3 * j + 1] - x[3 * i + 1],
+                           dz = x[3 * j + 2] - x[3 * i + 2];
+              const double dist2 = dx * dx + dy * dy + dz * dz;
+              const double rs = (rad[i] + rad[j]) * s;
+              if (dist2 >= rs * rs) continue;
+              const double dist = std::sqrt(dist2);
+              const double ov = rs - dist;
+              maxov = std::max(maxov, ov / (rmin * s));
+              double ux, uy, uz;
+              if (dist > 1e-12) {
+                ux = dx / dist; uy = dy / dist; uz = dz / dist;
+              } else {  // coincident centres: separate along x by index order
+                ux = j > i ? 1.0 : -1.0; uy = 0; uz = 0;
+              }
+              const double h = 0.5 * ov;
+              ddx -= h * ux; ddy -= h * uy; ddz -= h * uz;
+            }
+          }
+      d[3 * i] = ddx; d[3 * i + 1] = ddy; d[3 * i + 2] = ddz;
+    }
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; i++) {
+      const double r = rad[i] * s;
+      for (int c = 0; c < 3; c++) {
+        double v = x[3 * i + c] + 0.7 * d[3 * i + c];
+        v = std::min(std::max(v, r), box[c] - r);
+        x[3 * i + c] = v;
+      }
+    }
+    if (s >= 1.0 && maxov < 2e-3) break;
+    if (maxov < 0.05) s = std::min(1.0, s * 1.01);
+  }
+  memcpy(pos, x.data(), sizeof(double) * 3 * (size_t)n);
+  return maxov;
+}
+
+}  // extern "C"

@@ -37,7 +37,9 @@ def _cell_neighbours(pos, cutoff):
                 if cnt.sum() == 0:
                     continue
                 i = np.repeat(np.arange(len(pos)), cnt)
-                j = order[np.concatenate([np.arange(x, y) for x, y in zip(a, b) if y > x])]
+                first = np.repeat(a, cnt)
+                within = np.arange(int(cnt.sum())) - np.repeat(np.cumsum(cnt) - cnt, cnt)
+                j = order[first + within]
                 m = i < j
                 pairs.append(np.stack([i[m], j[m]], 1))
     if not pairs:
@@ -157,6 +159,22 @@ def bonded_compression(n=1000, seed=3, rmin=0.2, rmax=1.0, box=None, phi=0.3, dt
         "walls": [("bottom", (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), kr, (0.0, v_step, 0.0)),
                   ("top", (0.0, ly, 0.0), (0.0, -1.0, 0.0), kr, (0.0, -v_step, 0.0))],
     }
+
+
+def dense_bonded_compression(n=100000, seed=3, rmin=0.2, rmax=1.0, phi=0.58, max_sweeps=2000,
+                             **kw):
+    """bonded_compression on a dense grow-and-relax packing (geometry.py): radii
+    log-uniform in [rmin, rmax], a 1:2:1 prism sized for solid fraction phi."""
+    from . import geometry
+    rng = np.random.default_rng(seed)
+    rad = np.exp(rng.uniform(math.log(rmin), math.log(rmax), n))
+    vs = float(np.sum(4.0 / 3.0 * math.pi * rad ** 3))
+    s = (vs / phi / 2.0) ** (1.0 / 3.0)
+    box = (s, 2 * s, s)
+    pos, _ = geometry.dense_packing(rad, box, seed + 1, max_sweeps)
+    spec = bonded_compression(seed=seed, rmin=rmin, rmax=rmax, box=box, packing=(pos, rad), **kw)
+    spec["box"] = box
+    return spec
 
 
 def lattice_block(nx=8, ny=12, nz=8, r=0.5, jitter=1e-3, rad_spread=0.0, seed=7, dt=1e-4,

@@ -207,6 +207,21 @@ int esp_get_bond_breaks(esp_engine* e, int* step, int* id1, int* id2);
 /* Steps at which the neighbour table was rebuilt; returns the count when NULL. */
 int esp_get_rebuild_steps(esp_engine* e, int* steps);
 
+/* ---- measurement (no counterpart in the reference) ----------------------- */
+/* Kernel classes timed by esp_get_kernel_times. */
+#define ESP_KCLASS_BOND 0     /* rotational bond law + break test, per bond group */
+#define ESP_KCLASS_FRICTION 1 /* RotFriction law, per group */
+#define ESP_KCLASS_ELASTIC 2  /* NRotElastic / RotElastic laws, per group */
+#define ESP_KCLASS_GATHER 3   /* force gather + integration, per step */
+#define ESP_KCLASS_WALL 4     /* wall-force reduction, per step */
+#define ESP_KCLASS_COUNT 5
+/* Enable (1) or disable (0) HIP-event timing of every per-step kernel launch on
+ * the engine's stream; timing adds an event pair around each launch. */
+int esp_set_kernel_timing(esp_engine* e, int enable);
+/* Accumulated device time (ms) and launch count per kernel class, over the
+ * steps that completed since timing was enabled; resets the accumulators. */
+int esp_get_kernel_times(esp_engine* e, double* ms, long long* launches);
+
 #ifdef __cplusplus
 }
 #endif

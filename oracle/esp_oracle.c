@@ -1732,6 +1732,24 @@ int eso_get_bond_breaks(eso_engine* e, int* step, int* id1, int* id2)
   return n;
 }
 
+/* The oracle has no device kernels; timing reports zeros. */
+int eso_set_kernel_timing(eso_engine* e, int enable)
+{
+  (void)e;
+  (void)enable;
+  return ESP_OK;
+}
+
+int eso_get_kernel_times(eso_engine* e, double* ms, long long* launches)
+{
+  (void)e;
+  for (int k = 0; k < ESP_KCLASS_COUNT; k++) {
+    if (ms) ms[k] = 0.0;
+    if (launches) launches[k] = 0;
+  }
+  return ESP_OK;
+}
+
 int eso_get_rebuild_steps(eso_engine* e, int* steps)
 {
   if (steps) memcpy(steps, e->rebuild_steps, sizeof(int) * (size_t)e->nreb);
