@@ -131,4 +131,83 @@ double esp_geom_pack(int n, const double* rad, const double* box, unsigned long 
   return maxov;
 }
 
+// Moves centres until every listed (bonded) pair is at exact contact,
+// |x_i - x_j| = r_i + r_j, and no other pair overlaps, by Jacobi sweeps (each
+// particle gathers half of every residual it takes part in).  The bond network
+// of a packing has fewer constraints than degrees of freedom, so the sweeps
+// converge.  Returns the final maximum residual relative to r_i + r_j.
+double esp_geom_snap(int n, double* pos, const double* rad, int nb, const int* bonds,
+                     const double* box, int max_sweeps, double tol)
+{
+  std::vector<int> off(n + 1, 0), adj(2 * (size_t)nb);
+  for (int b = 0; b < nb; b++) {
+    off[bonds[2 * b] + 1]++;
+    off[bonds[2 * b + 1] + 1]++;
+  }
+  for (int i = 0; i < n; i++) off[i + 1] += off[i];
+  {
+    std::vector<int> fill(off.begin(), off.end() - 1);
+    for (int b = 0; b < nb; b++) {
+      adj[fill[bonds[2 * b]]++] = bonds[2 * b + 1];
+      adj[fill[bonds[2 * b + 1]]++] = bonds[2 * b];
+    }
+    for (int i = 0; i < n; i++) std::sort(adj.begin() + off[i], adj.begin() + off[i + 1]);
+  }
+  std::vector<double> x(pos, pos + 3 * (size_t)n), d(3 * (size_t)n);
+  double rmax = 0;
+  for (int i = 0; i < n; i++) rmax = std::max(rmax, rad[i]);
+  Grid g;
+  double maxres = 0;
+  for (int sweep = 0; sweep < max_sweeps; sweep++) {
+    g.build(x, n, box, 2 * rmax * 1.0001);
+    maxres = 0;
+#pragma omp parallel for schedule(static) reduction(max : maxres)
+    for (int i = 0; i < n; i++) {
+      double ddx = 0, ddy = 0, ddz = 0;
+      int active = 0;
+      auto push = [&](int j, double target, bool bonded) {
+        const double dx = x[3 * j] - x[3 * i], dy = x[3 * j + 1] - x[3 * i + 1],
+                     dz = x[3 * j + 2] - x[3 * i + 2];
+        const double dist = std::sqrt(dx * dx + dy * dy + dz * dz);
+        double res = dist - target;  // > 0: too far
+        if (!bonded && res >= 0) return;
+        maxres = std::max(maxres, std::fabs(res) / target);
+        if (dist <= 1e-12) return;
+        const double h = 0.5 * res / dist;
+        ddx += h * dx; ddy += h * dy; ddz += h * dz;
+        active++;
+      };
+      for (int k = off[i]; k < off[i + 1]; k++) push(adj[k], rad[i] + rad[adj[k]], true);
+      const int cx = std::min(g.nx - 1, std::max(0, (int)(x[3 * i] / g.cs)));
+      const int cy = std::min(g.ny - 1, std::max(0, (int)(x[3 * i + 1] / g.cs)));
+      const int cz = std::min(g.nz - 1, std::max(0, (int)(x[3 * i + 2] / g.cs)));
+      for (int ox = -1; ox <= 1; ox++)
+        for (int oy = -1; oy <= 1; oy++)
+          for (int oz = -1; oz <= 1; oz++) {
+            const int x2 = cx + ox, y2 = cy + oy, z2 = cz + oz;
+            if (x2 < 0 || y2 < 0 || z2 < 0 || x2 >= g.nx || y2 >= g.ny || z2 >= g.nz) continue;
+            const int c2 = (x2 * g.ny + y2) * g.nz + z2;
+            for (int b = g.start[c2]; b < g.start[c2 + 1]; b++) {
+              const int j = g.items[b];
+              if (j == i) continue;
+              if (std::binary_search(adj.begin() + off[i], adj.begin() + off[i + 1], j)) continue;
+              push(j, rad[i] + rad[j], false);
+            }
+          }
+      // averaged (not summed) corrections keep the sweep stable for any degree
+      const double w = active > 0 ? 1.0 / active : 0.0;
+      d[3 * i] = ddx * w; d[3 * i + 1] = ddy * w; d[3 * i + 2] = ddz * w;
+    }
+    if (maxres < tol) break;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; i++)
+      for (int c = 0; c < 3; c++) {
+        double v = x[3 * i + c] + 1.5 * d[3 * i + c];
+        x[3 * i + c] = std::min(std::max(v, rad[i]), box[c] - rad[i]);
+      }
+  }
+  memcpy(pos, x.data(), sizeof(double) * 3 * (size_t)n);
+  return maxres;
+}
+
 }  // extern "C"

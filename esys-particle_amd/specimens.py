@@ -162,7 +162,7 @@ def bonded_compression(n=1000, seed=3, rmin=0.2, rmax=1.0, box=None, phi=0.3, dt
 
 
 def dense_bonded_compression(n=100000, seed=3, rmin=0.2, rmax=1.0, phi=0.58, max_sweeps=2000,
-                             **kw):
+                             snap_sweeps=1500, **kw):
     """bonded_compression on a dense grow-and-relax packing (geometry.py): radii
     log-uniform in [rmin, rmax], a 1:2:1 prism sized for solid fraction phi."""
     from . import geometry
@@ -172,6 +172,8 @@ def dense_bonded_compression(n=100000, seed=3, rmin=0.2, rmax=1.0, phi=0.58, max
     s = (vs / phi / 2.0) ** (1.0 / 3.0)
     box = (s, 2 * s, s)
     pos, _ = geometry.dense_packing(rad, box, seed + 1, max_sweeps)
+    # relax the remaining overlaps so that contacts sit at (near) exact touching
+    pos, _ = geometry.snap_bonds(pos, rad, np.zeros((0, 2), np.int32), box, snap_sweeps, 1e-9)
     spec = bonded_compression(seed=seed, rmin=rmin, rmax=rmax, box=box, packing=(pos, rad), **kw)
     spec["box"] = box
     return spec
