@@ -143,6 +143,7 @@ struct esp_engine {
   int ncell = 0;
   hipStream_t stream = nullptr;
   int K = 32;
+  int kcur = 1;
 
   // host staging (until the first step) and host mirrors
   std::vector<int> h_id, h_tag;
@@ -167,6 +168,8 @@ struct esp_engine {
   DBuf<int2> tmp_ij, tmp_ij2;
   DBuf<long long> tmp_key, tmp_key2;
   DBuf<double4> tmp_ff, tmp_a, tmp_b, tmp_c;
+  DBuf<int> brk_list;
+  DBuf<long long> brk_keys, tmp_skeys;
   DBuf<int> inc_cnt, inc_off, inc_fill, inc_ent;
   int nrec = 0;
   DBuf<double> contrib;
@@ -610,6 +613,9 @@ void rebuild_csr(esp_engine* e)
                     e->inc_ent.p);
   }
   launch_inc_sort(e->stream, e->n, e->inc_off.p, e->inc_ent.p);
+  // records start at zero: a contact that is not touching keeps a zero record
+  // without rewriting it every step
+  zero(e, e->contrib.p, (size_t)std::max(rec, 1) * 9);
   HIPCHK(hipGetLastError());
 }
 
@@ -644,30 +650,25 @@ bool remove_broken_bonds(esp_engine* e, int nbroken)
     // device: flag particles, compact bond arrays
     e->keep.reserve(G.n + 1);
     e->keep_off.reserve(G.n + 1);
-    e->dev_int.reserve(4);
-    DBuf<int> dlist;
-    dlist.reserve(cnt);
-    h2d(e, dlist.p, &*lo, cnt);
-    launch_bond_mark_broken(e->stream, cnt, dlist.p, G.bond_off, G.n, e->keep.p, p, G.ij.p);
+    e->brk_list.reserve(cnt);
+    h2d(e, e->brk_list.p, &*lo, cnt);
+    launch_bond_mark_broken(e->stream, cnt, e->brk_list.p, G.bond_off, G.n, e->keep.p, p, G.ij.p);
     const int nb2 = scan_total(e, e->keep.p, e->keep_off.p, G.n);
-    DBuf<int2> ij2;
-    DBuf<double4> k2, m2, D2;
     const size_t c2 = std::max(nb2, 1);
-    ij2.reserve(c2); k2.reserve(c2); m2.reserve(c2); D2.reserve(c2);
+    e->tmp_ij.reserve(c2); e->tmp_a.reserve(c2); e->tmp_b.reserve(c2); e->tmp_c.reserve(c2);
     launch_compact_bonds(e->stream, G.n, e->keep.p, e->keep_off.p, G.ij.p, G.k4.p, G.mx4.p,
-                         G.D4.p, ij2.p, k2.p, m2.p, D2.p);
+                         G.D4.p, e->tmp_ij.p, e->tmp_a.p, e->tmp_b.p, e->tmp_c.p);
+    G.ij.swap(e->tmp_ij); G.k4.swap(e->tmp_a); G.mx4.swap(e->tmp_b); G.D4.swap(e->tmp_c);
     // sorted keys
-    DBuf<long long> drem;
-    drem.reserve(rem.size());
-    h2d(e, drem.p, rem.data(), rem.size());
-    launch_keys_mark_removed(e->stream, G.nkeys, G.skeys.p, (int)rem.size(), drem.p, e->keep.p);
+    e->brk_keys.reserve(rem.size());
+    h2d(e, e->brk_keys.p, rem.data(), rem.size());
+    launch_keys_mark_removed(e->stream, G.nkeys, G.skeys.p, (int)rem.size(), e->brk_keys.p,
+                             e->keep.p);
     const int nk2 = scan_total(e, e->keep.p, e->keep_off.p, G.nkeys);
-    DBuf<long long> sk2;
-    sk2.reserve(std::max(nk2, 1));
-    launch_keys_compact(e->stream, G.nkeys, e->keep.p, e->keep_off.p, G.skeys.p, sk2.p);
+    e->tmp_skeys.reserve(std::max(nk2, 1));
+    launch_keys_compact(e->stream, G.nkeys, e->keep.p, e->keep_off.p, G.skeys.p, e->tmp_skeys.p);
     HIPCHK(hipGetLastError());
-    sync(e);
-    G.ij.swap(ij2); G.k4.swap(k2); G.mx4.swap(m2); G.D4.swap(D2); G.skeys.swap(sk2);
+    G.skeys.swap(e->tmp_skeys);
     G.nkeys = nk2;
     int w = 0;
     for (int b = 0; b < G.n; b++)
@@ -890,7 +891,12 @@ void run_steps(esp_engine* e, int nsteps)
       done++;
       continue;
     }
-    done += fast_batch(e, std::min(e->K, nsteps - done));
+    // queue depth adapts to how often host bookkeeping interrupts the queue:
+    // it doubles after a full batch and drops to twice the completed run
+    const int k = std::min(e->kcur, nsteps - done);
+    const int got = fast_batch(e, k);
+    e->kcur = got == k ? std::min(e->K, 2 * e->kcur) : std::max(1, std::min(e->K, 2 * (got + 1)));
+    done += got;
   }
 }
 

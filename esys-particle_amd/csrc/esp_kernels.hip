@@ -87,12 +87,16 @@ __global__ __launch_bounds__(BS) void k_rot_friction(StepCtl ctl, PView p, int n
   double* r = contrib + 9 * (size_t)(rec_off + c);
   const dv3 Dq = sub3(xyz(P1), xyz(P2));
   const double eq = P1.w + P2.w;
-  if (!(dot3(Dq, Dq) < (eq * eq))) {  // not touching: history cleared, no force
-    ff[c] = make_double4(0.0, 0.0, 0.0, 0.0);
-    for (int t = 0; t < 9; t++) r[t] = 0.0;
+  const double4 st = ff[c];
+  if (!(dot3(Dq, Dq) < (eq * eq))) {
+    // not touching: history cleared, no force.  A pair that did not touch in
+    // the last step already holds a zero history and a zero record.
+    if (st.w != 0.0 || st.x != 0.0 || st.y != 0.0 || st.z != 0.0) {
+      ff[c] = make_double4(0.0, 0.0, 0.0, 0.0);
+      for (int t = 0; t < 9; t++) r[t] = 0.0;
+    }
     return;
   }
-  const double4 st = ff[c];
   const double4 V1 = p.vel[e.x], V2 = p.vel[e.y], W1 = p.ang[e.x], W2 = p.ang[e.y];
   double kk, kks;
   esp_friction_k(k, ks, scaling, meanR, P1.w, P2.w, do2d, kk, kks);
@@ -703,9 +707,36 @@ __global__ void k_inc_fill(int nc, const int2* ij, int rec_off, int has_m, const
   ent[off[e.y] + atomicAdd(&fill[e.y], 1)] = v | 1;
 }
 
-__global__ void k_inc_sort(int n, const int* off, int* ent)
+// Sorts each particle's run of CSR entries (ascending record index).  Runs of
+// consecutive particles are contiguous, so a block stages its 256 runs in LDS
+// when they fit and each thread insertion-sorts its own run there.
+#define INC_LDS 12288
+__global__ __launch_bounds__(BS) void k_inc_sort(int n, const int* off, int* ent)
 {
-  const int i = blockIdx.x * BS + threadIdx.x;
+  __shared__ int buf[INC_LDS];
+  const int p0 = blockIdx.x * BS;
+  const int p1 = min(p0 + BS, n);
+  const int base = off[p0], end = off[p1];
+  const int i = p0 + threadIdx.x;
+  if (end - base <= INC_LDS) {
+    for (int t = base + threadIdx.x; t < end; t += BS) buf[t - base] = ent[t];
+    __syncthreads();
+    if (i < n) {
+      const int a = off[i] - base, b = off[i + 1] - base;
+      for (int s = a + 1; s < b; s++) {
+        const int v = buf[s];
+        int j = s - 1;
+        while (j >= a && buf[j] > v) {
+          buf[j + 1] = buf[j];
+          j--;
+        }
+        buf[j + 1] = v;
+      }
+    }
+    __syncthreads();
+    for (int t = base + threadIdx.x; t < end; t += BS) ent[t] = buf[t - base];
+    return;
+  }
   if (i >= n) return;
   const int a = off[i], b = off[i + 1];
   for (int s = a + 1; s < b; s++) {
