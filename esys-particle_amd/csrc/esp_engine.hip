@@ -85,23 +85,26 @@ struct DynGroup {
   int kind;  // 1 NRotElastic, 2 RotElastic, 3 RotFriction
   double k = 0, ks = 0, mu_d = 0, mu_s = 0;
   int scaling = 1, meanR = 1;
-  int n = 0;
+  int n = 0;  // key-sorted main list; the appendix (device count napp) follows it
   DBuf<int2> ij;
   DBuf<long long> key;
   DBuf<double4> ff;
+  DBuf<int> napp;
+  DBuf<unsigned long long> hs;  // stored pairs (history groups)
+  unsigned hs_mask = 0;
   long long update_ts = 0;
   std::vector<int> ex;
-  int rec_off = 0;
+  int rec_off = 0, app_rec_off = 0;
 };
 
 struct BondGroup {
   std::string name;
   esp_rot_bond_prms prm;
   int n = 0;
-  DBuf<int2> ij;
+  DBuf<int2> ij;  // dead bonds hold (-1, -1) until the next compaction
   DBuf<double4> k4, mx4, D4;
-  DBuf<long long> skeys;  // sorted keys of live bonds (exclusion lookups)
-  int nkeys = 0;
+  DBuf<unsigned long long> hs;  // keys of live bonds (exclusion lookups)
+  unsigned hs_mask = 0;
   std::vector<int> hid1, hid2;  // ids in storage order
   std::vector<double4> h_k4, h_mx4, h_D4;  // staged until the first step
   int rec_off = 0, bond_off = 0;
@@ -168,9 +171,11 @@ struct esp_engine {
   DBuf<int2> tmp_ij, tmp_ij2;
   DBuf<long long> tmp_key, tmp_key2;
   DBuf<double4> tmp_ff, tmp_a, tmp_b, tmp_c;
-  DBuf<int> brk_list;
-  DBuf<long long> brk_keys, tmp_skeys;
   DBuf<int> inc_cnt, inc_off, inc_fill, inc_ent;
+  DBuf<int> xslot, xcnt, xdemand, flist, nidx;
+  DBuf<long long> nkey;
+  DBuf<int4> evt;
+  DBuf<UpdState> upd;
   int nrec = 0;
   DBuf<double> contrib;
   DBuf<StepStatus> ring;
@@ -188,6 +193,7 @@ struct esp_engine {
 
   long long t = 0, timestamp = 0, last_ns = -1;
   bool first_time = true, pending_host = false;
+  int pending_reason = 0;
   long long num_host_phases = 0;
   std::vector<int> rebuild_steps;
   std::vector<int> breaks;  // step, id1, id2
@@ -358,11 +364,53 @@ ExclSet excl_set(esp_engine* e, const DynGroup& G)
   ExclSet s;
   memset(&s, 0, sizeof s);
   for (int b : G.ex) {
-    s.keys[s.count] = e->bg[b].skeys.p;
-    s.n[s.count] = e->bg[b].nkeys;
+    s.h[s.count].t = e->bg[b].hs.p;
+    s.h[s.count].mask = e->bg[b].hs_mask;
     s.count++;
   }
   return s;
+}
+
+unsigned hash_capacity(size_t n)
+{
+  size_t c = 1024;
+  while (c < 2 * n + 2) c <<= 1;
+  return (unsigned)c;
+}
+
+// (re)build a hash set over `cnt` keys (device array)
+void build_hash(esp_engine* e, DBuf<unsigned long long>& hs, unsigned& mask, size_t capacity_for,
+                const long long* keys, int cnt)
+{
+  const unsigned cap = hash_capacity(capacity_for);
+  hs.reserve(cap);
+  mask = cap - 1;
+  HashRef h{hs.p, mask};
+  launch_hash_clear(e->stream, h);
+  launch_hash_insert(e->stream, h, keys, cnt);
+  HIPCHK(hipGetLastError());
+}
+
+void rebuild_bond_hash(esp_engine* e, BondGroup& G)
+{
+  e->tmp_key.reserve(std::max(G.n, 1));
+  launch_bond_keys(e->stream, G.n, G.ij.p, e->id.p, e->tmp_key.p);
+  build_hash(e, G.hs, G.hs_mask, G.n, e->tmp_key.p, G.n);
+}
+
+// dynamic-group arrays hold the main list plus room for the appendix
+void reserve_group(DynGroup& G, size_t n_main, bool hist)
+{
+  const size_t c = n_main + ESP_APP_CAP;
+  G.ij.reserve(c);
+  G.key.reserve(c);
+  if (hist) G.ff.reserve(c);
+}
+
+void rebuild_presence(esp_engine* e, DynGroup& G)
+{
+  if (G.kind != 3) return;
+  build_hash(e, G.hs, G.hs_mask, (size_t)G.n + ESP_APP_CAP, G.key.p, G.n);
 }
 
 // ---------------------------------------------------------------------------
@@ -412,32 +460,36 @@ void upload(esp_engine* e)
     G.bond_off = total_bonds;
     total_bonds += G.n;
     const size_t c = std::max(G.n, 1);
-    G.ij.reserve(c); G.k4.reserve(c); G.mx4.reserve(c); G.D4.reserve(c); G.skeys.reserve(c);
+    G.ij.reserve(c); G.k4.reserve(c); G.mx4.reserve(c); G.D4.reserve(c);
     std::vector<int2> ij(G.n);
-    std::vector<long long> keys(G.n);
-    for (int b = 0; b < G.n; b++) {
+    for (int b = 0; b < G.n; b++)
       ij[b] = make_int2(e->id2slot[G.hid1[b]], e->id2slot[G.hid2[b]]);
-      keys[b] = pkey(G.hid1[b], G.hid2[b]);
-    }
-    std::sort(keys.begin(), keys.end());
-    G.nkeys = G.n;
     h2d(e, G.ij.p, ij.data(), G.n);
     h2d(e, G.k4.p, G.h_k4.data(), G.n);
     h2d(e, G.mx4.p, G.h_mx4.data(), G.n);
     h2d(e, G.D4.p, G.h_D4.data(), G.n);
-    h2d(e, G.skeys.p, keys.data(), G.n);
+    rebuild_bond_hash(e, G);
     sync(e);
     std::vector<double4>().swap(G.h_k4);
     std::vector<double4>().swap(G.h_mx4);
     std::vector<double4>().swap(G.h_D4);
   }
+  for (auto& G : e->dg) {
+    G.napp.reserve(1);
+    zero(e, G.napp.p, 1);
+    reserve_group(G, 0, G.kind == 3);
+    rebuild_presence(e, G);
+  }
   e->broken_cap = std::max(total_bonds, 1);
   e->broken_list.reserve(e->broken_cap);
+  e->evt.reserve(e->broken_cap);
   e->ring_size = e->K + 2;
   e->ring.reserve(e->ring_size);
   zero(e, e->ring.p, e->ring_size);
   e->halted.reserve(1);
   zero(e, e->halted.p, 1);
+  e->upd.reserve(1);
+  zero(e, e->upd.p, 1);
   e->dev_int.reserve(4);
   e->cell_count.reserve(e->ncell + 1);
   e->cell_start.reserve(e->ncell + 1);
@@ -448,6 +500,14 @@ void upload(esp_engine* e)
   e->inc_cnt.reserve(cap + 1);
   e->inc_off.reserve(cap + 1);
   e->inc_fill.reserve(cap + 1);
+  e->xslot.reserve(cap * ESP_XSLOTS);
+  e->xcnt.reserve(cap);
+  e->xdemand.reserve(cap);
+  e->flist.reserve(cap);
+  e->nidx.reserve(2 * ESP_NEW_CAP);
+  e->nkey.reserve(ESP_NEW_CAP);
+  zero(e, e->xcnt.p, cap);
+  zero(e, e->xdemand.p, cap);
   const int nb = std::max(gather_blocks(n), 1);
   const int nwg = (int)e->wg.size();
   e->wall_part.reserve((size_t)nb * std::max(nwg, 1) * 3);
@@ -504,12 +564,115 @@ void new_pair_list(esp_engine* e)
   e->np_key.reserve(std::max(total, 1));
   launch_pairs(e->stream, 1, 1, p, e->grid, e->alpha, e->by_id.p, e->cell.p, e->cell_start.p,
                e->cell_items.p, nullptr, e->cand_off.p, e->np_p.p, e->np_q.p, e->np_key.p);
-  launch_clear_flag_bit(e->stream, p);
+  launch_clear_flag_bit(e->stream, p, e->upd.p);
   HIPCHK(hipGetLastError());
   e->nnew = total;
 }
 
+// Device bookkeeping folded back into plain lists: break events to the host
+// log, dead bonds compacted out, each dynamic group's appendix merged into its
+// key-sorted main list.  Results are unchanged; only storage is reorganised.
+void fold(esp_engine* e)
+{
+  UpdState st;
+  d2h(e, &st, e->upd.p, 1);
+  std::vector<int> napp(e->dg.size(), 0);
+  for (size_t g = 0; g < e->dg.size(); g++) d2h(e, &napp[g], e->dg[g].napp.p, 1);
+  sync(e);
+  if (st.evt_count > 0) {
+    std::vector<int4> ev(st.evt_count);
+    d2h(e, ev.data(), e->evt.p, st.evt_count);
+    const int zero_int = 0;
+    h2d(e, &e->upd.p->evt_count, &zero_int, 1);
+    sync(e);
+    for (const int4& v : ev) {
+      e->breaks.push_back(v.x);
+      e->breaks.push_back(v.y);
+      e->breaks.push_back(v.z);
+    }
+    // compact dead bonds
+    int off = 0;
+    for (int gi : e->bord) {
+      BondGroup& G = e->bg[gi];
+      if (G.n > 0) {
+        e->keep.reserve(G.n + 1);
+        e->keep_off.reserve(G.n + 1);
+        launch_alive_flags(e->stream, G.n, G.ij.p, e->keep.p);
+        const int nb2 = scan_total(e, e->keep.p, e->keep_off.p, G.n);
+        if (nb2 != G.n) {
+          std::vector<int> keep(G.n);
+          d2h(e, keep.data(), e->keep.p, G.n);
+          const size_t c2 = std::max(nb2, 1);
+          e->tmp_ij.reserve(c2); e->tmp_a.reserve(c2); e->tmp_b.reserve(c2); e->tmp_c.reserve(c2);
+          launch_compact_bonds(e->stream, G.n, e->keep.p, e->keep_off.p, G.ij.p, G.k4.p, G.mx4.p,
+                               G.D4.p, e->tmp_ij.p, e->tmp_a.p, e->tmp_b.p, e->tmp_c.p);
+          HIPCHK(hipGetLastError());
+          sync(e);
+          G.ij.swap(e->tmp_ij); G.k4.swap(e->tmp_a); G.mx4.swap(e->tmp_b); G.D4.swap(e->tmp_c);
+          int w = 0;
+          for (int b = 0; b < G.n; b++)
+            if (keep[b]) { G.hid1[w] = G.hid1[b]; G.hid2[w] = G.hid2[b]; w++; }
+          G.hid1.resize(w);
+          G.hid2.resize(w);
+          G.n = nb2;
+          rebuild_bond_hash(e, G);  // drops the erased slots
+        }
+      }
+      G.bond_off = off;
+      off += G.n;
+    }
+  }
+  for (size_t g = 0; g < e->dg.size(); g++) {
+    DynGroup& G = e->dg[g];
+    const int na = napp[g];
+    if (na <= 0) continue;
+    const bool hist = G.kind == 3;
+    // appendix: batches in key order, not sorted overall -> sort it on the host
+    std::vector<int2> aij(na);
+    std::vector<long long> akey(na);
+    std::vector<double4> aff(hist ? na : 0);
+    d2h(e, aij.data(), G.ij.p + G.n, na);
+    d2h(e, akey.data(), G.key.p + G.n, na);
+    if (hist) d2h(e, aff.data(), G.ff.p + G.n, na);
+    sync(e);
+    std::vector<int> perm(na);
+    for (int i = 0; i < na; i++) perm[i] = i;
+    std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return akey[a] < akey[b]; });
+    std::vector<int2> sij(na);
+    std::vector<long long> skey(na);
+    std::vector<double4> sff(hist ? na : 0);
+    for (int i = 0; i < na; i++) {
+      sij[i] = aij[perm[i]];
+      skey[i] = akey[perm[i]];
+      if (hist) sff[i] = aff[perm[i]];
+    }
+    e->tmp_ij2.reserve(na);
+    e->tmp_key2.reserve(na);
+    e->tmp_ff.reserve(na);
+    h2d(e, e->tmp_ij2.p, sij.data(), na);
+    h2d(e, e->tmp_key2.p, skey.data(), na);
+    if (hist) h2d(e, e->tmp_ff.p, sff.data(), na);
+    const int tot = G.n + na;
+    e->tmp_ij.reserve((size_t)tot + ESP_APP_CAP);
+    e->tmp_key.reserve((size_t)tot + ESP_APP_CAP);
+    e->tmp_a.reserve((size_t)tot + ESP_APP_CAP);
+    launch_merge(e->stream, G.n, G.ij.p, G.key.p, hist ? G.ff.p : nullptr, na, e->tmp_ij2.p,
+                 e->tmp_key2.p, hist ? e->tmp_ff.p : nullptr, e->tmp_ij.p, e->tmp_key.p,
+                 hist ? e->tmp_a.p : nullptr);
+    HIPCHK(hipGetLastError());
+    sync(e);
+    G.ij.swap(e->tmp_ij);
+    G.key.swap(e->tmp_key);
+    if (hist) G.ff.swap(e->tmp_a);
+    G.n = tot;
+    zero(e, G.napp.p, 1);
+    reserve_group(G, G.n, hist);  // keeps content when the capacity suffices
+  }
+  sync(e);
+}
+
 // PIS_NE::update / PIS_ED::update (pi_storage_ne.hpp:49-102, pi_storage_ed.hpp:84-161)
+// on the host bookkeeping path; the group's appendix is empty here (folded).
 void dyn_update(esp_engine* e, DynGroup& G, bool& new_list_taken)
 {
   const bool hist = G.kind == 3;
@@ -527,9 +690,9 @@ void dyn_update(esp_engine* e, DynGroup& G, bool& new_list_taken)
       launch_ne_mark(e->stream, nc, e->cand_key.p, ex, e->keep.p);
     }
     const int total = scan_total(e, e->keep.p, e->keep_off.p, nc);
-    e->tmp_ij.reserve(std::max(total, 1));
-    e->tmp_key.reserve(std::max(total, 1));
-    if (hist) e->tmp_ff.reserve(std::max(total, 1));
+    e->tmp_ij.reserve((size_t)total + ESP_APP_CAP);
+    e->tmp_key.reserve((size_t)total + ESP_APP_CAP);
+    if (hist) e->tmp_ff.reserve((size_t)total + ESP_APP_CAP);
     launch_pair_emit(e->stream, nc, e->keep.p, e->keep_off.p, e->cand_p.p, e->cand_q.p,
                      e->cand_key.p, hist ? e->src.p : nullptr, hist ? G.ff.p : nullptr,
                      e->tmp_ij.p, e->tmp_key.p, hist ? e->tmp_ff.p : nullptr);
@@ -538,6 +701,7 @@ void dyn_update(esp_engine* e, DynGroup& G, bool& new_list_taken)
     G.key.swap(e->tmp_key);
     if (hist) G.ff.swap(e->tmp_ff);
     G.n = total;
+    rebuild_presence(e, G);
   } else {
     int nc = 0;
     if (!new_list_taken) {
@@ -548,7 +712,8 @@ void dyn_update(esp_engine* e, DynGroup& G, bool& new_list_taken)
     if (nc > 0) {
       e->keep.reserve(nc + 1);
       e->keep_off.reserve(nc + 1);
-      launch_filter_new(e->stream, nc, e->np_key.p, ex, G.key.p, G.n, hist ? 1 : 0, e->keep.p);
+      launch_filter_new(e->stream, nc, e->np_key.p, ex, HashRef{G.hs.p, G.hs_mask}, hist ? 1 : 0,
+                        e->keep.p);
       const int add = scan_total(e, e->keep.p, e->keep_off.p, nc);
       if (add > 0) {
         e->tmp_ij2.reserve(add);
@@ -556,17 +721,19 @@ void dyn_update(esp_engine* e, DynGroup& G, bool& new_list_taken)
         launch_pack_pairs(e->stream, nc, e->keep.p, e->keep_off.p, e->np_p.p, e->np_q.p,
                           e->np_key.p, e->tmp_ij2.p, e->tmp_key2.p);
         const int tot = G.n + add;
-        e->tmp_ij.reserve(tot);
-        e->tmp_key.reserve(tot);
-        if (hist) e->tmp_ff.reserve(tot);
+        e->tmp_ij.reserve((size_t)tot + ESP_APP_CAP);
+        e->tmp_key.reserve((size_t)tot + ESP_APP_CAP);
+        if (hist) e->tmp_ff.reserve((size_t)tot + ESP_APP_CAP);
         launch_merge(e->stream, G.n, G.ij.p, G.key.p, hist ? G.ff.p : nullptr, add, e->tmp_ij2.p,
-                     e->tmp_key2.p, e->tmp_ij.p, e->tmp_key.p, hist ? e->tmp_ff.p : nullptr);
+                     e->tmp_key2.p, nullptr, e->tmp_ij.p, e->tmp_key.p,
+                     hist ? e->tmp_ff.p : nullptr);
         HIPCHK(hipGetLastError());
         sync(e);
         G.ij.swap(e->tmp_ij);
         G.key.swap(e->tmp_key);
         if (hist) G.ff.swap(e->tmp_ff);
         G.n = tot;
+        rebuild_presence(e, G);
       }
     }
   }
@@ -587,15 +754,18 @@ void wall_params(esp_engine* e, WallParam* out)
   }
 }
 
+// force records: main lists of the dynamic groups (name order), bonded groups
+// (name order), then one appendix region per dynamic group; the CSR covers the
+// first two, the per-particle overflow slots the appendices
 void rebuild_csr(esp_engine* e)
 {
-  int rec = 0;
-  for (int gi : e->dord) { e->dg[gi].rec_off = rec; rec += e->dg[gi].n; }
-  for (int gi : e->bord) { e->bg[gi].rec_off = rec; rec += e->bg[gi].n; }
-  if ((long long)rec >= (1ll << 29))
-    throw EspError(ESP_ERR_CAPACITY, "more than 2^29 interactions");
-  e->nrec = rec;
-  e->contrib.reserve((size_t)std::max(rec, 1) * 9);
+  long long rec = 0;
+  for (int gi : e->dord) { e->dg[gi].rec_off = (int)rec; rec += e->dg[gi].n; }
+  for (int gi : e->bord) { e->bg[gi].rec_off = (int)rec; rec += e->bg[gi].n; }
+  for (int gi : e->dord) { e->dg[gi].app_rec_off = (int)rec; rec += ESP_APP_CAP; }
+  if (rec >= (1ll << 29)) throw EspError(ESP_ERR_CAPACITY, "more than 2^29 interactions");
+  e->nrec = (int)rec;
+  e->contrib.reserve((size_t)std::max<long long>(rec, 1) * 9);
   zero(e, e->inc_cnt.p, e->n + 1);
   for (int gi : e->dord) launch_inc_count(e->stream, e->dg[gi].n, e->dg[gi].ij.p, e->inc_cnt.p);
   for (int gi : e->bord) launch_inc_count(e->stream, e->bg[gi].n, e->bg[gi].ij.p, e->inc_cnt.p);
@@ -615,112 +785,69 @@ void rebuild_csr(esp_engine* e)
   launch_inc_sort(e->stream, e->n, e->inc_off.p, e->inc_ent.p);
   // records start at zero: a contact that is not touching keeps a zero record
   // without rewriting it every step
-  zero(e, e->contrib.p, (size_t)std::max(rec, 1) * 9);
+  zero(e, e->contrib.p, (size_t)std::max<long long>(rec, 1) * 9);
+  zero(e, e->xcnt.p, std::max(e->n, 1));
   HIPCHK(hipGetLastError());
 }
 
-// bonds broken in the previous step leave their groups (PIS_EB::update)
-bool remove_broken_bonds(esp_engine* e, int nbroken)
+BondSet bond_set(esp_engine* e)
 {
-  if (nbroken <= 0) return false;
-  std::vector<int> list(nbroken);
-  d2h(e, list.data(), e->broken_list.p, nbroken);
-  sync(e);
-  std::sort(list.begin(), list.end());
-  PView p = e->view();
-  bool any = false;
+  BondSet bs;
+  memset(&bs, 0, sizeof bs);
   for (int gi : e->bord) {
     BondGroup& G = e->bg[gi];
-    auto lo = std::lower_bound(list.begin(), list.end(), G.bond_off);
-    auto hi = std::lower_bound(list.begin(), list.end(), G.bond_off + G.n);
-    const int cnt = (int)(hi - lo);
-    if (cnt == 0) continue;
-    any = true;
-    std::vector<long long> rem;
-    std::vector<char> gone(G.n, 0);
-    for (auto it = lo; it != hi; ++it) {
-      const int b = *it - G.bond_off;
-      gone[b] = 1;
-      rem.push_back(pkey(G.hid1[b], G.hid2[b]));
-      e->breaks.push_back((int)e->t);
-      e->breaks.push_back(G.hid1[b]);
-      e->breaks.push_back(G.hid2[b]);
-    }
-    std::sort(rem.begin(), rem.end());
-    // device: flag particles, compact bond arrays
-    e->keep.reserve(G.n + 1);
-    e->keep_off.reserve(G.n + 1);
-    e->brk_list.reserve(cnt);
-    h2d(e, e->brk_list.p, &*lo, cnt);
-    launch_bond_mark_broken(e->stream, cnt, e->brk_list.p, G.bond_off, G.n, e->keep.p, p, G.ij.p);
-    const int nb2 = scan_total(e, e->keep.p, e->keep_off.p, G.n);
-    const size_t c2 = std::max(nb2, 1);
-    e->tmp_ij.reserve(c2); e->tmp_a.reserve(c2); e->tmp_b.reserve(c2); e->tmp_c.reserve(c2);
-    launch_compact_bonds(e->stream, G.n, e->keep.p, e->keep_off.p, G.ij.p, G.k4.p, G.mx4.p,
-                         G.D4.p, e->tmp_ij.p, e->tmp_a.p, e->tmp_b.p, e->tmp_c.p);
-    G.ij.swap(e->tmp_ij); G.k4.swap(e->tmp_a); G.mx4.swap(e->tmp_b); G.D4.swap(e->tmp_c);
-    // sorted keys
-    e->brk_keys.reserve(rem.size());
-    h2d(e, e->brk_keys.p, rem.data(), rem.size());
-    launch_keys_mark_removed(e->stream, G.nkeys, G.skeys.p, (int)rem.size(), e->brk_keys.p,
-                             e->keep.p);
-    const int nk2 = scan_total(e, e->keep.p, e->keep_off.p, G.nkeys);
-    e->tmp_skeys.reserve(std::max(nk2, 1));
-    launch_keys_compact(e->stream, G.nkeys, e->keep.p, e->keep_off.p, G.skeys.p, e->tmp_skeys.p);
-    HIPCHK(hipGetLastError());
-    G.skeys.swap(e->tmp_skeys);
-    G.nkeys = nk2;
-    int w = 0;
-    for (int b = 0; b < G.n; b++)
-      if (!gone[b]) { G.hid1[w] = G.hid1[b]; G.hid2[w] = G.hid2[b]; w++; }
-    G.hid1.resize(w);
-    G.hid2.resize(w);
-    G.n = nb2;
+    const int k = bs.count++;
+    bs.off[k] = G.bond_off;
+    bs.n[k] = G.n;
+    bs.rec_off[k] = G.rec_off;
+    bs.ij[k] = G.ij.p;
+    bs.hs[k] = HashRef{G.hs.p, G.hs_mask};
   }
-  // renumber bond offsets
-  int off = 0;
-  for (int gi : e->bord) { e->bg[gi].bond_off = off; off += e->bg[gi].n; }
-  return any;
-}
-
-// TSubLattice::updateInteractions (SubLattice.hpp:2416-2466)
-void update_interactions(esp_engine* e, int nbroken_prev)
-{
-  bool need_update = remove_broken_bonds(e, nbroken_prev);
-  bool changed = need_update;
-  if (e->timestamp > e->last_ns || need_update) {
-    bool new_list_taken = false;
-    for (int gi : e->dord) dyn_update(e, e->dg[gi], new_list_taken);
-    if (!e->wg.empty()) {
-      WallParam w[ESP_MAX_WALL_GROUPS];
-      wall_params(e, w);
-      launch_wall_update(e->stream, e->view(), e->grid, (int)e->wg.size(), w);
-      HIPCHK(hipGetLastError());
-    }
-    e->last_ns = e->timestamp;
-    changed = true;
-  }
-  if (changed) rebuild_csr(e);
+  return bs;
 }
 
 // ---------------------------------------------------------------------------
 // one step's kernels
 // ---------------------------------------------------------------------------
-void launch_step(esp_engine* e, const StepCtl& ctl, double* whist_out)
+void launch_step(esp_engine* e, const StepCtl& ctl, double* whist_out, long long step)
 {
   PView p = e->view();
+  if (ctl.guarded) {
+    // bookkeeping of the previous step's bond breaks, on the device
+    if (!e->bg.empty())
+      launch_break_apply(e->stream, ctl, p, bond_set(e), e->broken_list.p, e->contrib.p, e->upd.p,
+                         e->flist.p, e->evt.p, step);
+    if (!e->dg.empty()) {
+      DynGroup& G = e->dg[e->dord[0]];
+      DynAppend da;
+      da.ij = G.ij.p;
+      da.key = G.key.p;
+      da.ff = G.kind == 3 ? G.ff.p : nullptr;
+      da.n_app = G.napp.p;
+      da.n_main = G.n;
+      da.rec_off = G.rec_off;
+      da.app_rec_off = G.app_rec_off;
+      da.is_hist = G.kind == 3;
+      da.present = HashRef{G.hs.p, G.hs_mask};
+      da.ex = excl_set(e, G);
+      launch_incremental_update(e->stream, ctl, p, e->grid, da, e->cell.p, e->cell_start.p,
+                                e->cell_items.p, e->upd.p, e->flist.p, e->xslot.p, e->xcnt.p,
+                                e->xdemand.p, e->nkey.p, e->nidx.p, 1);
+    }
+  }
   for (int gi : e->dord) {
     DynGroup& G = e->dg[gi];
     LaunchTimer lt(e, G.kind == 3 ? ESP_KCLASS_FRICTION : ESP_KCLASS_ELASTIC, ctl.step_id);
     if (G.kind == 1)
-      launch_nrot_elastic(e->stream, ctl, p, G.n, G.ij.p, G.k, G.scaling, e->do2d, e->contrib.p,
-                          G.rec_off);
+      launch_nrot_elastic(e->stream, ctl, p, G.n, G.napp.p, G.ij.p, G.k, G.scaling, e->do2d,
+                          e->contrib.p, G.rec_off, G.app_rec_off);
     else if (G.kind == 2)
-      launch_rot_elastic(e->stream, ctl, p, G.n, G.ij.p, G.k, G.scaling, e->do2d, e->contrib.p,
-                         G.rec_off);
+      launch_rot_elastic(e->stream, ctl, p, G.n, G.napp.p, G.ij.p, G.k, G.scaling, e->do2d,
+                         e->contrib.p, G.rec_off, G.app_rec_off);
     else
-      launch_rot_friction(e->stream, ctl, p, G.n, G.ij.p, G.ff.p, G.k, G.ks, G.scaling, G.meanR,
-                          e->do2d, G.mu_d, G.mu_s, e->dt, e->contrib.p, G.rec_off);
+      launch_rot_friction(e->stream, ctl, p, G.n, G.napp.p, G.ij.p, G.ff.p, G.k, G.ks, G.scaling,
+                          G.meanR, e->do2d, G.mu_d, G.mu_s, e->dt, e->contrib.p, G.rec_off,
+                          G.app_rec_off);
   }
   for (int gi : e->bord) {
     BondGroup& G = e->bg[gi];
@@ -733,6 +860,9 @@ void launch_step(esp_engine* e, const StepCtl& ctl, double* whist_out)
   memset(&gp, 0, sizeof gp);
   gp.inc_off = e->inc_off.p;
   gp.inc_ent = e->inc_ent.p;
+  gp.xslot = e->xslot.p;
+  gp.xcnt = e->xcnt.p;
+  gp.grid = e->grid;
   gp.contrib = e->contrib.p;
   gp.dt = e->dt;
   gp.rot = e->rot;
@@ -787,38 +917,64 @@ void apply_wall_motion(esp_engine* e)
       for (int c = 0; c < 3; c++) W.o[c] = W.o[c] + W.step[c];
 }
 
-// one step with host bookkeeping (first step, rebuild, broken bonds)
-void host_step(esp_engine* e)
+StepCtl make_ctl(esp_engine* e, long long step, int step_id, int guarded)
 {
-  e->num_host_phases++;
-  const long long step = e->t + 1;
-  int nbroken_prev = 0;
-  bool checked = false;
-  if (e->first_time) {
-    rebuild(e, step);  // forced search of the first runOneStep
-    e->first_time = false;
-    checked = true;
-  }
-  e->t = step;
-  apply_wall_motion(e);
-  if (!checked) {
-    StepStatus st;
-    d2h(e, &st, e->ring.p + ((step - 1) % e->ring_size), 1);
-    sync(e);
-    nbroken_prev = st.nbroken;
-    if (st.maxdisp2 > thresh_bits(e)) rebuild(e, step);
-  }
-  update_interactions(e, nbroken_prev);
-  launch_clear_status(e->stream, e->ring.p, e->ring_size, step, 1, e->halted.p);
   StepCtl ctl;
   ctl.prev = e->ring.p + ((step - 1) % e->ring_size);
   ctl.cur = e->ring.p + (step % e->ring_size);
   ctl.halted = e->halted.p;
   ctl.thresh = thresh_bits(e);
-  ctl.step_id = 1;
-  ctl.guarded = 0;
-  ctl.first = 1;
-  launch_step(e, ctl, e->wall_hist.p);
+  ctl.step_id = step_id;
+  ctl.guarded = guarded;
+  ctl.reason = &e->upd.p->halt_reason;
+  return ctl;
+}
+
+// one step with host bookkeeping: the first step, a rebuild, or an
+// incremental update that did not fit the device appendix
+void host_step(esp_engine* e, int reason)
+{
+  e->num_host_phases++;
+  const long long step = e->t + 1;
+  StepStatus prev;
+  memset(&prev, 0, sizeof prev);
+  if (!e->first_time) {
+    d2h(e, &prev, e->ring.p + ((step - 1) % e->ring_size), 1);
+    sync(e);
+  }
+  // PIS_EB::update for the previous step's breaks (already applied on the
+  // device when the queue stopped in the incremental update)
+  if (prev.nbroken > 0 && reason != 2 && !e->bg.empty()) {
+    StepCtl c0 = make_ctl(e, step, 1, 0);
+    launch_break_apply(e->stream, c0, e->view(), bond_set(e), e->broken_list.p, e->contrib.p,
+                       e->upd.p, e->flist.p, e->evt.p, step);
+    HIPCHK(hipGetLastError());
+  }
+  fold(e);
+  if (e->first_time) {
+    rebuild(e, step);  // forced search of the first runOneStep
+    e->first_time = false;
+  } else if (prev.maxdisp2 > thresh_bits(e)) {
+    rebuild(e, step);
+  }
+  e->t = step;
+  apply_wall_motion(e);
+  // TSubLattice::updateInteractions (SubLattice.hpp:2416-2466)
+  const bool need_update = prev.nbroken > 0;
+  if (e->timestamp > e->last_ns || need_update) {
+    bool new_list_taken = false;
+    for (int gi : e->dord) dyn_update(e, e->dg[gi], new_list_taken);
+    if (!e->wg.empty()) {
+      WallParam w[ESP_MAX_WALL_GROUPS];
+      wall_params(e, w);
+      launch_wall_update(e->stream, e->view(), e->grid, (int)e->wg.size(), w);
+      HIPCHK(hipGetLastError());
+    }
+    e->last_ns = e->timestamp;
+  }
+  rebuild_csr(e);
+  launch_clear_status(e->stream, e->ring.p, e->ring_size, step, 1, e->halted.p);
+  launch_step(e, make_ctl(e, step, 1, 0), e->wall_hist.p, step);
   const int nwg = (int)e->wg.size();
   std::vector<double> h((size_t)nwg * 3);
   if (nwg) d2h(e, h.data(), e->wall_hist.p, h.size());
@@ -828,7 +984,7 @@ void host_step(esp_engine* e)
 }
 
 // up to K steps queued without host bookkeeping; returns steps completed
-int fast_batch(esp_engine* e, int k)
+int fast_batch(esp_engine* e, int k, int* reason)
 {
   const long long step0 = e->t + 1;
   const int nwg = (int)e->wg.size();
@@ -842,22 +998,18 @@ int fast_batch(esp_engine* e, int k)
     for (auto& W : e->walls) snap.insert(snap.end(), W.o, W.o + 3);
     wall_o.push_back(snap);
     const long long step = step0 + s;
-    StepCtl ctl;
-    ctl.prev = e->ring.p + ((step - 1) % e->ring_size);
-    ctl.cur = e->ring.p + (step % e->ring_size);
-    ctl.halted = e->halted.p;
-    ctl.thresh = thresh_bits(e);
-    ctl.step_id = s + 1;
-    ctl.guarded = 1;
-    ctl.first = 1;
-    launch_step(e, ctl, e->wall_hist.p + (size_t)s * std::max(nwg, 1) * 3);
+    launch_step(e, make_ctl(e, step, s + 1, 1), e->wall_hist.p + (size_t)s * std::max(nwg, 1) * 3,
+                step);
   }
   int halted = 0;
+  UpdState st;
   d2h(e, &halted, e->halted.p, 1);
+  d2h(e, &st, e->upd.p, 1);
   std::vector<double> h((size_t)k * nwg * 3);
   if (nwg) d2h(e, h.data(), e->wall_hist.p, h.size());
   sync(e);
   const int done = halted ? halted - 1 : k;
+  *reason = halted ? st.halt_reason : 0;
   collect_timing(e, done);
   if (done < k) {  // walls stay where the last completed step left them
     const std::vector<double>& o = done > 0 ? wall_o[done - 1] : before;
@@ -878,6 +1030,8 @@ void run_steps(esp_engine* e, int nsteps)
       throw EspError(ESP_ERR_CAPACITY, "more than 8 wall interaction groups");
     if (e->grav.size() > ESP_MAX_GRAVITY)
       throw EspError(ESP_ERR_CAPACITY, "more than 4 gravity groups");
+    if (e->bg.size() > ESP_MAX_BOND_GROUPS)
+      throw EspError(ESP_ERR_CAPACITY, "more than 8 bonded groups");
     for (auto& G : e->dg)
       if (G.ex.size() > ESP_MAX_EXCL)
         throw EspError(ESP_ERR_CAPACITY, "more than 4 exclusions on one group");
@@ -886,7 +1040,7 @@ void run_steps(esp_engine* e, int nsteps)
   int done = 0;
   while (done < nsteps) {
     if (e->first_time || e->pending_host) {
-      host_step(e);
+      host_step(e, e->first_time ? 0 : e->pending_reason);
       e->pending_host = false;
       done++;
       continue;
@@ -894,7 +1048,9 @@ void run_steps(esp_engine* e, int nsteps)
     // queue depth adapts to how often host bookkeeping interrupts the queue:
     // it doubles after a full batch and drops to twice the completed run
     const int k = std::min(e->kcur, nsteps - done);
-    const int got = fast_batch(e, k);
+    int reason = 0;
+    const int got = fast_batch(e, k, &reason);
+    e->pending_reason = reason;
     e->kcur = got == k ? std::min(e->K, 2 * e->kcur) : std::max(1, std::min(e->K, 2 * (got + 1)));
     done += got;
   }
@@ -1296,6 +1452,7 @@ int esp_get_num_particles(esp_engine* e) { return e->n; }
 int esp_get_stats(esp_engine* e, esp_stats* o)
 {
   API_BEGIN
+  if (e->started) fold(e);
   memset(o, 0, sizeof *o);
   o->time_step = e->t;
   o->num_rebuilds = e->timestamp;
@@ -1422,16 +1579,20 @@ int esp_get_wall_force_history(esp_engine* e, const char* name, int n, double* o
 
 int esp_get_interaction_count(esp_engine* e, const char* name)
 {
+  API_BEGIN
+  if (e->started) fold(e);
   const int d = find_dg(e, name ? name : "");
   if (d >= 0) return e->dg[d].n;
   const int b = find_bg(e, name ? name : "");
   if (b >= 0) return e->bg[b].n;
   return -1;
+  API_END
 }
 
 int esp_get_interactions(esp_engine* e, const char* name, int* id1, int* id2)
 {
   API_BEGIN
+  if (e->started) fold(e);
   const int d = find_dg(e, name ? name : "");
   if (d >= 0) {
     DynGroup& G = e->dg[d];
@@ -1461,6 +1622,7 @@ int esp_get_interactions(esp_engine* e, const char* name, int* id1, int* id2)
 int esp_get_friction_state(esp_engine* e, const char* name, double* ffric, int* flg)
 {
   API_BEGIN
+  if (e->started) fold(e);
   const int d = find_dg(e, name ? name : "");
   if (d < 0 || e->dg[d].kind != 3) throw EspError(ESP_ERR_ARG, "not a RotFriction group");
   DynGroup& G = e->dg[d];
@@ -1496,6 +1658,7 @@ int esp_get_neighbour_pairs(esp_engine* e, int* id1, int* id2)
 int esp_get_bond_breaks(esp_engine* e, int* step, int* id1, int* id2)
 {
   API_BEGIN
+  if (e->started) fold(e);
   const int n = (int)e->breaks.size() / 3;
   if (step && id1 && id2) {
     std::vector<std::array<int, 3>> v(n);

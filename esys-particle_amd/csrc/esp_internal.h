@@ -45,7 +45,7 @@ struct StepCtl {
   unsigned long long thresh;  // bits of (verletDist/2)^2
   int step_id;              // 1 + index of this step in its batch
   int guarded;              // 0 for steps launched after host bookkeeping
-  int first;                // the kernel that records the halt
+  int* reason;              // why the queue halted (UpdState::halt_reason)
 };
 
 struct GridParams {
@@ -76,6 +76,9 @@ struct WallParam {
 struct GatherParams {
   const int* inc_off;
   const int* inc_ent;
+  const int* xslot;  // ESP_XSLOTS appended-record entries per particle
+  const int* xcnt;
+  GridParams grid;   // wall candidates are refreshed after bond breaks
   const double* contrib;
   double dt;
   int rot, do2d;
@@ -86,23 +89,86 @@ struct GatherParams {
   double* wall_part;  // [block][nwg][3]
 };
 
+// Open-addressing hash sets of pair keys (id1 << 32 | id2): live bonds of a
+// bonded group (exclusion lookups) and stored pairs of a dynamic group (isIn).
+#define HS_EMPTY 0xFFFFFFFFFFFFFFFFull
+#define HS_TOMB 0xFFFFFFFFFFFFFFFEull
+struct HashRef {
+  unsigned long long* t;
+  unsigned mask;
+};
+
 struct ExclSet {
-  const long long* keys[ESP_MAX_EXCL];
-  int n[ESP_MAX_EXCL];
+  HashRef h[ESP_MAX_EXCL];
   int count;
+};
+
+#define ESP_MAX_BOND_GROUPS 8
+// bonded groups as the break-application kernel sees them
+struct BondSet {
+  int count;
+  int off[ESP_MAX_BOND_GROUPS];  // global bond index of the group's first bond
+  int n[ESP_MAX_BOND_GROUPS];
+  int rec_off[ESP_MAX_BOND_GROUPS];
+  int2* ij[ESP_MAX_BOND_GROUPS];
+  HashRef hs[ESP_MAX_BOND_GROUPS];
+};
+
+// Device-side bookkeeping state shared by the update kernels.
+struct UpdState {
+  int fcount;       // particles in the flagged list
+  int evt_count;    // bond-break events logged on the device
+  int halt_reason;  // 1 rebuild due, 2 incremental update does not fit
+  int pad;
+};
+
+// Appendix of a dynamic group: pairs added between neighbour-table rebuilds
+// are stored after the key-sorted main list, with their force records in a
+// separate region and per-particle overflow slots in the CSR.
+#define ESP_APP_CAP 16384
+#define ESP_NEW_CAP 4096
+#define ESP_XSLOTS 4
+struct DynAppend {
+  int2* ij;          // whole list: main [0, n_main), appendix after it
+  long long* key;
+  double4* ff;       // shear history (history groups) or nullptr
+  int* n_app;        // device counter
+  int n_main;
+  int rec_off;       // records of the main list
+  int app_rec_off;   // records of the appendix
+  int is_hist;       // carries history / checks presence
+  HashRef present;   // stored pairs (history groups)
+  ExclSet ex;
 };
 
 // ---- per-step kernels ----
 void launch_rot_bond(hipStream_t s, StepCtl ctl, PView p, int nb, const int2* ij, const double4* k4,
                      const double4* mx4, const double4* D4, double truncated, double beta1,
                      double beta2, double* contrib, int rec_off, int bond_off, int* broken_list);
-void launch_rot_friction(hipStream_t s, StepCtl ctl, PView p, int nc, const int2* ij, double4* ff,
-                         double k, double ks, int scaling, int meanR, int do2d, double mu_d,
-                         double mu_s, double dt, double* contrib, int rec_off);
-void launch_nrot_elastic(hipStream_t s, StepCtl ctl, PView p, int nc, const int2* ij, double k,
-                         int scaling, int do2d, double* contrib, int rec_off);
-void launch_rot_elastic(hipStream_t s, StepCtl ctl, PView p, int nc, const int2* ij, double k,
-                        int scaling, int do2d, double* contrib, int rec_off);
+// nc = main-list size; appended pairs (device count) follow it
+void launch_rot_friction(hipStream_t s, StepCtl ctl, PView p, int nc, const int* n_app,
+                         const int2* ij, double4* ff, double k, double ks, int scaling, int meanR,
+                         int do2d, double mu_d, double mu_s, double dt, double* contrib,
+                         int rec_off, int app_rec_off);
+void launch_nrot_elastic(hipStream_t s, StepCtl ctl, PView p, int nc, const int* n_app,
+                         const int2* ij, double k, int scaling, int do2d, double* contrib,
+                         int rec_off, int app_rec_off);
+void launch_rot_elastic(hipStream_t s, StepCtl ctl, PView p, int nc, const int* n_app,
+                        const int2* ij, double k, int scaling, int do2d, double* contrib,
+                        int rec_off, int app_rec_off);
+// PIS_EB::update on the device: bonds of the previous step's break list die,
+// their particles are flagged, events are logged
+void launch_break_apply(hipStream_t s, StepCtl ctl, PView p, BondSet bs, const int* broken_list,
+                        double* contrib, UpdState* st, int* flist, int4* evt, long long step);
+// getNewList + PIS_NE/PIS_ED incremental update of the first dynamic group
+void launch_incremental_update(hipStream_t s, StepCtl ctl, PView p, GridParams g, DynAppend da,
+                               const int* cell, const int* cell_start, const int* cell_items,
+                               UpdState* st, int* flist, int* xslot, int* xcnt, int* xdemand,
+                               long long* nkey, int* nidx, int has_group);
+void launch_hash_clear(hipStream_t s, HashRef h);
+void launch_hash_insert(hipStream_t s, HashRef h, const long long* keys, int n);
+void launch_bond_keys(hipStream_t s, int nb, const int2* ij, const int* id, long long* keys);
+void launch_alive_flags(hipStream_t s, int nb, const int2* ij, int* keep);
 void launch_gather_integrate(hipStream_t s, StepCtl ctl, PView p, const GatherParams& gp);
 void launch_wall_finalize(hipStream_t s, StepCtl ctl, const double* wall_part, int nblocks,
                           int nwg, double* out);
@@ -119,7 +185,7 @@ void launch_cells_sort(hipStream_t s, int ncell, const int* cell_start, int* cel
 void launch_pairs(hipStream_t s, int mode_emit, int flagged, PView p, GridParams g, double alpha,
                   const int* by_id, const int* cell, const int* cell_start, const int* cell_items,
                   int* cnt, const int* off, int* out_p, int* out_q, long long* out_key);
-void launch_clear_flag_bit(hipStream_t s, PView p);
+void launch_clear_flag_bit(hipStream_t s, PView p, UpdState* st);
 void launch_ed_mark(hipStream_t s, int nc, const int* cp, const int* cq, const long long* ckey,
                     ExclSet ex, const long long* okey, int nold, const int2* oij, PView p,
                     int* keep, int* src);
@@ -127,23 +193,17 @@ void launch_ne_mark(hipStream_t s, int nc, const long long* ckey, ExclSet ex, in
 void launch_pair_emit(hipStream_t s, int nc, const int* keep, const int* off, const int* cp,
                       const int* cq, const long long* ckey, const int* src, const double4* off_ff,
                       int2* nij, long long* nkey, double4* nff);
-void launch_filter_new(hipStream_t s, int nc, const long long* ckey, ExclSet ex,
-                       const long long* okey, int nold, int check_present, int* keep);
+void launch_filter_new(hipStream_t s, int nc, const long long* ckey, ExclSet ex, HashRef present,
+                       int check_present, int* keep);
 void launch_merge(hipStream_t s, int nold, const int2* oij, const long long* okey,
                   const double4* off_, int nnew, const int2* nij_in, const long long* nkey_in,
-                  int2* ij_out, long long* key_out, double4* ff_out);
+                  const double4* nff_in, int2* ij_out, long long* key_out, double4* ff_out);
 void launch_pack_pairs(hipStream_t s, int n, const int* keep, const int* off, const int* cp,
                        const int* cq, const long long* ckey, int2* ij, long long* key);
-void launch_bond_mark_broken(hipStream_t s, int nbroken, const int* list, int bond_off, int nb,
-                             int* removed, PView p, const int2* ij);
 void launch_compact_bonds(hipStream_t s, int nb, const int* removed, const int* off,
                           const int2* ij, const double4* k4, const double4* mx4,
                           const double4* D4, int2* ij2, double4* k42, double4* mx42,
                           double4* D42);
-void launch_keys_mark_removed(hipStream_t s, int nk, const long long* keys, int nrem,
-                              const long long* rem_sorted, int* keep);
-void launch_keys_compact(hipStream_t s, int nk, const int* keep, const int* off,
-                         const long long* keys, long long* out);
 void launch_wall_update(hipStream_t s, PView p, GridParams g, int nwg, const WallParam* walls);
 void launch_inc_count(hipStream_t s, int nc, const int2* ij, int* inc_cnt);
 void launch_inc_fill(hipStream_t s, int nc, const int2* ij, int rec_off, int has_m,

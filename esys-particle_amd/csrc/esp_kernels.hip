@@ -32,12 +32,64 @@ __device__ __forceinline__ bool step_skip(const StepCtl& c)
 {
   if (!c.guarded) return false;
   if (*(volatile int*)c.halted) return true;
-  const unsigned long long md = c.prev->maxdisp2;
-  const int nb = c.prev->nbroken;
-  if (md > c.thresh || nb > 0) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) *c.halted = c.step_id;
+  if (c.prev->maxdisp2 > c.thresh) {  // neighbour table must be rebuilt: host bookkeeping
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      *c.reason = 1;
+      *c.halted = c.step_id;
+    }
     return true;
   }
+  return false;
+}
+
+__device__ __forceinline__ long long pair_key(int a, int b)
+{
+  return (long long)(((unsigned long long)(unsigned)a << 32) | (unsigned long long)(unsigned)b);
+}
+
+// ---------------------------------------------------------------------------
+// pair-key hash sets (linear probing; bounded probes)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ unsigned hs_slot(unsigned long long k, unsigned mask)
+{
+  k ^= k >> 33;
+  k *= 0xff51afd7ed558ccdull;
+  k ^= k >> 33;
+  k *= 0xc4ceb9fe1a85ec53ull;
+  k ^= k >> 33;
+  return (unsigned)k & mask;
+}
+__device__ __forceinline__ int hs_find(HashRef h, long long key)
+{
+  const unsigned long long k = (unsigned long long)key;
+  unsigned s = hs_slot(k, h.mask);
+  for (unsigned probe = 0; probe <= h.mask; probe++) {
+    const unsigned long long v = h.t[s];
+    if (v == k) return (int)s;
+    if (v == HS_EMPTY) return -1;
+    s = (s + 1) & h.mask;
+  }
+  return -1;
+}
+__device__ __forceinline__ void hs_insert(HashRef h, long long key)
+{
+  const unsigned long long k = (unsigned long long)key;
+  unsigned s = hs_slot(k, h.mask);
+  for (unsigned probe = 0; probe <= h.mask; probe++) {
+    const unsigned long long old = atomicCAS(&h.t[s], HS_EMPTY, k);
+    if (old == HS_EMPTY || old == k) return;
+    s = (s + 1) & h.mask;
+  }
+}
+__device__ __forceinline__ void hs_erase(HashRef h, long long key)
+{
+  const int s = hs_find(h, key);
+  if (s >= 0) h.t[s] = HS_TOMB;
+}
+__device__ __forceinline__ bool excluded(const ExclSet& ex, long long key)
+{
+  for (int x = 0; x < ex.count; x++)
+    if (hs_find(ex.h[x], key) >= 0) return true;
   return false;
 }
 
@@ -55,6 +107,7 @@ __global__ __launch_bounds__(BS) void k_rot_bond(StepCtl ctl, PView p, int nb, c
   const int b = blockIdx.x * BS + threadIdx.x;
   if (b >= nb) return;
   const int2 e = ij[b];
+  if (e.x < 0) return;  // broken earlier: removed from the group, record left at zero
   const double4 P1 = p.pos[e.x], P2 = p.pos[e.y];
   const double4 Q1 = p.quat[e.x], Q2 = p.quat[e.y];
   const double4 K = k4[b], D = D4[b];
@@ -73,18 +126,24 @@ __global__ __launch_bounds__(BS) void k_rot_bond(StepCtl ctl, PView p, int nb, c
   }
 }
 
-__global__ __launch_bounds__(BS) void k_rot_friction(StepCtl ctl, PView p, int nc, const int2* __restrict__ ij,
+__device__ __forceinline__ int record_of(int c, int nc, int rec_off, int app_rec_off)
+{
+  return c < nc ? rec_off + c : app_rec_off + (c - nc);
+}
+
+__global__ __launch_bounds__(BS) void k_rot_friction(StepCtl ctl, PView p, int nc, const int* n_app,
+                                                     const int2* __restrict__ ij,
                                                      double4* __restrict__ ff, double k, double ks,
                                                      int scaling, int meanR, int do2d, double mu_d,
                                                      double mu_s, double dt, double* contrib,
-                                                     int rec_off)
+                                                     int rec_off, int app_rec_off)
 {
   if (step_skip(ctl)) return;
   const int c = blockIdx.x * BS + threadIdx.x;
-  if (c >= nc) return;
+  if (c >= nc + *n_app) return;
   const int2 e = ij[c];
   const double4 P1 = p.pos[e.x], P2 = p.pos[e.y];
-  double* r = contrib + 9 * (size_t)(rec_off + c);
+  double* r = contrib + 9 * (size_t)record_of(c, nc, rec_off, app_rec_off);
   const dv3 Dq = sub3(xyz(P1), xyz(P2));
   const double eq = P1.w + P2.w;
   const double4 st = ff[c];
@@ -112,16 +171,17 @@ __global__ __launch_bounds__(BS) void k_rot_friction(StepCtl ctl, PView p, int n
 }
 
 // CElasticInteraction::calcForces (ElasticInteraction.cpp:54-71)
-__global__ __launch_bounds__(BS) void k_nrot_elastic(StepCtl ctl, PView p, int nc, const int2* __restrict__ ij,
-                                                     double k, int scaling, int do2d,
-                                                     double* contrib, int rec_off)
+__global__ __launch_bounds__(BS) void k_nrot_elastic(StepCtl ctl, PView p, int nc, const int* n_app,
+                                                     const int2* __restrict__ ij, double k,
+                                                     int scaling, int do2d, double* contrib,
+                                                     int rec_off, int app_rec_off)
 {
   if (step_skip(ctl)) return;
   const int c = blockIdx.x * BS + threadIdx.x;
-  if (c >= nc) return;
+  if (c >= nc + *n_app) return;
   const int2 e = ij[c];
   const double4 P1 = p.pos[e.x], P2 = p.pos[e.y];
-  double* r = contrib + 9 * (size_t)(rec_off + c);
+  double* r = contrib + 9 * (size_t)record_of(c, nc, rec_off, app_rec_off);
   const dv3 D = sub3(xyz(P1), xyz(P2));
   double dist = dot3(D, D);
   const double eq_dist = P1.w + P2.w;
@@ -136,16 +196,17 @@ __global__ __launch_bounds__(BS) void k_nrot_elastic(StepCtl ctl, PView p, int n
 }
 
 // CRotElasticInteraction::calcForces (RotElasticInteraction.cpp:69-88)
-__global__ __launch_bounds__(BS) void k_rot_elastic(StepCtl ctl, PView p, int nc, const int2* __restrict__ ij,
-                                                    double k, int scaling, int do2d,
-                                                    double* contrib, int rec_off)
+__global__ __launch_bounds__(BS) void k_rot_elastic(StepCtl ctl, PView p, int nc, const int* n_app,
+                                                    const int2* __restrict__ ij, double k,
+                                                    int scaling, int do2d, double* contrib,
+                                                    int rec_off, int app_rec_off)
 {
   if (step_skip(ctl)) return;
   const int c = blockIdx.x * BS + threadIdx.x;
-  if (c >= nc) return;
+  if (c >= nc + *n_app) return;
   const int2 e = ij[c];
   const double4 P1 = p.pos[e.x], P2 = p.pos[e.y];
-  double* r = contrib + 9 * (size_t)(rec_off + c);
+  double* r = contrib + 9 * (size_t)record_of(c, nc, rec_off, app_rec_off);
   const dv3 rb = sub3(xyz(P1), xyz(P2));
   const double rbNorm = norm3(rb);
   const double r_0Norm = P1.w + P2.w;
@@ -156,6 +217,49 @@ __global__ __launch_bounds__(BS) void k_rot_elastic(StepCtl ctl, PView p, int nc
   } else {
     r[0] = 0.0; r[1] = 0.0; r[2] = 0.0;
   }
+}
+
+// NeighborTable::index (ntable.hpp:128-139): floor((x - x0)/dim) - global index,
+// with values outside the int range (and NaN) mapped to "outside" as x86's
+// conversion does.
+__device__ __forceinline__ bool cell_coords(const GridParams& g, double x, double y, double z,
+                                            int& ix, int& iy, int& iz)
+{
+  const double fx = floor((x - g.x0) / g.dim);
+  const double fy = floor((y - g.y0) / g.dim);
+  const double fz = floor((z - g.z0) / g.dim);
+  if (!(fx >= -2147483648.0 && fx < 2147483648.0 && fy >= -2147483648.0 && fy < 2147483648.0 &&
+        fz >= -2147483648.0 && fz < 2147483648.0))
+    return false;
+  ix = (int)fx - g.gx;
+  iy = (int)fy - g.gy;
+  iz = (int)fz - g.gz;
+  return true;
+}
+
+// Wall-group membership and inner flags of one particle (CEWallInteractionGroup::
+// Update, EWallInteractionGroup.hpp:57-82, with isInInner, ntable.hpp:207-222).
+__device__ __forceinline__ unsigned wall_membership(const GridParams& g, int nwg,
+                                                    const WallParam* W, double4 P, unsigned fl)
+{
+  fl &= ~((0xFFu << PF_WALL_SHIFT) | (0xFFu << PF_INNER_SHIFT));
+  int ix = 0, iy = 0, iz = 0;
+  bool inner = false;
+  if (cell_coords(g, P.x, P.y, P.z, ix, iy, iz)) {
+    if (g.nz < 3) inner = (ix > 0) && (ix < g.nx - 1) && (iy > 0) && (iy < g.ny - 1);
+    else
+      inner = (ix > 0) && (ix < g.nx - 1) && (iy > 0) && (iy < g.ny - 1) && (iz > 0) &&
+              (iz < g.nz - 1);
+  }
+  for (int w = 0; w < nwg; w++) {
+    const dv3 n = mk3(W[w].nrm[0], W[w].nrm[1], W[w].nrm[2]);
+    const double dist = dot3(sub3(xyz(P), mk3(W[w].o[0], W[w].o[1], W[w].o[2])), n);
+    if (fabs(dist) < g.dim) {
+      fl |= 1u << (PF_WALL_SHIFT + w);
+      if (inner) fl |= 1u << (PF_INNER_SHIFT + w);
+    }
+  }
+  return fl;
 }
 
 // ---------------------------------------------------------------------------
@@ -187,9 +291,16 @@ __global__ __launch_bounds__(BS) void k_gather_integrate(StepCtl ctl, PView p, G
     dv3 x = xyz(P), v = xyz(Vv);
     const double r = P.w, invm = Vv.w, mass = O.w;
     dv3 F = mk3(0.0, 0.0, 0.0), M = mk3(0.0, 0.0, 0.0);
+    unsigned flw = fl;
+    if (gp.nwg > 0 && ctl.guarded && ctl.prev->nbroken > 0) {
+      // bonds broke in the previous step: the reference's updateInteractions
+      // re-Updates the wall groups (EWallInteractionGroup.hpp:57-82) before forces
+      flw = wall_membership(gp.grid, gp.nwg, gp.wall, P, fl);
+      p.flags[i] = flw;
+    }
     // wall interaction groups, name order (EWallInteraction.hpp:42-78)
     for (int w = 0; w < gp.nwg; w++) {
-      if (!((fl >> (PF_WALL_SHIFT + w)) & 1u)) continue;
+      if (!((flw >> (PF_WALL_SHIFT + w)) & 1u)) continue;
       const WallParam& W = gp.wall[w];
       const dv3 n = mk3(W.nrm[0], W.nrm[1], W.nrm[2]);
       const double dist = dot3(sub3(x, mk3(W.o[0], W.o[1], W.o[2])), n);
@@ -198,7 +309,7 @@ __global__ __launch_bounds__(BS) void k_gather_integrate(StepCtl ctl, PView p, G
         if (!gp.do2d) f *= 3.141592654 * r;
         const dv3 force = mul3(n, (f * W.k) * (r - dist));
         F = add3(F, force);
-        if ((fl >> (PF_INNER_SHIFT + w)) & 1u) {
+        if ((flw >> (PF_INNER_SHIFT + w)) & 1u) {
           wf[w][0] = force.x; wf[w][1] = force.y; wf[w][2] = force.z;
         }
       }
@@ -207,8 +318,9 @@ __global__ __launch_bounds__(BS) void k_gather_integrate(StepCtl ctl, PView p, G
     for (int g = 0; g < gp.ngrav; g++) F = add3(F, mul3(mk3(gp.g[g][0], gp.g[g][1], gp.g[g][2]), mass));
     // interaction records: dynamic groups then bonded groups, each in storage order
     const int e0 = gp.inc_off[i], e1 = gp.inc_off[i + 1];
-    for (int t = e0; t < e1; t++) {
-      const int ent = gp.inc_ent[t];
+    const int xc = gp.xcnt[i];
+    for (int t = e0; t < e1 + xc; t++) {
+      const int ent = t < e1 ? gp.inc_ent[t] : gp.xslot[ESP_XSLOTS * i + (t - e1)];
       const double* rec = gp.contrib + 9 * (size_t)(ent >> 2);
       dv3 f = mk3(rec[0], rec[1], rec[2]);
       if (ent & 1) f = neg3(f);
@@ -322,24 +434,6 @@ __global__ void k_clear_status(StepStatus* ring, int ring_size, long long first_
 // ---------------------------------------------------------------------------
 // neighbour table
 // ---------------------------------------------------------------------------
-// NeighborTable::index (ntable.hpp:128-139): floor((x - x0)/dim) - global index,
-// with values outside the int range (and NaN) mapped to "outside" as x86's
-// conversion does.
-__device__ __forceinline__ bool cell_coords(const GridParams& g, double x, double y, double z,
-                                            int& ix, int& iy, int& iz)
-{
-  const double fx = floor((x - g.x0) / g.dim);
-  const double fy = floor((y - g.y0) / g.dim);
-  const double fz = floor((z - g.z0) / g.dim);
-  if (!(fx >= -2147483648.0 && fx < 2147483648.0 && fy >= -2147483648.0 && fy < 2147483648.0 &&
-        fz >= -2147483648.0 && fz < 2147483648.0))
-    return false;
-  ix = (int)fx - g.gx;
-  iy = (int)fy - g.gy;
-  iz = (int)fz - g.gz;
-  return true;
-}
-
 __global__ void k_cells_count(PView p, GridParams g, int* cell, int* cell_count, int* n_out)
 {
   const int i = blockIdx.x * BS + threadIdx.x;
@@ -385,10 +479,6 @@ __global__ void k_cells_sort(int ncell, const int* cell_start, int* items)
   }
 }
 
-__device__ __forceinline__ long long pair_key(int a, int b)
-{
-  return (long long)(((unsigned long long)(unsigned)a << 32) | (unsigned long long)(unsigned)b);
-}
 
 // Pairs of particle p (taken in ascending id order) with every particle q of
 // higher id in the 27 cells around p's cell: the full list keeps pairs with
@@ -464,10 +554,11 @@ __global__ void k_pairs(int emit, int flagged, PView p, GridParams g, double alp
   }
 }
 
-__global__ void k_clear_flag_bit(PView p)
+__global__ void k_clear_flag_bit(PView p, UpdState* st)
 {
   const int i = blockIdx.x * BS + threadIdx.x;
   if (i < p.n) p.flags[i] &= ~PF_FLAG;
+  if (i == 0) st->fcount = 0;
 }
 
 __device__ __forceinline__ int bsearch_key(const long long* a, int n, long long key)
@@ -502,12 +593,6 @@ __device__ __forceinline__ int upper_bound_key(const long long* a, int n, long l
     else hi = mid;
   }
   return lo;
-}
-__device__ __forceinline__ bool excluded(const ExclSet& ex, long long key)
-{
-  for (int x = 0; x < ex.count; x++)
-    if (bsearch_key(ex.keys[x], ex.n[x], key) >= 0) return true;
-  return false;
 }
 
 // PIS_ED::update on a rebuilt table (pi_storage_ed.hpp:97-136): a candidate is
@@ -562,14 +647,14 @@ __global__ void k_pair_emit(int nc, const int* keep, const int* off, const int* 
 
 // pairs of the flagged list that a group takes: not excluded and (for history
 // groups) not already stored (pi_storage_ed.hpp:137-156, pi_storage_ne.hpp:79-96)
-__global__ void k_filter_new(int nc, const long long* ckey, ExclSet ex, const long long* okey,
-                             int nold, int check_present, int* keep)
+__global__ void k_filter_new(int nc, const long long* ckey, ExclSet ex, HashRef present,
+                             int check_present, int* keep)
 {
   const int c = blockIdx.x * BS + threadIdx.x;
   if (c >= nc) return;
   const long long key = ckey[c];
   int k = excluded(ex, key) ? 0 : 1;
-  if (k && check_present && bsearch_key(okey, nold, key) >= 0) k = 0;
+  if (k && check_present && hs_find(present, key) >= 0) k = 0;
   keep[c] = k;
 }
 
@@ -585,8 +670,8 @@ __global__ void k_pack_pairs(int n, const int* keep, const int* off, const int* 
 
 // merge of two key-sorted lists; equal keys keep stored entries first
 __global__ void k_merge(int nold, const int2* oij, const long long* okey, const double4* off_,
-                        int nnew, const int2* nij_in, const long long* nkey_in, int2* ij_out,
-                        long long* key_out, double4* ff_out)
+                        int nnew, const int2* nij_in, const long long* nkey_in,
+                        const double4* nff_in, int2* ij_out, long long* key_out, double4* ff_out)
 {
   const int t = blockIdx.x * BS + threadIdx.x;
   if (t < nold) {
@@ -601,24 +686,10 @@ __global__ void k_merge(int nold, const int2* oij, const long long* okey, const 
     const int w = j + upper_bound_key(okey, nold, key);
     ij_out[w] = nij_in[j];
     key_out[w] = key;
-    if (ff_out) ff_out[w] = make_double4(0.0, 0.0, 0.0, 0.0);
+    if (ff_out) ff_out[w] = nff_in ? nff_in[j] : make_double4(0.0, 0.0, 0.0, 0.0);
   }
 }
 
-// PIS_EB::update (pi_storage_eb.hpp:23-49): broken bonds leave; both
-// particles are flagged for the next flagged pair list.
-__global__ void k_bond_mark_broken(int nbroken, const int* list, int bond_off, int nb, int* keep,
-                                   PView p, const int2* ij)
-{
-  const int t = blockIdx.x * BS + threadIdx.x;
-  if (t >= nbroken) return;
-  const int b = list[t] - bond_off;
-  if (b < 0 || b >= nb) return;
-  keep[b] = 0;
-  const int2 e = ij[b];
-  atomicOr(&p.flags[e.x], PF_FLAG);
-  atomicOr(&p.flags[e.y], PF_FLAG);
-}
 
 __global__ void k_fill_int(int n, int* a, int v)
 {
@@ -639,19 +710,7 @@ __global__ void k_compact_bonds(int nb, const int* keep, const int* off, const i
   D42[w] = D4[b];
 }
 
-__global__ void k_keys_mark_removed(int nk, const long long* keys, int nrem,
-                                    const long long* rem_sorted, int* keep)
-{
-  const int t = blockIdx.x * BS + threadIdx.x;
-  if (t < nk) keep[t] = bsearch_key(rem_sorted, nrem, keys[t]) >= 0 ? 0 : 1;
-}
 
-__global__ void k_keys_compact(int nk, const int* keep, const int* off, const long long* keys,
-                               long long* out)
-{
-  const int t = blockIdx.x * BS + threadIdx.x;
-  if (t < nk && keep[t]) out[off[t]] = keys[t];
-}
 
 // CEWallInteractionGroup::Update (EWallInteractionGroup.hpp:57-82): particles
 // with |(x - o).n| < gridSpacing become candidates; isInInner (ntable.hpp:207-222)
@@ -663,25 +722,210 @@ __global__ void k_wall_update(PView p, GridParams g, int nwg, WallParam w0, Wall
   const int i = blockIdx.x * BS + threadIdx.x;
   if (i >= p.n) return;
   const WallParam W[ESP_MAX_WALL_GROUPS] = {w0, w1, w2, w3, w4, w5, w6, w7};
-  const double4 P = p.pos[i];
-  unsigned fl = p.flags[i] & ~((0xFFu << PF_WALL_SHIFT) | (0xFFu << PF_INNER_SHIFT));
-  int ix = 0, iy = 0, iz = 0;
-  bool inner = false;
-  if (cell_coords(g, P.x, P.y, P.z, ix, iy, iz)) {
-    if (g.nz < 3) inner = (ix > 0) && (ix < g.nx - 1) && (iy > 0) && (iy < g.ny - 1);
-    else
-      inner = (ix > 0) && (ix < g.nx - 1) && (iy > 0) && (iy < g.ny - 1) && (iz > 0) &&
-              (iz < g.nz - 1);
+  p.flags[i] = wall_membership(g, nwg, W, p.pos[i], p.flags[i]);
+}
+
+// ---------------------------------------------------------------------------
+// bookkeeping of bond breaks on the device
+// ---------------------------------------------------------------------------
+// PIS_EB::update (pi_storage_eb.hpp:23-49) for the bonds whose criterion held in
+// the previous step: the bond leaves its group (marked dead, record zeroed, key
+// erased from the exclusion set), both particles are flagged
+// (RotBondedInteraction.cpp:211-213) and the event is logged.
+__global__ void k_break_apply(StepCtl ctl, PView p, BondSet bs, const int* broken_list,
+                              double* contrib, UpdState* st, int* flist, int4* evt, long long step)
+{
+  if (step_skip(ctl)) return;
+  const int nb = ctl.prev->nbroken;
+  for (int t = blockIdx.x * BS + threadIdx.x; t < nb; t += gridDim.x * BS) {
+    const int gb = broken_list[t];
+    int g = 0;
+    while (g + 1 < bs.count && gb >= bs.off[g + 1]) g++;
+    const int b = gb - bs.off[g];
+    if (b < 0 || b >= bs.n[g]) continue;
+    const int2 e = bs.ij[g][b];
+    if (e.x < 0) continue;
+    const int id1 = p.id[e.x], id2 = p.id[e.y];
+    hs_erase(bs.hs[g], pair_key(id1, id2));
+    bs.ij[g][b] = make_int2(-1, -1);
+    double* r = contrib + 9 * (size_t)(bs.rec_off[g] + b);
+    for (int k = 0; k < 9; k++) r[k] = 0.0;
+    if (!(atomicOr(&p.flags[e.x], PF_FLAG) & PF_FLAG)) flist[atomicAdd(&st->fcount, 1)] = e.x;
+    if (!(atomicOr(&p.flags[e.y], PF_FLAG) & PF_FLAG)) flist[atomicAdd(&st->fcount, 1)] = e.y;
+    evt[atomicAdd(&st->evt_count, 1)] = make_int4((int)step, id1, id2, 0);
   }
-  for (int w = 0; w < nwg; w++) {
-    const dv3 n = mk3(W[w].nrm[0], W[w].nrm[1], W[w].nrm[2]);
-    const double dist = dot3(sub3(xyz(P), mk3(W[w].o[0], W[w].o[1], W[w].o[2])), n);
-    if (fabs(dist) < g.dim) {
-      fl |= 1u << (PF_WALL_SHIFT + w);
-      if (inner) fl |= 1u << (PF_INNER_SHIFT + w);
+}
+
+// getNewList (ntable.hpp:628-664) + the first dynamic group's incremental update
+// (pi_storage_ed.hpp:137-156 / pi_storage_ne.hpp:79-96) in one block: pairs of
+// flagged particles in neighbouring cells of the last build, without a distance
+// test, minus excluded (and, for history groups, stored) pairs, are appended in
+// key order; then every flag is cleared.  When the pairs do not fit the
+// appendix, the queue halts and the host does the update.
+#define U2_BS 1024
+#define NEW_SENTINEL 0x7fffffffffffffffll
+__global__ __launch_bounds__(U2_BS) void k_incremental_update(
+    StepCtl ctl, PView p, GridParams g, DynAppend da, const int* cell, const int* cell_start,
+    const int* cell_items, UpdState* st, int* flist, int* xslot, int* xcnt, int* xdemand,
+    long long* nkey_unused, int* nidx, int has_group)
+{
+  if (step_skip(ctl)) return;
+  if (ctl.prev->nbroken <= 0 || !has_group) return;
+  __shared__ long long s_key[ESP_NEW_CAP];
+  __shared__ int s_idx[ESP_NEW_CAP];
+  __shared__ int s_cnt, s_over, s_keep;
+  const int tid = threadIdx.x;
+  if (tid == 0) {
+    s_cnt = 0;
+    s_over = 0;
+  }
+  __syncthreads();
+  const int nf = st->fcount;
+  for (int t = tid; t < nf; t += U2_BS) {
+    const int pi = flist[t];
+    const int c = cell[pi];
+    const int cx = c / (g.ny * g.nz);
+    const int rem = c - cx * (g.ny * g.nz);
+    const int cy = rem / g.nz;
+    const int cz = rem - cy * g.nz;
+    const int idp = p.id[pi];
+    const int x0 = cx > 0 ? cx - 1 : cx, x1 = cx < g.nx - 1 ? cx + 1 : cx;
+    const int y0 = cy > 0 ? cy - 1 : cy, y1 = cy < g.ny - 1 ? cy + 1 : cy;
+    const int z0 = cz > 0 ? cz - 1 : cz, z1 = cz < g.nz - 1 ? cz + 1 : cz;
+    for (int x = x0; x <= x1; x++)
+      for (int y = y0; y <= y1; y++)
+        for (int z = z0; z <= z1; z++) {
+          const int c2 = g.ny * g.nz * x + g.nz * y + z;
+          const int t1 = cell_start[c2 + 1];
+          for (int u = cell_start[c2]; u < t1; u++) {
+            const int q = cell_items[u];
+            if (!(p.flags[q] & PF_FLAG)) continue;
+            const int idq = p.id[q];
+            if (idq <= idp) continue;
+            const int k = atomicAdd(&s_cnt, 1);
+            if (k < ESP_NEW_CAP) {
+              s_key[k] = pair_key(idp, idq);
+              nidx[2 * k] = pi;
+              nidx[2 * k + 1] = q;
+            } else {
+              s_over = 1;
+            }
+          }
+        }
+  }
+  __syncthreads();
+  const int n = min(s_cnt, ESP_NEW_CAP);
+  // keep: not excluded, not stored (history groups); others sort to the end
+  int m = 1;
+  while (m < n) m <<= 1;
+  for (int t = tid; t < m; t += U2_BS) {
+    if (t < n) {
+      const long long key = s_key[t];
+      const bool keep = !excluded(da.ex, key) && !(da.is_hist && hs_find(da.present, key) >= 0);
+      if (!keep) s_key[t] = NEW_SENTINEL;
+    } else {
+      s_key[t] = NEW_SENTINEL;
     }
+    s_idx[t] = t;
   }
-  p.flags[i] = fl;
+  __syncthreads();
+  for (int k = 2; k <= m; k <<= 1)
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = tid; i < m; i += U2_BS) {
+        const int ixj = i ^ j;
+        if (ixj > i) {
+          const bool up = (i & k) == 0;
+          const long long a = s_key[i], b = s_key[ixj];
+          if ((a > b) == up) {
+            s_key[i] = b;
+            s_key[ixj] = a;
+            const int ti = s_idx[i];
+            s_idx[i] = s_idx[ixj];
+            s_idx[ixj] = ti;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  if (tid == 0) {
+    int kept = 0;
+    while (kept < n && s_key[kept] != NEW_SENTINEL) kept++;
+    s_keep = kept;
+    if (*da.n_app + kept > ESP_APP_CAP) s_over = 1;
+  }
+  __syncthreads();
+  const int kept = s_keep;
+  // per-particle overflow slots must have room for the new records
+  for (int t = tid; t < kept; t += U2_BS) {
+    atomicAdd(&xdemand[nidx[2 * s_idx[t]]], 1);
+    atomicAdd(&xdemand[nidx[2 * s_idx[t] + 1]], 1);
+  }
+  __syncthreads();
+  for (int t = tid; t < kept; t += U2_BS) {
+    const int a = nidx[2 * s_idx[t]], b = nidx[2 * s_idx[t] + 1];
+    if (xcnt[a] + xdemand[a] > ESP_XSLOTS || xcnt[b] + xdemand[b] > ESP_XSLOTS) s_over = 1;
+  }
+  __syncthreads();
+  for (int t = tid; t < kept; t += U2_BS) {
+    xdemand[nidx[2 * s_idx[t]]] = 0;
+    xdemand[nidx[2 * s_idx[t] + 1]] = 0;
+  }
+  __syncthreads();
+  if (s_over) {
+    if (tid == 0) {
+      *ctl.reason = 2;
+      *ctl.halted = ctl.step_id;
+    }
+    return;
+  }
+  const int napp = *da.n_app;
+  for (int t = tid; t < kept; t += U2_BS) {
+    const int a = nidx[2 * s_idx[t]], b = nidx[2 * s_idx[t] + 1];
+    const int w = da.n_main + napp + t;
+    da.ij[w] = make_int2(a, b);
+    da.key[w] = s_key[t];
+    if (da.ff) da.ff[w] = make_double4(0.0, 0.0, 0.0, 0.0);
+    if (da.is_hist) hs_insert(da.present, s_key[t]);
+  }
+  if (tid == 0) {
+    const int hm = da.is_hist ? 2 : 0;  // history groups carry moments
+    for (int t = 0; t < kept; t++) {
+      const int a = nidx[2 * s_idx[t]], b = nidx[2 * s_idx[t] + 1];
+      const int ent = ((da.app_rec_off + napp + t) << 2) | hm;
+      xslot[ESP_XSLOTS * a + xcnt[a]++] = ent;
+      xslot[ESP_XSLOTS * b + xcnt[b]++] = ent | 1;
+    }
+    *da.n_app = napp + kept;
+  }
+  __syncthreads();
+  for (int t = tid; t < nf; t += U2_BS) p.flags[flist[t]] &= ~PF_FLAG;
+  if (tid == 0) st->fcount = 0;
+}
+
+__global__ void k_hash_clear(HashRef h)
+{
+  const unsigned t = blockIdx.x * BS + threadIdx.x;
+  if (t <= h.mask) h.t[t] = HS_EMPTY;
+}
+
+__global__ void k_hash_insert(HashRef h, const long long* keys, int n)
+{
+  const int t = blockIdx.x * BS + threadIdx.x;
+  if (t < n) hs_insert(h, keys[t]);
+}
+
+__global__ void k_bond_keys(int nb, const int2* ij, const int* id, long long* keys)
+{
+  const int b = blockIdx.x * BS + threadIdx.x;
+  if (b >= nb) return;
+  const int2 e = ij[b];
+  keys[b] = e.x < 0 ? -1ll : pair_key(id[e.x], id[e.y]);
+}
+
+__global__ void k_alive_flags(int nb, const int2* ij, int* keep)
+{
+  const int b = blockIdx.x * BS + threadIdx.x;
+  if (b < nb) keep[b] = ij[b].x >= 0 ? 1 : 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -692,6 +936,7 @@ __global__ void k_inc_count(int nc, const int2* ij, int* cnt)
   const int c = blockIdx.x * BS + threadIdx.x;
   if (c >= nc) return;
   const int2 e = ij[c];
+  if (e.x < 0) return;
   atomicAdd(&cnt[e.x], 1);
   atomicAdd(&cnt[e.y], 1);
 }
@@ -702,6 +947,7 @@ __global__ void k_inc_fill(int nc, const int2* ij, int rec_off, int has_m, const
   const int c = blockIdx.x * BS + threadIdx.x;
   if (c >= nc) return;
   const int2 e = ij[c];
+  if (e.x < 0) return;
   const int v = ((rec_off + c) << 2) | (has_m << 1);
   ent[off[e.x] + atomicAdd(&fill[e.x], 1)] = v;
   ent[off[e.y] + atomicAdd(&fill[e.y], 1)] = v | 1;
@@ -829,29 +1075,71 @@ void launch_rot_bond(hipStream_t s, StepCtl ctl, PView p, int nb, const int2* ij
                      truncated, beta1, beta2, contrib, rec_off, bond_off, broken_list);
 }
 
-void launch_rot_friction(hipStream_t s, StepCtl ctl, PView p, int nc, const int2* ij, double4* ff,
-                         double k, double ks, int scaling, int meanR, int do2d, double mu_d,
-                         double mu_s, double dt, double* contrib, int rec_off)
+void launch_rot_friction(hipStream_t s, StepCtl ctl, PView p, int nc, const int* n_app,
+                         const int2* ij, double4* ff, double k, double ks, int scaling, int meanR,
+                         int do2d, double mu_d, double mu_s, double dt, double* contrib,
+                         int rec_off, int app_rec_off)
 {
-  if (nc <= 0) return;
-  hipLaunchKernelGGL(k_rot_friction, dim3(nblk(nc)), dim3(BS), 0, s, ctl, p, nc, ij, ff, k, ks,
-                     scaling, meanR, do2d, mu_d, mu_s, dt, contrib, rec_off);
+  hipLaunchKernelGGL(k_rot_friction, dim3(nblk((long long)nc + ESP_APP_CAP)), dim3(BS), 0, s, ctl,
+                     p, nc, n_app, ij, ff, k, ks, scaling, meanR, do2d, mu_d, mu_s, dt, contrib,
+                     rec_off, app_rec_off);
 }
 
-void launch_nrot_elastic(hipStream_t s, StepCtl ctl, PView p, int nc, const int2* ij, double k,
-                         int scaling, int do2d, double* contrib, int rec_off)
+void launch_nrot_elastic(hipStream_t s, StepCtl ctl, PView p, int nc, const int* n_app,
+                         const int2* ij, double k, int scaling, int do2d, double* contrib,
+                         int rec_off, int app_rec_off)
 {
-  if (nc <= 0) return;
-  hipLaunchKernelGGL(k_nrot_elastic, dim3(nblk(nc)), dim3(BS), 0, s, ctl, p, nc, ij, k, scaling,
-                     do2d, contrib, rec_off);
+  hipLaunchKernelGGL(k_nrot_elastic, dim3(nblk((long long)nc + ESP_APP_CAP)), dim3(BS), 0, s, ctl,
+                     p, nc, n_app, ij, k, scaling, do2d, contrib, rec_off, app_rec_off);
 }
 
-void launch_rot_elastic(hipStream_t s, StepCtl ctl, PView p, int nc, const int2* ij, double k,
-                        int scaling, int do2d, double* contrib, int rec_off)
+void launch_rot_elastic(hipStream_t s, StepCtl ctl, PView p, int nc, const int* n_app,
+                        const int2* ij, double k, int scaling, int do2d, double* contrib,
+                        int rec_off, int app_rec_off)
 {
-  if (nc <= 0) return;
-  hipLaunchKernelGGL(k_rot_elastic, dim3(nblk(nc)), dim3(BS), 0, s, ctl, p, nc, ij, k, scaling,
-                     do2d, contrib, rec_off);
+  hipLaunchKernelGGL(k_rot_elastic, dim3(nblk((long long)nc + ESP_APP_CAP)), dim3(BS), 0, s, ctl,
+                     p, nc, n_app, ij, k, scaling, do2d, contrib, rec_off, app_rec_off);
+}
+
+void launch_break_apply(hipStream_t s, StepCtl ctl, PView p, BondSet bs, const int* broken_list,
+                        double* contrib, UpdState* st, int* flist, int4* evt, long long step)
+{
+  if (bs.count <= 0) return;
+  hipLaunchKernelGGL(k_break_apply, dim3(16), dim3(BS), 0, s, ctl, p, bs, broken_list, contrib, st,
+                     flist, evt, step);
+}
+
+void launch_incremental_update(hipStream_t s, StepCtl ctl, PView p, GridParams g, DynAppend da,
+                               const int* cell, const int* cell_start, const int* cell_items,
+                               UpdState* st, int* flist, int* xslot, int* xcnt, int* xdemand,
+                               long long* nkey, int* nidx, int has_group)
+{
+  hipLaunchKernelGGL(k_incremental_update, dim3(1), dim3(U2_BS), 0, s, ctl, p, g, da, cell,
+                     cell_start, cell_items, st, flist, xslot, xcnt, xdemand, nkey, nidx,
+                     has_group);
+}
+
+void launch_hash_clear(hipStream_t s, HashRef h)
+{
+  hipLaunchKernelGGL(k_hash_clear, dim3(nblk((long long)h.mask + 1)), dim3(BS), 0, s, h);
+}
+
+void launch_hash_insert(hipStream_t s, HashRef h, const long long* keys, int n)
+{
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_hash_insert, dim3(nblk(n)), dim3(BS), 0, s, h, keys, n);
+}
+
+void launch_bond_keys(hipStream_t s, int nb, const int2* ij, const int* id, long long* keys)
+{
+  if (nb <= 0) return;
+  hipLaunchKernelGGL(k_bond_keys, dim3(nblk(nb)), dim3(BS), 0, s, nb, ij, id, keys);
+}
+
+void launch_alive_flags(hipStream_t s, int nb, const int2* ij, int* keep)
+{
+  if (nb <= 0) return;
+  hipLaunchKernelGGL(k_alive_flags, dim3(nblk(nb)), dim3(BS), 0, s, nb, ij, keep);
 }
 
 void launch_gather_integrate(hipStream_t s, StepCtl ctl, PView p, const GatherParams& gp)
@@ -904,10 +1192,9 @@ void launch_pairs(hipStream_t s, int mode_emit, int flagged, PView p, GridParams
                      by_id, cell, cell_start, cell_items, cnt, off, out_p, out_q, out_key);
 }
 
-void launch_clear_flag_bit(hipStream_t s, PView p)
+void launch_clear_flag_bit(hipStream_t s, PView p, UpdState* st)
 {
-  if (p.n <= 0) return;
-  hipLaunchKernelGGL(k_clear_flag_bit, dim3(nblk(p.n)), dim3(BS), 0, s, p);
+  hipLaunchKernelGGL(k_clear_flag_bit, dim3(nblk(p.n > 0 ? p.n : 1)), dim3(BS), 0, s, p, st);
 }
 
 void launch_ed_mark(hipStream_t s, int nc, const int* cp, const int* cq, const long long* ckey,
@@ -934,21 +1221,21 @@ void launch_pair_emit(hipStream_t s, int nc, const int* keep, const int* off, co
                      off_ff, nij, nkey, nff);
 }
 
-void launch_filter_new(hipStream_t s, int nc, const long long* ckey, ExclSet ex,
-                       const long long* okey, int nold, int check_present, int* keep)
+void launch_filter_new(hipStream_t s, int nc, const long long* ckey, ExclSet ex, HashRef present,
+                       int check_present, int* keep)
 {
   if (nc <= 0) return;
-  hipLaunchKernelGGL(k_filter_new, dim3(nblk(nc)), dim3(BS), 0, s, nc, ckey, ex, okey, nold,
+  hipLaunchKernelGGL(k_filter_new, dim3(nblk(nc)), dim3(BS), 0, s, nc, ckey, ex, present,
                      check_present, keep);
 }
 
 void launch_merge(hipStream_t s, int nold, const int2* oij, const long long* okey,
                   const double4* off_, int nnew, const int2* nij_in, const long long* nkey_in,
-                  int2* ij_out, long long* key_out, double4* ff_out)
+                  const double4* nff_in, int2* ij_out, long long* key_out, double4* ff_out)
 {
   if (nold + nnew <= 0) return;
   hipLaunchKernelGGL(k_merge, dim3(nblk((long long)nold + nnew)), dim3(BS), 0, s, nold, oij, okey,
-                     off_, nnew, nij_in, nkey_in, ij_out, key_out, ff_out);
+                     off_, nnew, nij_in, nkey_in, nff_in, ij_out, key_out, ff_out);
 }
 
 void launch_pack_pairs(hipStream_t s, int n, const int* keep, const int* off, const int* cp,
@@ -958,15 +1245,6 @@ void launch_pack_pairs(hipStream_t s, int n, const int* keep, const int* off, co
   hipLaunchKernelGGL(k_pack_pairs, dim3(nblk(n)), dim3(BS), 0, s, n, keep, off, cp, cq, ckey, ij, key);
 }
 
-void launch_bond_mark_broken(hipStream_t s, int nbroken, const int* list, int bond_off, int nb,
-                             int* keep, PView p, const int2* ij)
-{
-  if (nb <= 0) return;
-  hipLaunchKernelGGL(k_fill_int, dim3(nblk(nb)), dim3(BS), 0, s, nb, keep, 1);
-  if (nbroken <= 0) return;
-  hipLaunchKernelGGL(k_bond_mark_broken, dim3(nblk(nbroken)), dim3(BS), 0, s, nbroken, list,
-                     bond_off, nb, keep, p, ij);
-}
 
 void launch_compact_bonds(hipStream_t s, int nb, const int* keep, const int* off,
                           const int2* ij, const double4* k4, const double4* mx4,
@@ -978,20 +1256,7 @@ void launch_compact_bonds(hipStream_t s, int nb, const int* keep, const int* off
                      D4, ij2, k42, mx42, D42);
 }
 
-void launch_keys_mark_removed(hipStream_t s, int nk, const long long* keys, int nrem,
-                              const long long* rem_sorted, int* keep)
-{
-  if (nk <= 0) return;
-  hipLaunchKernelGGL(k_keys_mark_removed, dim3(nblk(nk)), dim3(BS), 0, s, nk, keys, nrem,
-                     rem_sorted, keep);
-}
 
-void launch_keys_compact(hipStream_t s, int nk, const int* keep, const int* off,
-                         const long long* keys, long long* out)
-{
-  if (nk <= 0) return;
-  hipLaunchKernelGGL(k_keys_compact, dim3(nblk(nk)), dim3(BS), 0, s, nk, keep, off, keys, out);
-}
 
 void launch_wall_update(hipStream_t s, PView p, GridParams g, int nwg, const WallParam* w)
 {
