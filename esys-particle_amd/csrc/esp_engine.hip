@@ -197,6 +197,7 @@ struct esp_engine {
   long long num_host_phases = 0;
   std::vector<int> rebuild_steps;
   std::vector<int> breaks;  // step, id1, id2
+  long long dead_pending = 0;  // bonds marked dead on the device, not yet compacted
   std::string err;
 
   // kernel timing: one (start, stop, class) record per launch of a queued step
@@ -569,27 +570,39 @@ void new_pair_list(esp_engine* e)
   e->nnew = total;
 }
 
-// Device bookkeeping folded back into plain lists: break events to the host
-// log, dead bonds compacted out, each dynamic group's appendix merged into its
-// key-sorted main list.  Results are unchanged; only storage is reorganised.
-void fold(esp_engine* e)
+// Break events logged on the device move to the host log.  Touches nothing the
+// steps read, so queries may call it at any time.
+void fold_events(esp_engine* e)
 {
   UpdState st;
   d2h(e, &st, e->upd.p, 1);
+  sync(e);
+  if (st.evt_count <= 0) return;
+  std::vector<int4> ev(st.evt_count);
+  d2h(e, ev.data(), e->evt.p, st.evt_count);
+  const int zero_int = 0;
+  h2d(e, &e->upd.p->evt_count, &zero_int, 1);
+  sync(e);
+  for (const int4& v : ev) {
+    e->breaks.push_back(v.x);
+    e->breaks.push_back(v.y);
+    e->breaks.push_back(v.z);
+  }
+  e->dead_pending += st.evt_count;
+}
+
+// Device bookkeeping folded back into plain lists: dead bonds compacted out,
+// each dynamic group's appendix merged into its key-sorted main list.  Record
+// indices change, so this runs only in host bookkeeping, right before the
+// force-record layout and the CSR are rebuilt (rebuild_csr).
+void fold(esp_engine* e)
+{
+  fold_events(e);
   std::vector<int> napp(e->dg.size(), 0);
   for (size_t g = 0; g < e->dg.size(); g++) d2h(e, &napp[g], e->dg[g].napp.p, 1);
   sync(e);
-  if (st.evt_count > 0) {
-    std::vector<int4> ev(st.evt_count);
-    d2h(e, ev.data(), e->evt.p, st.evt_count);
-    const int zero_int = 0;
-    h2d(e, &e->upd.p->evt_count, &zero_int, 1);
-    sync(e);
-    for (const int4& v : ev) {
-      e->breaks.push_back(v.x);
-      e->breaks.push_back(v.y);
-      e->breaks.push_back(v.z);
-    }
+  if (e->dead_pending > 0) {
+    e->dead_pending = 0;
     // compact dead bonds
     int off = 0;
     for (int gi : e->bord) {
@@ -667,6 +680,7 @@ void fold(esp_engine* e)
     G.n = tot;
     zero(e, G.napp.p, 1);
     reserve_group(G, G.n, hist);  // keeps content when the capacity suffices
+    rebuild_presence(e, G);       // room for the next appendix
   }
   sync(e);
 }
@@ -1067,6 +1081,59 @@ std::vector<int> id_order(esp_engine* e)
   return order;
 }
 
+// Read-only views for queries.  Dead bonds stay in storage and appended pairs
+// stay in their appendix until host bookkeeping folds them (fold), so queries
+// assemble the lists the way fold would and leave the device state alone.
+std::vector<char> bond_alive(esp_engine* e, BondGroup& G)
+{
+  std::vector<char> a(G.n, 1);
+  if (!e->started || G.n == 0) return a;
+  std::vector<int2> ij(G.n);
+  d2h(e, ij.data(), G.ij.p, G.n);
+  sync(e);
+  for (int b = 0; b < G.n; b++) a[b] = ij[b].x >= 0;
+  return a;
+}
+
+int bond_live_count(esp_engine* e, BondGroup& G)
+{
+  int c = 0;
+  for (char a : bond_alive(e, G)) c += a;
+  return c;
+}
+
+int dyn_appended(esp_engine* e, DynGroup& G)
+{
+  if (!e->started) return 0;
+  int na = 0;
+  d2h(e, &na, G.napp.p, 1);
+  sync(e);
+  return na;
+}
+
+// stored pairs of a dynamic group in key order (main list merged with the
+// appendix, main first on equal keys, as k_merge orders them)
+void dyn_view(esp_engine* e, DynGroup& G, std::vector<long long>& key, std::vector<double4>* ff)
+{
+  const int tot = G.n + dyn_appended(e, G);
+  key.assign(tot, 0);
+  std::vector<long long> k(std::max(tot, 1));
+  std::vector<double4> f(ff ? std::max(tot, 1) : 0);
+  if (tot > 0) {
+    d2h(e, k.data(), G.key.p, tot);
+    if (ff) d2h(e, f.data(), G.ff.p, tot);
+    sync(e);
+  }
+  std::vector<int> perm(tot);
+  for (int i = 0; i < tot; i++) perm[i] = i;
+  std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return k[a] < k[b]; });
+  if (ff) ff->resize(tot);
+  for (int i = 0; i < tot; i++) {
+    key[i] = k[perm[i]];
+    if (ff) (*ff)[i] = f[perm[i]];
+  }
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -1452,13 +1519,13 @@ int esp_get_num_particles(esp_engine* e) { return e->n; }
 int esp_get_stats(esp_engine* e, esp_stats* o)
 {
   API_BEGIN
-  if (e->started) fold(e);
+  if (e->started) fold_events(e);
   memset(o, 0, sizeof *o);
   o->time_step = e->t;
   o->num_rebuilds = e->timestamp;
   o->num_particles = e->n;
-  for (auto& G : e->bg) o->num_bonds += G.n;
-  for (auto& G : e->dg) o->num_dynamic += G.n;
+  for (auto& G : e->bg) o->num_bonds += bond_live_count(e, G);
+  for (auto& G : e->dg) o->num_dynamic += G.n + dyn_appended(e, G);
   o->num_candidates = e->ncand;
   o->num_broken = (long long)e->breaks.size() / 3;
   o->num_host_phases = e->num_host_phases;
@@ -1580,11 +1647,10 @@ int esp_get_wall_force_history(esp_engine* e, const char* name, int n, double* o
 int esp_get_interaction_count(esp_engine* e, const char* name)
 {
   API_BEGIN
-  if (e->started) fold(e);
   const int d = find_dg(e, name ? name : "");
-  if (d >= 0) return e->dg[d].n;
+  if (d >= 0) return e->dg[d].n + dyn_appended(e, e->dg[d]);
   const int b = find_bg(e, name ? name : "");
-  if (b >= 0) return e->bg[b].n;
+  if (b >= 0) return bond_live_count(e, e->bg[b]);
   return -1;
   API_END
 }
@@ -1592,50 +1658,42 @@ int esp_get_interaction_count(esp_engine* e, const char* name)
 int esp_get_interactions(esp_engine* e, const char* name, int* id1, int* id2)
 {
   API_BEGIN
-  if (e->started) fold(e);
   const int d = find_dg(e, name ? name : "");
+  std::vector<long long> k;
   if (d >= 0) {
-    DynGroup& G = e->dg[d];
-    std::vector<long long> k(std::max(G.n, 1));
-    d2h(e, k.data(), G.key.p, G.n);
-    sync(e);
-    for (int i = 0; i < G.n; i++) {
-      id1[i] = (int)(k[i] >> 32);
-      id2[i] = (int)(k[i] & 0xffffffffLL);
-    }
-    return G.n;
+    dyn_view(e, e->dg[d], k, nullptr);
+  } else {
+    const int b = find_bg(e, name ? name : "");
+    if (b < 0) throw EspError(ESP_ERR_ARG, "unknown interaction group");
+    BondGroup& G = e->bg[b];
+    const std::vector<char> alive = bond_alive(e, G);
+    for (int i = 0; i < G.n; i++)
+      if (alive[i]) k.push_back(pkey(G.hid1[i], G.hid2[i]));
+    std::sort(k.begin(), k.end());
   }
-  const int b = find_bg(e, name ? name : "");
-  if (b < 0) throw EspError(ESP_ERR_ARG, "unknown interaction group");
-  BondGroup& G = e->bg[b];
-  std::vector<long long> k(G.n);
-  for (int i = 0; i < G.n; i++) k[i] = pkey(G.hid1[i], G.hid2[i]);
-  std::sort(k.begin(), k.end());
-  for (int i = 0; i < G.n; i++) {
+  for (size_t i = 0; i < k.size(); i++) {
     id1[i] = (int)(k[i] >> 32);
     id2[i] = (int)(k[i] & 0xffffffffLL);
   }
-  return G.n;
+  return (int)k.size();
   API_END
 }
 
 int esp_get_friction_state(esp_engine* e, const char* name, double* ffric, int* flg)
 {
   API_BEGIN
-  if (e->started) fold(e);
   const int d = find_dg(e, name ? name : "");
   if (d < 0 || e->dg[d].kind != 3) throw EspError(ESP_ERR_ARG, "not a RotFriction group");
-  DynGroup& G = e->dg[d];
-  std::vector<double4> f(std::max(G.n, 1));
-  d2h(e, f.data(), G.ff.p, G.n);
-  sync(e);
-  for (int i = 0; i < G.n; i++) {
+  std::vector<long long> k;
+  std::vector<double4> f;
+  dyn_view(e, e->dg[d], k, &f);
+  for (size_t i = 0; i < f.size(); i++) {
     ffric[3 * i] = f[i].x;
     ffric[3 * i + 1] = f[i].y;
     ffric[3 * i + 2] = f[i].z;
     flg[i] = (int)f[i].w;
   }
-  return G.n;
+  return (int)f.size();
   API_END
 }
 
@@ -1658,7 +1716,7 @@ int esp_get_neighbour_pairs(esp_engine* e, int* id1, int* id2)
 int esp_get_bond_breaks(esp_engine* e, int* step, int* id1, int* id2)
 {
   API_BEGIN
-  if (e->started) fold(e);
+  if (e->started) fold_events(e);
   const int n = (int)e->breaks.size() / 3;
   if (step && id1 && id2) {
     std::vector<std::array<int, 3>> v(n);

@@ -153,3 +153,38 @@ def test_neighbour_set_bitexact_large(pkg, orc, spm):
         lists.append(sim.engine.neighbour_pairs())
     np.testing.assert_array_equal(lists[0], lists[1])
     assert len(lists[0]) > 10000
+
+
+def test_dense_bonded_queries_between_runs(pkg, orc, spm):
+    """Dense bonded compression run in chunks with every query in between:
+    bond breaks are applied, and new friction pairs appended, on the device
+    between host phases; queries read that state without reorganising it, so
+    the chunked run is bitwise equal to a straight run and follows the oracle
+    event for event."""
+    spec = spm.dense_bonded_compression(n=6000, seed=3)
+    g = pkg.LsmMpi(stepsPerBatch=64)
+    spm.apply(g, spec)
+    o = orc.oracle_lsm_class(pkg)()
+    spm.apply(o, spec)
+    r = float(np.mean(spec["rad"]))
+    print("case: dense bonded n=6000 chunks of 100", flush=True)
+    for chunk in range(8):
+        g.runTimeSteps(100)
+        o.runTimeSteps(100)
+        np.testing.assert_array_equal(g.getBondBreaks(), o.getBondBreaks())
+        for name in ("bonded", "friction"):
+            np.testing.assert_array_equal(g.getInteractionPairs(name), o.getInteractionPairs(name))
+            assert g.getNumInteractions(name) == o.getNumInteractions(name)
+        assert g.getStats()["num_bonds"] == o.getStats()["num_bonds"]
+        gf, gfl = g.engine.friction_state("friction")
+        of, ofl = o.engine.friction_state("friction")
+        np.testing.assert_array_equal(gfl, ofl)
+        assert np.max(np.abs(g.getParticleData("pos") - o.getParticleData("pos"))) <= TRAJ_TOL * r
+    # breaks after the initial relaxation went through the device path
+    assert (o.getBondBreaks()[:, 0] > 100).sum() > 20
+    assert g.getStats()["num_host_phases"] < 10
+    s = pkg.LsmMpi(stepsPerBatch=64)
+    spm.apply(s, spec)
+    s.runTimeSteps(800)
+    assert s.getParticleData("pos").tobytes() == g.getParticleData("pos").tobytes()
+    np.testing.assert_array_equal(s.getBondBreaks(), g.getBondBreaks())
