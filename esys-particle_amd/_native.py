@@ -114,7 +114,7 @@ SIGNATURES = {
     "get_kernel_times": (_i, [_vp, _pd, ctypes.POINTER(ctypes.c_longlong)]),
 }
 
-KERNEL_CLASSES = ("bond", "friction", "elastic", "gather_integrate", "wall_reduce")
+KERNEL_CLASSES = ("bond", "friction", "elastic", "gather_integrate", "update")
 
 
 class NativeLib:

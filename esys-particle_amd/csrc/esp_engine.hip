@@ -173,7 +173,6 @@ struct esp_engine {
   DBuf<double4> tmp_ff, tmp_a, tmp_b, tmp_c;
   DBuf<int> inc_cnt, inc_off, inc_fill, inc_ent;
   DBuf<int> xslot, xcnt, xdemand, flist, nidx;
-  DBuf<long long> nkey;
   DBuf<int4> evt;
   DBuf<UpdState> upd;
   int nrec = 0;
@@ -506,12 +505,11 @@ void upload(esp_engine* e)
   e->xdemand.reserve(cap);
   e->flist.reserve(cap);
   e->nidx.reserve(2 * ESP_NEW_CAP);
-  e->nkey.reserve(ESP_NEW_CAP);
   zero(e, e->xcnt.p, cap);
   zero(e, e->xdemand.p, cap);
   const int nb = std::max(gather_blocks(n), 1);
   const int nwg = (int)e->wg.size();
-  e->wall_part.reserve((size_t)nb * std::max(nwg, 1) * 3);
+  e->wall_part.reserve((size_t)(e->K + 1) * nb * std::max(nwg, 1) * 3);
   e->wall_hist.reserve((size_t)(e->K + 1) * std::max(nwg, 1) * 3);
   sync(e);
   e->started = true;
@@ -823,17 +821,15 @@ BondSet bond_set(esp_engine* e)
 // ---------------------------------------------------------------------------
 // one step's kernels
 // ---------------------------------------------------------------------------
-void launch_step(esp_engine* e, const StepCtl& ctl, double* whist_out, long long step)
+void launch_step(esp_engine* e, const StepCtl& ctl, long long step)
 {
   PView p = e->view();
-  if (ctl.guarded) {
+  if (ctl.guarded && !e->bg.empty()) {
     // bookkeeping of the previous step's bond breaks, on the device
-    if (!e->bg.empty())
-      launch_break_apply(e->stream, ctl, p, bond_set(e), e->broken_list.p, e->contrib.p, e->upd.p,
-                         e->flist.p, e->evt.p, step);
+    DynAppend da;
+    memset(&da, 0, sizeof da);
     if (!e->dg.empty()) {
       DynGroup& G = e->dg[e->dord[0]];
-      DynAppend da;
       da.ij = G.ij.p;
       da.key = G.key.p;
       da.ff = G.kind == 3 ? G.ff.p : nullptr;
@@ -844,10 +840,12 @@ void launch_step(esp_engine* e, const StepCtl& ctl, double* whist_out, long long
       da.is_hist = G.kind == 3;
       da.present = HashRef{G.hs.p, G.hs_mask};
       da.ex = excl_set(e, G);
-      launch_incremental_update(e->stream, ctl, p, e->grid, da, e->cell.p, e->cell_start.p,
-                                e->cell_items.p, e->upd.p, e->flist.p, e->xslot.p, e->xcnt.p,
-                                e->xdemand.p, e->nkey.p, e->nidx.p, 1);
     }
+    LaunchTimer lt(e, ESP_KCLASS_UPDATE, ctl.step_id);
+    launch_incremental_update(e->stream, ctl, p, bond_set(e), e->broken_list.p, e->contrib.p,
+                              e->evt.p, step, e->grid, da, e->cell.p, e->cell_start.p,
+                              e->cell_items.p, e->upd.p, e->flist.p, e->xslot.p, e->xcnt.p,
+                              e->xdemand.p, e->nidx.p, e->dg.empty() ? 0 : 1);
   }
   for (int gi : e->dord) {
     DynGroup& G = e->dg[gi];
@@ -886,14 +884,10 @@ void launch_step(esp_engine* e, const StepCtl& ctl, double* whist_out, long long
   gp.ngrav = (int)e->gord.size();
   for (size_t k = 0; k < e->gord.size(); k++)
     for (int c = 0; c < 3; c++) gp.g[k][c] = e->grav[e->gord[k]].g[c];
-  gp.wall_part = e->wall_part.p;
+  gp.wall_part = e->wall_part.p + (size_t)(ctl.step_id - 1) * gather_blocks(e->n) * gp.nwg * 3;
   {
     LaunchTimer lt(e, ESP_KCLASS_GATHER, ctl.step_id);
     launch_gather_integrate(e->stream, ctl, p, gp);
-  }
-  if (gp.nwg) {
-    LaunchTimer lt(e, ESP_KCLASS_WALL, ctl.step_id);
-    launch_wall_finalize(e->stream, ctl, e->wall_part.p, gather_blocks(e->n), gp.nwg, whist_out);
   }
   HIPCHK(hipGetLastError());
 }
@@ -988,8 +982,9 @@ void host_step(esp_engine* e, int reason)
   }
   rebuild_csr(e);
   launch_clear_status(e->stream, e->ring.p, e->ring_size, step, 1, e->halted.p);
-  launch_step(e, make_ctl(e, step, 1, 0), e->wall_hist.p, step);
+  launch_step(e, make_ctl(e, step, 1, 0), step);
   const int nwg = (int)e->wg.size();
+  launch_wall_finalize(e->stream, e->wall_part.p, gather_blocks(e->n), nwg, 1, e->wall_hist.p);
   std::vector<double> h((size_t)nwg * 3);
   if (nwg) d2h(e, h.data(), e->wall_hist.p, h.size());
   sync(e);
@@ -1012,9 +1007,9 @@ int fast_batch(esp_engine* e, int k, int* reason)
     for (auto& W : e->walls) snap.insert(snap.end(), W.o, W.o + 3);
     wall_o.push_back(snap);
     const long long step = step0 + s;
-    launch_step(e, make_ctl(e, step, s + 1, 1), e->wall_hist.p + (size_t)s * std::max(nwg, 1) * 3,
-                step);
+    launch_step(e, make_ctl(e, step, s + 1, 1), step);
   }
+  launch_wall_finalize(e->stream, e->wall_part.p, gather_blocks(e->n), nwg, k, e->wall_hist.p);
   int halted = 0;
   UpdState st;
   d2h(e, &halted, e->halted.p, 1);

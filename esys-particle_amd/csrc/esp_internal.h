@@ -86,7 +86,7 @@ struct GatherParams {
   WallParam wall[ESP_MAX_WALL_GROUPS];
   int ngrav;
   double g[ESP_MAX_GRAVITY][3];
-  double* wall_part;  // [block][nwg][3]
+  double* wall_part;  // this step's slot: [block][nwg][3]
 };
 
 // Open-addressing hash sets of pair keys (id1 << 32 | id2): live bonds of a
@@ -160,18 +160,21 @@ void launch_rot_elastic(hipStream_t s, StepCtl ctl, PView p, int nc, const int* 
 // their particles are flagged, events are logged
 void launch_break_apply(hipStream_t s, StepCtl ctl, PView p, BondSet bs, const int* broken_list,
                         double* contrib, UpdState* st, int* flist, int4* evt, long long step);
-// getNewList + PIS_NE/PIS_ED incremental update of the first dynamic group
-void launch_incremental_update(hipStream_t s, StepCtl ctl, PView p, GridParams g, DynAppend da,
-                               const int* cell, const int* cell_start, const int* cell_items,
-                               UpdState* st, int* flist, int* xslot, int* xcnt, int* xdemand,
-                               long long* nkey, int* nidx, int has_group);
+// queued steps: the same break application, then getNewList + PIS_NE/PIS_ED
+// incremental update of the first dynamic group (has_group), in one launch
+void launch_incremental_update(hipStream_t s, StepCtl ctl, PView p, BondSet bs,
+                               const int* broken_list, double* contrib, int4* evt, long long step,
+                               GridParams g, DynAppend da, const int* cell, const int* cell_start,
+                               const int* cell_items, UpdState* st, int* flist, int* xslot,
+                               int* xcnt, int* xdemand, int* nidx, int has_group);
 void launch_hash_clear(hipStream_t s, HashRef h);
 void launch_hash_insert(hipStream_t s, HashRef h, const long long* keys, int n);
 void launch_bond_keys(hipStream_t s, int nb, const int2* ij, const int* id, long long* keys);
 void launch_alive_flags(hipStream_t s, int nb, const int2* ij, int* keep);
 void launch_gather_integrate(hipStream_t s, StepCtl ctl, PView p, const GatherParams& gp);
-void launch_wall_finalize(hipStream_t s, StepCtl ctl, const double* wall_part, int nblocks,
-                          int nwg, double* out);
+// wall forces [step][nwg][3] from the partials [step][block][nwg][3]
+void launch_wall_finalize(hipStream_t s, const double* part, int nblocks, int nwg, int nsteps,
+                          double* out);
 void launch_clear_status(hipStream_t s, StepStatus* ring, int ring_size, long long first_step,
                          int count, int* halted);
 int gather_blocks(int n);

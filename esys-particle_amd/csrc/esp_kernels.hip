@@ -6,9 +6,10 @@
 //   k_gather_integrate
 //       one thread per particle: wall and body forces, then the records of its
 //       interactions in a fixed order (CSR), then the CParticle/CRotParticle
-//       first-order update and the max-displacement reduction
-//   k_wall_finalize
-//       fixed-order sum of the per-block wall-force partials
+//       first-order update and the max-displacement reduction; per-block
+//       partials of the wall forces
+//   k_wall_finalize (once per queued batch)
+//       fixed-order sums of the wall-force partials of every step of the batch
 // Every per-step kernel first evaluates the previous step's status; if that
 // step displaced a particle by more than verletDist/2 or broke a bond, the host
 // must do bookkeeping first, so the kernel records the halt and returns, and
@@ -274,33 +275,56 @@ __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
   return v;
 }
 
+// Fixed-order sum of (a, b, c) over the block, valid in thread 0: xor butterfly
+// inside each wave (every lane ends with the same value; a + b == b + a), then
+// the waves in index order.  Every thread of the block must call it.
+__device__ __forceinline__ void block_sum3(double& a, double& b, double& c, double* lds)
+{
+  for (int o = 32; o > 0; o >>= 1) {
+    a = a + __shfl_xor(a, o, 64);
+    b = b + __shfl_xor(b, o, 64);
+    c = c + __shfl_xor(c, o, 64);
+  }
+  __syncthreads();  // lds free from the previous call
+  if ((threadIdx.x & 63) == 0) {
+    const int w = threadIdx.x >> 6;
+    lds[3 * w] = a;
+    lds[3 * w + 1] = b;
+    lds[3 * w + 2] = c;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int w = 1; w < BS / 64; w++) {
+      a = a + lds[3 * w];
+      b = b + lds[3 * w + 1];
+      c = c + lds[3 * w + 2];
+    }
+}
+
 __global__ __launch_bounds__(BS) void k_gather_integrate(StepCtl ctl, PView p, GatherParams gp)
 {
   if (step_skip(ctl)) return;
-  __shared__ double red[BS];
+  __shared__ double lds3[3 * (BS / 64)];
   __shared__ unsigned long long mred[BS / 64];
   const int i = blockIdx.x * BS + threadIdx.x;
-  double wf[ESP_MAX_WALL_GROUPS][3];
-  for (int w = 0; w < ESP_MAX_WALL_GROUPS; w++) wf[w][0] = wf[w][1] = wf[w][2] = 0.0;
-  unsigned long long md = 0;
-  if (i < p.n) {
-    const double4 P = p.pos[i];
-    const double4 Vv = p.vel[i];
-    const double4 O = p.old[i];
-    const unsigned fl = p.flags[i];
-    dv3 x = xyz(P), v = xyz(Vv);
-    const double r = P.w, invm = Vv.w, mass = O.w;
-    dv3 F = mk3(0.0, 0.0, 0.0), M = mk3(0.0, 0.0, 0.0);
-    unsigned flw = fl;
-    if (gp.nwg > 0 && ctl.guarded && ctl.prev->nbroken > 0) {
-      // bonds broke in the previous step: the reference's updateInteractions
-      // re-Updates the wall groups (EWallInteractionGroup.hpp:57-82) before forces
-      flw = wall_membership(gp.grid, gp.nwg, gp.wall, P, fl);
-      p.flags[i] = flw;
-    }
-    // wall interaction groups, name order (EWallInteraction.hpp:42-78)
-    for (int w = 0; w < gp.nwg; w++) {
-      if (!((flw >> (PF_WALL_SHIFT + w)) & 1u)) continue;
+  const bool act = i < p.n;
+  const double4 P = act ? p.pos[i] : make_double4(0.0, 0.0, 0.0, 0.0);
+  const unsigned fl = act ? p.flags[i] : 0u;
+  dv3 x = xyz(P);
+  const double r = P.w;
+  dv3 F = mk3(0.0, 0.0, 0.0), M = mk3(0.0, 0.0, 0.0);
+  unsigned flw = fl;
+  if (act && gp.nwg > 0 && ctl.guarded && ctl.prev->nbroken > 0) {
+    // bonds broke in the previous step: the reference's updateInteractions
+    // re-Updates the wall groups (EWallInteractionGroup.hpp:57-82) before forces
+    flw = wall_membership(gp.grid, gp.nwg, gp.wall, P, fl);
+    p.flags[i] = flw;
+  }
+  // wall interaction groups, name order (EWallInteraction.hpp:42-78); the
+  // forces on inner particles go into this block's partial of the wall force
+  for (int w = 0; w < gp.nwg; w++) {
+    double wx = 0.0, wy = 0.0, wz = 0.0;
+    if (act && ((flw >> (PF_WALL_SHIFT + w)) & 1u)) {
       const WallParam& W = gp.wall[w];
       const dv3 n = mk3(W.nrm[0], W.nrm[1], W.nrm[2]);
       const double dist = dot3(sub3(x, mk3(W.o[0], W.o[1], W.o[2])), n);
@@ -310,10 +334,22 @@ __global__ __launch_bounds__(BS) void k_gather_integrate(StepCtl ctl, PView p, G
         const dv3 force = mul3(n, (f * W.k) * (r - dist));
         F = add3(F, force);
         if ((flw >> (PF_INNER_SHIFT + w)) & 1u) {
-          wf[w][0] = force.x; wf[w][1] = force.y; wf[w][2] = force.z;
+          wx = force.x; wy = force.y; wz = force.z;
         }
       }
     }
+    block_sum3(wx, wy, wz, lds3);
+    if (threadIdx.x == 0) {
+      double* q = gp.wall_part + ((size_t)blockIdx.x * gp.nwg + w) * 3;
+      q[0] = wx; q[1] = wy; q[2] = wz;
+    }
+  }
+  unsigned long long md = 0;
+  if (act) {
+    const double4 Vv = p.vel[i];
+    const double4 O = p.old[i];
+    dv3 v = xyz(Vv);
+    const double invm = Vv.w, mass = O.w;
     // body forces, name order (BodyForceGroup.hpp:41-65)
     for (int g = 0; g < gp.ngrav; g++) F = add3(F, mul3(mk3(gp.g[g][0], gp.g[g][1], gp.g[g][2]), mass));
     // interaction records: dynamic groups then bonded groups, each in storage order
@@ -383,39 +419,32 @@ __global__ __launch_bounds__(BS) void k_gather_integrate(StepCtl ctl, PView p, G
     for (int w = 1; w < BS / 64; w++) m = mred[w] > m ? mred[w] : m;
     if (m) atomicMax(&ctl.cur->maxdisp2, m);
   }
-  // wall-force partials: fixed-order tree per block
-  for (int w = 0; w < gp.nwg; w++) {
-    for (int c = 0; c < 3; c++) {
-      __syncthreads();
-      red[threadIdx.x] = wf[w][c];
-      __syncthreads();
-      for (int s = BS / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + s];
-        __syncthreads();
-      }
-      if (threadIdx.x == 0) gp.wall_part[((size_t)blockIdx.x * gp.nwg + w) * 3 + c] = red[0];
-    }
-  }
 }
 
-__global__ __launch_bounds__(BS) void k_wall_finalize(StepCtl ctl, const double* part, int nblocks, int nwg,
+// Wall forces of a run of steps: block s sums step s's per-block partials in
+// block order.  Launched once per queued batch; slots of steps that were
+// skipped hold stale values and are not read by the host.
+__global__ __launch_bounds__(BS) void k_wall_finalize(const double* part, int nblocks, int nwg,
                                                       double* out)
 {
-  if (step_skip(ctl)) return;
-  __shared__ double red[BS];
-  for (int w = 0; w < nwg; w++)
-    for (int c = 0; c < 3; c++) {
-      double s = 0.0;
-      for (int b = threadIdx.x; b < nblocks; b += BS) s = s + part[((size_t)b * nwg + w) * 3 + c];
-      __syncthreads();
-      red[threadIdx.x] = s;
-      __syncthreads();
-      for (int h = BS / 2; h > 0; h >>= 1) {
-        if (threadIdx.x < h) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + h];
-        __syncthreads();
-      }
-      if (threadIdx.x == 0) out[w * 3 + c] = red[0];
+  __shared__ double lds3[3 * (BS / 64)];
+  const double* ps = part + (size_t)blockIdx.x * nblocks * nwg * 3;
+  for (int w = 0; w < nwg; w++) {
+    double a = 0.0, b = 0.0, c = 0.0;
+    for (int k = threadIdx.x; k < nblocks; k += BS) {
+      const double* q = ps + ((size_t)k * nwg + w) * 3;
+      a = a + q[0];
+      b = b + q[1];
+      c = c + q[2];
     }
+    block_sum3(a, b, c, lds3);
+    if (threadIdx.x == 0) {
+      double* o = out + ((size_t)blockIdx.x * nwg + w) * 3;
+      o[0] = a;
+      o[1] = b;
+      o[2] = c;
+    }
+  }
 }
 
 __global__ void k_clear_status(StepStatus* ring, int ring_size, long long first_step, int count,
@@ -728,91 +757,107 @@ __global__ void k_wall_update(PView p, GridParams g, int nwg, WallParam w0, Wall
 // ---------------------------------------------------------------------------
 // bookkeeping of bond breaks on the device
 // ---------------------------------------------------------------------------
-// PIS_EB::update (pi_storage_eb.hpp:23-49) for the bonds whose criterion held in
+// PIS_EB::update (pi_storage_eb.hpp:23-49) for one bond whose criterion held in
 // the previous step: the bond leaves its group (marked dead, record zeroed, key
 // erased from the exclusion set), both particles are flagged
 // (RotBondedInteraction.cpp:211-213) and the event is logged.
+__device__ __forceinline__ void break_one(PView p, const BondSet& bs, int gb, double* contrib,
+                                          UpdState* st, int* flist, int4* evt, long long step)
+{
+  int g = 0;
+  while (g + 1 < bs.count && gb >= bs.off[g + 1]) g++;
+  const int b = gb - bs.off[g];
+  if (b < 0 || b >= bs.n[g]) return;
+  const int2 e = bs.ij[g][b];
+  if (e.x < 0) return;
+  const int id1 = p.id[e.x], id2 = p.id[e.y];
+  hs_erase(bs.hs[g], pair_key(id1, id2));
+  bs.ij[g][b] = make_int2(-1, -1);
+  double* r = contrib + 9 * (size_t)(bs.rec_off[g] + b);
+  for (int k = 0; k < 9; k++) r[k] = 0.0;
+  if (!(atomicOr(&p.flags[e.x], PF_FLAG) & PF_FLAG)) flist[atomicAdd(&st->fcount, 1)] = e.x;
+  if (!(atomicOr(&p.flags[e.y], PF_FLAG) & PF_FLAG)) flist[atomicAdd(&st->fcount, 1)] = e.y;
+  evt[atomicAdd(&st->evt_count, 1)] = make_int4((int)step, id1, id2, 0);
+}
+
+// the previous step's breaks on the host bookkeeping path
 __global__ void k_break_apply(StepCtl ctl, PView p, BondSet bs, const int* broken_list,
                               double* contrib, UpdState* st, int* flist, int4* evt, long long step)
 {
   if (step_skip(ctl)) return;
   const int nb = ctl.prev->nbroken;
-  for (int t = blockIdx.x * BS + threadIdx.x; t < nb; t += gridDim.x * BS) {
-    const int gb = broken_list[t];
-    int g = 0;
-    while (g + 1 < bs.count && gb >= bs.off[g + 1]) g++;
-    const int b = gb - bs.off[g];
-    if (b < 0 || b >= bs.n[g]) continue;
-    const int2 e = bs.ij[g][b];
-    if (e.x < 0) continue;
-    const int id1 = p.id[e.x], id2 = p.id[e.y];
-    hs_erase(bs.hs[g], pair_key(id1, id2));
-    bs.ij[g][b] = make_int2(-1, -1);
-    double* r = contrib + 9 * (size_t)(bs.rec_off[g] + b);
-    for (int k = 0; k < 9; k++) r[k] = 0.0;
-    if (!(atomicOr(&p.flags[e.x], PF_FLAG) & PF_FLAG)) flist[atomicAdd(&st->fcount, 1)] = e.x;
-    if (!(atomicOr(&p.flags[e.y], PF_FLAG) & PF_FLAG)) flist[atomicAdd(&st->fcount, 1)] = e.y;
-    evt[atomicAdd(&st->evt_count, 1)] = make_int4((int)step, id1, id2, 0);
-  }
+  for (int t = blockIdx.x * BS + threadIdx.x; t < nb; t += gridDim.x * BS)
+    break_one(p, bs, broken_list[t], contrib, st, flist, evt, step);
 }
 
-// getNewList (ntable.hpp:628-664) + the first dynamic group's incremental update
-// (pi_storage_ed.hpp:137-156 / pi_storage_ne.hpp:79-96) in one block: pairs of
-// flagged particles in neighbouring cells of the last build, without a distance
-// test, minus excluded (and, for history groups, stored) pairs, are appended in
-// key order; then every flag is cleared.  When the pairs do not fit the
-// appendix, the queue halts and the host does the update.
+// TSubLattice::updateInteractions after bond breaks, in one block: the previous
+// step's breaks are applied (break_one), then getNewList (ntable.hpp:628-664)
+// and the first dynamic group's incremental update (pi_storage_ed.hpp:137-156 /
+// pi_storage_ne.hpp:79-96): pairs of flagged particles in neighbouring cells of
+// the last build, without a distance test, minus excluded (and, for history
+// groups, stored) pairs, are appended in key order; then every flag is
+// cleared.  When the pairs do not fit the appendix, the queue halts and the
+// host does the update.
 #define U2_BS 1024
 #define NEW_SENTINEL 0x7fffffffffffffffll
 __global__ __launch_bounds__(U2_BS) void k_incremental_update(
-    StepCtl ctl, PView p, GridParams g, DynAppend da, const int* cell, const int* cell_start,
+    StepCtl ctl, PView p, BondSet bs, const int* broken_list, double* contrib, int4* evt,
+    long long step, GridParams g, DynAppend da, const int* cell, const int* cell_start,
     const int* cell_items, UpdState* st, int* flist, int* xslot, int* xcnt, int* xdemand,
-    long long* nkey_unused, int* nidx, int has_group)
+    int* nidx, int has_group)
 {
   if (step_skip(ctl)) return;
-  if (ctl.prev->nbroken <= 0 || !has_group) return;
+  const int nbk = ctl.prev->nbroken;
+  if (nbk <= 0) return;
+  const int tid = threadIdx.x;
+  for (int t = tid; t < nbk; t += U2_BS) break_one(p, bs, broken_list[t], contrib, st, flist, evt, step);
+  if (!has_group) return;
   __shared__ long long s_key[ESP_NEW_CAP];
   __shared__ int s_idx[ESP_NEW_CAP];
   __shared__ int s_cnt, s_over, s_keep;
-  const int tid = threadIdx.x;
+  // flags, flagged list and its count complete and visible to the whole block
+  __threadfence();
+  __syncthreads();
+  __threadfence();
   if (tid == 0) {
     s_cnt = 0;
     s_over = 0;
   }
   __syncthreads();
   const int nf = st->fcount;
-  for (int t = tid; t < nf; t += U2_BS) {
-    const int pi = flist[t];
+  // one wave per flagged particle; lane l scans half (l / 27) of the items of
+  // neighbour cell l % 27, so the dependent loads of a cell scan run in parallel
+  const int lane = tid & 63;
+  for (int f = tid >> 6; f < nf; f += U2_BS / 64) {
+    const int pi = flist[f];
     const int c = cell[pi];
     const int cx = c / (g.ny * g.nz);
     const int rem = c - cx * (g.ny * g.nz);
     const int cy = rem / g.nz;
     const int cz = rem - cy * g.nz;
     const int idp = p.id[pi];
-    const int x0 = cx > 0 ? cx - 1 : cx, x1 = cx < g.nx - 1 ? cx + 1 : cx;
-    const int y0 = cy > 0 ? cy - 1 : cy, y1 = cy < g.ny - 1 ? cy + 1 : cy;
-    const int z0 = cz > 0 ? cz - 1 : cz, z1 = cz < g.nz - 1 ? cz + 1 : cz;
-    for (int x = x0; x <= x1; x++)
-      for (int y = y0; y <= y1; y++)
-        for (int z = z0; z <= z1; z++) {
-          const int c2 = g.ny * g.nz * x + g.nz * y + z;
-          const int t1 = cell_start[c2 + 1];
-          for (int u = cell_start[c2]; u < t1; u++) {
-            const int q = cell_items[u];
-            if (!(p.flags[q] & PF_FLAG)) continue;
-            const int idq = p.id[q];
-            if (idq <= idp) continue;
-            const int k = atomicAdd(&s_cnt, 1);
-            if (k < ESP_NEW_CAP) {
-              s_key[k] = pair_key(idp, idq);
-              nidx[2 * k] = pi;
-              nidx[2 * k + 1] = q;
-            } else {
-              s_over = 1;
-            }
-          }
-        }
+    if (lane >= 54) continue;
+    const int nc = lane % 27, half = lane / 27;
+    const int x = cx + nc / 9 - 1, y = cy + (nc / 3) % 3 - 1, z = cz + nc % 3 - 1;
+    if (x < 0 || x >= g.nx || y < 0 || y >= g.ny || z < 0 || z >= g.nz) continue;
+    const int c2 = g.ny * g.nz * x + g.nz * y + z;
+    const int t1 = cell_start[c2 + 1];
+    for (int u = cell_start[c2] + half; u < t1; u += 2) {
+      const int q = cell_items[u];
+      if (!(p.flags[q] & PF_FLAG)) continue;
+      const int idq = p.id[q];
+      if (idq <= idp) continue;
+      const int k = atomicAdd(&s_cnt, 1);
+      if (k < ESP_NEW_CAP) {
+        s_key[k] = pair_key(idp, idq);
+        nidx[2 * k] = pi;
+        nidx[2 * k + 1] = q;
+      } else {
+        s_over = 1;
+      }
+    }
   }
+  __threadfence();
   __syncthreads();
   const int n = min(s_cnt, ESP_NEW_CAP);
   // keep: not excluded, not stored (history groups); others sort to the end
@@ -1109,14 +1154,15 @@ void launch_break_apply(hipStream_t s, StepCtl ctl, PView p, BondSet bs, const i
                      flist, evt, step);
 }
 
-void launch_incremental_update(hipStream_t s, StepCtl ctl, PView p, GridParams g, DynAppend da,
-                               const int* cell, const int* cell_start, const int* cell_items,
-                               UpdState* st, int* flist, int* xslot, int* xcnt, int* xdemand,
-                               long long* nkey, int* nidx, int has_group)
+void launch_incremental_update(hipStream_t s, StepCtl ctl, PView p, BondSet bs,
+                               const int* broken_list, double* contrib, int4* evt, long long step,
+                               GridParams g, DynAppend da, const int* cell, const int* cell_start,
+                               const int* cell_items, UpdState* st, int* flist, int* xslot,
+                               int* xcnt, int* xdemand, int* nidx, int has_group)
 {
-  hipLaunchKernelGGL(k_incremental_update, dim3(1), dim3(U2_BS), 0, s, ctl, p, g, da, cell,
-                     cell_start, cell_items, st, flist, xslot, xcnt, xdemand, nkey, nidx,
-                     has_group);
+  hipLaunchKernelGGL(k_incremental_update, dim3(1), dim3(U2_BS), 0, s, ctl, p, bs, broken_list,
+                     contrib, evt, step, g, da, cell, cell_start, cell_items, st, flist, xslot,
+                     xcnt, xdemand, nidx, has_group);
 }
 
 void launch_hash_clear(hipStream_t s, HashRef h)
@@ -1148,12 +1194,13 @@ void launch_gather_integrate(hipStream_t s, StepCtl ctl, PView p, const GatherPa
   hipLaunchKernelGGL(k_gather_integrate, dim3(nblk(p.n)), dim3(BS), 0, s, ctl, p, gp);
 }
 
-void launch_wall_finalize(hipStream_t s, StepCtl ctl, const double* wall_part, int nblocks,
-                          int nwg, double* out)
+void launch_wall_finalize(hipStream_t s, const double* part, int nblocks, int nwg, int nsteps,
+                          double* out)
 {
-  if (nwg <= 0) return;
-  hipLaunchKernelGGL(k_wall_finalize, dim3(1), dim3(BS), 0, s, ctl, wall_part, nblocks, nwg, out);
+  if (nwg <= 0 || nsteps <= 0) return;
+  hipLaunchKernelGGL(k_wall_finalize, dim3(nsteps), dim3(BS), 0, s, part, nblocks, nwg, out);
 }
+
 
 void launch_clear_status(hipStream_t s, StepStatus* ring, int ring_size, long long first_step,
                          int count, int* halted)

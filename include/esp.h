@@ -213,7 +213,7 @@ int esp_get_rebuild_steps(esp_engine* e, int* steps);
 #define ESP_KCLASS_FRICTION 1 /* RotFriction law, per group */
 #define ESP_KCLASS_ELASTIC 2  /* NRotElastic / RotElastic laws, per group */
 #define ESP_KCLASS_GATHER 3   /* force gather + integration, per step */
-#define ESP_KCLASS_WALL 4     /* wall-force reduction, per step */
+#define ESP_KCLASS_UPDATE 4   /* bond-break bookkeeping of a queued step (break list, new pairs) */
 #define ESP_KCLASS_COUNT 5
 /* Enable (1) or disable (0) HIP-event timing of every per-step kernel launch on
  * the engine's stream; timing adds an event pair around each launch. */
