@@ -85,7 +85,7 @@ struct DynGroup {
   int kind;  // 1 NRotElastic, 2 RotElastic, 3 RotFriction
   double k = 0, ks = 0, mu_d = 0, mu_s = 0;
   int scaling = 1, meanR = 1;
-  int n = 0;  // key-sorted main list; the appendix (device count napp) follows it
+  int n = 0;  // main list sorted by slot key; the appendix (device count napp) follows it
   DBuf<int2> ij;
   DBuf<long long> key;
   DBuf<double4> ff;
@@ -103,9 +103,10 @@ struct BondGroup {
   int n = 0;
   DBuf<int2> ij;  // dead bonds hold (-1, -1) until the next compaction
   DBuf<double4> k4, mx4, D4;
+  DBuf<int> rank;  // position in connection order (the order bonded forces are summed in)
   DBuf<unsigned long long> hs;  // keys of live bonds (exclusion lookups)
   unsigned hs_mask = 0;
-  std::vector<int> hid1, hid2;  // ids in storage order
+  std::vector<int> hid1, hid2;  // ids in storage order (sorted by particle slots)
   std::vector<double4> h_k4, h_mx4, h_D4;  // staged until the first step
   int rec_off = 0, bond_off = 0;
 };
@@ -159,7 +160,7 @@ struct esp_engine {
 
   DBuf<double4> pos, vel, ang, quat, old, frc, mom;
   DBuf<unsigned> flags;
-  DBuf<int> id, by_id, cell;
+  DBuf<int> id, cell;
   DBuf<int> cell_count, cell_start, cell_fill, cell_items;
   DBuf<int> cand_cnt, cand_off, cand_p, cand_q;
   DBuf<long long> cand_key;
@@ -171,7 +172,8 @@ struct esp_engine {
   DBuf<int2> tmp_ij, tmp_ij2;
   DBuf<long long> tmp_key, tmp_key2;
   DBuf<double4> tmp_ff, tmp_a, tmp_b, tmp_c;
-  DBuf<int> inc_cnt, inc_off, inc_fill, inc_ent;
+  DBuf<int> inc_cnt, inc_off, inc_fill, inc_ent, tmp_rank;
+  DBuf<unsigned long long> inc_ent64;
   DBuf<int> xslot, xcnt, xdemand, flist, nidx;
   DBuf<int4> evt;
   DBuf<UpdState> upd;
@@ -416,12 +418,63 @@ void rebuild_presence(esp_engine* e, DynGroup& G)
 // ---------------------------------------------------------------------------
 // device upload at the first step
 // ---------------------------------------------------------------------------
+unsigned long long morton3(unsigned x, unsigned y, unsigned z)
+{
+  auto spread = [](unsigned long long v) {
+    v &= 0x1fffffull;
+    v = (v | v << 32) & 0x1f00000000ffffull;
+    v = (v | v << 16) & 0x1f0000ff0000ffull;
+    v = (v | v << 8) & 0x100f00f00f00f00full;
+    v = (v | v << 4) & 0x10c30c30c30c30c3ull;
+    v = (v | v << 2) & 0x1249249249249249ull;
+    return v;
+  };
+  return spread(x) | spread(y) << 1 | spread(z) << 2;
+}
+
+// Particles are stored in Morton order of their initial grid cells, so that
+// interacting particles sit close together in memory.  The storage order
+// changes no result: lists are keyed by ids or slots, and per-particle sums
+// follow the reference's group and connection orders.
+void spatial_order(esp_engine* e)
+{
+  const int n = e->n;
+  const double o[3] = {e->grid.x0, e->grid.y0, e->grid.z0};
+  std::vector<unsigned long long> code(n);
+  for (int i = 0; i < n; i++) {
+    unsigned c[3];
+    for (int a = 0; a < 3; a++) {
+      const double f = std::floor((e->h_pos[3 * i + a] - o[a]) / e->grid.dim) + 1.0;
+      c[a] = (unsigned)std::max(0.0, std::min(f, 2097151.0));
+    }
+    code[i] = morton3(c[0], c[1], c[2]);
+  }
+  std::vector<int> perm(n);  // new slot -> old slot
+  for (int i = 0; i < n; i++) perm[i] = i;
+  std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return code[a] < code[b]; });
+  auto permute = [&](auto& v, int w) {
+    auto old = v;
+    for (int i = 0; i < n; i++)
+      for (int c = 0; c < w; c++) v[(size_t)w * i + c] = old[(size_t)w * perm[i] + c];
+  };
+  permute(e->h_id, 1);
+  permute(e->h_tag, 1);
+  permute(e->h_pos, 3);
+  permute(e->h_vel, 3);
+  permute(e->h_rad, 1);
+  permute(e->h_mass, 1);
+  permute(e->h_flags, 1);
+  e->id2slot.clear();
+  for (int i = 0; i < n; i++) e->id2slot[e->h_id[i]] = i;
+}
+
 void upload(esp_engine* e)
 {
+  spatial_order(e);
   const int n = e->n;
   const size_t cap = std::max(n, 1);
   e->pos.reserve(cap); e->vel.reserve(cap); e->old.reserve(cap); e->frc.reserve(cap);
-  e->flags.reserve(cap); e->id.reserve(cap); e->by_id.reserve(cap); e->cell.reserve(cap);
+  e->flags.reserve(cap); e->id.reserve(cap); e->cell.reserve(cap);
   if (e->rot) { e->ang.reserve(cap); e->quat.reserve(cap); e->mom.reserve(cap); }
   std::vector<double4> P(n), Vv(n), O(n), A(n), Q(n), Z(n);
   for (int i = 0; i < n; i++) {
@@ -449,10 +502,6 @@ void upload(esp_engine* e)
   }
   h2d(e, e->flags.p, e->h_flags.data(), n);
   h2d(e, e->id.p, e->h_id.data(), n);
-  std::vector<int> order(n);
-  for (int i = 0; i < n; i++) order[i] = i;
-  std::sort(order.begin(), order.end(), [&](int a, int b) { return e->h_id[a] < e->h_id[b]; });
-  h2d(e, e->by_id.p, order.data(), n);
 
   int total_bonds = 0;
   for (int gi : e->bord) {
@@ -460,14 +509,36 @@ void upload(esp_engine* e)
     G.bond_off = total_bonds;
     total_bonds += G.n;
     const size_t c = std::max(G.n, 1);
-    G.ij.reserve(c); G.k4.reserve(c); G.mx4.reserve(c); G.D4.reserve(c);
-    std::vector<int2> ij(G.n);
+    G.ij.reserve(c); G.k4.reserve(c); G.mx4.reserve(c); G.D4.reserve(c); G.rank.reserve(c);
+    if (G.n >= (1 << 27)) throw EspError(ESP_ERR_CAPACITY, "more than 2^27 bonds in one group");
+    // stored in slot order; rank keeps the connection order for the force sums
+    std::vector<int2> ij0(G.n);
     for (int b = 0; b < G.n; b++)
-      ij[b] = make_int2(e->id2slot[G.hid1[b]], e->id2slot[G.hid2[b]]);
+      ij0[b] = make_int2(e->id2slot[G.hid1[b]], e->id2slot[G.hid2[b]]);
+    std::vector<int> rank(G.n);
+    for (int b = 0; b < G.n; b++) rank[b] = b;
+    std::stable_sort(rank.begin(), rank.end(), [&](int a, int b) {
+      return ij0[a].x != ij0[b].x ? ij0[a].x < ij0[b].x : ij0[a].y < ij0[b].y;
+    });
+    std::vector<int2> ij(G.n);
+    std::vector<int> h1(G.n), h2(G.n);
+    std::vector<double4> k4(G.n), mx4(G.n), D4(G.n);
+    for (int b = 0; b < G.n; b++) {
+      const int r = rank[b];
+      ij[b] = ij0[r];
+      h1[b] = G.hid1[r];
+      h2[b] = G.hid2[r];
+      k4[b] = G.h_k4[r];
+      mx4[b] = G.h_mx4[r];
+      D4[b] = G.h_D4[r];
+    }
+    G.hid1.swap(h1);
+    G.hid2.swap(h2);
     h2d(e, G.ij.p, ij.data(), G.n);
-    h2d(e, G.k4.p, G.h_k4.data(), G.n);
-    h2d(e, G.mx4.p, G.h_mx4.data(), G.n);
-    h2d(e, G.D4.p, G.h_D4.data(), G.n);
+    h2d(e, G.k4.p, k4.data(), G.n);
+    h2d(e, G.mx4.p, mx4.data(), G.n);
+    h2d(e, G.D4.p, D4.data(), G.n);
+    h2d(e, G.rank.p, rank.data(), G.n);
     rebuild_bond_hash(e, G);
     sync(e);
     std::vector<double4>().swap(G.h_k4);
@@ -537,13 +608,13 @@ void rebuild(esp_engine* e, long long step)
   launch_cells_fill(e->stream, e->n, e->cell.p, e->cell_start.p, e->cell_fill.p, e->cell_items.p);
   launch_cells_sort(e->stream, e->ncell, e->cell_start.p, e->cell_items.p);
   // full candidate list (getFullList), sorted by (id1, id2)
-  launch_pairs(e->stream, 0, 0, p, e->grid, e->alpha, e->by_id.p, e->cell.p, e->cell_start.p,
+  launch_pairs(e->stream, 0, 0, p, e->grid, e->alpha, e->cell.p, e->cell_start.p,
                e->cell_items.p, e->cand_cnt.p, nullptr, nullptr, nullptr, nullptr);
   const int total = scan_total(e, e->cand_cnt.p, e->cand_off.p, e->n);
   e->cand_p.reserve(std::max(total, 1));
   e->cand_q.reserve(std::max(total, 1));
   e->cand_key.reserve(std::max(total, 1));
-  launch_pairs(e->stream, 1, 0, p, e->grid, e->alpha, e->by_id.p, e->cell.p, e->cell_start.p,
+  launch_pairs(e->stream, 1, 0, p, e->grid, e->alpha, e->cell.p, e->cell_start.p,
                e->cell_items.p, nullptr, e->cand_off.p, e->cand_p.p, e->cand_q.p, e->cand_key.p);
   HIPCHK(hipGetLastError());
   e->ncand = total;
@@ -555,13 +626,13 @@ void rebuild(esp_engine* e, long long step)
 void new_pair_list(esp_engine* e)
 {
   PView p = e->view();
-  launch_pairs(e->stream, 0, 1, p, e->grid, e->alpha, e->by_id.p, e->cell.p, e->cell_start.p,
+  launch_pairs(e->stream, 0, 1, p, e->grid, e->alpha, e->cell.p, e->cell_start.p,
                e->cell_items.p, e->cand_cnt.p, nullptr, nullptr, nullptr, nullptr);
   const int total = scan_total(e, e->cand_cnt.p, e->cand_off.p, e->n);
   e->np_p.reserve(std::max(total, 1));
   e->np_q.reserve(std::max(total, 1));
   e->np_key.reserve(std::max(total, 1));
-  launch_pairs(e->stream, 1, 1, p, e->grid, e->alpha, e->by_id.p, e->cell.p, e->cell_start.p,
+  launch_pairs(e->stream, 1, 1, p, e->grid, e->alpha, e->cell.p, e->cell_start.p,
                e->cell_items.p, nullptr, e->cand_off.p, e->np_p.p, e->np_q.p, e->np_key.p);
   launch_clear_flag_bit(e->stream, p, e->upd.p);
   HIPCHK(hipGetLastError());
@@ -615,11 +686,14 @@ void fold(esp_engine* e)
           d2h(e, keep.data(), e->keep.p, G.n);
           const size_t c2 = std::max(nb2, 1);
           e->tmp_ij.reserve(c2); e->tmp_a.reserve(c2); e->tmp_b.reserve(c2); e->tmp_c.reserve(c2);
+          e->tmp_rank.reserve(c2);
           launch_compact_bonds(e->stream, G.n, e->keep.p, e->keep_off.p, G.ij.p, G.k4.p, G.mx4.p,
-                               G.D4.p, e->tmp_ij.p, e->tmp_a.p, e->tmp_b.p, e->tmp_c.p);
+                               G.D4.p, G.rank.p, e->tmp_ij.p, e->tmp_a.p, e->tmp_b.p, e->tmp_c.p,
+                               e->tmp_rank.p);
           HIPCHK(hipGetLastError());
           sync(e);
           G.ij.swap(e->tmp_ij); G.k4.swap(e->tmp_a); G.mx4.swap(e->tmp_b); G.D4.swap(e->tmp_c);
+          G.rank.swap(e->tmp_rank);
           int w = 0;
           for (int b = 0; b < G.n; b++)
             if (keep[b]) { G.hid1[w] = G.hid1[b]; G.hid2[w] = G.hid2[b]; w++; }
@@ -699,7 +773,7 @@ void dyn_update(esp_engine* e, DynGroup& G, bool& new_list_taken)
       launch_ed_mark(e->stream, nc, e->cand_p.p, e->cand_q.p, e->cand_key.p, ex, G.key.p, G.n,
                      G.ij.p, p, e->keep.p, e->src.p);
     } else {
-      launch_ne_mark(e->stream, nc, e->cand_key.p, ex, e->keep.p);
+      launch_ne_mark(e->stream, nc, e->cand_p.p, e->cand_q.p, e->id.p, ex, e->keep.p);
     }
     const int total = scan_total(e, e->keep.p, e->keep_off.p, nc);
     e->tmp_ij.reserve((size_t)total + ESP_APP_CAP);
@@ -724,8 +798,8 @@ void dyn_update(esp_engine* e, DynGroup& G, bool& new_list_taken)
     if (nc > 0) {
       e->keep.reserve(nc + 1);
       e->keep_off.reserve(nc + 1);
-      launch_filter_new(e->stream, nc, e->np_key.p, ex, HashRef{G.hs.p, G.hs_mask}, hist ? 1 : 0,
-                        e->keep.p);
+      launch_filter_new(e->stream, nc, e->np_p.p, e->np_q.p, e->id.p, e->np_key.p, ex,
+                        HashRef{G.hs.p, G.hs_mask}, hist ? 1 : 0, e->keep.p);
       const int add = scan_total(e, e->keep.p, e->keep_off.p, nc);
       if (add > 0) {
         e->tmp_ij2.reserve(add);
@@ -776,6 +850,8 @@ void rebuild_csr(esp_engine* e)
   for (int gi : e->bord) { e->bg[gi].rec_off = (int)rec; rec += e->bg[gi].n; }
   for (int gi : e->dord) { e->dg[gi].app_rec_off = (int)rec; rec += ESP_APP_CAP; }
   if (rec >= (1ll << 29)) throw EspError(ESP_ERR_CAPACITY, "more than 2^29 interactions");
+  for (int gi : e->dord)
+    if (e->dg[gi].n >= (1 << 27)) throw EspError(ESP_ERR_CAPACITY, "more than 2^27 pairs in one group");
   e->nrec = (int)rec;
   e->contrib.reserve((size_t)std::max<long long>(rec, 1) * 9);
   zero(e, e->inc_cnt.p, e->n + 1);
@@ -783,18 +859,22 @@ void rebuild_csr(esp_engine* e)
   for (int gi : e->bord) launch_inc_count(e->stream, e->bg[gi].n, e->bg[gi].ij.p, e->inc_cnt.p);
   const int total = scan_total(e, e->inc_cnt.p, e->inc_off.p, e->n);
   e->inc_ent.reserve(std::max(total, 1));
+  e->inc_ent64.reserve(std::max(total, 1));
   zero(e, e->inc_fill.p, e->n + 1);
+  // summation order: dynamic groups by name (storage order), then bonded groups
+  // by name (connection order)
+  int grp = 0;
   for (int gi : e->dord) {
     DynGroup& G = e->dg[gi];
-    launch_inc_fill(e->stream, G.n, G.ij.p, G.rec_off, G.kind == 3 ? 1 : 0, e->inc_off.p,
-                    e->inc_fill.p, e->inc_ent.p);
+    launch_inc_fill(e->stream, G.n, G.ij.p, G.rec_off, G.kind == 3 ? 1 : 0, nullptr, grp++,
+                    e->inc_off.p, e->inc_fill.p, e->inc_ent64.p);
   }
   for (int gi : e->bord) {
     BondGroup& G = e->bg[gi];
-    launch_inc_fill(e->stream, G.n, G.ij.p, G.rec_off, 1, e->inc_off.p, e->inc_fill.p,
-                    e->inc_ent.p);
+    launch_inc_fill(e->stream, G.n, G.ij.p, G.rec_off, 1, G.rank.p, grp++, e->inc_off.p,
+                    e->inc_fill.p, e->inc_ent64.p);
   }
-  launch_inc_sort(e->stream, e->n, e->inc_off.p, e->inc_ent.p);
+  launch_inc_sort(e->stream, e->n, e->inc_off.p, e->inc_ent64.p, e->inc_ent.p);
   // records start at zero: a contact that is not touching keeps a zero record
   // without rewriting it every step
   zero(e, e->contrib.p, (size_t)std::max<long long>(rec, 1) * 9);
@@ -1041,6 +1121,8 @@ void run_steps(esp_engine* e, int nsteps)
       throw EspError(ESP_ERR_CAPACITY, "more than 4 gravity groups");
     if (e->bg.size() > ESP_MAX_BOND_GROUPS)
       throw EspError(ESP_ERR_CAPACITY, "more than 8 bonded groups");
+    if (e->bg.size() + e->dg.size() > 32)
+      throw EspError(ESP_ERR_CAPACITY, "more than 32 interaction groups");
     for (auto& G : e->dg)
       if (G.ex.size() > ESP_MAX_EXCL)
         throw EspError(ESP_ERR_CAPACITY, "more than 4 exclusions on one group");
@@ -1106,8 +1188,15 @@ int dyn_appended(esp_engine* e, DynGroup& G)
   return na;
 }
 
-// stored pairs of a dynamic group in key order (main list merged with the
-// appendix, main first on equal keys, as k_merge orders them)
+// slot-pair key -> id-pair key
+long long id_key(esp_engine* e, long long skey)
+{
+  return pkey(e->h_id[(int)(skey >> 32)], e->h_id[(int)(skey & 0xffffffffLL)]);
+}
+
+// stored pairs of a dynamic group (main list and appendix) as id keys in
+// ascending order; equal keys (NRotElastic may store a pair twice) keep
+// storage order
 void dyn_view(esp_engine* e, DynGroup& G, std::vector<long long>& key, std::vector<double4>* ff)
 {
   const int tot = G.n + dyn_appended(e, G);
@@ -1119,6 +1208,7 @@ void dyn_view(esp_engine* e, DynGroup& G, std::vector<long long>& key, std::vect
     if (ff) d2h(e, f.data(), G.ff.p, tot);
     sync(e);
   }
+  for (int i = 0; i < tot; i++) k[i] = id_key(e, k[i]);
   std::vector<int> perm(tot);
   for (int i = 0; i < tot; i++) perm[i] = i;
   std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return k[a] < k[b]; });
@@ -1699,6 +1789,8 @@ int esp_get_neighbour_pairs(esp_engine* e, int* id1, int* id2)
     std::vector<long long> k(e->ncand);
     d2h(e, k.data(), e->cand_key.p, e->ncand);
     sync(e);
+    for (int i = 0; i < e->ncand; i++) k[i] = id_key(e, k[i]);
+    std::sort(k.begin(), k.end());
     for (int i = 0; i < e->ncand; i++) {
       id1[i] = (int)(k[i] >> 32);
       id2[i] = (int)(k[i] & 0xffffffffLL);

@@ -89,8 +89,10 @@ struct GatherParams {
   double* wall_part;  // this step's slot: [block][nwg][3]
 };
 
-// Open-addressing hash sets of pair keys (id1 << 32 | id2): live bonds of a
-// bonded group (exclusion lookups) and stored pairs of a dynamic group (isIn).
+// Open-addressing hash sets of pair keys (a << 32 | b): live bonds of a bonded
+// group keyed by particle ids (exclusion lookups), stored pairs of a dynamic
+// group keyed by particle slots (isIn).  A pair is oriented (lower id, higher
+// id); dynamic lists are sorted by the slot key of that orientation.
 #define HS_EMPTY 0xFFFFFFFFFFFFFFFFull
 #define HS_TOMB 0xFFFFFFFFFFFFFFFEull
 struct HashRef {
@@ -185,33 +187,40 @@ void launch_cells_count(hipStream_t s, PView p, GridParams g, int* cell, int* ce
 void launch_cells_fill(hipStream_t s, int n, const int* cell, const int* cell_start,
                        int* cell_fill, int* cell_items);
 void launch_cells_sort(hipStream_t s, int ncell, const int* cell_start, int* cell_items);
+// candidate pairs from each particle in slot order, oriented (lower id, higher
+// id), keyed by the slot pair; flagged = getNewList
 void launch_pairs(hipStream_t s, int mode_emit, int flagged, PView p, GridParams g, double alpha,
-                  const int* by_id, const int* cell, const int* cell_start, const int* cell_items,
-                  int* cnt, const int* off, int* out_p, int* out_q, long long* out_key);
+                  const int* cell, const int* cell_start, const int* cell_items, int* cnt,
+                  const int* off, int* out_p, int* out_q, long long* out_key);
 void launch_clear_flag_bit(hipStream_t s, PView p, UpdState* st);
 void launch_ed_mark(hipStream_t s, int nc, const int* cp, const int* cq, const long long* ckey,
                     ExclSet ex, const long long* okey, int nold, const int2* oij, PView p,
                     int* keep, int* src);
-void launch_ne_mark(hipStream_t s, int nc, const long long* ckey, ExclSet ex, int* keep);
+void launch_ne_mark(hipStream_t s, int nc, const int* cp, const int* cq, const int* id, ExclSet ex,
+                    int* keep);
 void launch_pair_emit(hipStream_t s, int nc, const int* keep, const int* off, const int* cp,
                       const int* cq, const long long* ckey, const int* src, const double4* off_ff,
                       int2* nij, long long* nkey, double4* nff);
-void launch_filter_new(hipStream_t s, int nc, const long long* ckey, ExclSet ex, HashRef present,
-                       int check_present, int* keep);
+void launch_filter_new(hipStream_t s, int nc, const int* cp, const int* cq, const int* id,
+                       const long long* ckey, ExclSet ex, HashRef present, int check_present,
+                       int* keep);
 void launch_merge(hipStream_t s, int nold, const int2* oij, const long long* okey,
                   const double4* off_, int nnew, const int2* nij_in, const long long* nkey_in,
                   const double4* nff_in, int2* ij_out, long long* key_out, double4* ff_out);
 void launch_pack_pairs(hipStream_t s, int n, const int* keep, const int* off, const int* cp,
                        const int* cq, const long long* ckey, int2* ij, long long* key);
-void launch_compact_bonds(hipStream_t s, int nb, const int* removed, const int* off,
+void launch_compact_bonds(hipStream_t s, int nb, const int* keep, const int* off,
                           const int2* ij, const double4* k4, const double4* mx4,
-                          const double4* D4, int2* ij2, double4* k42, double4* mx42,
-                          double4* D42);
+                          const double4* D4, const int* rank, int2* ij2, double4* k42,
+                          double4* mx42, double4* D42, int* rank2);
 void launch_wall_update(hipStream_t s, PView p, GridParams g, int nwg, const WallParam* walls);
 void launch_inc_count(hipStream_t s, int nc, const int2* ij, int* inc_cnt);
-void launch_inc_fill(hipStream_t s, int nc, const int2* ij, int rec_off, int has_m,
-                     const int* inc_off, int* inc_fill, int* inc_ent);
-void launch_inc_sort(hipStream_t s, int n, const int* inc_off, int* inc_ent);
+// rank: the interactions' positions in the group's reference order (nullptr:
+// storage order); grp: the group's position in the summation order (< 32)
+void launch_inc_fill(hipStream_t s, int nc, const int2* ij, int rec_off, int has_m, const int* rank,
+                     int grp, const int* inc_off, int* inc_fill, unsigned long long* ent64);
+void launch_inc_sort(hipStream_t s, int n, const int* inc_off, unsigned long long* ent64,
+                     int* inc_ent);
 
 // exclusive scan of n ints into out (out[n] = total); tmp sized by scan_tmp_size
 size_t scan_tmp_size(int n);

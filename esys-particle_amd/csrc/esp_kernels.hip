@@ -516,13 +516,12 @@ __global__ void k_cells_sort(int ncell, const int* cell_start, int* items)
 // (ntable.hpp:532-583).  The emit pass sorts each particle's run by the partner
 // id, so the whole list comes out sorted by (id1, id2).
 __global__ void k_pairs(int emit, int flagged, PView p, GridParams g, double alpha,
-                        const int* by_id, const int* cell, const int* cell_start,
-                        const int* cell_items, int* cnt, const int* off, int* out_p, int* out_q,
-                        long long* out_key)
+                        const int* cell, const int* cell_start, const int* cell_items, int* cnt,
+                        const int* off, int* out_p, int* out_q, long long* out_key)
 {
   const int k = blockIdx.x * BS + threadIdx.x;
   if (k >= p.n) return;
-  const int pi = by_id[k];
+  const int pi = k;
   if (flagged && !(p.flags[pi] & PF_FLAG)) {
     if (!emit) cnt[k] = 0;
     return;
@@ -560,7 +559,7 @@ __global__ void k_pairs(int emit, int flagged, PView p, GridParams g, double alp
           if (emit) {
             out_p[base + count] = pi;
             out_q[base + count] = q;
-            out_key[base + count] = pair_key(idp, idq);
+            out_key[base + count] = pair_key(pi, q);
           }
           count++;
         }
@@ -634,7 +633,7 @@ __global__ void k_ed_mark(int nc, const int* cp, const int* cq, const long long*
   const int c = blockIdx.x * BS + threadIdx.x;
   if (c >= nc) return;
   const long long key = ckey[c];
-  if (excluded(ex, key)) {
+  if (excluded(ex, pair_key(p.id[cp[c]], p.id[cq[c]]))) {
     keep[c] = 0;
     src[c] = -1;
     return;
@@ -653,10 +652,10 @@ __global__ void k_ed_mark(int nc, const int* cp, const int* cq, const long long*
   src[c] = s;
 }
 
-__global__ void k_ne_mark(int nc, const long long* ckey, ExclSet ex, int* keep)
+__global__ void k_ne_mark(int nc, const int* cp, const int* cq, const int* id, ExclSet ex, int* keep)
 {
   const int c = blockIdx.x * BS + threadIdx.x;
-  if (c < nc) keep[c] = excluded(ex, ckey[c]) ? 0 : 1;
+  if (c < nc) keep[c] = excluded(ex, pair_key(id[cp[c]], id[cq[c]])) ? 0 : 1;
 }
 
 __global__ void k_pair_emit(int nc, const int* keep, const int* off, const int* cp, const int* cq,
@@ -676,14 +675,14 @@ __global__ void k_pair_emit(int nc, const int* keep, const int* off, const int* 
 
 // pairs of the flagged list that a group takes: not excluded and (for history
 // groups) not already stored (pi_storage_ed.hpp:137-156, pi_storage_ne.hpp:79-96)
-__global__ void k_filter_new(int nc, const long long* ckey, ExclSet ex, HashRef present,
-                             int check_present, int* keep)
+__global__ void k_filter_new(int nc, const int* cp, const int* cq, const int* id,
+                             const long long* ckey, ExclSet ex, HashRef present, int check_present,
+                             int* keep)
 {
   const int c = blockIdx.x * BS + threadIdx.x;
   if (c >= nc) return;
-  const long long key = ckey[c];
-  int k = excluded(ex, key) ? 0 : 1;
-  if (k && check_present && hs_find(present, key) >= 0) k = 0;
+  int k = excluded(ex, pair_key(id[cp[c]], id[cq[c]])) ? 0 : 1;
+  if (k && check_present && hs_find(present, ckey[c]) >= 0) k = 0;
   keep[c] = k;
 }
 
@@ -728,7 +727,8 @@ __global__ void k_fill_int(int n, int* a, int v)
 
 __global__ void k_compact_bonds(int nb, const int* keep, const int* off, const int2* ij,
                                 const double4* k4, const double4* mx4, const double4* D4,
-                                int2* ij2, double4* k42, double4* mx42, double4* D42)
+                                const int* rank, int2* ij2, double4* k42, double4* mx42,
+                                double4* D42, int* rank2)
 {
   const int b = blockIdx.x * BS + threadIdx.x;
   if (b >= nb || !keep[b]) return;
@@ -737,6 +737,7 @@ __global__ void k_compact_bonds(int nb, const int* keep, const int* off, const i
   k42[w] = k4[b];
   mx42[w] = mx4[b];
   D42[w] = D4[b];
+  rank2[w] = rank[b];
 }
 
 
@@ -815,10 +816,9 @@ __global__ __launch_bounds__(U2_BS) void k_incremental_update(
   __shared__ long long s_key[ESP_NEW_CAP];
   __shared__ int s_idx[ESP_NEW_CAP];
   __shared__ int s_cnt, s_over, s_keep;
-  // flags, flagged list and its count complete and visible to the whole block
-  __threadfence();
+  // flags, flagged list and its count complete: within one workgroup a barrier
+  // orders global memory (the waves share one vector L1)
   __syncthreads();
-  __threadfence();
   if (tid == 0) {
     s_cnt = 0;
     s_over = 0;
@@ -849,7 +849,7 @@ __global__ __launch_bounds__(U2_BS) void k_incremental_update(
       if (idq <= idp) continue;
       const int k = atomicAdd(&s_cnt, 1);
       if (k < ESP_NEW_CAP) {
-        s_key[k] = pair_key(idp, idq);
+        s_key[k] = pair_key(pi, q);
         nidx[2 * k] = pi;
         nidx[2 * k + 1] = q;
       } else {
@@ -857,7 +857,6 @@ __global__ __launch_bounds__(U2_BS) void k_incremental_update(
       }
     }
   }
-  __threadfence();
   __syncthreads();
   const int n = min(s_cnt, ESP_NEW_CAP);
   // keep: not excluded, not stored (history groups); others sort to the end
@@ -866,7 +865,8 @@ __global__ __launch_bounds__(U2_BS) void k_incremental_update(
   for (int t = tid; t < m; t += U2_BS) {
     if (t < n) {
       const long long key = s_key[t];
-      const bool keep = !excluded(da.ex, key) && !(da.is_hist && hs_find(da.present, key) >= 0);
+      const long long idkey = pair_key(p.id[nidx[2 * t]], p.id[nidx[2 * t + 1]]);
+      const bool keep = !excluded(da.ex, idkey) && !(da.is_hist && hs_find(da.present, key) >= 0);
       if (!keep) s_key[t] = NEW_SENTINEL;
     } else {
       s_key[t] = NEW_SENTINEL;
@@ -986,36 +986,45 @@ __global__ void k_inc_count(int nc, const int2* ij, int* cnt)
   atomicAdd(&cnt[e.y], 1);
 }
 
-__global__ void k_inc_fill(int nc, const int2* ij, int rec_off, int has_m, const int* off,
-                           int* fill, int* ent)
+// CSR entry of an interaction for each of its particles: the record reference
+// (rec << 2 | has-moments << 1 | second-particle) in the low 32 bits, above it
+// the summation order: group position (5 bits) and the interaction's rank in
+// the group's reference order (27 bits; the storage index where storage order
+// is that order).
+__global__ void k_inc_fill(int nc, const int2* ij, int rec_off, int has_m, const int* rank,
+                           unsigned long long grp, const int* off, int* fill,
+                           unsigned long long* ent)
 {
   const int c = blockIdx.x * BS + threadIdx.x;
   if (c >= nc) return;
   const int2 e = ij[c];
   if (e.x < 0) return;
-  const int v = ((rec_off + c) << 2) | (has_m << 1);
-  ent[off[e.x] + atomicAdd(&fill[e.x], 1)] = v;
-  ent[off[e.y] + atomicAdd(&fill[e.y], 1)] = v | 1;
+  const unsigned v = ((unsigned)(rec_off + c) << 2) | ((unsigned)has_m << 1);
+  const unsigned long long hi = (grp << 59) | ((unsigned long long)(rank ? rank[c] : c) << 32);
+  ent[off[e.x] + atomicAdd(&fill[e.x], 1)] = hi | v;
+  ent[off[e.y] + atomicAdd(&fill[e.y], 1)] = hi | v | 1u;
 }
 
-// Sorts each particle's run of CSR entries (ascending record index).  Runs of
-// consecutive particles are contiguous, so a block stages its 256 runs in LDS
-// when they fit and each thread insertion-sorts its own run there.
-#define INC_LDS 12288
-__global__ __launch_bounds__(BS) void k_inc_sort(int n, const int* off, int* ent)
+// Sorts each particle's run of CSR entries into summation order and keeps the
+// record references.  Runs of consecutive particles are contiguous, so a block
+// stages its 256 runs in LDS when they fit and each thread insertion-sorts its
+// own run there.
+#define INC_LDS 6144
+__global__ __launch_bounds__(BS) void k_inc_sort(int n, const int* off, unsigned long long* ent64,
+                                                 int* ent)
 {
-  __shared__ int buf[INC_LDS];
+  __shared__ unsigned long long buf[INC_LDS];
   const int p0 = blockIdx.x * BS;
   const int p1 = min(p0 + BS, n);
   const int base = off[p0], end = off[p1];
   const int i = p0 + threadIdx.x;
   if (end - base <= INC_LDS) {
-    for (int t = base + threadIdx.x; t < end; t += BS) buf[t - base] = ent[t];
+    for (int t = base + threadIdx.x; t < end; t += BS) buf[t - base] = ent64[t];
     __syncthreads();
     if (i < n) {
       const int a = off[i] - base, b = off[i + 1] - base;
       for (int s = a + 1; s < b; s++) {
-        const int v = buf[s];
+        const unsigned long long v = buf[s];
         int j = s - 1;
         while (j >= a && buf[j] > v) {
           buf[j + 1] = buf[j];
@@ -1025,20 +1034,21 @@ __global__ __launch_bounds__(BS) void k_inc_sort(int n, const int* off, int* ent
       }
     }
     __syncthreads();
-    for (int t = base + threadIdx.x; t < end; t += BS) ent[t] = buf[t - base];
+    for (int t = base + threadIdx.x; t < end; t += BS) ent[t] = (int)(unsigned)buf[t - base];
     return;
   }
   if (i >= n) return;
   const int a = off[i], b = off[i + 1];
   for (int s = a + 1; s < b; s++) {
-    const int v = ent[s];
+    const unsigned long long v = ent64[s];
     int j = s - 1;
-    while (j >= a && ent[j] > v) {
-      ent[j + 1] = ent[j];
+    while (j >= a && ent64[j] > v) {
+      ent64[j + 1] = ent64[j];
       j--;
     }
-    ent[j + 1] = v;
+    ent64[j + 1] = v;
   }
+  for (int s = a; s < b; s++) ent[s] = (int)(unsigned)ent64[s];
 }
 
 // ---------------------------------------------------------------------------
@@ -1231,12 +1241,12 @@ void launch_cells_sort(hipStream_t s, int ncell, const int* cell_start, int* cel
 }
 
 void launch_pairs(hipStream_t s, int mode_emit, int flagged, PView p, GridParams g, double alpha,
-                  const int* by_id, const int* cell, const int* cell_start, const int* cell_items,
-                  int* cnt, const int* off, int* out_p, int* out_q, long long* out_key)
+                  const int* cell, const int* cell_start, const int* cell_items, int* cnt,
+                  const int* off, int* out_p, int* out_q, long long* out_key)
 {
   if (p.n <= 0) return;
   hipLaunchKernelGGL(k_pairs, dim3(nblk(p.n)), dim3(BS), 0, s, mode_emit, flagged, p, g, alpha,
-                     by_id, cell, cell_start, cell_items, cnt, off, out_p, out_q, out_key);
+                     cell, cell_start, cell_items, cnt, off, out_p, out_q, out_key);
 }
 
 void launch_clear_flag_bit(hipStream_t s, PView p, UpdState* st)
@@ -1253,10 +1263,11 @@ void launch_ed_mark(hipStream_t s, int nc, const int* cp, const int* cq, const l
                      oij, p, keep, src);
 }
 
-void launch_ne_mark(hipStream_t s, int nc, const long long* ckey, ExclSet ex, int* keep)
+void launch_ne_mark(hipStream_t s, int nc, const int* cp, const int* cq, const int* id, ExclSet ex,
+                    int* keep)
 {
   if (nc <= 0) return;
-  hipLaunchKernelGGL(k_ne_mark, dim3(nblk(nc)), dim3(BS), 0, s, nc, ckey, ex, keep);
+  hipLaunchKernelGGL(k_ne_mark, dim3(nblk(nc)), dim3(BS), 0, s, nc, cp, cq, id, ex, keep);
 }
 
 void launch_pair_emit(hipStream_t s, int nc, const int* keep, const int* off, const int* cp,
@@ -1268,12 +1279,13 @@ void launch_pair_emit(hipStream_t s, int nc, const int* keep, const int* off, co
                      off_ff, nij, nkey, nff);
 }
 
-void launch_filter_new(hipStream_t s, int nc, const long long* ckey, ExclSet ex, HashRef present,
-                       int check_present, int* keep)
+void launch_filter_new(hipStream_t s, int nc, const int* cp, const int* cq, const int* id,
+                       const long long* ckey, ExclSet ex, HashRef present, int check_present,
+                       int* keep)
 {
   if (nc <= 0) return;
-  hipLaunchKernelGGL(k_filter_new, dim3(nblk(nc)), dim3(BS), 0, s, nc, ckey, ex, present,
-                     check_present, keep);
+  hipLaunchKernelGGL(k_filter_new, dim3(nblk(nc)), dim3(BS), 0, s, nc, cp, cq, id, ckey, ex,
+                     present, check_present, keep);
 }
 
 void launch_merge(hipStream_t s, int nold, const int2* oij, const long long* okey,
@@ -1295,12 +1307,12 @@ void launch_pack_pairs(hipStream_t s, int n, const int* keep, const int* off, co
 
 void launch_compact_bonds(hipStream_t s, int nb, const int* keep, const int* off,
                           const int2* ij, const double4* k4, const double4* mx4,
-                          const double4* D4, int2* ij2, double4* k42, double4* mx42,
-                          double4* D42)
+                          const double4* D4, const int* rank, int2* ij2, double4* k42,
+                          double4* mx42, double4* D42, int* rank2)
 {
   if (nb <= 0) return;
   hipLaunchKernelGGL(k_compact_bonds, dim3(nblk(nb)), dim3(BS), 0, s, nb, keep, off, ij, k4, mx4,
-                     D4, ij2, k42, mx42, D42);
+                     D4, rank, ij2, k42, mx42, D42, rank2);
 }
 
 
@@ -1318,16 +1330,17 @@ void launch_inc_count(hipStream_t s, int nc, const int2* ij, int* inc_cnt)
   hipLaunchKernelGGL(k_inc_count, dim3(nblk(nc)), dim3(BS), 0, s, nc, ij, inc_cnt);
 }
 
-void launch_inc_fill(hipStream_t s, int nc, const int2* ij, int rec_off, int has_m,
-                     const int* inc_off, int* inc_fill, int* inc_ent)
+void launch_inc_fill(hipStream_t s, int nc, const int2* ij, int rec_off, int has_m, const int* rank,
+                     int grp, const int* inc_off, int* inc_fill, unsigned long long* ent64)
 {
   if (nc <= 0) return;
-  hipLaunchKernelGGL(k_inc_fill, dim3(nblk(nc)), dim3(BS), 0, s, nc, ij, rec_off, has_m, inc_off,
-                     inc_fill, inc_ent);
+  hipLaunchKernelGGL(k_inc_fill, dim3(nblk(nc)), dim3(BS), 0, s, nc, ij, rec_off, has_m, rank,
+                     (unsigned long long)grp, inc_off, inc_fill, ent64);
 }
 
-void launch_inc_sort(hipStream_t s, int n, const int* inc_off, int* inc_ent)
+void launch_inc_sort(hipStream_t s, int n, const int* inc_off, unsigned long long* ent64,
+                     int* inc_ent)
 {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_inc_sort, dim3(nblk(n)), dim3(BS), 0, s, n, inc_off, inc_ent);
+  hipLaunchKernelGGL(k_inc_sort, dim3(nblk(n)), dim3(BS), 0, s, n, inc_off, ent64, inc_ent);
 }
