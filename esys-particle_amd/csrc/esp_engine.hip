@@ -174,7 +174,7 @@ struct esp_engine {
   DBuf<double4> tmp_ff, tmp_a, tmp_b, tmp_c;
   DBuf<int> inc_cnt, inc_off, inc_fill, inc_ent, tmp_rank;
   DBuf<unsigned long long> inc_ent64;
-  DBuf<int> xslot, xcnt, xdemand, flist, nidx;
+  DBuf<int> xslot, xcnt, flist;
   DBuf<int4> evt;
   DBuf<UpdState> upd;
   int nrec = 0;
@@ -573,11 +573,8 @@ void upload(esp_engine* e)
   e->inc_fill.reserve(cap + 1);
   e->xslot.reserve(cap * ESP_XSLOTS);
   e->xcnt.reserve(cap);
-  e->xdemand.reserve(cap);
   e->flist.reserve(cap);
-  e->nidx.reserve(2 * ESP_NEW_CAP);
   zero(e, e->xcnt.p, cap);
-  zero(e, e->xdemand.p, cap);
   const int nb = std::max(gather_blocks(n), 1);
   const int nwg = (int)e->wg.size();
   e->wall_part.reserve((size_t)(e->K + 1) * nb * std::max(nwg, 1) * 3);
@@ -925,7 +922,7 @@ void launch_step(esp_engine* e, const StepCtl& ctl, long long step)
     launch_incremental_update(e->stream, ctl, p, bond_set(e), e->broken_list.p, e->contrib.p,
                               e->evt.p, step, e->grid, da, e->cell.p, e->cell_start.p,
                               e->cell_items.p, e->upd.p, e->flist.p, e->xslot.p, e->xcnt.p,
-                              e->xdemand.p, e->nidx.p, e->dg.empty() ? 0 : 1);
+                              e->dg.empty() ? 0 : 1);
   }
   for (int gi : e->dord) {
     DynGroup& G = e->dg[gi];

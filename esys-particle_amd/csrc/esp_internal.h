@@ -168,7 +168,7 @@ void launch_incremental_update(hipStream_t s, StepCtl ctl, PView p, BondSet bs,
                                const int* broken_list, double* contrib, int4* evt, long long step,
                                GridParams g, DynAppend da, const int* cell, const int* cell_start,
                                const int* cell_items, UpdState* st, int* flist, int* xslot,
-                               int* xcnt, int* xdemand, int* nidx, int has_group);
+                               int* xcnt, int has_group);
 void launch_hash_clear(hipStream_t s, HashRef h);
 void launch_hash_insert(hipStream_t s, HashRef h, const long long* keys, int n);
 void launch_bond_keys(hipStream_t s, int nb, const int2* ij, const int* id, long long* keys);

@@ -301,6 +301,7 @@ __device__ __forceinline__ void block_sum3(double& a, double& b, double& c, doub
     }
 }
 
+#define GATHER_CHUNK 4
 __global__ __launch_bounds__(BS) void k_gather_integrate(StepCtl ctl, PView p, GatherParams gp)
 {
   if (step_skip(ctl)) return;
@@ -352,18 +353,32 @@ __global__ __launch_bounds__(BS) void k_gather_integrate(StepCtl ctl, PView p, G
     const double invm = Vv.w, mass = O.w;
     // body forces, name order (BodyForceGroup.hpp:41-65)
     for (int g = 0; g < gp.ngrav; g++) F = add3(F, mul3(mk3(gp.g[g][0], gp.g[g][1], gp.g[g][2]), mass));
-    // interaction records: dynamic groups then bonded groups, each in storage order
+    // interaction records in summation order (CSR, then the appendix slots),
+    // fetched GATHER_CHUNK at a time so their loads are in flight together;
+    // the sum itself stays sequential
     const int e0 = gp.inc_off[i], e1 = gp.inc_off[i + 1];
-    const int xc = gp.xcnt[i];
-    for (int t = e0; t < e1 + xc; t++) {
-      const int ent = t < e1 ? gp.inc_ent[t] : gp.xslot[ESP_XSLOTS * i + (t - e1)];
-      const double* rec = gp.contrib + 9 * (size_t)(ent >> 2);
-      dv3 f = mk3(rec[0], rec[1], rec[2]);
-      if (ent & 1) f = neg3(f);
-      F = add3(F, f);
-      if (ent & 2) {
-        const double* m = rec + 3 + 3 * (ent & 1);
-        M = add3(M, mk3(m[0], m[1], m[2]));
+    const int cnt = e1 - e0 + gp.xcnt[i];
+    for (int b = 0; b < cnt; b += GATHER_CHUNK) {
+      int ent[GATHER_CHUNK];
+#pragma unroll
+      for (int k = 0; k < GATHER_CHUNK; k++) {
+        const int t = e0 + b + k;
+        ent[k] = b + k >= cnt ? -1 : t < e1 ? gp.inc_ent[t] : gp.xslot[ESP_XSLOTS * i + (t - e1)];
+      }
+      dv3 f[GATHER_CHUNK], m[GATHER_CHUNK];
+#pragma unroll
+      for (int k = 0; k < GATHER_CHUNK; k++) {
+        const int en = ent[k] < 0 ? 0 : ent[k];
+        const double* rec = gp.contrib + 9 * (size_t)(en >> 2);
+        const double* mr = rec + 3 + 3 * (en & 1);
+        f[k] = mk3(rec[0], rec[1], rec[2]);
+        m[k] = mk3(mr[0], mr[1], mr[2]);
+      }
+#pragma unroll
+      for (int k = 0; k < GATHER_CHUNK; k++) {
+        if (ent[k] < 0) break;
+        F = add3(F, (ent[k] & 1) ? neg3(f[k]) : f[k]);
+        if (ent[k] & 2) M = add3(M, m[k]);
       }
     }
     const double dt = gp.dt;
@@ -804,8 +819,7 @@ __global__ void k_break_apply(StepCtl ctl, PView p, BondSet bs, const int* broke
 __global__ __launch_bounds__(U2_BS) void k_incremental_update(
     StepCtl ctl, PView p, BondSet bs, const int* broken_list, double* contrib, int4* evt,
     long long step, GridParams g, DynAppend da, const int* cell, const int* cell_start,
-    const int* cell_items, UpdState* st, int* flist, int* xslot, int* xcnt, int* xdemand,
-    int* nidx, int has_group)
+    const int* cell_items, UpdState* st, int* flist, int* xslot, int* xcnt, int has_group)
 {
   if (step_skip(ctl)) return;
   const int nbk = ctl.prev->nbroken;
@@ -814,7 +828,6 @@ __global__ __launch_bounds__(U2_BS) void k_incremental_update(
   for (int t = tid; t < nbk; t += U2_BS) break_one(p, bs, broken_list[t], contrib, st, flist, evt, step);
   if (!has_group) return;
   __shared__ long long s_key[ESP_NEW_CAP];
-  __shared__ int s_idx[ESP_NEW_CAP];
   __shared__ int s_cnt, s_over, s_keep;
   // flags, flagged list and its count complete: within one workgroup a barrier
   // orders global memory (the waves share one vector L1)
@@ -822,6 +835,7 @@ __global__ __launch_bounds__(U2_BS) void k_incremental_update(
   if (tid == 0) {
     s_cnt = 0;
     s_over = 0;
+    s_keep = 0;
   }
   __syncthreads();
   const int nf = st->fcount;
@@ -850,8 +864,6 @@ __global__ __launch_bounds__(U2_BS) void k_incremental_update(
       const int k = atomicAdd(&s_cnt, 1);
       if (k < ESP_NEW_CAP) {
         s_key[k] = pair_key(pi, q);
-        nidx[2 * k] = pi;
-        nidx[2 * k + 1] = q;
       } else {
         s_over = 1;
       }
@@ -859,19 +871,21 @@ __global__ __launch_bounds__(U2_BS) void k_incremental_update(
   }
   __syncthreads();
   const int n = min(s_cnt, ESP_NEW_CAP);
-  // keep: not excluded, not stored (history groups); others sort to the end
+  // keep: not excluded, not stored (history groups); others sort to the end.
+  // A key holds the pair's slots (lower-id particle first).
   int m = 1;
   while (m < n) m <<= 1;
   for (int t = tid; t < m; t += U2_BS) {
     if (t < n) {
       const long long key = s_key[t];
-      const long long idkey = pair_key(p.id[nidx[2 * t]], p.id[nidx[2 * t + 1]]);
-      const bool keep = !excluded(da.ex, idkey) && !(da.is_hist && hs_find(da.present, key) >= 0);
-      if (!keep) s_key[t] = NEW_SENTINEL;
+      const int a = (int)(key >> 32), b = (int)(key & 0xffffffffll);
+      const bool keep = !excluded(da.ex, pair_key(p.id[a], p.id[b])) &&
+                        !(da.is_hist && hs_find(da.present, key) >= 0);
+      if (keep) atomicAdd(&s_keep, 1);
+      else s_key[t] = NEW_SENTINEL;
     } else {
       s_key[t] = NEW_SENTINEL;
     }
-    s_idx[t] = t;
   }
   __syncthreads();
   for (int k = 2; k <= m; k <<= 1)
@@ -884,36 +898,36 @@ __global__ __launch_bounds__(U2_BS) void k_incremental_update(
           if ((a > b) == up) {
             s_key[i] = b;
             s_key[ixj] = a;
-            const int ti = s_idx[i];
-            s_idx[i] = s_idx[ixj];
-            s_idx[ixj] = ti;
           }
         }
       }
       __syncthreads();
     }
-  if (tid == 0) {
-    int kept = 0;
-    while (kept < n && s_key[kept] != NEW_SENTINEL) kept++;
-    s_keep = kept;
-    if (*da.n_app + kept > ESP_APP_CAP) s_over = 1;
-  }
-  __syncthreads();
   const int kept = s_keep;
-  // per-particle overflow slots must have room for the new records
-  for (int t = tid; t < kept; t += U2_BS) {
-    atomicAdd(&xdemand[nidx[2 * s_idx[t]]], 1);
-    atomicAdd(&xdemand[nidx[2 * s_idx[t] + 1]], 1);
-  }
-  __syncthreads();
-  for (int t = tid; t < kept; t += U2_BS) {
-    const int a = nidx[2 * s_idx[t]], b = nidx[2 * s_idx[t] + 1];
-    if (xcnt[a] + xdemand[a] > ESP_XSLOTS || xcnt[b] + xdemand[b] > ESP_XSLOTS) s_over = 1;
-  }
-  __syncthreads();
-  for (int t = tid; t < kept; t += U2_BS) {
-    xdemand[nidx[2 * s_idx[t]]] = 0;
-    xdemand[nidx[2 * s_idx[t] + 1]] = 0;
+  const int napp = *da.n_app;
+  if (napp + kept > ESP_APP_CAP) s_over = 1;
+  // each new record takes the next appendix slot of both its particles, pairs
+  // in key order: the slot is the count of earlier kept pairs on that particle
+  int pa[ESP_NEW_CAP / U2_BS], pb[ESP_NEW_CAP / U2_BS];
+#pragma unroll
+  for (int r = 0; r < ESP_NEW_CAP / U2_BS; r++) {
+    const int t = tid + r * U2_BS;
+    pa[r] = pb[r] = 0;
+    if (t >= kept) continue;
+    const int a = (int)(s_key[t] >> 32), b = (int)(s_key[t] & 0xffffffffll);
+    int da_ = 0, db_ = 0;
+    for (int u = 0; u < kept; u++) {
+      const long long ku = s_key[u];
+      const int a2 = (int)(ku >> 32), b2 = (int)(ku & 0xffffffffll);
+      const int ia = (a2 == a) | (b2 == a), ib = (a2 == b) | (b2 == b);
+      if (u < t) {
+        pa[r] += ia;
+        pb[r] += ib;
+      }
+      da_ += ia;
+      db_ += ib;
+    }
+    if (xcnt[a] + da_ > ESP_XSLOTS || xcnt[b] + db_ > ESP_XSLOTS) s_over = 1;
   }
   __syncthreads();
   if (s_over) {
@@ -923,26 +937,32 @@ __global__ __launch_bounds__(U2_BS) void k_incremental_update(
     }
     return;
   }
-  const int napp = *da.n_app;
-  for (int t = tid; t < kept; t += U2_BS) {
-    const int a = nidx[2 * s_idx[t]], b = nidx[2 * s_idx[t] + 1];
+  const int hm = da.is_hist ? 2 : 0;  // history groups carry moments
+#pragma unroll
+  for (int r = 0; r < ESP_NEW_CAP / U2_BS; r++) {
+    const int t = tid + r * U2_BS;
+    if (t >= kept) continue;
+    const long long key = s_key[t];
+    const int a = (int)(key >> 32), b = (int)(key & 0xffffffffll);
     const int w = da.n_main + napp + t;
     da.ij[w] = make_int2(a, b);
-    da.key[w] = s_key[t];
+    da.key[w] = key;
     if (da.ff) da.ff[w] = make_double4(0.0, 0.0, 0.0, 0.0);
-    if (da.is_hist) hs_insert(da.present, s_key[t]);
+    if (da.is_hist) hs_insert(da.present, key);
+    const int ent = ((da.app_rec_off + napp + t) << 2) | hm;
+    xslot[ESP_XSLOTS * a + xcnt[a] + pa[r]] = ent;
+    xslot[ESP_XSLOTS * b + xcnt[b] + pb[r]] = ent | 1;
   }
-  if (tid == 0) {
-    const int hm = da.is_hist ? 2 : 0;  // history groups carry moments
-    for (int t = 0; t < kept; t++) {
-      const int a = nidx[2 * s_idx[t]], b = nidx[2 * s_idx[t] + 1];
-      const int ent = ((da.app_rec_off + napp + t) << 2) | hm;
-      xslot[ESP_XSLOTS * a + xcnt[a]++] = ent;
-      xslot[ESP_XSLOTS * b + xcnt[b]++] = ent | 1;
-    }
-    *da.n_app = napp + kept;
+  __syncthreads();  // every xcnt read above precedes the increments
+#pragma unroll
+  for (int r = 0; r < ESP_NEW_CAP / U2_BS; r++) {
+    const int t = tid + r * U2_BS;
+    if (t >= kept) continue;
+    const long long key = s_key[t];
+    atomicAdd(&xcnt[(int)(key >> 32)], 1);
+    atomicAdd(&xcnt[(int)(key & 0xffffffffll)], 1);
   }
-  __syncthreads();
+  if (tid == 0) *da.n_app = napp + kept;
   for (int t = tid; t < nf; t += U2_BS) p.flags[flist[t]] &= ~PF_FLAG;
   if (tid == 0) st->fcount = 0;
 }
@@ -1168,11 +1188,11 @@ void launch_incremental_update(hipStream_t s, StepCtl ctl, PView p, BondSet bs,
                                const int* broken_list, double* contrib, int4* evt, long long step,
                                GridParams g, DynAppend da, const int* cell, const int* cell_start,
                                const int* cell_items, UpdState* st, int* flist, int* xslot,
-                               int* xcnt, int* xdemand, int* nidx, int has_group)
+                               int* xcnt, int has_group)
 {
   hipLaunchKernelGGL(k_incremental_update, dim3(1), dim3(U2_BS), 0, s, ctl, p, bs, broken_list,
                      contrib, evt, step, g, da, cell, cell_start, cell_items, st, flist, xslot,
-                     xcnt, xdemand, nidx, has_group);
+                     xcnt, has_group);
 }
 
 void launch_hash_clear(hipStream_t s, HashRef h)
