@@ -53,6 +53,31 @@ def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
+def copy_probe(mib):
+    """A plain device-to-device copy of known size, for scaling the FETCH_SIZE /
+    WRITE_SIZE counters (each copy reads and writes mib MiB)."""
+    import torch
+    a = torch.ones(mib << 20, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(10):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log("copy probe: 10 x %d MiB" % mib)
+
+
+def save_specimen(path, spec):
+    """Cache of a specimen this script built (its own file, read back below)."""
+    import pickle
+    with open(path, "wb") as f:
+        pickle.dump(spec, f)
+
+
+def load_specimen(path):
+    import pickle
+    with open(path, "rb") as f:
+        return pickle.load(f)
+
+
 def cpu_baseline(pkg, spm, spec, budget_s):
     """The oracle (C restatement of the reference path, one thread) stepping the
     same specimen from its initial state until ~budget_s seconds have passed."""
@@ -79,16 +104,27 @@ def main():
     ap.add_argument("--batch", type=int, default=64, help="steps queued per host sync")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--specimen", default=None,
+                    help="npz file: load the specimen from it if present, else build and save it")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device to device 10 times (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the engine holds one subdomain on one device "
               "(multi-GPU runs are not part of this build)", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
 
     pkg = load_pkg()
     spm = importlib.import_module("esys_particle_amd.specimens")
     t0 = time.perf_counter()
-    spec = spm.dense_bonded_compression(n=args.particles, seed=3)
+    spec = load_specimen(args.specimen) if args.specimen and os.path.exists(args.specimen) else None
+    if spec is None:
+        spec = spm.dense_bonded_compression(n=args.particles, seed=3)
+        if args.specimen:
+            save_specimen(args.specimen, spec)
     log("specimen: %d particles, %d bonds, box %s (%.1f s)" % (
         len(spec["ids"]), len(spec["bonds"]), tuple(round(b, 2) for b in spec["box"]),
         time.perf_counter() - t0))
@@ -121,7 +157,7 @@ def main():
 
     nb = 0.5 * (s1["num_bonds"] + s2["num_bonds"])
     nd = 0.5 * (s1["num_dynamic"] + s2["num_dynamic"])
-    per_launch = {"bond": nb * B_BOND, "friction": nd * B_FRICTION,
+    per_launch = {"forces": nb * B_BOND + nd * B_FRICTION,
                   "gather_integrate": n * B_PARTICLE_ROT}
     dom = max((k for k in per_launch if kt[k][1] > 0), key=lambda k: kt[k][0])
     avg_s = kt[dom][0] / kt[dom][1] * 1e-3

@@ -5,7 +5,7 @@ integration) is hand-written HIP in ``csrc/`` compiled into ``libesp.so``; this
 package exposes it through the reference's scripting interface (``lsm.py``).
 """
 from .lsm import (  # noqa: F401
-    BoundingBox, BrittleBeamPrms, GravityPrms, LsmMpi, NRotElasticPrms,
+    BoundingBox, BrittleBeamPrms, BrittleBeamSCPrms, GravityPrms, LsmMpi, NRotElasticPrms,
     NRotElasticWallPrms, NRotSphere, RotBondPrms, RotElasticPrms, RotFrictionPrms,
     RotSphere, Runnable, TaggedIdConnection, Vec3,
 )

@@ -50,6 +50,15 @@ class EspRotBondPrms(ctypes.Structure):
     ]
 
 
+class EspBrittleBeamSCPrms(ctypes.Structure):
+    _fields_ = [
+        ("kr", ctypes.c_double), ("ks", ctypes.c_double),
+        ("kt", ctypes.c_double), ("kb", ctypes.c_double),
+        ("cohesion", ctypes.c_double), ("friction_angle", ctypes.c_double),
+        ("tension_cutoff", ctypes.c_double), ("tag", ctypes.c_int),
+    ]
+
+
 class EspRotFrictionPrms(ctypes.Structure):
     _fields_ = [
         ("k_n", ctypes.c_double), ("mu_d", ctypes.c_double),
@@ -87,6 +96,7 @@ SIGNATURES = {
     "set_particle_non_rotational": (_i, [_vp, _i]),
     "add_connections": (_i, [_vp, _i, _pi, _pi, _pi]),
     "add_rot_bond_group": (_i, [_vp, _s, ctypes.POINTER(EspRotBondPrms), _pd, _i]),
+    "add_brittle_beam_sc_group": (_i, [_vp, _s, ctypes.POINTER(EspBrittleBeamSCPrms)]),
     "add_nrot_elastic_group": (_i, [_vp, _s, _d, _i]),
     "add_rot_elastic_group": (_i, [_vp, _s, _d, _i]),
     "add_rot_friction_group": (_i, [_vp, _s, ctypes.POINTER(EspRotFrictionPrms)]),
@@ -108,13 +118,14 @@ SIGNATURES = {
     "get_interactions": (_i, [_vp, _s, _pi, _pi]),
     "get_friction_state": (_i, [_vp, _s, _pd, _pi]),
     "get_neighbour_pairs": (_i, [_vp, _pi, _pi]),
+    "get_particle_stress": (_i, [_vp, _pd]),
     "get_bond_breaks": (_i, [_vp, _pi, _pi, _pi]),
     "get_rebuild_steps": (_i, [_vp, _pi]),
     "set_kernel_timing": (_i, [_vp, _i]),
     "get_kernel_times": (_i, [_vp, _pd, ctypes.POINTER(ctypes.c_longlong)]),
 }
 
-KERNEL_CLASSES = ("bond", "friction", "elastic", "gather_integrate", "update")
+KERNEL_CLASSES = ("forces", "gather_integrate", "update")
 
 
 class NativeLib:
@@ -245,6 +256,20 @@ class Engine:
             n = len(pb)
         self._check(self.lib.add_rot_bond_group(self.h, _b(name), ctypes.byref(c), _dp(pb), n),
                     "createInteractionGroup(RotBondPrms)")
+
+    def add_brittle_beam_sc_group(self, name, prms):
+        c = EspBrittleBeamSCPrms()
+        for f, _ in EspBrittleBeamSCPrms._fields_:
+            setattr(c, f, prms[f])
+        self._check(self.lib.add_brittle_beam_sc_group(self.h, _b(name), ctypes.byref(c)),
+                    "createInteractionGroup(BrittleBeamSCPrms)")
+
+    def particle_stress(self):
+        """Unscaled stress sums of the last step, (n, 3, 3), ascending id order."""
+        n = self.lib.get_num_particles(self.h)
+        out = np.zeros(9 * max(n, 1))
+        self._check(self.lib.get_particle_stress(self.h, _dp(out)), "getParticleStress")
+        return out[: 9 * n].reshape(n, 3, 3)
 
     def add_nrot_elastic_group(self, name, k, scaling=True):
         self._check(self.lib.add_nrot_elastic_group(self.h, _b(name), float(k), int(scaling)),

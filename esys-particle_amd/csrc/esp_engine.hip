@@ -109,6 +109,10 @@ struct BondGroup {
   std::vector<int> hid1, hid2;  // ids in storage order (sorted by particle slots)
   std::vector<double4> h_k4, h_mx4, h_D4;  // staged until the first step
   int rec_off = 0, bond_off = 0;
+  // BrittleBeamSC: broken by the particle stresses (tension cutoff, cohesion,
+  // tan of the friction angle) instead of the bond forces
+  int sc = 0;
+  double sc_sigma3 = 0, sc_cohesion = 0, sc_tan = 0;
 };
 
 struct Wall {
@@ -174,11 +178,15 @@ struct esp_engine {
   DBuf<double4> tmp_ff, tmp_a, tmp_b, tmp_c;
   DBuf<int> inc_cnt, inc_off, inc_fill, inc_ent, tmp_rank;
   DBuf<unsigned long long> inc_ent64;
-  DBuf<int> xslot, xcnt, flist;
+  DBuf<int> xcnt, flist;
   DBuf<int4> evt;
   DBuf<UpdState> upd;
   int nrec = 0;
-  DBuf<double> contrib;
+  DBuf<double> slots;     // per-particle slots: CSR runs, then ESP_XSLOTS appendix slots each
+  DBuf<int2> slot_of;     // record -> its two slots
+  int ss = 6;             // doubles per slot (9 when particle stresses are tracked)
+  int xbase = 0;          // first appendix slot
+  DBuf<double> sig;       // stress sums [particle][9] when tracked
   DBuf<StepStatus> ring;
   int ring_size = 0;
   DBuf<int> halted, broken_list, dev_int;
@@ -211,8 +219,8 @@ struct esp_engine {
   std::vector<TimedLaunch> tl;
   std::vector<hipEvent_t> ev_pool;
   size_t ev_used = 0;
-  double k_ms[ESP_KCLASS_COUNT] = {0, 0, 0, 0, 0};
-  long long k_n[ESP_KCLASS_COUNT] = {0, 0, 0, 0, 0};
+  double k_ms[ESP_KCLASS_COUNT] = {};
+  long long k_n[ESP_KCLASS_COUNT] = {};
 
   PView view()
   {
@@ -471,6 +479,8 @@ void spatial_order(esp_engine* e)
 void upload(esp_engine* e)
 {
   spatial_order(e);
+  for (auto& G : e->bg)
+    if (G.sc) e->ss = 9;
   const int n = e->n;
   const size_t cap = std::max(n, 1);
   e->pos.reserve(cap); e->vel.reserve(cap); e->old.reserve(cap); e->frc.reserve(cap);
@@ -571,13 +581,16 @@ void upload(esp_engine* e)
   e->inc_cnt.reserve(cap + 1);
   e->inc_off.reserve(cap + 1);
   e->inc_fill.reserve(cap + 1);
-  e->xslot.reserve(cap * ESP_XSLOTS);
   e->xcnt.reserve(cap);
   e->flist.reserve(cap);
   zero(e, e->xcnt.p, cap);
   const int nb = std::max(gather_blocks(n), 1);
   const int nwg = (int)e->wg.size();
   e->wall_part.reserve((size_t)(e->K + 1) * nb * std::max(nwg, 1) * 3);
+  if (e->ss == 9) {
+    e->sig.reserve((size_t)std::max(n, 1) * 9);
+    zero(e, e->sig.p, (size_t)std::max(n, 1) * 9);
+  }
   e->wall_hist.reserve((size_t)(e->K + 1) * std::max(nwg, 1) * 3);
   sync(e);
   e->started = true;
@@ -850,7 +863,7 @@ void rebuild_csr(esp_engine* e)
   for (int gi : e->dord)
     if (e->dg[gi].n >= (1 << 27)) throw EspError(ESP_ERR_CAPACITY, "more than 2^27 pairs in one group");
   e->nrec = (int)rec;
-  e->contrib.reserve((size_t)std::max<long long>(rec, 1) * 9);
+  e->slot_of.reserve((size_t)std::max<long long>(rec, 1));
   zero(e, e->inc_cnt.p, e->n + 1);
   for (int gi : e->dord) launch_inc_count(e->stream, e->dg[gi].n, e->dg[gi].ij.p, e->inc_cnt.p);
   for (int gi : e->bord) launch_inc_count(e->stream, e->bg[gi].n, e->bg[gi].ij.p, e->inc_cnt.p);
@@ -872,9 +885,13 @@ void rebuild_csr(esp_engine* e)
                     e->inc_fill.p, e->inc_ent64.p);
   }
   launch_inc_sort(e->stream, e->n, e->inc_off.p, e->inc_ent64.p, e->inc_ent.p);
-  // records start at zero: a contact that is not touching keeps a zero record
-  // without rewriting it every step
-  zero(e, e->contrib.p, (size_t)std::max<long long>(rec, 1) * 9);
+  launch_slot_of(e->stream, total, e->inc_ent.p, e->slot_of.p);
+  // slots start at zero: a contact that is not touching keeps zero slots
+  // without rewriting them every step
+  e->xbase = total;
+  const size_t nslot = (size_t)total + (size_t)std::max(e->n, 1) * ESP_XSLOTS;
+  e->slots.reserve(nslot * e->ss);
+  zero(e, e->slots.p, nslot * e->ss);
   zero(e, e->xcnt.p, std::max(e->n, 1));
   HIPCHK(hipGetLastError());
 }
@@ -883,6 +900,9 @@ BondSet bond_set(esp_engine* e)
 {
   BondSet bs;
   memset(&bs, 0, sizeof bs);
+  bs.ss = e->ss;
+  bs.slots = e->slots.p;
+  bs.slot_of = e->slot_of.p;
   for (int gi : e->bord) {
     BondGroup& G = e->bg[gi];
     const int k = bs.count++;
@@ -898,6 +918,84 @@ BondSet bond_set(esp_engine* e)
 // ---------------------------------------------------------------------------
 // one step's kernels
 // ---------------------------------------------------------------------------
+// segments of the fused force-law launch: bonded groups, then dynamic groups
+ForceLaunch force_launch(esp_engine* e)
+{
+  ForceLaunch fl;
+  memset(&fl, 0, sizeof fl);
+  fl.do2d = e->do2d;
+  fl.ss = e->ss;
+  fl.dt = e->dt;
+  fl.slots = e->slots.p;
+  fl.slot_of = e->slot_of.p;
+  fl.broken_list = e->broken_list.p;
+  int blk = 0;
+  for (int gi : e->bord) {
+    BondGroup& G = e->bg[gi];
+    if (G.n <= 0) continue;
+    ForceSeg& s = fl.seg[fl.count++];
+    s.kind = ESP_SEG_ROT_BOND;
+    s.blk0 = blk;
+    s.n = G.n;
+    s.rec_off = G.rec_off;
+    s.bond_off = G.bond_off;
+    s.ij = G.ij.p;
+    s.k4 = G.k4.p;
+    s.mx4 = G.mx4.p;
+    s.D4 = G.D4.p;
+    s.truncated = G.prm.truncated;
+    s.beta1 = G.prm.beta1;
+    s.beta2 = G.prm.beta2;
+    s.sc = G.sc;
+    blk += force_blocks(G.n);
+  }
+  for (int gi : e->dord) {
+    DynGroup& G = e->dg[gi];
+    ForceSeg& s = fl.seg[fl.count++];
+    s.kind = G.kind == 1 ? ESP_SEG_NROT_ELASTIC : G.kind == 2 ? ESP_SEG_ROT_ELASTIC
+                                                              : ESP_SEG_ROT_FRICTION;
+    s.blk0 = blk;
+    s.n = G.n;
+    s.rec_off = G.rec_off;
+    s.app_rec_off = G.app_rec_off;
+    s.n_app = G.napp.p;
+    s.ij = G.ij.p;
+    s.ff = G.kind == 3 ? G.ff.p : nullptr;
+    s.k = G.k;
+    s.ks = G.ks;
+    s.mu_d = G.mu_d;
+    s.mu_s = G.mu_s;
+    s.scaling = G.scaling;
+    s.meanR = G.meanR;
+    blk += force_blocks(G.n + ESP_APP_CAP);
+  }
+  fl.nblocks = blk;
+  return fl;
+}
+
+ScLaunch sc_launch(esp_engine* e)
+{
+  ScLaunch sl;
+  memset(&sl, 0, sizeof sl);
+  sl.sig = e->sig.p;
+  sl.broken_list = e->broken_list.p;
+  int blk = 0;
+  for (int gi : e->bord) {
+    BondGroup& G = e->bg[gi];
+    if (!G.sc || G.n <= 0) continue;
+    const int k = sl.count++;
+    sl.blk0[k] = blk;
+    sl.n[k] = G.n;
+    sl.bond_off[k] = G.bond_off;
+    sl.ij[k] = G.ij.p;
+    sl.sigma3[k] = G.sc_sigma3;
+    sl.cohesion[k] = G.sc_cohesion;
+    sl.tan_angle[k] = G.sc_tan;
+    blk += force_blocks(G.n);
+  }
+  sl.nblocks = blk;
+  return sl;
+}
 void launch_step(esp_engine* e, const StepCtl& ctl, long long step)
 {
   PView p = e->view();
@@ -917,42 +1015,25 @@ void launch_step(esp_engine* e, const StepCtl& ctl, long long step)
       da.is_hist = G.kind == 3;
       da.present = HashRef{G.hs.p, G.hs_mask};
       da.ex = excl_set(e, G);
+      da.slot_of = e->slot_of.p;
+      da.xbase = e->xbase;
     }
     LaunchTimer lt(e, ESP_KCLASS_UPDATE, ctl.step_id);
-    launch_incremental_update(e->stream, ctl, p, bond_set(e), e->broken_list.p, e->contrib.p,
-                              e->evt.p, step, e->grid, da, e->cell.p, e->cell_start.p,
-                              e->cell_items.p, e->upd.p, e->flist.p, e->xslot.p, e->xcnt.p,
-                              e->dg.empty() ? 0 : 1);
+    launch_incremental_update(e->stream, ctl, p, bond_set(e), e->broken_list.p, e->evt.p, step,
+                              e->grid, da, e->cell.p, e->cell_start.p, e->cell_items.p, e->upd.p,
+                              e->flist.p, e->xcnt.p, e->dg.empty() ? 0 : 1);
   }
-  for (int gi : e->dord) {
-    DynGroup& G = e->dg[gi];
-    LaunchTimer lt(e, G.kind == 3 ? ESP_KCLASS_FRICTION : ESP_KCLASS_ELASTIC, ctl.step_id);
-    if (G.kind == 1)
-      launch_nrot_elastic(e->stream, ctl, p, G.n, G.napp.p, G.ij.p, G.k, G.scaling, e->do2d,
-                          e->contrib.p, G.rec_off, G.app_rec_off);
-    else if (G.kind == 2)
-      launch_rot_elastic(e->stream, ctl, p, G.n, G.napp.p, G.ij.p, G.k, G.scaling, e->do2d,
-                         e->contrib.p, G.rec_off, G.app_rec_off);
-    else
-      launch_rot_friction(e->stream, ctl, p, G.n, G.napp.p, G.ij.p, G.ff.p, G.k, G.ks, G.scaling,
-                          G.meanR, e->do2d, G.mu_d, G.mu_s, e->dt, e->contrib.p, G.rec_off,
-                          G.app_rec_off);
-  }
-  for (int gi : e->bord) {
-    BondGroup& G = e->bg[gi];
-    LaunchTimer lt(e, ESP_KCLASS_BOND, ctl.step_id);
-    launch_rot_bond(e->stream, ctl, p, G.n, G.ij.p, G.k4.p, G.mx4.p, G.D4.p, G.prm.truncated,
-                    G.prm.beta1, G.prm.beta2, e->contrib.p, G.rec_off, G.bond_off,
-                    e->broken_list.p);
+  {
+    LaunchTimer lt(e, ESP_KCLASS_FORCES, ctl.step_id);
+    launch_forces(e->stream, ctl, p, force_launch(e));
   }
   GatherParams gp;
   memset(&gp, 0, sizeof gp);
   gp.inc_off = e->inc_off.p;
-  gp.inc_ent = e->inc_ent.p;
-  gp.xslot = e->xslot.p;
   gp.xcnt = e->xcnt.p;
+  gp.xbase = e->xbase;
   gp.grid = e->grid;
-  gp.contrib = e->contrib.p;
+  gp.slots = e->slots.p;
   gp.dt = e->dt;
   gp.rot = e->rot;
   gp.do2d = e->do2d;
@@ -962,10 +1043,12 @@ void launch_step(esp_engine* e, const StepCtl& ctl, long long step)
   for (size_t k = 0; k < e->gord.size(); k++)
     for (int c = 0; c < 3; c++) gp.g[k][c] = e->grav[e->gord[k]].g[c];
   gp.wall_part = e->wall_part.p + (size_t)(ctl.step_id - 1) * gather_blocks(e->n) * gp.nwg * 3;
+  gp.sig = e->ss == 9 ? e->sig.p : nullptr;
   {
     LaunchTimer lt(e, ESP_KCLASS_GATHER, ctl.step_id);
     launch_gather_integrate(e->stream, ctl, p, gp);
   }
+  if (e->ss == 9) launch_sc_break(e->stream, ctl, p, sc_launch(e));
   HIPCHK(hipGetLastError());
 }
 
@@ -1031,8 +1114,8 @@ void host_step(esp_engine* e, int reason)
   // device when the queue stopped in the incremental update)
   if (prev.nbroken > 0 && reason != 2 && !e->bg.empty()) {
     StepCtl c0 = make_ctl(e, step, 1, 0);
-    launch_break_apply(e->stream, c0, e->view(), bond_set(e), e->broken_list.p, e->contrib.p,
-                       e->upd.p, e->flist.p, e->evt.p, step);
+    launch_break_apply(e->stream, c0, e->view(), bond_set(e), e->broken_list.p, e->upd.p,
+                       e->flist.p, e->evt.p, step);
     HIPCHK(hipGetLastError());
   }
   fold(e);
@@ -1118,8 +1201,8 @@ void run_steps(esp_engine* e, int nsteps)
       throw EspError(ESP_ERR_CAPACITY, "more than 4 gravity groups");
     if (e->bg.size() > ESP_MAX_BOND_GROUPS)
       throw EspError(ESP_ERR_CAPACITY, "more than 8 bonded groups");
-    if (e->bg.size() + e->dg.size() > 32)
-      throw EspError(ESP_ERR_CAPACITY, "more than 32 interaction groups");
+    if (e->bg.size() + e->dg.size() > ESP_MAX_FORCE_SEGS)
+      throw EspError(ESP_ERR_CAPACITY, "more than 16 interaction groups");
     for (auto& G : e->dg)
       if (G.ex.size() > ESP_MAX_EXCL)
         throw EspError(ESP_ERR_CAPACITY, "more than 4 exclusions on one group");
@@ -1415,10 +1498,10 @@ int esp_add_connections(esp_engine* e, int n, const int* id1, const int* id2, co
   API_END
 }
 
-int esp_add_rot_bond_group(esp_engine* e, const char* name, const esp_rot_bond_prms* prm,
-                           const double* per_bond_max, int n_per_bond)
+namespace {
+BondGroup& add_bond_group(esp_engine* e, const char* name, const esp_rot_bond_prms* prm,
+                          const double* per_bond_max, int n_per_bond)
 {
-  API_BEGIN
   require_setup(e, "createInteractionGroup");
   if (!e->rot) throw EspError(ESP_ERR_STATE, "rotational bonds need particle type RotSphere");
   if (!name || !prm) throw EspError(ESP_ERR_ARG, "bad group");
@@ -1458,6 +1541,42 @@ int esp_add_rot_bond_group(esp_engine* e, const char* name, const esp_rot_bond_p
   }
   G.n = (int)G.hid1.size();
   e->bg.push_back(std::move(G));
+  return e->bg.back();
+}
+}  // namespace
+
+int esp_add_rot_bond_group(esp_engine* e, const char* name, const esp_rot_bond_prms* prm,
+                           const double* per_bond_max, int n_per_bond)
+{
+  API_BEGIN
+  add_bond_group(e, name, prm, per_bond_max, n_per_bond);
+  return ESP_OK;
+  API_END
+}
+
+int esp_add_brittle_beam_sc_group(esp_engine* e, const char* name,
+                                  const esp_brittle_beam_sc_prms* sc)
+{
+  API_BEGIN
+  if (!sc) throw EspError(ESP_ERR_ARG, "bad group");
+  // BrittleBeamSCIGP -> ARotBondedIGP(kr, ks, kt, kb, tag, scaling=true)
+  // (BrittleBeamSC.cpp:46-56); strengths do not apply
+  esp_rot_bond_prms prm;
+  memset(&prm, 0, sizeof prm);
+  prm.kr = sc->kr;
+  prm.ks = sc->ks;
+  prm.kt = sc->kt;
+  prm.kb = sc->kb;
+  prm.max_n = prm.max_s = prm.max_t = prm.max_b = 1.0;
+  prm.tag = sc->tag;
+  prm.scaling = 1;
+  prm.meanR_scaling = 1;
+  prm.truncated = prm.beta1 = prm.beta2 = 1.0;
+  BondGroup& G = add_bond_group(e, name, &prm, nullptr, 0);
+  G.sc = 1;
+  G.sc_sigma3 = sc->tension_cutoff;
+  G.sc_cohesion = sc->cohesion;
+  G.sc_tan = std::tan(sc->friction_angle / 57.2957795);  // BrittleBeamSC.cpp:78
   return ESP_OK;
   API_END
 }
@@ -1776,6 +1895,23 @@ int esp_get_friction_state(esp_engine* e, const char* name, double* ffric, int* 
     flg[i] = (int)f[i].w;
   }
   return (int)f.size();
+  API_END
+}
+
+int esp_get_particle_stress(esp_engine* e, double* out)
+{
+  API_BEGIN
+  if (!e->started || e->ss != 9)
+    throw EspError(ESP_ERR_STATE, "particle stresses are tracked only with a BrittleBeamSC group, "
+                                  "after the first time step");
+  HIPCHK(hipSetDevice(e->cfg.device));
+  std::vector<double> s((size_t)std::max(e->n, 1) * 9);
+  d2h(e, s.data(), e->sig.p, (size_t)e->n * 9);
+  sync(e);
+  std::vector<int> order = id_order(e);
+  for (int k = 0; k < e->n; k++)
+    for (int c = 0; c < 9; c++) out[9 * k + c] = s[9 * (size_t)order[k] + c];
+  return ESP_OK;
   API_END
 }
 

@@ -11,9 +11,13 @@
 //   flags[i]  = dynamic | rotational<<1 | bond-broken flag<<2 |
 //               wall-group membership<<8 (8 bits) | wall-group inner<<16 (8 bits)
 // Interactions are structure-of-arrays per group (pair slots int2, history
-// double4); every interaction writes one 9-double record (F1, M1, M2) per step
-// into a shared buffer, and a per-particle CSR of record references sums them
-// in a fixed order - the deterministic, atomic-free force reduction.
+// double4).  Every interaction has a record index; at each rebuild a
+// per-particle CSR orders a particle's interactions for summation and slot_of
+// maps each record to its two CSR positions.  Per step an interaction writes
+// each particle's share (force, moment: 6 doubles; with particle stresses
+// tracked also the contact point relative to the particle: 9 doubles) into that
+// particle's slot, and each particle sums its contiguous run of slots in order -
+// the deterministic, atomic-free force reduction.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -74,12 +78,11 @@ struct WallParam {
 };
 
 struct GatherParams {
-  const int* inc_off;
-  const int* inc_ent;
-  const int* xslot;  // ESP_XSLOTS appended-record entries per particle
-  const int* xcnt;
-  GridParams grid;   // wall candidates are refreshed after bond breaks
-  const double* contrib;
+  const int* inc_off;  // CSR runs of slots
+  const int* xcnt;     // appendix slots in use per particle (ESP_XSLOTS each, from xbase)
+  int xbase;
+  GridParams grid;     // wall candidates are refreshed after bond breaks
+  const double* slots;
   double dt;
   int rot, do2d;
   int nwg;
@@ -87,6 +90,7 @@ struct GatherParams {
   int ngrav;
   double g[ESP_MAX_GRAVITY][3];
   double* wall_part;  // this step's slot: [block][nwg][3]
+  double* sig;        // stress sums [particle][9] (slots of 9 doubles only)
 };
 
 // Open-addressing hash sets of pair keys (a << 32 | b): live bonds of a bonded
@@ -109,6 +113,9 @@ struct ExclSet {
 // bonded groups as the break-application kernel sees them
 struct BondSet {
   int count;
+  int ss;               // doubles per slot
+  double* slots;
+  const int2* slot_of;  // record -> its two slots
   int off[ESP_MAX_BOND_GROUPS];  // global bond index of the group's first bond
   int n[ESP_MAX_BOND_GROUPS];
   int rec_off[ESP_MAX_BOND_GROUPS];
@@ -131,6 +138,8 @@ struct UpdState {
 #define ESP_NEW_CAP 4096
 #define ESP_XSLOTS 4
 struct DynAppend {
+  int2* slot_of;     // record -> slots; appended records get appendix slots
+  int xbase;         // first appendix slot
   int2* ij;          // whole list: main [0, n_main), appendix after it
   long long* key;
   double4* ff;       // shear history (history groups) or nullptr
@@ -144,31 +153,64 @@ struct DynAppend {
 };
 
 // ---- per-step kernels ----
-void launch_rot_bond(hipStream_t s, StepCtl ctl, PView p, int nb, const int2* ij, const double4* k4,
-                     const double4* mx4, const double4* D4, double truncated, double beta1,
-                     double beta2, double* contrib, int rec_off, int bond_off, int* broken_list);
-// nc = main-list size; appended pairs (device count) follow it
-void launch_rot_friction(hipStream_t s, StepCtl ctl, PView p, int nc, const int* n_app,
-                         const int2* ij, double4* ff, double k, double ks, int scaling, int meanR,
-                         int do2d, double mu_d, double mu_s, double dt, double* contrib,
-                         int rec_off, int app_rec_off);
-void launch_nrot_elastic(hipStream_t s, StepCtl ctl, PView p, int nc, const int* n_app,
-                         const int2* ij, double k, int scaling, int do2d, double* contrib,
-                         int rec_off, int app_rec_off);
-void launch_rot_elastic(hipStream_t s, StepCtl ctl, PView p, int nc, const int* n_app,
-                        const int2* ij, double k, int scaling, int do2d, double* contrib,
-                        int rec_off, int app_rec_off);
+// One block segment per interaction group in the fused force-law launch.
+#define ESP_SEG_NROT_ELASTIC 1
+#define ESP_SEG_ROT_ELASTIC 2
+#define ESP_SEG_ROT_FRICTION 3
+#define ESP_SEG_ROT_BOND 4
+#define ESP_MAX_FORCE_SEGS 16
+struct ForceSeg {
+  int kind;
+  int blk0;         // first block of the segment
+  int n;            // bonds, or pairs of a dynamic group's main list
+  int rec_off;      // records of the bonds / of the main list
+  int app_rec_off;  // records of the appendix (dynamic groups)
+  int bond_off;     // global index of the first bond (break list entries)
+  int scaling, meanR;
+  int sc;            // bonded group broken by the stress criterion (no force test)
+  const int* n_app;  // appended pairs, device count (dynamic groups)
+  const int2* ij;
+  double4* ff;       // shear history (RotFriction)
+  const double4* k4;
+  const double4* mx4;
+  const double4* D4;
+  double k, ks, mu_d, mu_s;
+  double truncated, beta1, beta2;
+};
+struct ForceLaunch {
+  int count, nblocks, do2d;
+  int ss;  // doubles per slot (6, or 9 with the contact point)
+  double dt;
+  double* slots;
+  const int2* slot_of;
+  int* broken_list;
+  ForceSeg seg[ESP_MAX_FORCE_SEGS];
+};
+// blocks a segment of n interactions takes (dynamic groups: n + ESP_APP_CAP)
+int force_blocks(int n);
+void launch_forces(hipStream_t s, StepCtl ctl, PView p, const ForceLaunch& fl);
+// BrittleBeamSC groups: the stress test after the step's integration, one
+// segment of blocks per group
+struct ScLaunch {
+  int count, nblocks;
+  int blk0[ESP_MAX_BOND_GROUPS], n[ESP_MAX_BOND_GROUPS], bond_off[ESP_MAX_BOND_GROUPS];
+  const int2* ij[ESP_MAX_BOND_GROUPS];
+  double sigma3[ESP_MAX_BOND_GROUPS], cohesion[ESP_MAX_BOND_GROUPS], tan_angle[ESP_MAX_BOND_GROUPS];
+  const double* sig;
+  int* broken_list;
+};
+void launch_sc_break(hipStream_t s, StepCtl ctl, PView p, const ScLaunch& sl);
 // PIS_EB::update on the device: bonds of the previous step's break list die,
 // their particles are flagged, events are logged
 void launch_break_apply(hipStream_t s, StepCtl ctl, PView p, BondSet bs, const int* broken_list,
-                        double* contrib, UpdState* st, int* flist, int4* evt, long long step);
+                        UpdState* st, int* flist, int4* evt, long long step);
 // queued steps: the same break application, then getNewList + PIS_NE/PIS_ED
 // incremental update of the first dynamic group (has_group), in one launch
 void launch_incremental_update(hipStream_t s, StepCtl ctl, PView p, BondSet bs,
-                               const int* broken_list, double* contrib, int4* evt, long long step,
-                               GridParams g, DynAppend da, const int* cell, const int* cell_start,
-                               const int* cell_items, UpdState* st, int* flist, int* xslot,
-                               int* xcnt, int has_group);
+                               const int* broken_list, int4* evt, long long step, GridParams g,
+                               DynAppend da, const int* cell, const int* cell_start,
+                               const int* cell_items, UpdState* st, int* flist, int* xcnt,
+                               int has_group);
 void launch_hash_clear(hipStream_t s, HashRef h);
 void launch_hash_insert(hipStream_t s, HashRef h, const long long* keys, int n);
 void launch_bond_keys(hipStream_t s, int nb, const int2* ij, const int* id, long long* keys);
@@ -221,6 +263,8 @@ void launch_inc_fill(hipStream_t s, int nc, const int2* ij, int rec_off, int has
                      int grp, const int* inc_off, int* inc_fill, unsigned long long* ent64);
 void launch_inc_sort(hipStream_t s, int n, const int* inc_off, unsigned long long* ent64,
                      int* inc_ent);
+// slot_of[record] = (CSR position on its first particle, on its second)
+void launch_slot_of(hipStream_t s, int total, const int* inc_ent, int2* slot_of);
 
 // exclusive scan of n ints into out (out[n] = total); tmp sized by scan_tmp_size
 size_t scan_tmp_size(int n);

@@ -1,8 +1,10 @@
 // esp_kernels.hip - HIP kernels of the per-timestep DEM engine (gfx950, FP64).
 //
 // Per step (launched back to back on one stream, no host synchronisation):
-//   k_rot_bond / k_rot_friction / k_nrot_elastic / k_rot_elastic
-//       one thread per interaction; each writes its force record (F1, M1, M2)
+//   k_forces
+//       the force laws of every interaction group in one launch (one block
+//       segment per group), one thread per interaction; each writes its force
+//       record (F1, M1, M2)
 //   k_gather_integrate
 //       one thread per particle: wall and body forces, then the records of its
 //       interactions in a fixed order (CSR), then the CParticle/CRotParticle
@@ -32,8 +34,10 @@ __device__ __forceinline__ dv3 xyz(double4 a) { return mk3(a.x, a.y, a.z); }
 __device__ __forceinline__ bool step_skip(const StepCtl& c)
 {
   if (!c.guarded) return false;
-  if (*(volatile int*)c.halted) return true;
-  if (c.prev->maxdisp2 > c.thresh) {  // neighbour table must be rebuilt: host bookkeeping
+  const int halted = *(volatile int*)c.halted;
+  const unsigned long long md = c.prev->maxdisp2;  // both loads in flight together
+  if (halted) return true;
+  if (md > c.thresh) {  // neighbour table must be rebuilt: host bookkeeping
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       *c.reason = 1;
       *c.halted = c.step_id;
@@ -97,33 +101,60 @@ __device__ __forceinline__ bool excluded(const ExclSet& ex, long long key)
 // ---------------------------------------------------------------------------
 // per-step interaction kernels
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(BS) void k_rot_bond(StepCtl ctl, PView p, int nb, const int2* __restrict__ ij,
-                                                 const double4* __restrict__ k4,
-                                                 const double4* __restrict__ mx4,
-                                                 const double4* __restrict__ D4, double truncated,
-                                                 double beta1, double beta2, double* contrib,
-                                                 int rec_off, int bond_off, int* broken_list)
+// Each interaction writes its result per side straight into the two
+// particles' CSR slots: force and moment on that particle (and, with stresses
+// tracked, the contact point relative to it).  Slot positions are fixed at each
+// rebuild (slot_of), so a particle later reads one contiguous run.
+__device__ __forceinline__ void put3(double* d, dv3 v)
 {
-  if (step_skip(ctl)) return;
-  const int b = blockIdx.x * BS + threadIdx.x;
+  d[0] = v.x;
+  d[1] = v.y;
+  d[2] = v.z;
+}
+__device__ __forceinline__ void zero_slot(double* d, int ss)
+{
+  for (int k = 0; k < ss; k++) d[k] = 0.0;
+}
+// contact point p2 + r2 / (r1 + r2) (p1 - p2) of the pair laws
+// (ARotBondedInteraction.cpp:437-440, RotFricInteraction.cpp:185, ElasticInteraction.cpp:61),
+// stored relative to each particle
+__device__ __forceinline__ void put_rel(double* a, double* b, double4 P1, double4 P2)
+{
+  const dv3 D = sub3(xyz(P1), xyz(P2));
+  const dv3 c = add3(xyz(P2), mul3(D, P2.w / (P1.w + P2.w)));
+  put3(a + 6, sub3(c, xyz(P1)));
+  put3(b + 6, sub3(c, xyz(P2)));
+}
+
+// ARotBondedInteraction::calcForces + CRotBondedInteraction::broken
+// (ARotBondedInteraction.cpp:307-448, RotBondedInteraction.cpp:175-219)
+__device__ __forceinline__ void rot_bond_item(const StepCtl& ctl, const PView& p, int b,
+                                              const ForceSeg& g, const ForceLaunch& fl)
+{
+  const int nb = g.n;
+  const int2* __restrict__ ij = g.ij;
   if (b >= nb) return;
   const int2 e = ij[b];
-  if (e.x < 0) return;  // broken earlier: removed from the group, record left at zero
+  const int2 so = fl.slot_of[g.rec_off + b];
+  if (e.x < 0) return;  // broken earlier: removed from the group, slots left at zero
   const double4 P1 = p.pos[e.x], P2 = p.pos[e.y];
   const double4 Q1 = p.quat[e.x], Q2 = p.quat[e.y];
-  const double4 K = k4[b], D = D4[b];
+  const double4 K = g.k4[b], D = g.D4[b], M = g.mx4[b];
   BondResult o;
   esp_rot_bond_law(xyz(P1), dquat{Q1.x, mk3(Q1.y, Q1.z, Q1.w)}, P1.w, xyz(P2),
                    dquat{Q2.x, mk3(Q2.y, Q2.z, Q2.w)}, P2.w, K.x, K.y, K.z, K.w, xyz(D), o);
-  double* r = contrib + 9 * (size_t)(rec_off + b);
-  r[0] = o.f1.x; r[1] = o.f1.y; r[2] = o.f1.z;
-  r[3] = o.m1.x; r[4] = o.m1.y; r[5] = o.m1.z;
-  r[6] = o.m2.x; r[7] = o.m2.y; r[8] = o.m2.z;
-  const double4 M = mx4[b];
-  if (esp_bond_broken(o.nForce, o.shForce, o.tMoment, o.bMoment, M.x, M.y, M.z, M.w, truncated,
-                      beta1, beta2)) {
+  double* s0 = fl.slots + (size_t)fl.ss * so.x;
+  double* s1 = fl.slots + (size_t)fl.ss * so.y;
+  put3(s0, o.f1);
+  put3(s0 + 3, o.m1);
+  put3(s1, neg3(o.f1));
+  put3(s1 + 3, o.m2);
+  if (fl.ss == 9) put_rel(s0, s1, P1, P2);
+  if (g.sc) return;  // broken by the particle stresses after the integration
+  if (esp_bond_broken(o.nForce, o.shForce, o.tMoment, o.bMoment, M.x, M.y, M.z, M.w, g.truncated,
+                      g.beta1, g.beta2)) {
     const int slot = atomicAdd(&ctl.cur->nbroken, 1);
-    broken_list[slot] = bond_off + b;
+    fl.broken_list[slot] = g.bond_off + b;
   }
 }
 
@@ -132,91 +163,117 @@ __device__ __forceinline__ int record_of(int c, int nc, int rec_off, int app_rec
   return c < nc ? rec_off + c : app_rec_off + (c - nc);
 }
 
-__global__ __launch_bounds__(BS) void k_rot_friction(StepCtl ctl, PView p, int nc, const int* n_app,
-                                                     const int2* __restrict__ ij,
-                                                     double4* __restrict__ ff, double k, double ks,
-                                                     int scaling, int meanR, int do2d, double mu_d,
-                                                     double mu_s, double dt, double* contrib,
-                                                     int rec_off, int app_rec_off)
+// CRotFrictionInteraction::calcForces (RotFricInteraction.cpp:103-231)
+__device__ __forceinline__ void rot_friction_item(const PView& p, int c, const ForceSeg& g,
+                                                  const ForceLaunch& fl)
 {
-  if (step_skip(ctl)) return;
-  const int c = blockIdx.x * BS + threadIdx.x;
-  if (c >= nc + *n_app) return;
-  const int2 e = ij[c];
+  const int nc = g.n;
+  double4* __restrict__ ff = g.ff;
+  if (c >= nc + *g.n_app) return;
+  const int2 e = g.ij[c];
+  const int2 so = fl.slot_of[record_of(c, nc, g.rec_off, g.app_rec_off)];
   const double4 P1 = p.pos[e.x], P2 = p.pos[e.y];
-  double* r = contrib + 9 * (size_t)record_of(c, nc, rec_off, app_rec_off);
+  const double4 st = ff[c];
+  double* s0 = fl.slots + (size_t)fl.ss * so.x;
+  double* s1 = fl.slots + (size_t)fl.ss * so.y;
   const dv3 Dq = sub3(xyz(P1), xyz(P2));
   const double eq = P1.w + P2.w;
-  const double4 st = ff[c];
   if (!(dot3(Dq, Dq) < (eq * eq))) {
     // not touching: history cleared, no force.  A pair that did not touch in
-    // the last step already holds a zero history and a zero record.
+    // the last step already holds a zero history and zero slots.
     if (st.w != 0.0 || st.x != 0.0 || st.y != 0.0 || st.z != 0.0) {
       ff[c] = make_double4(0.0, 0.0, 0.0, 0.0);
-      for (int t = 0; t < 9; t++) r[t] = 0.0;
+      zero_slot(s0, fl.ss);
+      zero_slot(s1, fl.ss);
     }
     return;
   }
   const double4 V1 = p.vel[e.x], V2 = p.vel[e.y], W1 = p.ang[e.x], W2 = p.ang[e.y];
   double kk, kks;
-  esp_friction_k(k, ks, scaling, meanR, P1.w, P2.w, do2d, kk, kks);
+  esp_friction_k(g.k, g.ks, g.scaling, g.meanR, P1.w, P2.w, fl.do2d, kk, kks);
   dv3 ffr = mk3(st.x, st.y, st.z);
   bool slipping = (((int)st.w) & 1) != 0;
   dv3 f1, m1, m2;
   esp_rot_friction_law(xyz(P1), xyz(V1), xyz(W1), P1.w, xyz(P2), xyz(V2), xyz(W2), P2.w, kk, kks,
-                       mu_d, mu_s, dt, ffr, slipping, f1, m1, m2);
+                       g.mu_d, g.mu_s, fl.dt, ffr, slipping, f1, m1, m2);
   ff[c] = make_double4(ffr.x, ffr.y, ffr.z, (double)((slipping ? 1 : 0) | 2));
-  r[0] = f1.x; r[1] = f1.y; r[2] = f1.z;
-  r[3] = m1.x; r[4] = m1.y; r[5] = m1.z;
-  r[6] = m2.x; r[7] = m2.y; r[8] = m2.z;
+  put3(s0, f1);
+  put3(s0 + 3, m1);
+  put3(s1, neg3(f1));
+  put3(s1 + 3, m2);
+  if (fl.ss == 9) put_rel(s0, s1, P1, P2);
 }
 
-// CElasticInteraction::calcForces (ElasticInteraction.cpp:54-71)
-__global__ __launch_bounds__(BS) void k_nrot_elastic(StepCtl ctl, PView p, int nc, const int* n_app,
-                                                     const int2* __restrict__ ij, double k,
-                                                     int scaling, int do2d, double* contrib,
-                                                     int rec_off, int app_rec_off)
+// CElasticInteraction::calcForces (ElasticInteraction.cpp:54-71); no moments
+// (their slot parts stay zero)
+__device__ __forceinline__ void nrot_elastic_item(const PView& p, int c, const ForceSeg& g,
+                                                  const ForceLaunch& fl)
 {
-  if (step_skip(ctl)) return;
-  const int c = blockIdx.x * BS + threadIdx.x;
-  if (c >= nc + *n_app) return;
-  const int2 e = ij[c];
+  const int nc = g.n;
+  if (c >= nc + *g.n_app) return;
+  const int2 e = g.ij[c];
+  const int2 so = fl.slot_of[record_of(c, nc, g.rec_off, g.app_rec_off)];
   const double4 P1 = p.pos[e.x], P2 = p.pos[e.y];
-  double* r = contrib + 9 * (size_t)record_of(c, nc, rec_off, app_rec_off);
+  double* s0 = fl.slots + (size_t)fl.ss * so.x;
+  double* s1 = fl.slots + (size_t)fl.ss * so.y;
   const dv3 D = sub3(xyz(P1), xyz(P2));
   double dist = dot3(D, D);
   const double eq_dist = P1.w + P2.w;
   if (dist < (eq_dist * eq_dist)) {
-    const double kk = esp_elastic_k(k, scaling, P1.w, P2.w, do2d);
+    const double kk = esp_elastic_k(g.k, g.scaling, P1.w, P2.w, fl.do2d);
     dist = sqrt(dist);
     const dv3 force = mul3(D, kk * (dist - eq_dist) / dist);
-    r[0] = -force.x; r[1] = -force.y; r[2] = -force.z;
+    put3(s0, neg3(force));
+    put3(s1, force);
+    if (fl.ss == 9) put_rel(s0, s1, P1, P2);
   } else {
-    r[0] = 0.0; r[1] = 0.0; r[2] = 0.0;
+    put3(s0, mk3(0.0, 0.0, 0.0));
+    put3(s1, mk3(0.0, 0.0, 0.0));
   }
 }
 
 // CRotElasticInteraction::calcForces (RotElasticInteraction.cpp:69-88)
-__global__ __launch_bounds__(BS) void k_rot_elastic(StepCtl ctl, PView p, int nc, const int* n_app,
-                                                    const int2* __restrict__ ij, double k,
-                                                    int scaling, int do2d, double* contrib,
-                                                    int rec_off, int app_rec_off)
+__device__ __forceinline__ void rot_elastic_item(const PView& p, int c, const ForceSeg& g,
+                                                 const ForceLaunch& fl)
 {
-  if (step_skip(ctl)) return;
-  const int c = blockIdx.x * BS + threadIdx.x;
-  if (c >= nc + *n_app) return;
-  const int2 e = ij[c];
+  const int nc = g.n;
+  if (c >= nc + *g.n_app) return;
+  const int2 e = g.ij[c];
+  const int2 so = fl.slot_of[record_of(c, nc, g.rec_off, g.app_rec_off)];
   const double4 P1 = p.pos[e.x], P2 = p.pos[e.y];
-  double* r = contrib + 9 * (size_t)record_of(c, nc, rec_off, app_rec_off);
+  double* s0 = fl.slots + (size_t)fl.ss * so.x;
+  double* s1 = fl.slots + (size_t)fl.ss * so.y;
   const dv3 rb = sub3(xyz(P1), xyz(P2));
   const double rbNorm = norm3(rb);
   const double r_0Norm = P1.w + P2.w;
   if (rbNorm < r_0Norm) {
     const dv3 delta_r = div3(mul3(rb, rbNorm - r_0Norm), rbNorm);
-    const dv3 force = mul3(delta_r, esp_rotelastic_k(k, scaling, P1.w, P2.w, do2d));
-    r[0] = -force.x; r[1] = -force.y; r[2] = -force.z;
+    const dv3 force = mul3(delta_r, esp_rotelastic_k(g.k, g.scaling, P1.w, P2.w, fl.do2d));
+    put3(s0, neg3(force));
+    put3(s1, force);
+    if (fl.ss == 9) put_rel(s0, s1, P1, P2);
   } else {
-    r[0] = 0.0; r[1] = 0.0; r[2] = 0.0;
+    put3(s0, mk3(0.0, 0.0, 0.0));
+    put3(s1, mk3(0.0, 0.0, 0.0));
+  }
+}
+
+// The force laws of every interaction group in one launch: each segment of
+// blocks covers one group (bonded groups first, their law is the longest), so
+// the groups' memory latencies overlap.  Every interaction writes its slots.
+__global__ __launch_bounds__(BS) void k_forces(StepCtl ctl, PView p, ForceLaunch fl)
+{
+  if (step_skip(ctl)) return;
+  int s = 0;
+  while (s + 1 < fl.count && (int)blockIdx.x >= fl.seg[s + 1].blk0) s++;
+  const ForceSeg& g = fl.seg[s];
+  const int c = ((int)blockIdx.x - g.blk0) * BS + (int)threadIdx.x;
+  switch (g.kind) {
+    case ESP_SEG_ROT_BOND: rot_bond_item(ctl, p, c, g, fl); break;
+    case ESP_SEG_ROT_FRICTION: rot_friction_item(p, c, g, fl); break;
+    case ESP_SEG_NROT_ELASTIC: nrot_elastic_item(p, c, g, fl); break;
+    case ESP_SEG_ROT_ELASTIC: rot_elastic_item(p, c, g, fl); break;
+    default: break;
   }
 }
 
@@ -301,90 +358,106 @@ __device__ __forceinline__ void block_sum3(double& a, double& b, double& c, doub
     }
 }
 
-#define GATHER_CHUNK 4
+// sigma(i,j) -= f_i rel_j (CParticle::applyForce, Particle.cpp:391-400)
+__device__ __forceinline__ void stress_add(double (&sg)[3][3], dv3 f, dv3 rel)
+{
+  const double fv[3] = {f.x, f.y, f.z}, rv[3] = {rel.x, rel.y, rel.z};
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) sg[i][j] -= fv[i] * rv[j];
+}
+
+// STRESS: records carry contact points and every applied force also updates
+// the particle's stress sum, in the order the forces are summed
+template <bool STRESS>
 __global__ __launch_bounds__(BS) void k_gather_integrate(StepCtl ctl, PView p, GatherParams gp)
 {
+  constexpr int GATHER_CHUNK = STRESS ? 2 : 4;
+  double sg[3][3];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) sg[i][j] = 0.0;
   if (step_skip(ctl)) return;
   __shared__ double lds3[3 * (BS / 64)];
   __shared__ unsigned long long mred[BS / 64];
   const int i = blockIdx.x * BS + threadIdx.x;
   const bool act = i < p.n;
-  const double4 P = act ? p.pos[i] : make_double4(0.0, 0.0, 0.0, 0.0);
-  const unsigned fl = act ? p.flags[i] : 0u;
-  dv3 x = xyz(P);
+  const int ii = act ? i : 0;
+  // every load that does not depend on another is issued here, together
+  const double4 P = p.pos[ii];
+  const unsigned fl = p.flags[ii];
+  const double4 Vv = p.vel[ii];
+  const double4 O = p.old[ii];
+  const int e0 = gp.inc_off[ii], e1 = gp.inc_off[ii + 1];
+  const int xcnt = gp.xcnt[ii];
+  double4 A = make_double4(0.0, 0.0, 0.0, 0.0), Q = A;
+  if (gp.rot) {
+    A = p.ang[ii];
+    Q = p.quat[ii];
+  }
+  const int refresh = gp.nwg > 0 && ctl.guarded && ctl.prev->nbroken > 0;
+  const dv3 x0 = xyz(P);
+  dv3 x = x0;
   const double r = P.w;
   dv3 F = mk3(0.0, 0.0, 0.0), M = mk3(0.0, 0.0, 0.0);
   unsigned flw = fl;
-  if (act && gp.nwg > 0 && ctl.guarded && ctl.prev->nbroken > 0) {
+  if (act && refresh) {
     // bonds broke in the previous step: the reference's updateInteractions
     // re-Updates the wall groups (EWallInteractionGroup.hpp:57-82) before forces
     flw = wall_membership(gp.grid, gp.nwg, gp.wall, P, fl);
     p.flags[i] = flw;
   }
-  // wall interaction groups, name order (EWallInteraction.hpp:42-78); the
-  // forces on inner particles go into this block's partial of the wall force
+  // wall interaction groups, name order (EWallInteraction.hpp:42-78)
   for (int w = 0; w < gp.nwg; w++) {
-    double wx = 0.0, wy = 0.0, wz = 0.0;
-    if (act && ((flw >> (PF_WALL_SHIFT + w)) & 1u)) {
-      const WallParam& W = gp.wall[w];
-      const dv3 n = mk3(W.nrm[0], W.nrm[1], W.nrm[2]);
-      const double dist = dot3(sub3(x, mk3(W.o[0], W.o[1], W.o[2])), n);
-      if (dist < r) {
-        double f = 1.0;
-        if (!gp.do2d) f *= 3.141592654 * r;
-        const dv3 force = mul3(n, (f * W.k) * (r - dist));
-        F = add3(F, force);
-        if ((flw >> (PF_INNER_SHIFT + w)) & 1u) {
-          wx = force.x; wy = force.y; wz = force.z;
-        }
-      }
-    }
-    block_sum3(wx, wy, wz, lds3);
-    if (threadIdx.x == 0) {
-      double* q = gp.wall_part + ((size_t)blockIdx.x * gp.nwg + w) * 3;
-      q[0] = wx; q[1] = wy; q[2] = wz;
+    if (!act || !((flw >> (PF_WALL_SHIFT + w)) & 1u)) continue;
+    const WallParam& W = gp.wall[w];
+    const dv3 n = mk3(W.nrm[0], W.nrm[1], W.nrm[2]);
+    const double dist = dot3(sub3(x, mk3(W.o[0], W.o[1], W.o[2])), n);
+    if (dist < r) {
+      double f = 1.0;
+      if (!gp.do2d) f *= 3.141592654 * r;
+      const dv3 force = mul3(n, (f * W.k) * (r - dist));
+      F = add3(F, force);
+      if (STRESS) stress_add(sg, force, sub3(sub3(x, mul3(n, dist)), x));
     }
   }
   unsigned long long md = 0;
   if (act) {
-    const double4 Vv = p.vel[i];
-    const double4 O = p.old[i];
     dv3 v = xyz(Vv);
     const double invm = Vv.w, mass = O.w;
-    // body forces, name order (BodyForceGroup.hpp:41-65)
-    for (int g = 0; g < gp.ngrav; g++) F = add3(F, mul3(mk3(gp.g[g][0], gp.g[g][1], gp.g[g][2]), mass));
-    // interaction records in summation order (CSR, then the appendix slots),
-    // fetched GATHER_CHUNK at a time so their loads are in flight together;
-    // the sum itself stays sequential
-    const int e0 = gp.inc_off[i], e1 = gp.inc_off[i + 1];
-    const int cnt = e1 - e0 + gp.xcnt[i];
+    // body forces, name order (BodyForceGroup.hpp:41-65), applied at the centre
+    for (int g = 0; g < gp.ngrav; g++) {
+      const dv3 fg = mul3(mk3(gp.g[g][0], gp.g[g][1], gp.g[g][2]), mass);
+      F = add3(F, fg);
+      if (STRESS) stress_add(sg, fg, sub3(x, x));
+    }
+    // the particle's slots in summation order (its CSR run, then its appendix
+    // slots), fetched GATHER_CHUNK at a time so their loads are in flight
+    // together; the sum itself stays sequential.  Moments of laws without
+    // moments are zeros, and a sum that starts at +0 never becomes -0, so
+    // adding them changes nothing.
+    const int nmain = e1 - e0;
+    const int cnt = nmain + xcnt;
+    const int ss = STRESS ? 9 : 6;
     for (int b = 0; b < cnt; b += GATHER_CHUNK) {
-      int ent[GATHER_CHUNK];
+      dv3 f[GATHER_CHUNK], m[GATHER_CHUNK], rl[GATHER_CHUNK];
 #pragma unroll
       for (int k = 0; k < GATHER_CHUNK; k++) {
-        const int t = e0 + b + k;
-        ent[k] = b + k >= cnt ? -1 : t < e1 ? gp.inc_ent[t] : gp.xslot[ESP_XSLOTS * i + (t - e1)];
-      }
-      dv3 f[GATHER_CHUNK], m[GATHER_CHUNK];
-#pragma unroll
-      for (int k = 0; k < GATHER_CHUNK; k++) {
-        const int en = ent[k] < 0 ? 0 : ent[k];
-        const double* rec = gp.contrib + 9 * (size_t)(en >> 2);
-        const double* mr = rec + 3 + 3 * (en & 1);
-        f[k] = mk3(rec[0], rec[1], rec[2]);
-        m[k] = mk3(mr[0], mr[1], mr[2]);
+        const int t = b + k;
+        const int slot = t >= cnt ? 0 : t < nmain ? e0 + t : gp.xbase + ESP_XSLOTS * i + (t - nmain);
+        const double* q = gp.slots + (size_t)ss * slot;
+        f[k] = mk3(q[0], q[1], q[2]);
+        m[k] = mk3(q[3], q[4], q[5]);
+        if (STRESS) rl[k] = mk3(q[6], q[7], q[8]);
       }
 #pragma unroll
       for (int k = 0; k < GATHER_CHUNK; k++) {
-        if (ent[k] < 0) break;
-        F = add3(F, (ent[k] & 1) ? neg3(f[k]) : f[k]);
-        if (ent[k] & 2) M = add3(M, m[k]);
+        if (b + k >= cnt) break;
+        F = add3(F, f[k]);
+        if (STRESS) stress_add(sg, f[k], rl[k]);
+        M = add3(M, m[k]);
       }
     }
     const double dt = gp.dt;
     if (gp.rot) {
-      const double4 A = p.ang[i];
-      const double4 Q = p.quat[i];
       dv3 w = xyz(A);
       dquat q = dquat{Q.x, mk3(Q.y, Q.z, Q.w)};
       if (fl & PF_ROT) {
@@ -422,6 +495,11 @@ __global__ __launch_bounds__(BS) void k_gather_integrate(StepCtl ctl, PView p, G
     p.pos[i] = make_double4(x.x, x.y, x.z, r);
     p.vel[i] = make_double4(v.x, v.y, v.z, invm);
     p.frc[i] = make_double4(F.x, F.y, F.z, 0.0);
+    if (STRESS) {
+      double* o = gp.sig + 9 * (size_t)i;
+      for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) o[3 * a + b] = sg[a][b];
+    }
     const dv3 d = sub3(x, xyz(O));
     md = (unsigned long long)__double_as_longlong(dot3(d, d));
   }
@@ -433,6 +511,27 @@ __global__ __launch_bounds__(BS) void k_gather_integrate(StepCtl ctl, PView p, G
     unsigned long long m = mred[0];
     for (int w = 1; w < BS / 64; w++) m = mred[w] > m ? mred[w] : m;
     if (m) atomicMax(&ctl.cur->maxdisp2, m);
+  }
+  // this block's partial of each wall force: the forces on inner particles,
+  // recomputed from the same values as above, in a fixed-order block sum
+  for (int w = 0; w < gp.nwg; w++) {
+    double wx = 0.0, wy = 0.0, wz = 0.0;
+    if (act && ((flw >> (PF_WALL_SHIFT + w)) & 1u) && ((flw >> (PF_INNER_SHIFT + w)) & 1u)) {
+      const WallParam& W = gp.wall[w];
+      const dv3 n = mk3(W.nrm[0], W.nrm[1], W.nrm[2]);
+      const double dist = dot3(sub3(x0, mk3(W.o[0], W.o[1], W.o[2])), n);
+      if (dist < r) {
+        double f = 1.0;
+        if (!gp.do2d) f *= 3.141592654 * r;
+        const dv3 force = mul3(n, (f * W.k) * (r - dist));
+        wx = force.x; wy = force.y; wz = force.z;
+      }
+    }
+    block_sum3(wx, wy, wz, lds3);
+    if (threadIdx.x == 0) {
+      double* q = gp.wall_part + ((size_t)blockIdx.x * gp.nwg + w) * 3;
+      q[0] = wx; q[1] = wy; q[2] = wz;
+    }
   }
 }
 
@@ -777,8 +876,8 @@ __global__ void k_wall_update(PView p, GridParams g, int nwg, WallParam w0, Wall
 // the previous step: the bond leaves its group (marked dead, record zeroed, key
 // erased from the exclusion set), both particles are flagged
 // (RotBondedInteraction.cpp:211-213) and the event is logged.
-__device__ __forceinline__ void break_one(PView p, const BondSet& bs, int gb, double* contrib,
-                                          UpdState* st, int* flist, int4* evt, long long step)
+__device__ __forceinline__ void break_one(PView p, const BondSet& bs, int gb, UpdState* st,
+                                          int* flist, int4* evt, long long step)
 {
   int g = 0;
   while (g + 1 < bs.count && gb >= bs.off[g + 1]) g++;
@@ -789,8 +888,13 @@ __device__ __forceinline__ void break_one(PView p, const BondSet& bs, int gb, do
   const int id1 = p.id[e.x], id2 = p.id[e.y];
   hs_erase(bs.hs[g], pair_key(id1, id2));
   bs.ij[g][b] = make_int2(-1, -1);
-  double* r = contrib + 9 * (size_t)(bs.rec_off[g] + b);
-  for (int k = 0; k < 9; k++) r[k] = 0.0;
+  const int2 so = bs.slot_of[bs.rec_off[g] + b];
+  double* s0 = bs.slots + (size_t)bs.ss * so.x;
+  double* s1 = bs.slots + (size_t)bs.ss * so.y;
+  for (int k = 0; k < bs.ss; k++) {
+    s0[k] = 0.0;
+    s1[k] = 0.0;
+  }
   if (!(atomicOr(&p.flags[e.x], PF_FLAG) & PF_FLAG)) flist[atomicAdd(&st->fcount, 1)] = e.x;
   if (!(atomicOr(&p.flags[e.y], PF_FLAG) & PF_FLAG)) flist[atomicAdd(&st->fcount, 1)] = e.y;
   evt[atomicAdd(&st->evt_count, 1)] = make_int4((int)step, id1, id2, 0);
@@ -798,12 +902,12 @@ __device__ __forceinline__ void break_one(PView p, const BondSet& bs, int gb, do
 
 // the previous step's breaks on the host bookkeeping path
 __global__ void k_break_apply(StepCtl ctl, PView p, BondSet bs, const int* broken_list,
-                              double* contrib, UpdState* st, int* flist, int4* evt, long long step)
+                              UpdState* st, int* flist, int4* evt, long long step)
 {
   if (step_skip(ctl)) return;
   const int nb = ctl.prev->nbroken;
   for (int t = blockIdx.x * BS + threadIdx.x; t < nb; t += gridDim.x * BS)
-    break_one(p, bs, broken_list[t], contrib, st, flist, evt, step);
+    break_one(p, bs, broken_list[t], st, flist, evt, step);
 }
 
 // TSubLattice::updateInteractions after bond breaks, in one block: the previous
@@ -817,15 +921,15 @@ __global__ void k_break_apply(StepCtl ctl, PView p, BondSet bs, const int* broke
 #define U2_BS 1024
 #define NEW_SENTINEL 0x7fffffffffffffffll
 __global__ __launch_bounds__(U2_BS) void k_incremental_update(
-    StepCtl ctl, PView p, BondSet bs, const int* broken_list, double* contrib, int4* evt,
-    long long step, GridParams g, DynAppend da, const int* cell, const int* cell_start,
-    const int* cell_items, UpdState* st, int* flist, int* xslot, int* xcnt, int has_group)
+    StepCtl ctl, PView p, BondSet bs, const int* broken_list, int4* evt, long long step,
+    GridParams g, DynAppend da, const int* cell, const int* cell_start, const int* cell_items,
+    UpdState* st, int* flist, int* xcnt, int has_group)
 {
   if (step_skip(ctl)) return;
   const int nbk = ctl.prev->nbroken;
   if (nbk <= 0) return;
   const int tid = threadIdx.x;
-  for (int t = tid; t < nbk; t += U2_BS) break_one(p, bs, broken_list[t], contrib, st, flist, evt, step);
+  for (int t = tid; t < nbk; t += U2_BS) break_one(p, bs, broken_list[t], st, flist, evt, step);
   if (!has_group) return;
   __shared__ long long s_key[ESP_NEW_CAP];
   __shared__ int s_cnt, s_over, s_keep;
@@ -855,17 +959,28 @@ __global__ __launch_bounds__(U2_BS) void k_incremental_update(
     const int x = cx + nc / 9 - 1, y = cy + (nc / 3) % 3 - 1, z = cz + nc % 3 - 1;
     if (x < 0 || x >= g.nx || y < 0 || y >= g.ny || z < 0 || z >= g.nz) continue;
     const int c2 = g.ny * g.nz * x + g.nz * y + z;
-    const int t1 = cell_start[c2 + 1];
-    for (int u = cell_start[c2] + half; u < t1; u += 2) {
-      const int q = cell_items[u];
-      if (!(p.flags[q] & PF_FLAG)) continue;
-      const int idq = p.id[q];
-      if (idq <= idp) continue;
-      const int k = atomicAdd(&s_cnt, 1);
-      if (k < ESP_NEW_CAP) {
-        s_key[k] = pair_key(pi, q);
-      } else {
-        s_over = 1;
+    const int t0 = cell_start[c2], t1 = cell_start[c2 + 1];
+    // this lane's items u = t0 + half + 2 j, four at a time with their loads in flight together
+    for (int u = t0 + half; u < t1; u += 8) {
+      int q[4];
+      unsigned fq[4];
+      int idq[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++) q[j] = u + 2 * j < t1 ? cell_items[u + 2 * j] : -1;
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        fq[j] = q[j] >= 0 ? p.flags[q[j]] : 0u;
+        idq[j] = q[j] >= 0 ? p.id[q[j]] : 0;
+      }
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        if (!(fq[j] & PF_FLAG) || idq[j] <= idp) continue;
+        const int k = atomicAdd(&s_cnt, 1);
+        if (k < ESP_NEW_CAP) {
+          s_key[k] = pair_key(pi, q[j]);
+        } else {
+          s_over = 1;
+        }
       }
     }
   }
@@ -937,7 +1052,6 @@ __global__ __launch_bounds__(U2_BS) void k_incremental_update(
     }
     return;
   }
-  const int hm = da.is_hist ? 2 : 0;  // history groups carry moments
 #pragma unroll
   for (int r = 0; r < ESP_NEW_CAP / U2_BS; r++) {
     const int t = tid + r * U2_BS;
@@ -949,9 +1063,9 @@ __global__ __launch_bounds__(U2_BS) void k_incremental_update(
     da.key[w] = key;
     if (da.ff) da.ff[w] = make_double4(0.0, 0.0, 0.0, 0.0);
     if (da.is_hist) hs_insert(da.present, key);
-    const int ent = ((da.app_rec_off + napp + t) << 2) | hm;
-    xslot[ESP_XSLOTS * a + xcnt[a] + pa[r]] = ent;
-    xslot[ESP_XSLOTS * b + xcnt[b] + pb[r]] = ent | 1;
+    // appendix slots start at zero (cleared at the last rebuild, used once)
+    da.slot_of[da.app_rec_off + napp + t] =
+        make_int2(da.xbase + ESP_XSLOTS * a + xcnt[a] + pa[r], da.xbase + ESP_XSLOTS * b + xcnt[b] + pb[r]);
   }
   __syncthreads();  // every xcnt read above precedes the increments
 #pragma unroll
@@ -1071,6 +1185,15 @@ __global__ __launch_bounds__(BS) void k_inc_sort(int n, const int* off, unsigned
   for (int s = a; s < b; s++) ent[s] = (int)(unsigned)ent64[s];
 }
 
+__global__ void k_slot_of(int total, const int* ent, int2* slot_of)
+{
+  const int t = blockIdx.x * BS + threadIdx.x;
+  if (t >= total) return;
+  const int v = ent[t];
+  int* so = reinterpret_cast<int*>(slot_of + (v >> 2));
+  so[v & 1] = t;
+}
+
 // ---------------------------------------------------------------------------
 // exclusive scan (1024 elements per block, recursive over block sums)
 // ---------------------------------------------------------------------------
@@ -1141,58 +1264,30 @@ void launch_exclusive_scan(hipStream_t s, const int* in, int* out, int n, int* t
 // ---------------------------------------------------------------------------
 int gather_blocks(int n) { return n > 0 ? nblk(n) : 0; }
 
-void launch_rot_bond(hipStream_t s, StepCtl ctl, PView p, int nb, const int2* ij, const double4* k4,
-                     const double4* mx4, const double4* D4, double truncated, double beta1,
-                     double beta2, double* contrib, int rec_off, int bond_off, int* broken_list)
-{
-  if (nb <= 0) return;
-  hipLaunchKernelGGL(k_rot_bond, dim3(nblk(nb)), dim3(BS), 0, s, ctl, p, nb, ij, k4, mx4, D4,
-                     truncated, beta1, beta2, contrib, rec_off, bond_off, broken_list);
-}
+int force_blocks(int n) { return nblk(n); }
 
-void launch_rot_friction(hipStream_t s, StepCtl ctl, PView p, int nc, const int* n_app,
-                         const int2* ij, double4* ff, double k, double ks, int scaling, int meanR,
-                         int do2d, double mu_d, double mu_s, double dt, double* contrib,
-                         int rec_off, int app_rec_off)
+void launch_forces(hipStream_t s, StepCtl ctl, PView p, const ForceLaunch& fl)
 {
-  hipLaunchKernelGGL(k_rot_friction, dim3(nblk((long long)nc + ESP_APP_CAP)), dim3(BS), 0, s, ctl,
-                     p, nc, n_app, ij, ff, k, ks, scaling, meanR, do2d, mu_d, mu_s, dt, contrib,
-                     rec_off, app_rec_off);
-}
-
-void launch_nrot_elastic(hipStream_t s, StepCtl ctl, PView p, int nc, const int* n_app,
-                         const int2* ij, double k, int scaling, int do2d, double* contrib,
-                         int rec_off, int app_rec_off)
-{
-  hipLaunchKernelGGL(k_nrot_elastic, dim3(nblk((long long)nc + ESP_APP_CAP)), dim3(BS), 0, s, ctl,
-                     p, nc, n_app, ij, k, scaling, do2d, contrib, rec_off, app_rec_off);
-}
-
-void launch_rot_elastic(hipStream_t s, StepCtl ctl, PView p, int nc, const int* n_app,
-                        const int2* ij, double k, int scaling, int do2d, double* contrib,
-                        int rec_off, int app_rec_off)
-{
-  hipLaunchKernelGGL(k_rot_elastic, dim3(nblk((long long)nc + ESP_APP_CAP)), dim3(BS), 0, s, ctl,
-                     p, nc, n_app, ij, k, scaling, do2d, contrib, rec_off, app_rec_off);
+  if (fl.nblocks <= 0) return;
+  hipLaunchKernelGGL(k_forces, dim3(fl.nblocks), dim3(BS), 0, s, ctl, p, fl);
 }
 
 void launch_break_apply(hipStream_t s, StepCtl ctl, PView p, BondSet bs, const int* broken_list,
-                        double* contrib, UpdState* st, int* flist, int4* evt, long long step)
+                        UpdState* st, int* flist, int4* evt, long long step)
 {
   if (bs.count <= 0) return;
-  hipLaunchKernelGGL(k_break_apply, dim3(16), dim3(BS), 0, s, ctl, p, bs, broken_list, contrib, st,
-                     flist, evt, step);
+  hipLaunchKernelGGL(k_break_apply, dim3(16), dim3(BS), 0, s, ctl, p, bs, broken_list, st, flist,
+                     evt, step);
 }
 
 void launch_incremental_update(hipStream_t s, StepCtl ctl, PView p, BondSet bs,
-                               const int* broken_list, double* contrib, int4* evt, long long step,
-                               GridParams g, DynAppend da, const int* cell, const int* cell_start,
-                               const int* cell_items, UpdState* st, int* flist, int* xslot,
-                               int* xcnt, int has_group)
+                               const int* broken_list, int4* evt, long long step, GridParams g,
+                               DynAppend da, const int* cell, const int* cell_start,
+                               const int* cell_items, UpdState* st, int* flist, int* xcnt,
+                               int has_group)
 {
   hipLaunchKernelGGL(k_incremental_update, dim3(1), dim3(U2_BS), 0, s, ctl, p, bs, broken_list,
-                     contrib, evt, step, g, da, cell, cell_start, cell_items, st, flist, xslot,
-                     xcnt, has_group);
+                     evt, step, g, da, cell, cell_start, cell_items, st, flist, xcnt, has_group);
 }
 
 void launch_hash_clear(hipStream_t s, HashRef h)
@@ -1221,7 +1316,36 @@ void launch_alive_flags(hipStream_t s, int nb, const int2* ij, int* keep)
 void launch_gather_integrate(hipStream_t s, StepCtl ctl, PView p, const GatherParams& gp)
 {
   if (p.n <= 0) return;
-  hipLaunchKernelGGL(k_gather_integrate, dim3(nblk(p.n)), dim3(BS), 0, s, ctl, p, gp);
+  if (gp.sig)
+    hipLaunchKernelGGL(k_gather_integrate<true>, dim3(nblk(p.n)), dim3(BS), 0, s, ctl, p, gp);
+  else
+    hipLaunchKernelGGL(k_gather_integrate<false>, dim3(nblk(p.n)), dim3(BS), 0, s, ctl, p, gp);
+}
+
+// BrittleBeamSCInteraction::broken for every live bond of the stress-criterion
+// groups, on the positions after this step's integration and the stresses of
+// its forces - the state the reference's next updateInteractions sees.
+__global__ __launch_bounds__(BS) void k_sc_break(StepCtl ctl, PView p, ScLaunch sl)
+{
+  if (step_skip(ctl)) return;
+  int g = 0;
+  while (g + 1 < sl.count && (int)blockIdx.x >= sl.blk0[g + 1]) g++;
+  const int b = ((int)blockIdx.x - sl.blk0[g]) * BS + (int)threadIdx.x;
+  if (b >= sl.n[g]) return;
+  const int2 e = sl.ij[g][b];
+  if (e.x < 0) return;
+  const double4 P1 = p.pos[e.x], P2 = p.pos[e.y];
+  if (esp_sc_broken(xyz(P1), P1.w, sl.sig + 9 * (size_t)e.x, xyz(P2), P2.w,
+                    sl.sig + 9 * (size_t)e.y, sl.sigma3[g], sl.cohesion[g], sl.tan_angle[g])) {
+    const int slot = atomicAdd(&ctl.cur->nbroken, 1);
+    sl.broken_list[slot] = sl.bond_off[g] + b;
+  }
+}
+
+void launch_sc_break(hipStream_t s, StepCtl ctl, PView p, const ScLaunch& sl)
+{
+  if (sl.nblocks <= 0) return;
+  hipLaunchKernelGGL(k_sc_break, dim3(sl.nblocks), dim3(BS), 0, s, ctl, p, sl);
 }
 
 void launch_wall_finalize(hipStream_t s, const double* part, int nblocks, int nwg, int nsteps,
@@ -1356,6 +1480,12 @@ void launch_inc_fill(hipStream_t s, int nc, const int2* ij, int rec_off, int has
   if (nc <= 0) return;
   hipLaunchKernelGGL(k_inc_fill, dim3(nblk(nc)), dim3(BS), 0, s, nc, ij, rec_off, has_m, rank,
                      (unsigned long long)grp, inc_off, inc_fill, ent64);
+}
+
+void launch_slot_of(hipStream_t s, int total, const int* inc_ent, int2* slot_of)
+{
+  if (total <= 0) return;
+  hipLaunchKernelGGL(k_slot_of, dim3(nblk(total)), dim3(BS), 0, s, total, inc_ent, slot_of);
 }
 
 void launch_inc_sort(hipStream_t s, int n, const int* inc_off, unsigned long long* ent64,

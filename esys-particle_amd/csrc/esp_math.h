@@ -306,3 +306,50 @@ ESP_HD bool esp_rot_friction_law(dv3 pos1, dv3 vel1, dv3 w1, double r1, dv3 pos2
   slipping = false;
   return false;
 }
+
+// 1 / particle volume (CParticle::m_div_vol, Particle.cpp:52)
+ESP_HD double esp_div_vol(double r) { return 1.0 / (4.0 / 3.0 * M_PI * r * r * r); }
+
+// BrittleBeamSCInteraction::broken (BrittleBeamSC.cpp:103-148): truncated
+// Mohr-Coulomb test on the mean of the two particles' stresses
+// (CParticle::sigma = m_sigma / volume, Particle.cpp:405-408; sig1/sig2 are the
+// unscaled sums, row-major) in the frame of the current bond direction.
+ESP_HD bool esp_sc_broken(dv3 pos1, double r1, const double* sig1, dv3 pos2, double r2,
+                          const double* sig2, double sigma3, double cohesion, double tan_angle)
+{
+  const dv3 d = sub3(pos2, pos1);
+  const dv3 n = div3(d, norm3(d));
+  dv3 s;
+  if (n.x != 1.0 || n.y != 0.0 || n.z != 0.0) {
+    const dv3 c = cross3(n, mk3(1.0, 0.0, 0.0));
+    s = div3(c, norm3(c));
+  } else {
+    const dv3 c = cross3(n, mk3(0.0, 1.0, 0.0));
+    s = div3(c, norm3(c));
+  }
+  const dv3 t = cross3(n, s);
+  // Matrix3(n,s,t).trans(): the axes as rows
+  const double tm[3][3] = {{n.x, n.y, n.z}, {s.x, s.y, s.z}, {t.x, t.y, t.z}};
+  const double dv1 = esp_div_vol(r1), dv2 = esp_div_vol(r2);
+  double avg[3][3], sym[3][3];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) avg[i][j] = 0.5 * (sig1[3 * i + j] * dv1 + sig2[3 * i + j] * dv2);
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) sym[i][j] = 0.5 * (avg[i][j] + avg[j][i]);
+  // first row of tm . sym . tm^T (Matrix3::dot sums from zero, Matrix3.cpp:320-333)
+  double a0[3];
+  for (int k = 0; k < 3; k++) {
+    double v = 0.0;
+    for (int m = 0; m < 3; m++) v += tm[0][m] * sym[m][k];
+    a0[k] = v;
+  }
+  double rot0[3];
+  for (int j = 0; j < 3; j++) {
+    double v = 0.0;
+    for (int k = 0; k < 3; k++) v += a0[k] * tm[j][k];
+    rot0[j] = v;
+  }
+  const double sigma_n = rot0[0];
+  const double tau = sqrt(rot0[1] * rot0[1] + rot0[2] * rot0[2]);
+  return (sigma_n < -1.0 * sigma3) || (tau > cohesion + sigma_n * tan_angle);
+}

@@ -223,6 +223,32 @@ class BrittleBeamPrms(RotBondPrms):
                              truncated, perBondStrengths)
 
 
+class BrittleBeamSCPrms(InteractionPrms):
+    """BrittleBeamSCPrms(name, youngsModulus, poissonsRatio, cohesion, frictionAngle,
+    tensileCutoff, tag) (InteractionParamsPy.cpp:680-693, 1689-1730): elastic
+    beam bonds (ARotBondedInteraction) that break by a truncated Mohr-Coulomb
+    test on the mean stress of their two particles (BrittleBeamSC.cpp:103-148).
+    frictionAngle in degrees.  Stiffnesses as ARotBondedIGP(E, nu)
+    (ARotBondedInteraction.cpp:94-110)."""
+
+    def __init__(self, name, youngsModulus, poissonsRatio, cohesion, frictionAngle,
+                 tensileCutoff, tag):
+        shear_modulus = youngsModulus / (2.0 * (1. + poissonsRatio))
+        self.name = name
+        self.youngsModulus, self.poissonsRatio = youngsModulus, poissonsRatio
+        self.cohesion, self.frictionAngle, self.tensileCutoff = cohesion, frictionAngle, tensileCutoff
+        self.tag = tag
+        self.kr = math.pi * youngsModulus
+        self.ks = math.pi * shear_modulus
+        self.kb = math.pi * youngsModulus
+        self.kt = math.pi * shear_modulus
+
+    def _c(self):
+        return {"kr": self.kr, "ks": self.ks, "kt": self.kt, "kb": self.kb,
+                "cohesion": self.cohesion, "friction_angle": self.frictionAngle,
+                "tension_cutoff": self.tensileCutoff, "tag": self.tag}
+
+
 class GravityPrms(InteractionPrms):
     """GravityPrms(name, acceleration) (InteractionParamsPy.cpp:917-925)."""
 
@@ -373,6 +399,8 @@ class LsmMpi:
         e = self._e()
         if isinstance(prms, RotBondPrms):
             e.add_rot_bond_group(prms.name, prms._c(), prms.perBondStrengths)
+        elif isinstance(prms, BrittleBeamSCPrms):
+            e.add_brittle_beam_sc_group(prms.name, prms._c())
         elif isinstance(prms, NRotElasticPrms):
             e.add_nrot_elastic_group(prms.name, prms.normalK, prms.scaling)
         elif isinstance(prms, RotElasticPrms):
@@ -465,6 +493,12 @@ class LsmMpi:
 
     def getBondBreaks(self):
         return self._e().bond_breaks()
+
+    def getParticleStress(self):
+        """Per-particle stress sums of the last step (CParticle::m_sigma,
+        Particle.cpp:391-400), shape (n, 3, 3), ascending id order; needs a
+        BrittleBeamSC group.  Divide by the particle volume for CParticle::sigma."""
+        return self._e().particle_stress()
 
     def getStats(self):
         return self._e().stats()

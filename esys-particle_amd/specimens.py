@@ -218,9 +218,12 @@ def apply(sim, spec, steps_per_batch=None):
         ids = spec["ids"]
         sim.createConnectionArrays(ids[b[:, 0]], ids[b[:, 1]], np.zeros(len(b), np.int32))
         bp = spec["bond_prms"]
-        sim.createInteractionGroup(lsm.RotBondPrms(
-            bp["name"], bp["normalK"], bp["shearK"], bp["torsionK"], bp["bendingK"],
-            1.0, 1.0, 1.0, 1.0, bp["tag"], perBondStrengths=bp.get("strengths")))
+        if "sc" in bp:  # BrittleBeamSC bonds: E, nu, cohesion, friction angle, tension cutoff
+            sim.createInteractionGroup(lsm.BrittleBeamSCPrms(bp["name"], *bp["sc"], bp["tag"]))
+        else:
+            sim.createInteractionGroup(lsm.RotBondPrms(
+                bp["name"], bp["normalK"], bp["shearK"], bp["torsionK"], bp["bendingK"],
+                1.0, 1.0, 1.0, 1.0, bp["tag"], perBondStrengths=bp.get("strengths")))
     for name, k in spec.get("elastic", []):
         sim.createInteractionGroup(lsm.NRotElasticPrms(name, k))
     if "friction" in spec:

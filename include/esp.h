@@ -75,6 +75,18 @@ typedef struct esp_rot_bond_prms {
   double beta1, beta2;                   /* bending / torsion weights (default 1) */
 } esp_rot_bond_prms;
 
+/* BrittleBeamSCPrms(name, youngsModulus, poissonsRatio, cohesion, frictionAngle,
+ * tensileCutoff, tag) after the Python-side mapping: kr = kb = pi E,
+ * ks = kt = pi E / (2 (1 + nu)) (InteractionParamsPy.cpp:680-693,
+ * ARotBondedInteraction.cpp:94-110, BrittleBeamSC.cpp:29-56). */
+typedef struct esp_brittle_beam_sc_prms {
+  double kr, ks, kt, kb;   /* stiffnesses before size scaling */
+  double cohesion;         /* Mohr-Coulomb cohesion */
+  double friction_angle;   /* degrees */
+  double tension_cutoff;   /* tensile strength */
+  int tag;                 /* connection tag the group binds */
+} esp_brittle_beam_sc_prms;
+
 /* RotFrictionPrms (InteractionParamsPy.cpp:772-833, RotFricInteraction.cpp:34-57). */
 typedef struct esp_rot_friction_prms {
   double k_n;        /* normalK */
@@ -142,6 +154,16 @@ int esp_add_rot_bond_group(esp_engine* e, const char* name,
                            const esp_rot_bond_prms* prms,
                            const double* per_bond_max, int n_per_bond);
 
+/* createInteractionGroup(BrittleBeamSCPrms): LsmMpiPy::createBrittleBeamSCInteractGrp
+ * (LsmMpiPy.cpp:645-649) -> TRotSubLattice::addBrittleBeamSCIG
+ * (RotSubLattice.hpp:188-225): ARotBondedInteraction bonds from the connections
+ * with the tag; a bond breaks on the mean stress of its two particles
+ * (BrittleBeamSCInteraction::broken, BrittleBeamSC.cpp:103-148).  With such a
+ * group the engine accumulates every particle's stress tensor
+ * (CParticle::applyForce, Particle.cpp:391-400). */
+int esp_add_brittle_beam_sc_group(esp_engine* e, const char* name,
+                                  const esp_brittle_beam_sc_prms* prms);
+
 /* ---- dynamic pair groups ------------------------------------------------ */
 /* createInteractionGroup(NRotElasticPrms): ParallelInteractionStorage_NE
  * <T,CElasticInteraction> (SubLattice.hpp:1038-1054, ElasticInteraction.cpp:25-71). */
@@ -201,6 +223,10 @@ int esp_get_friction_state(esp_engine* e, const char* name, double* ffric, int* 
 /* Pairs of the last full neighbour list (NeighborTable::getFullList,
  * ntable.hpp:590-621), sorted by (id1, id2); returns the count when ids are NULL. */
 int esp_get_neighbour_pairs(esp_engine* e, int* id1, int* id2);
+/* Stress sums of the last step, 9 doubles per particle (row-major), ascending
+ * id order: CParticle::m_sigma (Particle.cpp:391-400) before the division by
+ * the particle volume.  ESP_ERR_STATE unless a BrittleBeamSC group exists. */
+int esp_get_particle_stress(esp_engine* e, double* out);
 /* Bond-break events (step, id1, id2) in (step, id1, id2) order; returns the count
  * when the pointers are NULL. */
 int esp_get_bond_breaks(esp_engine* e, int* step, int* id1, int* id2);
@@ -209,12 +235,10 @@ int esp_get_rebuild_steps(esp_engine* e, int* steps);
 
 /* ---- measurement (no counterpart in the reference) ----------------------- */
 /* Kernel classes timed by esp_get_kernel_times. */
-#define ESP_KCLASS_BOND 0     /* rotational bond law + break test, per bond group */
-#define ESP_KCLASS_FRICTION 1 /* RotFriction law, per group */
-#define ESP_KCLASS_ELASTIC 2  /* NRotElastic / RotElastic laws, per group */
-#define ESP_KCLASS_GATHER 3   /* force gather + integration, per step */
-#define ESP_KCLASS_UPDATE 4   /* bond-break bookkeeping of a queued step (break list, new pairs) */
-#define ESP_KCLASS_COUNT 5
+#define ESP_KCLASS_FORCES 0 /* force laws of every interaction group (+ break test), per step */
+#define ESP_KCLASS_GATHER 1 /* force gather + integration, per step */
+#define ESP_KCLASS_UPDATE 2 /* bond-break bookkeeping of a queued step (break list, new pairs) */
+#define ESP_KCLASS_COUNT 3
 /* Enable (1) or disable (0) HIP-event timing of every per-step kernel launch on
  * the engine's stream; timing adds an event pair around each launch. */
 int esp_set_kernel_timing(esp_engine* e, int enable);

@@ -172,6 +172,9 @@ typedef struct {
 typedef struct {
   char name[128];
   esp_rot_bond_prms prm;
+  /* BrittleBeamSC groups: stress criterion instead of CRotBondedInteraction::broken */
+  int sc;
+  double sc_sigma3, sc_cohesion, sc_tan;
   obond* v;
   int n, cap;
   long long* keys; /* sorted (id1,id2) keys of live bonds: the storage's m_set */
@@ -1005,6 +1008,65 @@ static int check_neighbours(eso_engine* e)
 }
 
 /* TSubLattice::updateInteractions (SubLattice.hpp:2416-2466). */
+/* CParticle::sigma (Particle.cpp:405-408, m_div_vol from Particle.cpp:52). */
+static mat3 particle_sigma(const opart* p)
+{
+  const double div_vol = 1.0 / (4.0 / 3.0 * M_PI * p->rad * p->rad * p->rad);
+  mat3 r;
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) r.m[i][j] = p->sigma[i][j] * div_vol;
+  return r;
+}
+
+/* Matrix3::dot (Matrix3.cpp:320-333): res(i,j) += a(i,k) b(k,j) from zero. */
+static mat3 mdot(const mat3* a, const mat3* b)
+{
+  mat3 r;
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      r.m[i][j] = 0.0;
+      for (int k = 0; k < 3; k++) r.m[i][j] += a->m[i][k] * b->m[k][j];
+    }
+  return r;
+}
+
+/* BrittleBeamSCInteraction::broken (BrittleBeamSC.cpp:103-148): truncated
+ * Mohr-Coulomb test on the two particles' mean stress in the bond frame. */
+static int sc_broken(const eso_engine* e, const obgroup* G, const obond* b)
+{
+  const opart* p1 = &e->P[b->i1];
+  const opart* p2 = &e->P[b->i2];
+  v3 d = vsub(p2->pos, p1->pos);
+  v3 n = vdiv(d, vnorm(d));
+  v3 s;
+  if (n.x != 1.0 || n.y != 0.0 || n.z != 0.0) {
+    v3 c = vcross(n, V(1.0, 0.0, 0.0));
+    s = vdiv(c, vnorm(c));
+  } else {
+    v3 c = vcross(n, V(0.0, 1.0, 0.0));
+    s = vdiv(c, vnorm(c));
+  }
+  v3 t = vcross(n, s);
+  /* Matrix3(n,s,t) holds the vectors as columns; its transpose as rows */
+  mat3 tm;
+  tm.m[0][0] = n.x; tm.m[0][1] = n.y; tm.m[0][2] = n.z;
+  tm.m[1][0] = s.x; tm.m[1][1] = s.y; tm.m[1][2] = s.z;
+  tm.m[2][0] = t.x; tm.m[2][1] = t.y; tm.m[2][2] = t.z;
+  mat3 s1 = particle_sigma(p1), s2 = particle_sigma(p2), avg, sym;
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) avg.m[i][j] = 0.5 * (s1.m[i][j] + s2.m[i][j]);
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) sym.m[i][j] = 0.5 * (avg.m[i][j] + avg.m[j][i]);
+  mat3 tt = mtrans(&tm);
+  mat3 a = mdot(&tm, &sym);
+  mat3 rot = mdot(&a, &tt);
+  double sigma_n = rot.m[0][0];
+  double tau_x = rot.m[0][1];
+  double tau_y = rot.m[0][2];
+  double tau = sqrt(tau_x * tau_x + tau_y * tau_y);
+  return (sigma_n < -1.0 * G->sc_sigma3) || (tau > G->sc_cohesion + sigma_n * G->sc_tan);
+}
+
 static void update_interactions(eso_engine* e)
 {
   int need_update = 0;
@@ -1014,7 +1076,7 @@ static void update_interactions(eso_engine* e)
     int w = 0, res = 0;
     for (int i = 0; i < G->n; i++) {
       obond b = G->v[i];
-      if (bond_broken(&b, &G->prm)) {
+      if (G->sc ? sc_broken(e, G, &b) : bond_broken(&b, &G->prm)) {
         res = 1;
         e->P[b.i1].flag = 1;
         e->P[b.i2].flag = 1;
@@ -1359,8 +1421,40 @@ int eso_add_connections(eso_engine* e, int n, const int* id1, const int* id2, co
   return ESP_OK;
 }
 
+static int add_bond_group(eso_engine* e, const char* name, const esp_rot_bond_prms* prm,
+                          const double* per_bond_max, int n_per_bond,
+                          const esp_brittle_beam_sc_prms* sc);
+
 int eso_add_rot_bond_group(eso_engine* e, const char* name, const esp_rot_bond_prms* prm,
                            const double* per_bond_max, int n_per_bond)
+{
+  return add_bond_group(e, name, prm, per_bond_max, n_per_bond, NULL);
+}
+
+/* TRotSubLattice::addBrittleBeamSCIG (RotSubLattice.hpp:188-225): the bonds of
+ * ARotBondedInteraction (scaled stiffnesses, ARotBondedInteraction.cpp:141-179)
+ * with the stress criterion of BrittleBeamSCInteraction (BrittleBeamSC.cpp:73-79). */
+int eso_add_brittle_beam_sc_group(eso_engine* e, const char* name,
+                                  const esp_brittle_beam_sc_prms* sc)
+{
+  if (!sc) return fail(e, ESP_ERR_ARG, "bad group");
+  esp_rot_bond_prms prm;
+  memset(&prm, 0, sizeof prm);
+  prm.kr = sc->kr;
+  prm.ks = sc->ks;
+  prm.kt = sc->kt;
+  prm.kb = sc->kb;
+  prm.max_n = prm.max_s = prm.max_t = prm.max_b = 1.0;
+  prm.tag = sc->tag;
+  prm.scaling = 1;
+  prm.meanR_scaling = 1;
+  prm.truncated = prm.beta1 = prm.beta2 = 1.0;
+  return add_bond_group(e, name, &prm, NULL, 0, sc);
+}
+
+static int add_bond_group(eso_engine* e, const char* name, const esp_rot_bond_prms* prm,
+                          const double* per_bond_max, int n_per_bond,
+                          const esp_brittle_beam_sc_prms* sc)
 {
   if (!e->rot) return fail(e, ESP_ERR_STATE, "rotational bonds need particle type RotSphere");
   if (!name || !prm || strlen(name) >= 128) return fail(e, ESP_ERR_ARG, "bad group");
@@ -1371,6 +1465,12 @@ int eso_add_rot_bond_group(eso_engine* e, const char* name, const esp_rot_bond_p
   strcpy(G->name, name);
   G->prm = *prm;
   G->keys_dirty = 1;
+  if (sc) {
+    G->sc = 1;
+    G->sc_sigma3 = sc->tension_cutoff;
+    G->sc_cohesion = sc->cohesion;
+    G->sc_tan = tan(sc->friction_angle / 57.2957795);
+  }
   /* connections of this tag, in arrival order; a repeated (sorted) id pair is
    * rejected by isIn at its second occurrence */
   int nc = 0;
@@ -1580,7 +1680,7 @@ int eso_get_particle_data(eso_engine* e, int field, double* out)
   return ESP_OK;
 }
 
-int eso_get_particle_sigma(eso_engine* e, double* out)
+int eso_get_particle_stress(eso_engine* e, double* out)
 {
   for (int k = 0; k < e->nidkey; k++) {
     const opart* p = &e->P[e->idkey[k] & 0xffffffffLL];

@@ -66,7 +66,9 @@ int eso_get_kernel_times(eso_engine* e, double* ms, long long* launches);
 /* Oracle-only extras used by the tests. */
 /* Particle stress tensors (9 doubles, row-major, unscaled sigma of
  * CParticle::applyForce, Particle.cpp:391-400), ascending id order. */
-int eso_get_particle_sigma(eso_engine* e, double* out);
+int eso_get_particle_stress(eso_engine* e, double* out);
+int eso_add_brittle_beam_sc_group(eso_engine* e, const char* name,
+                                  const esp_brittle_beam_sc_prms* sc);
 /* Particle-id pairs of a group in storage order, i.e. the std::list order of
  * the reference's interaction storage; returns the count. */
 int eso_get_group_list_order(eso_engine* e, const char* name, int* id1, int* id2);

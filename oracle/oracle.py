@@ -16,7 +16,6 @@ _pi = ctypes.POINTER(ctypes.c_int)
 _vp = ctypes.c_void_p
 
 ORACLE_EXTRA = {
-    "get_particle_sigma": (ctypes.c_int, [_vp, _pd]),
     "get_group_list_order": (ctypes.c_int, [_vp, ctypes.c_char_p, _pi, _pi]),
     "law_rot_bond": (None, [_pd, _pd, _pd]),
     "law_rot_friction": (None, [_pd, _pd, _pd, _pd]),

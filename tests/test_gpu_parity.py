@@ -188,3 +188,27 @@ def test_dense_bonded_queries_between_runs(pkg, orc, spm):
     s.runTimeSteps(800)
     assert s.getParticleData("pos").tobytes() == g.getParticleData("pos").tobytes()
     np.testing.assert_array_equal(s.getBondBreaks(), g.getBondBreaks())
+
+
+def test_brittle_beam_sc_stress_and_events(pkg, orc, spm):
+    """BrittleBeamSC bonds (truncated Mohr-Coulomb test on the particles' mean
+    stress) in the compressed lattice: stress sums after one step within 1e-12
+    (normwise relative), break events bit-exact and trajectories within 1e-9
+    after 200 steps."""
+    spec = lattice_spec(spm)
+    bp = dict(spec["bond_prms"])
+    bp.pop("strengths", None)
+    bp["sc"] = (3e4, 0.25, 20.0, 30.0, 20.0)  # E, nu, cohesion, angle (deg), tension cutoff
+    spec["bond_prms"] = bp
+    g, o = run_pair(pkg, orc, spm, spec, 1)
+    sg = g.getParticleStress().reshape(-1, 9)
+    so = o.getParticleStress().reshape(-1, 9)
+    assert np.abs(so).max() > 0
+    assert rel_field(sg, so) <= FORCE_TOL
+    g.runTimeSteps(199)
+    o.runTimeSteps(199)
+    assert len(o.getBondBreaks()) > 50
+    assert_events_equal(g, o, ["bonded", "friction"])
+    r = float(np.mean(spec["rad"]))
+    assert np.max(np.abs(g.getParticleData("pos") - o.getParticleData("pos"))) <= TRAJ_TOL * r
+    assert rel_field(g.getParticleStress().reshape(-1, 9), o.getParticleStress().reshape(-1, 9)) <= TRAJ_TOL
