@@ -32,6 +32,13 @@ B_BOND = 96           # i, j (2 x int32) + kr ks kt kb + 4 strengths + D0 (3 dou
 B_FRICTION = 64       # i, j + shear spring (3) + flags, read and write
 B_ELASTIC = 8         # i, j
 
+# HBM bytes one launch moves on this workload, per unit: rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE over steady-state steps, each scaled by the same
+# counter's reading for a plain 256 MiB device copy (FETCH_SIZE x2.000,
+# WRITE_SIZE x1.000); profiles/step_kernels_bonded100k.md
+TRAFFIC_GATHER_PER_PARTICLE = 737.0
+TRAFFIC_FORCES_PER_INTERACTION = 205.0
+
 
 def _load(name, path, pkgdir=None):
     if name in sys.modules:
@@ -159,6 +166,8 @@ def main():
     nd = 0.5 * (s1["num_dynamic"] + s2["num_dynamic"])
     per_launch = {"forces": nb * B_BOND + nd * B_FRICTION,
                   "gather_integrate": n * B_PARTICLE_ROT}
+    traffic = {"forces": (nb + nd) * TRAFFIC_FORCES_PER_INTERACTION,
+               "gather_integrate": n * TRAFFIC_GATHER_PER_PARTICLE}
     dom = max((k for k in per_launch if kt[k][1] > 0), key=lambda k: kt[k][0])
     avg_s = kt[dom][0] / kt[dom][1] * 1e-3
     achieved = per_launch[dom] / avg_s / 1e9
@@ -194,7 +203,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic[dom],
+            "traffic_GBs": traffic[dom] / avg_s / 1e9,
             "algorithmic_bytes_per_launch": per_launch[dom],
             "avg_launch_us": avg_s * 1e6,
         },

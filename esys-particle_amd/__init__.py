@@ -7,6 +7,6 @@ package exposes it through the reference's scripting interface (``lsm.py``).
 from .lsm import (  # noqa: F401
     BoundingBox, BrittleBeamPrms, BrittleBeamSCPrms, GravityPrms, LsmMpi, NRotElasticPrms,
     NRotElasticWallPrms, NRotSphere, RotBondPrms, RotElasticPrms, RotFrictionPrms,
-    RotSphere, Runnable, TaggedIdConnection, Vec3,
+    RestartCheckPointPrms, RotSphere, Runnable, TaggedIdConnection, Vec3,
 )
 from ._native import Engine, load_hip_library, LIB_PATH  # noqa: F401

@@ -119,6 +119,8 @@ SIGNATURES = {
     "get_friction_state": (_i, [_vp, _s, _pd, _pi]),
     "get_neighbour_pairs": (_i, [_vp, _pi, _pi]),
     "get_particle_stress": (_i, [_vp, _pd]),
+    "save_checkpoint": (_i, [_vp, _s, _i, _i]),
+    "load_checkpoint": (_i, [_vp, _s]),
     "get_bond_breaks": (_i, [_vp, _pi, _pi, _pi]),
     "get_rebuild_steps": (_i, [_vp, _pi]),
     "set_kernel_timing": (_i, [_vp, _i]),
@@ -263,6 +265,13 @@ class Engine:
             setattr(c, f, prms[f])
         self._check(self.lib.add_brittle_beam_sc_group(self.h, _b(name), ctypes.byref(c)),
                     "createInteractionGroup(BrittleBeamSCPrms)")
+
+    def save_checkpoint(self, prefix, num_time_steps=0, precision=12):
+        self._check(self.lib.save_checkpoint(self.h, _b(prefix), int(num_time_steps), int(precision)),
+                    "saveCheckPoint")
+
+    def load_checkpoint(self, path):
+        self._check(self.lib.load_checkpoint(self.h, _b(path)), "loadCheckPoint")
 
     def particle_stress(self):
         """Unscaled stress sums of the last step, (n, 3, 3), ascending id order."""

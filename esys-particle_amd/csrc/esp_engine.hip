@@ -15,6 +15,8 @@
 
 #include <algorithm>
 #include <array>
+#include <fstream>
+#include <sstream>
 #include <functional>
 #include <cmath>
 #include <cstdio>
@@ -95,6 +97,10 @@ struct DynGroup {
   long long update_ts = 0;
   std::vector<int> ex;
   int rec_off = 0, app_rec_off = 0;
+  // pairs and shear histories loaded from a restart checkpoint, stored at the
+  // first step (PIS_ED::loadCheckPointData, pi_storage_ed.hpp:225-255)
+  std::vector<int> pre_id1, pre_id2;
+  std::vector<double4> pre_ff;
 };
 
 struct BondGroup {
@@ -118,6 +124,7 @@ struct BondGroup {
 struct Wall {
   std::string name;
   double o[3], n[3], step[3];
+  double old[3];  // CWall::m_oldpos (Wall.cpp:38): the origin at creation
   bool has_motion = false;
   std::vector<double> hist;
   double force[3] = {0, 0, 0};
@@ -157,6 +164,10 @@ struct esp_engine {
   std::vector<int> h_id, h_tag;
   std::vector<double> h_pos, h_rad, h_mass, h_vel;
   std::vector<unsigned> h_flags;
+  // state a restart checkpoint may carry beyond createParticles (per slot)
+  std::vector<double> h_quat, h_ang, h_inert;  // h_inert 0: 0.4 m r^2 (0.5 m r^2 in 2-D)
+  std::vector<double> h_init;                  // CParticle::m_initpos
+  bool forced_search = false;  // a checkpoint forced the neighbour search after the last step
   std::vector<int> h_conn;
   std::unordered_map<int, int> id2slot;
   bool started = false;
@@ -472,6 +483,10 @@ void spatial_order(esp_engine* e)
   permute(e->h_rad, 1);
   permute(e->h_mass, 1);
   permute(e->h_flags, 1);
+  permute(e->h_init, 3);
+  permute(e->h_quat, 4);
+  permute(e->h_ang, 3);
+  permute(e->h_inert, 1);
   e->id2slot.clear();
   for (int i = 0; i < n; i++) e->id2slot[e->h_id[i]] = i;
 }
@@ -496,9 +511,11 @@ void upload(esp_engine* e)
     Z[i] = make_double4(0, 0, 0, 0);
     if (e->rot) {
       // CRotParticle: I = 0.4 m r^2 (3-D) / 0.5 m r^2 (2-D) (RotParticle.cpp:80-124, 160-177)
-      const double I = e->do2d ? 0.5 * m * r * r : 0.4 * m * r * r;
-      A[i] = make_double4(0, 0, 0, 1.0 / I);
-      Q[i] = make_double4(1.0, 0, 0, 0);
+      const double I = e->h_inert[i] != 0.0 ? e->h_inert[i]
+                                            : (e->do2d ? 0.5 * m * r * r : 0.4 * m * r * r);
+      A[i] = make_double4(e->h_ang[3 * i], e->h_ang[3 * i + 1], e->h_ang[3 * i + 2], 1.0 / I);
+      Q[i] = make_double4(e->h_quat[4 * i], e->h_quat[4 * i + 1], e->h_quat[4 * i + 2],
+                          e->h_quat[4 * i + 3]);
     }
   }
   h2d(e, e->pos.p, P.data(), n);
@@ -558,7 +575,36 @@ void upload(esp_engine* e)
   for (auto& G : e->dg) {
     G.napp.reserve(1);
     zero(e, G.napp.p, 1);
-    reserve_group(G, 0, G.kind == 3);
+    // pairs loaded from a checkpoint form the stored list the first update
+    // starts from (their histories persist while touching)
+    std::vector<std::pair<long long, int>> pk;
+    for (size_t k = 0; k < G.pre_id1.size(); k++) {
+      auto a = e->id2slot.find(G.pre_id1[k]), b = e->id2slot.find(G.pre_id2[k]);
+      if (a == e->id2slot.end() || b == e->id2slot.end()) continue;
+      pk.push_back({pkey(a->second, b->second), (int)k});
+    }
+    std::sort(pk.begin(), pk.end());
+    pk.erase(std::unique(pk.begin(), pk.end(),
+                         [](const std::pair<long long, int>& x, const std::pair<long long, int>& y) {
+                           return x.first == y.first;
+                         }),
+             pk.end());
+    const int np = (int)pk.size();
+    reserve_group(G, np, G.kind == 3);
+    if (np > 0) {
+      std::vector<int2> ij(np);
+      std::vector<long long> key(np);
+      std::vector<double4> ff(np);
+      for (int k = 0; k < np; k++) {
+        key[k] = pk[k].first;
+        ij[k] = make_int2((int)(pk[k].first >> 32), (int)(pk[k].first & 0xffffffffLL));
+        ff[k] = G.pre_ff[pk[k].second];
+      }
+      h2d(e, G.ij.p, ij.data(), np);
+      h2d(e, G.key.p, key.data(), np);
+      if (G.kind == 3) h2d(e, G.ff.p, ff.data(), np);
+    }
+    G.n = np;
     rebuild_presence(e, G);
   }
   e->broken_cap = std::max(total_bonds, 1);
@@ -1122,9 +1168,12 @@ void host_step(esp_engine* e, int reason)
   if (e->first_time) {
     rebuild(e, step);  // forced search of the first runOneStep
     e->first_time = false;
+  } else if (e->forced_search) {
+    // a checkpoint searched after the last step; displacements count from there
   } else if (prev.maxdisp2 > thresh_bits(e)) {
     rebuild(e, step);
   }
+  e->forced_search = false;
   e->t = step;
   apply_wall_motion(e);
   // TSubLattice::updateInteractions (SubLattice.hpp:2416-2466)
@@ -1318,6 +1367,438 @@ void dyn_view(esp_engine* e, DynGroup& G, std::vector<long long>& key, std::vect
     return ESP_ERR_STATE;                                        \
   }
 
+namespace {
+BondGroup& add_bond_group(esp_engine* e, const char* name, const esp_rot_bond_prms* prm,
+                          const double* per_bond_max, int n_per_bond)
+{
+  require_setup(e, "createInteractionGroup");
+  if (!e->rot) throw EspError(ESP_ERR_STATE, "rotational bonds need particle type RotSphere");
+  if (!name || !prm) throw EspError(ESP_ERR_ARG, "bad group");
+  if (name_taken(e, name)) throw EspError(ESP_ERR_ARG, std::string("interaction group '") + name + "' exists");
+  BondGroup G;
+  G.name = name;
+  G.prm = *prm;
+  // connections of this tag in arrival order; repeated id pairs are rejected by
+  // isIn at their second occurrence (pi_storage_e.hpp:216-250)
+  std::vector<int> cidx;
+  for (size_t c = 0; c < e->h_conn.size(); c += 3)
+    if (e->h_conn[c] == prm->tag) cidx.push_back((int)c);
+  if (per_bond_max && (int)cidx.size() != n_per_bond)
+    throw EspError(ESP_ERR_ARG, "per_bond_max has " + std::to_string(n_per_bond) + " entries for " +
+                                    std::to_string(cidx.size()) + " connections");
+  std::unordered_map<long long, int> seen;
+  for (size_t j = 0; j < cidx.size(); j++) {
+    const int a = e->h_conn[cidx[j] + 1], b = e->h_conn[cidx[j] + 2];
+    const int id1 = std::min(a, b), id2 = std::max(a, b);
+    const long long key = pkey(id1, id2);
+    if (seen.count(key)) continue;
+    seen[key] = 1;
+    auto i1 = e->id2slot.find(id1), i2 = e->id2slot.find(id2);
+    if (i1 == e->id2slot.end() || i2 == e->id2slot.end()) continue;
+    const int s1 = i1->second, s2 = i2->second;
+    double maxes[4] = {prm->max_n, prm->max_s, prm->max_t, prm->max_b};
+    if (per_bond_max) memcpy(maxes, per_bond_max + 4 * j, sizeof maxes);
+    const BondConst bc = esp_bond_constants(
+        mk3(e->h_pos[3 * s1], e->h_pos[3 * s1 + 1], e->h_pos[3 * s1 + 2]), e->h_rad[s1],
+        mk3(e->h_pos[3 * s2], e->h_pos[3 * s2 + 1], e->h_pos[3 * s2 + 2]), e->h_rad[s2], prm->kr,
+        prm->ks, prm->kt, prm->kb, maxes, prm->scaling, prm->meanR_scaling, e->do2d);
+    G.hid1.push_back(id1);
+    G.hid2.push_back(id2);
+    G.h_k4.push_back(make_double4(bc.kr, bc.ks, bc.kt, bc.kb));
+    G.h_mx4.push_back(make_double4(bc.max_n, bc.max_s, bc.max_t, bc.max_b));
+    G.h_D4.push_back(make_double4(bc.D.x, bc.D.y, bc.D.z, 0.0));
+  }
+  G.n = (int)G.hid1.size();
+  e->bg.push_back(std::move(G));
+  return e->bg.back();
+}
+}  // namespace
+
+
+// ===========================================================================
+// restart checkpoints in the reference's text format
+// ===========================================================================
+namespace {
+// Foundation/version.h:19-20, bzrversion.h:1
+const int kCheckPointVersion = 3;
+const char* kVersionLine = "ESyS-Particle_2.3.5_bzr_rev. 1218";
+
+std::string cp_file_name(const std::string& prefix, long long t, int rank)
+{
+  std::ostringstream s;
+  s << prefix << "_t=" << t << "_" << rank << ".txt";  // CheckPointParams.cpp:37-43
+  return s.str();
+}
+
+// TSubLattice::saveCheckPointData (SubLattice.hpp:3599-3672) for the one subdomain
+void cp_write(esp_engine* e, const std::string& prefix, int num_time_steps, int precision)
+{
+  if (!e->started) refresh_orders(e);
+  const int n = e->n;
+  std::vector<double4> P(n), V(n), O(n), F(n), A(n), Q(n);
+  std::vector<unsigned> fl(n);
+  if (e->started) {
+    d2h(e, P.data(), e->pos.p, n);
+    d2h(e, V.data(), e->vel.p, n);
+    d2h(e, O.data(), e->old.p, n);
+    d2h(e, F.data(), e->frc.p, n);
+    if (e->rot) {
+      d2h(e, A.data(), e->ang.p, n);
+      d2h(e, Q.data(), e->quat.p, n);
+    }
+    d2h(e, fl.data(), e->flags.p, n);
+    sync(e);
+  } else {
+    for (int i = 0; i < n; i++) {
+      P[i] = make_double4(e->h_pos[3 * i], e->h_pos[3 * i + 1], e->h_pos[3 * i + 2], e->h_rad[i]);
+      V[i] = make_double4(e->h_vel[3 * i], e->h_vel[3 * i + 1], e->h_vel[3 * i + 2], 0.0);
+      O[i] = make_double4(P[i].x, P[i].y, P[i].z, e->h_mass[i]);
+      F[i] = make_double4(0.0, 0.0, 0.0, 0.0);
+      A[i] = make_double4(e->h_ang[3 * i], e->h_ang[3 * i + 1], e->h_ang[3 * i + 2], 0.0);
+      Q[i] = make_double4(e->h_quat[4 * i], e->h_quat[4 * i + 1], e->h_quat[4 * i + 2],
+                          e->h_quat[4 * i + 3]);
+      fl[i] = e->h_flags[i];
+    }
+  }
+  const std::string data_name = cp_file_name(prefix, e->t, 1);
+  {
+    // CheckPointInfo::write (CheckPointInfo.cpp:73-96), default stream precision
+    std::ofstream o(cp_file_name(prefix, e->t, 0));
+    if (!o) throw EspError(ESP_ERR_ARG, "cannot write " + cp_file_name(prefix, e->t, 0));
+    o << "V " << kCheckPointVersion << "\n" << num_time_steps << "\n" << e->dt << "\n" << e->t
+      << "\n" << kVersionLine << "\n";
+    // GeometryInfo::writeWithoutVersion (GeometryInfo.cpp:134-155)
+    o << "BoundingBox " << e->cfg.domain_min[0] << " " << e->cfg.domain_min[1] << " "
+      << e->cfg.domain_min[2] << " " << e->cfg.domain_max[0] << " " << e->cfg.domain_max[1]
+      << " " << e->cfg.domain_max[2] << "\n";
+    o << "PeriodicBoundaries 0 0 0\nDimension " << (e->do2d ? "2D" : "3D") << "\n" << data_name;
+  }
+  std::ofstream o(data_name);
+  if (!o) throw EspError(ESP_ERR_ARG, "cannot write " + data_name);
+  o.precision(precision);
+  // ParallelParticleArray::saveCheckPointData (pp_array.hpp:1216-1235)
+  o << e->grid.x0 << " " << e->grid.y0 << " " << e->grid.z0 << "\n";
+  o << e->grid.gx << " " << e->grid.gy << " " << e->grid.gz << "\n";
+  o << n << "\n";
+  for (int i : id_order(e)) {
+    // CParticle / CRotParticle::saveCheckPointData (Particle.cpp:461-475, RotParticle.cpp:510-519)
+    const double m = O[i].w, r = P[i].w;
+    o << P[i].x << " " << P[i].y << " " << P[i].z << " " << r << " " << e->h_id[i] << " "
+      << e->h_tag[i] << " " << m << " " << e->h_init[3 * i] << " " << e->h_init[3 * i + 1] << " "
+      << e->h_init[3 * i + 2] << " " << O[i].x << " " << O[i].y << " " << O[i].z << " " << V[i].x
+      << " " << V[i].y << " " << V[i].z << " " << F[i].x << " " << F[i].y << " " << F[i].z
+      << " 0 0 0 " << ((fl[i] & PF_DYN) ? 1 : 0);
+    if (e->rot) {
+      const double I = e->h_inert[i] != 0.0 ? e->h_inert[i]
+                                            : (e->do2d ? 0.5 * m * r * r : 0.4 * m * r * r);
+      o << " " << I << " " << Q[i].x << " " << Q[i].y << " " << Q[i].z << " " << Q[i].w << " "
+        << A[i].x << " " << A[i].y << " " << A[i].z << " " << ((fl[i] & PF_ROT) ? 1 : 0);
+    }
+    o << "\n";
+  }
+  // bonded groups, name order (PIS_EB::saveCheckPointData, pi_storage_eb.hpp:78-97)
+  o << e->bg.size() << "\n";
+  for (int gi : e->bord) {
+    BondGroup& G = e->bg[gi];
+    std::vector<int2> ij(G.n);
+    std::vector<int> rank(G.n);
+    std::vector<double4> k4(G.n), mx4(G.n), D4(G.n);
+    if (e->started && G.n > 0) {
+      d2h(e, ij.data(), G.ij.p, G.n);
+      d2h(e, rank.data(), G.rank.p, G.n);
+      d2h(e, k4.data(), G.k4.p, G.n);
+      d2h(e, mx4.data(), G.mx4.p, G.n);
+      d2h(e, D4.data(), G.D4.p, G.n);
+      sync(e);
+    } else {
+      for (int b = 0; b < G.n; b++) {
+        ij[b] = make_int2(0, 0);
+        rank[b] = b;
+        k4[b] = G.h_k4[b];
+        mx4[b] = G.h_mx4[b];
+        D4[b] = G.h_D4[b];
+      }
+    }
+    std::vector<int> live;
+    for (int b = 0; b < G.n; b++)
+      if (ij[b].x >= 0) live.push_back(b);
+    std::sort(live.begin(), live.end(), [&](int a, int b) { return rank[a] < rank[b]; });
+    o << "RotBonded\n" << live.size();  // BrittleBeamSC's type string is RotBonded too
+    for (int b : live) {
+      o << "\n" << G.hid1[b] << " " << G.hid2[b] << " 1 " << k4[b].x << " " << k4[b].y << " "
+        << k4[b].w << " " << k4[b].z << " ";
+      if (G.sc) {
+        // BrittleBeamSCInteraction::saveRestartData (BrittleBeamSC.cpp:287-302)
+        o << G.prm.scaling << " " << D4[b].x << " " << D4[b].y << " " << D4[b].z << " "
+          << G.prm.tag << " " << G.sc_sigma3 << " " << G.sc_cohesion << " " << G.sc_tan
+          << std::endl;
+      } else {
+        // CRotBondedInteraction::saveRestartData (RotBondedInteraction.cpp:379-399)
+        o << G.prm.scaling << " " << G.prm.meanR_scaling << " " << G.prm.truncated << " "
+          << G.prm.beta1 << " " << G.prm.beta2 << " " << mx4[b].x << " " << mx4[b].y << " "
+          << mx4[b].z << " " << mx4[b].w << " " << D4[b].x << " " << D4[b].y << " " << D4[b].z
+          << " " << G.prm.tag << std::endl;
+      }
+    }
+    o << "\n";
+  }
+  // dynamic groups that save (PIS_ED: RotFriction), name order
+  // (PIS_ED::saveCheckPointData, pi_storage_ed.hpp:201-218).  The reference writes
+  // the first record straight after the count; a newline separates them here.
+  int nsave = 0;
+  for (auto& G : e->dg) nsave += G.kind == 3;
+  o << nsave << "\n";
+  for (int gi : e->dord) {
+    DynGroup& G = e->dg[gi];
+    if (G.kind != 3) continue;
+    std::vector<long long> keys;
+    std::vector<double4> ff;
+    if (e->started) {
+      dyn_view(e, G, keys, &ff);
+    } else {
+      for (size_t k = 0; k < G.pre_id1.size(); k++) {
+        keys.push_back(pkey(G.pre_id1[k], G.pre_id2[k]));
+        ff.push_back(G.pre_ff[k]);
+      }
+    }
+    o << "RotFriction" << std::endl << keys.size() << "\n";
+    for (size_t k = 0; k < keys.size(); k++) {
+      const int id1 = (int)(keys[k] >> 32), id2 = (int)(keys[k] & 0xffffffffLL);
+      const int s1 = e->id2slot[id1], s2 = e->id2slot[id2];
+      const double r1 = e->h_rad[s1], r2 = e->h_rad[s2];
+      double kk, kks;
+      esp_friction_k(G.k, G.ks, G.scaling, G.meanR, r1, r2, e->do2d, kk, kks);
+      const int flg = (int)ff[k].w;
+      // CRotFrictionInteraction::saveRestartData (RotFricInteraction.cpp:597-614)
+      o << kk << " " << (r1 + r2) << " " << G.mu_d << " " << G.mu_s << " " << kks << " " << e->dt
+        << " " << G.scaling << " " << G.meanR << " " << id1 << " " << id2 << " " << (flg & 1)
+        << " " << ((flg >> 1) & 1) << " " << ff[k].x << " " << ff[k].y << " " << ff[k].z
+        << std::endl;
+    }
+    o << "\n";
+  }
+  // walls, name order (SubLattice.hpp:3636-3643, CWall::writeCheckPoint, Wall.cpp:118-121)
+  std::vector<int> word(e->walls.size());
+  for (size_t w = 0; w < word.size(); w++) word[w] = (int)w;
+  std::sort(word.begin(), word.end(),
+            [&](int a, int b) { return e->walls[a].name < e->walls[b].name; });
+  o << "Walls " << e->walls.size() << "\n";
+  for (int w : word) {
+    const Wall& W = e->walls[w];
+    o << W.name << "\n"
+      << W.o[0] << " " << W.o[1] << " " << W.o[2] << " " << W.old[0] << " " << W.old[1] << " "
+      << W.old[2] << " " << W.n[0] << " " << W.n[1] << " " << W.n[2] << "\n";
+  }
+  o << "Spheres 0\nTriMesh 0\nMesh2D 0\n";
+  if (!o) throw EspError(ESP_ERR_ARG, "writing " + data_name + " failed");
+}
+
+// whitespace-separated tokens of a file
+std::vector<std::string> cp_tokens(const std::string& path)
+{
+  std::ifstream f(path);
+  if (!f) throw EspError(ESP_ERR_ARG, "cannot open " + path);
+  std::vector<std::string> t;
+  std::string s;
+  while (f >> s) t.push_back(s);
+  return t;
+}
+
+struct CpReader {
+  std::vector<std::string> t;
+  size_t k = 0;
+  std::string file;
+  const std::string& next()
+  {
+    if (k >= t.size()) throw EspError(ESP_ERR_ARG, file + ": unexpected end of checkpoint data");
+    return t[k++];
+  }
+  double d() { return std::stod(next()); }
+  long long i() { return std::stoll(next()); }
+  void expect(const char* w)
+  {
+    const std::string& s = next();
+    if (s != w) throw EspError(ESP_ERR_ARG, file + ": expected '" + w + "', found '" + s + "'");
+  }
+};
+
+// CLatticeMaster::loadCheckPointData (LatticeMaster.cpp:780-820) and
+// TSubLattice::loadCheckPointData (SubLattice.hpp:3675-3790)
+void cp_load(esp_engine* e, const std::string& info_path)
+{
+  refresh_orders(e);
+  CpReader in{cp_tokens(info_path), 0, info_path};
+  in.expect("V");
+  in.i();  // checkpoint format version
+  in.i();  // number of time steps of the saving run
+  in.d();  // time step size of the saving run (the script sets its own)
+  const long long t0 = in.i();
+  const std::string head = in.next();
+  if (head.compare(0, 4, "ESyS") != 0 && head != "LSMGeometry")
+    throw EspError(ESP_ERR_ARG, info_path + ": not a checkpoint info file");
+  in.next();  // version
+  in.expect("BoundingBox");
+  for (int c = 0; c < 6; c++) in.d();
+  in.expect("PeriodicBoundaries");
+  for (int c = 0; c < 3; c++)
+    if (in.i() != 0) throw EspError(ESP_ERR_ARG, "periodic boundaries are not supported");
+  in.expect("Dimension");
+  in.next();
+  std::vector<std::string> files(in.t.begin() + in.k, in.t.end());
+  if (files.empty()) throw EspError(ESP_ERR_ARG, info_path + ": no subdomain files listed");
+  const size_t slash = info_path.find_last_of('/');
+  const std::string dir = slash == std::string::npos ? "" : info_path.substr(0, slash + 1);
+  for (const std::string& name : files) {
+    std::string path = name;
+    if (!std::ifstream(path) && !dir.empty()) path = dir + name.substr(name.find_last_of('/') + 1);
+    CpReader r{cp_tokens(path), 0, path};
+    for (int c = 0; c < 6; c++) r.d();  // table base point and base index
+    const long long np = r.i();
+    for (long long q = 0; q < np; q++) {
+      double pos[3], vel[3], init[3], v3[3];
+      for (int c = 0; c < 3; c++) pos[c] = r.d();
+      const double rad = r.d();
+      const int id = (int)r.i();
+      const int tag = (int)r.i();
+      const double mass = r.d();
+      for (int c = 0; c < 3; c++) init[c] = r.d();
+      for (int c = 0; c < 3; c++) v3[c] = r.d();  // position at the last search
+      for (int c = 0; c < 3; c++) vel[c] = r.d();
+      for (int c = 0; c < 3; c++) v3[c] = r.d();  // force of the last step
+      for (int c = 0; c < 3; c++) v3[c] = r.d();  // circular shift
+      const bool dyn = r.i() != 0;
+      if (esp_add_particles(e, 1, &id, &tag, pos, &rad, &mass, vel) != ESP_OK)
+        throw EspError(ESP_ERR_ARG, e->err);
+      auto it = e->id2slot.find(id);
+      const int s = it == e->id2slot.end() ? -1 : it->second;  // -1: outside the table
+      if (s >= 0)
+        for (int c = 0; c < 3; c++) e->h_init[3 * s + c] = init[c];
+      if (e->rot) {
+        double rs[8];
+        for (int c = 0; c < 8; c++) rs[c] = r.d();  // inertia, quaternion, angular velocity
+        const bool rot = r.i() != 0;
+        if (s < 0) continue;
+        e->h_inert[s] = rs[0];
+        for (int c = 0; c < 4; c++) e->h_quat[4 * s + c] = rs[1 + c];
+        for (int c = 0; c < 3; c++) e->h_ang[3 * s + c] = rs[5 + c];
+        if (!rot) e->h_flags[s] &= ~PF_ROT;
+      }
+      if (s >= 0 && !dyn) e->h_flags[s] &= ~(PF_DYN | PF_ROT);
+    }
+    // bonded groups, matched to the script's groups in name order
+    const long long nbg = r.i();
+    if (nbg != (long long)e->bg.size())
+      throw EspError(ESP_ERR_ARG, "the checkpoint holds " + std::to_string(nbg) +
+                                      " bonded groups, the script created " +
+                                      std::to_string(e->bg.size()));
+    for (int gi : e->bord) {
+      BondGroup& G = e->bg[gi];
+      r.expect("RotBonded");
+      const long long nb = r.i();
+      std::unordered_map<long long, int> have;
+      for (int b = 0; b < (int)G.hid1.size(); b++) have[pkey(G.hid1[b], G.hid2[b])] = 1;
+      for (long long b = 0; b < nb; b++) {
+        int id1 = (int)r.i(), id2 = (int)r.i();
+        r.i();  // initialised flag
+        const double kr = r.d(), ks = r.d(), kb = r.d(), kt = r.d();
+        double mx[4] = {1.0, 1.0, 1.0, 1.0}, D[3];
+        if (G.sc) {
+          r.i();  // scaling
+          for (int c = 0; c < 3; c++) D[c] = r.d();
+          r.i();  // tag
+          G.sc_sigma3 = r.d();
+          G.sc_cohesion = r.d();
+          G.sc_tan = r.d();
+        } else {
+          r.i();  // scaling
+          r.i();  // mean-radius scaling
+          for (int c = 0; c < 3; c++) r.d();  // truncation, beta1, beta2 (the group's apply)
+          for (int c = 0; c < 4; c++) mx[c] = r.d();
+          for (int c = 0; c < 3; c++) D[c] = r.d();
+          r.i();  // tag
+        }
+        if (id1 > id2) std::swap(id1, id2);
+        if (!e->id2slot.count(id1) || !e->id2slot.count(id2)) continue;
+        if (!have.emplace(pkey(id1, id2), 1).second) continue;  // isIn (pi_storage_e.hpp:216-250)
+        G.hid1.push_back(id1);
+        G.hid2.push_back(id2);
+        G.h_k4.push_back(make_double4(kr, ks, kt, kb));
+        G.h_mx4.push_back(make_double4(mx[0], mx[1], mx[2], mx[3]));
+        G.h_D4.push_back(make_double4(D[0], D[1], D[2], 0.0));
+      }
+      G.n = (int)G.hid1.size();
+    }
+    // dynamic groups that save: the script's RotFriction groups in name order
+    const long long ndg = r.i();
+    std::vector<int> fric;
+    for (int gi : e->dord)
+      if (e->dg[gi].kind == 3) fric.push_back(gi);
+    if (ndg != (long long)fric.size())
+      throw EspError(ESP_ERR_ARG, "the checkpoint holds " + std::to_string(ndg) +
+                                      " saved dynamic groups, the script created " +
+                                      std::to_string(fric.size()) + " RotFriction groups");
+    for (int gi : fric) {
+      DynGroup& G = e->dg[gi];
+      r.expect("RotFriction");
+      // the reference's writer puts the first record right after the count:
+      // the section's records are 15 fields each up to the next section name
+      size_t end = r.k;
+      while (end < r.t.size() && r.t[end] != "RotFriction" && r.t[end] != "Walls") end++;
+      const size_t nt = end - r.k;
+      long long nc = 0;
+      if (nt % 15 == 1) {
+        nc = r.i();
+      } else if (nt % 15 == 0 && nt > 0) {
+        nc = (long long)(nt / 15);
+        const std::string c = std::to_string(nc);
+        if (r.t[r.k].compare(0, c.size(), c) != 0)
+          throw EspError(ESP_ERR_ARG, r.file + ": RotFriction count and records disagree");
+        r.t[r.k] = r.t[r.k].substr(c.size());
+      } else {
+        throw EspError(ESP_ERR_ARG, r.file + ": malformed RotFriction section");
+      }
+      for (long long c = 0; c < nc; c++) {
+        for (int q = 0; q < 8; q++) r.d();  // k, r0, mu_d, mu_s, ks, dt, scaling, mean-R scaling
+        int id1 = (int)r.i(), id2 = (int)r.i();
+        const int slip = (int)r.i(), touch = (int)r.i();
+        const double fx = r.d(), fy = r.d(), fz = r.d();
+        if (id1 > id2) std::swap(id1, id2);
+        G.pre_id1.push_back(id1);
+        G.pre_id2.push_back(id2);
+        G.pre_ff.push_back(make_double4(fx, fy, fz, (double)((slip ? 1 : 0) | (touch ? 2 : 0))));
+      }
+    }
+    // walls
+    r.expect("Walls");
+    const long long nw = r.i();
+    for (long long w = 0; w < nw; w++) {
+      const std::string name = r.next();
+      double v[9];
+      for (int c = 0; c < 9; c++) v[c] = r.d();
+      int wi = find_wall(e, name.c_str());
+      if (wi < 0) {
+        Wall W;
+        W.name = name;
+        for (int c = 0; c < 3; c++) W.step[c] = 0.0;
+        e->walls.push_back(W);
+        wi = (int)e->walls.size() - 1;
+      }
+      Wall& W = e->walls[wi];
+      for (int c = 0; c < 3; c++) {
+        W.o[c] = v[c];
+        W.old[c] = v[3 + c];
+        W.n[c] = v[6 + c];
+      }
+    }
+    for (const char* kind : {"Spheres", "TriMesh", "Mesh2D"}) {
+      r.expect(kind);
+      if (r.i() != 0) throw EspError(ESP_ERR_ARG, std::string(kind) + " are not supported");
+    }
+  }
+  e->t = t0;
+}
+}  // namespace
+
 extern "C" {
 
 const char* esp_create_error(void) { return g_create_err.c_str(); }
@@ -1445,6 +1926,10 @@ int esp_add_particles(esp_engine* e, int n, const int* id, const int* tag, const
     e->h_rad.push_back(r);
     e->h_mass.push_back(mass ? mass[i] : (e->do2d ? r * r : r * r * r));
     e->h_flags.push_back(PF_DYN | (e->rot ? PF_ROT : 0u));
+    for (int c = 0; c < 3; c++) e->h_init.push_back(pos[3 * i + c]);
+    e->h_quat.insert(e->h_quat.end(), {1.0, 0.0, 0.0, 0.0});
+    e->h_ang.insert(e->h_ang.end(), {0.0, 0.0, 0.0});
+    e->h_inert.push_back(0.0);
   }
   return ESP_OK;
   API_END
@@ -1498,52 +1983,6 @@ int esp_add_connections(esp_engine* e, int n, const int* id1, const int* id2, co
   API_END
 }
 
-namespace {
-BondGroup& add_bond_group(esp_engine* e, const char* name, const esp_rot_bond_prms* prm,
-                          const double* per_bond_max, int n_per_bond)
-{
-  require_setup(e, "createInteractionGroup");
-  if (!e->rot) throw EspError(ESP_ERR_STATE, "rotational bonds need particle type RotSphere");
-  if (!name || !prm) throw EspError(ESP_ERR_ARG, "bad group");
-  if (name_taken(e, name)) throw EspError(ESP_ERR_ARG, std::string("interaction group '") + name + "' exists");
-  BondGroup G;
-  G.name = name;
-  G.prm = *prm;
-  // connections of this tag in arrival order; repeated id pairs are rejected by
-  // isIn at their second occurrence (pi_storage_e.hpp:216-250)
-  std::vector<int> cidx;
-  for (size_t c = 0; c < e->h_conn.size(); c += 3)
-    if (e->h_conn[c] == prm->tag) cidx.push_back((int)c);
-  if (per_bond_max && (int)cidx.size() != n_per_bond)
-    throw EspError(ESP_ERR_ARG, "per_bond_max has " + std::to_string(n_per_bond) + " entries for " +
-                                    std::to_string(cidx.size()) + " connections");
-  std::unordered_map<long long, int> seen;
-  for (size_t j = 0; j < cidx.size(); j++) {
-    const int a = e->h_conn[cidx[j] + 1], b = e->h_conn[cidx[j] + 2];
-    const int id1 = std::min(a, b), id2 = std::max(a, b);
-    const long long key = pkey(id1, id2);
-    if (seen.count(key)) continue;
-    seen[key] = 1;
-    auto i1 = e->id2slot.find(id1), i2 = e->id2slot.find(id2);
-    if (i1 == e->id2slot.end() || i2 == e->id2slot.end()) continue;
-    const int s1 = i1->second, s2 = i2->second;
-    double maxes[4] = {prm->max_n, prm->max_s, prm->max_t, prm->max_b};
-    if (per_bond_max) memcpy(maxes, per_bond_max + 4 * j, sizeof maxes);
-    const BondConst bc = esp_bond_constants(
-        mk3(e->h_pos[3 * s1], e->h_pos[3 * s1 + 1], e->h_pos[3 * s1 + 2]), e->h_rad[s1],
-        mk3(e->h_pos[3 * s2], e->h_pos[3 * s2 + 1], e->h_pos[3 * s2 + 2]), e->h_rad[s2], prm->kr,
-        prm->ks, prm->kt, prm->kb, maxes, prm->scaling, prm->meanR_scaling, e->do2d);
-    G.hid1.push_back(id1);
-    G.hid2.push_back(id2);
-    G.h_k4.push_back(make_double4(bc.kr, bc.ks, bc.kt, bc.kb));
-    G.h_mx4.push_back(make_double4(bc.max_n, bc.max_s, bc.max_t, bc.max_b));
-    G.h_D4.push_back(make_double4(bc.D.x, bc.D.y, bc.D.z, 0.0));
-  }
-  G.n = (int)G.hid1.size();
-  e->bg.push_back(std::move(G));
-  return e->bg.back();
-}
-}  // namespace
 
 int esp_add_rot_bond_group(esp_engine* e, const char* name, const esp_rot_bond_prms* prm,
                            const double* per_bond_max, int n_per_bond)
@@ -1650,7 +2089,7 @@ int esp_add_wall(esp_engine* e, const char* name, const double o[3], const doubl
   require_setup(e, "createWall");
   Wall W;
   W.name = name;
-  for (int c = 0; c < 3; c++) { W.o[c] = o[c]; W.n[c] = n[c]; W.step[c] = 0.0; }
+  for (int c = 0; c < 3; c++) { W.o[c] = o[c]; W.n[c] = n[c]; W.step[c] = 0.0; W.old[c] = o[c]; }
   e->walls.push_back(W);
   return ESP_OK;
   API_END
@@ -1895,6 +2334,36 @@ int esp_get_friction_state(esp_engine* e, const char* name, double* ffric, int* 
     flg[i] = (int)f[i].w;
   }
   return (int)f.size();
+  API_END
+}
+
+int esp_save_checkpoint(esp_engine* e, const char* prefix, int num_time_steps, int precision)
+{
+  API_BEGIN
+  if (!prefix || !*prefix) throw EspError(ESP_ERR_ARG, "empty checkpoint file prefix");
+  if (precision < 1 || precision > 17) throw EspError(ESP_ERR_ARG, "precision must be 1..17");
+  if (e->started) {
+    HIPCHK(hipSetDevice(e->cfg.device));
+    // the reference searches neighbours before every checkpoint
+    // (LatticeMaster.cpp:2697-2700); the next step then updates every group
+    rebuild(e, e->t);
+    e->forced_search = true;
+    e->pending_host = true;
+    e->pending_reason = 0;
+  }
+  cp_write(e, prefix, num_time_steps, precision);
+  return ESP_OK;
+  API_END
+}
+
+int esp_load_checkpoint(esp_engine* e, const char* info_file)
+{
+  API_BEGIN
+  require_setup(e, "loadCheckPoint");
+  if (!info_file) throw EspError(ESP_ERR_ARG, "no checkpoint file");
+  if (e->n > 0) throw EspError(ESP_ERR_STATE, "loadCheckPoint needs a model without particles");
+  cp_load(e, info_file);
+  return ESP_OK;
   API_END
 }
 

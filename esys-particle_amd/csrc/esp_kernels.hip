@@ -264,10 +264,11 @@ __device__ __forceinline__ void rot_elastic_item(const PView& p, int c, const Fo
 __global__ __launch_bounds__(BS) void k_forces(StepCtl ctl, PView p, ForceLaunch fl)
 {
   if (step_skip(ctl)) return;
+  const int vb = (int)blockIdx.x;
   int s = 0;
-  while (s + 1 < fl.count && (int)blockIdx.x >= fl.seg[s + 1].blk0) s++;
+  while (s + 1 < fl.count && vb >= fl.seg[s + 1].blk0) s++;
   const ForceSeg& g = fl.seg[s];
-  const int c = ((int)blockIdx.x - g.blk0) * BS + (int)threadIdx.x;
+  const int c = (vb - g.blk0) * BS + (int)threadIdx.x;
   switch (g.kind) {
     case ESP_SEG_ROT_BOND: rot_bond_item(ctl, p, c, g, fl); break;
     case ESP_SEG_ROT_FRICTION: rot_friction_item(p, c, g, fl); break;
@@ -885,19 +886,27 @@ __device__ __forceinline__ void break_one(PView p, const BondSet& bs, int gb, Up
   if (b < 0 || b >= bs.n[g]) return;
   const int2 e = bs.ij[g][b];
   if (e.x < 0) return;
+  // independent memory operations first, so their latencies overlap
   const int id1 = p.id[e.x], id2 = p.id[e.y];
-  hs_erase(bs.hs[g], pair_key(id1, id2));
-  bs.ij[g][b] = make_int2(-1, -1);
   const int2 so = bs.slot_of[bs.rec_off[g] + b];
+  const unsigned ox = atomicOr(&p.flags[e.x], PF_FLAG);
+  const unsigned oy = atomicOr(&p.flags[e.y], PF_FLAG);
+  bs.ij[g][b] = make_int2(-1, -1);
+  const int ev = atomicAdd(&st->evt_count, 1);
+  const int nx = (ox & PF_FLAG) ? 0 : 1, ny = (oy & PF_FLAG) ? 0 : 1;
+  if (nx + ny) {
+    const int f0 = atomicAdd(&st->fcount, nx + ny);
+    if (nx) flist[f0] = e.x;
+    if (ny) flist[f0 + nx] = e.y;
+  }
   double* s0 = bs.slots + (size_t)bs.ss * so.x;
   double* s1 = bs.slots + (size_t)bs.ss * so.y;
   for (int k = 0; k < bs.ss; k++) {
     s0[k] = 0.0;
     s1[k] = 0.0;
   }
-  if (!(atomicOr(&p.flags[e.x], PF_FLAG) & PF_FLAG)) flist[atomicAdd(&st->fcount, 1)] = e.x;
-  if (!(atomicOr(&p.flags[e.y], PF_FLAG) & PF_FLAG)) flist[atomicAdd(&st->fcount, 1)] = e.y;
-  evt[atomicAdd(&st->evt_count, 1)] = make_int4((int)step, id1, id2, 0);
+  evt[ev] = make_int4((int)step, id1, id2, 0);
+  hs_erase(bs.hs[g], pair_key(id1, id2));
 }
 
 // the previous step's breaks on the host bookkeeping path
@@ -1003,21 +1012,36 @@ __global__ __launch_bounds__(U2_BS) void k_incremental_update(
     }
   }
   __syncthreads();
-  for (int k = 2; k <= m; k <<= 1)
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int i = tid; i < m; i += U2_BS) {
-        const int ixj = i ^ j;
-        if (ixj > i) {
-          const bool up = (i & k) == 0;
-          const long long a = s_key[i], b = s_key[ixj];
-          if ((a > b) == up) {
-            s_key[i] = b;
-            s_key[ixj] = a;
+  if (n <= U2_BS) {
+    // few candidates: each kept key moves to its rank among the kept keys (keys
+    // are distinct, the sentinel is larger than any), one pass
+    long long mine = NEW_SENTINEL;
+    int rank = 0;
+    if (tid < n) {
+      mine = s_key[tid];
+      if (mine != NEW_SENTINEL)
+        for (int u = 0; u < n; u++) rank += s_key[u] < mine ? 1 : 0;
+    }
+    __syncthreads();
+    if (mine != NEW_SENTINEL) s_key[rank] = mine;
+    __syncthreads();
+  } else {
+    for (int k = 2; k <= m; k <<= 1)
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        for (int i = tid; i < m; i += U2_BS) {
+          const int ixj = i ^ j;
+          if (ixj > i) {
+            const bool up = (i & k) == 0;
+            const long long a = s_key[i], b = s_key[ixj];
+            if ((a > b) == up) {
+              s_key[i] = b;
+              s_key[ixj] = a;
+            }
           }
         }
+        __syncthreads();
       }
-      __syncthreads();
-    }
+  }
   const int kept = s_keep;
   const int napp = *da.n_app;
   if (napp + kept > ESP_APP_CAP) s_over = 1;

@@ -249,6 +249,31 @@ class BrittleBeamSCPrms(InteractionPrms):
                 "tension_cutoff": self.tensileCutoff, "tag": self.tag}
 
 
+class RestartCheckPointPrms:
+    """RestartCheckPointPrms(fileNamePrefix, beginTimeStep, endTimeStep,
+    timeStepIncr, precision=12) (CheckPointParamsPy.cpp:140-160): restartable
+    checkpoints after every step t with begin <= t <= end and
+    (t - begin) % incr == 0, and after step end (CheckPointController.cpp:349-363)."""
+
+    def __init__(self, fileNamePrefix, beginTimeStep, endTimeStep, timeStepIncr, precision=12):
+        self.fileNamePrefix = fileNamePrefix
+        self.beginTimeStep, self.endTimeStep = int(beginTimeStep), int(endTimeStep)
+        self.timeStepIncr, self.precision = int(timeStepIncr), int(precision)
+
+    def isCheckPoint(self, t):
+        b, e, i = self.beginTimeStep, self.endTimeStep, self.timeStepIncr
+        return (b <= t <= e and (t - b) % i == 0) or t == e
+
+    def nextCheckPoint(self, t):
+        """First checkpoint step after step t, or None."""
+        b, e, i = self.beginTimeStep, self.endTimeStep, self.timeStepIncr
+        if t < b:
+            return b if b <= e else e
+        if t >= e:
+            return None
+        return min(b + ((t - b) // i + 1) * i, e)
+
+
 class GravityPrms(InteractionPrms):
     """GravityPrms(name, acceleration) (InteractionParamsPy.cpp:917-925)."""
 
@@ -285,6 +310,7 @@ class LsmMpi:
         self._engine = None
         self._pre = []
         self._post = []
+        self._checkpointer = None
 
     # the library this facade binds; the product path is the HIP engine
     def _open_native(self):
@@ -448,21 +474,55 @@ class LsmMpi:
     def addPostTimeStepRunnable(self, runnable):
         self._post.append(runnable)
 
+    # ---- restart checkpoints (LsmMpiPy.cpp:846-883) ----
+    def createRestartCheckPointer(self, prms):
+        """Restartable checkpoints in the reference's text format; a later call
+        replaces the earlier checkpointer, as the reference's single controller does."""
+        self._checkpointer = prms
+
+    def loadCheckPoint(self, fileName):
+        """Load <prefix>_t=<step>_0.txt (and the subdomain files it names) into a
+        model whose groups and walls the script has created, before the first step."""
+        self._e().load_checkpoint(fileName)
+
+    def _save_if_due(self, t):
+        cp = self._checkpointer
+        if cp is not None and cp.isCheckPoint(t):
+            self._e().save_checkpoint(cp.fileNamePrefix, self._nts, cp.precision)
+
+    def _first_step_checkpoint(self):
+        # CLatticeMaster::runOneStep saves step 0 after the first search (LatticeMaster.cpp:2635-2640)
+        if self._checkpointer is not None and self._e().stats()["time_step"] == 0:
+            self._save_if_due(0)
+
     def runTimeStep(self):
+        self._first_step_checkpoint()
         for r in self._pre:
             r.run()
         self._e().run(1)
+        self._save_if_due(self.getTimeStep())
         for r in self._post:
             r.run()
 
     def runTimeSteps(self, n):
         """n x runTimeStep; queued on the device in one call when no Python
-        runnables are registered."""
+        runnables are registered (split at checkpoint steps)."""
+        n = int(n)
         if self._pre or self._post:
-            for _ in range(int(n)):
+            for _ in range(n):
                 self.runTimeStep()
-        else:
-            self._e().run(int(n))
+            return
+        if self._checkpointer is None:
+            self._e().run(n)
+            return
+        self._first_step_checkpoint()
+        while n > 0:
+            t = self.getTimeStep()
+            nxt = self._checkpointer.nextCheckPoint(t)
+            k = n if nxt is None else min(n, nxt - t)
+            self._e().run(k)
+            n -= k
+            self._save_if_due(t + k)
 
     def run(self):
         remaining = self._nts - self.getTimeStep()

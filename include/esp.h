@@ -223,6 +223,22 @@ int esp_get_friction_state(esp_engine* e, const char* name, double* ffric, int* 
 /* Pairs of the last full neighbour list (NeighborTable::getFullList,
  * ntable.hpp:590-621), sorted by (id1, id2); returns the count when ids are NULL. */
 int esp_get_neighbour_pairs(esp_engine* e, int* id1, int* id2);
+/* Restart checkpoint in the reference's text format: <prefix>_t=<step>_0.txt
+ * (CheckPointInfo::write, CheckPointInfo.cpp:73-96) naming <prefix>_t=<step>_1.txt
+ * (TSubLattice::saveCheckPointData, SubLattice.hpp:3599-3672: particles, bonded
+ * groups, RotFriction groups, walls), `precision` significant digits (the
+ * reference's RestartCheckPointPrms default: 12).  As the reference does before
+ * a checkpoint (LatticeMaster.cpp:2697-2700), the neighbour table is rebuilt
+ * first, so the next step updates every interaction group.  num_time_steps is
+ * recorded in the info file (setNumTimeSteps of the saving script). */
+int esp_save_checkpoint(esp_engine* e, const char* prefix, int num_time_steps, int precision);
+/* LsmMpiPy::loadCheckPoint (LsmMpiPy.cpp:880-883) -> CLatticeMaster::loadCheckPointData
+ * (LatticeMaster.cpp:780-820): before the first time step, with the script's
+ * interaction groups and walls created and no particles yet: particles with their
+ * state, bonds with their stored constants (matched to the bonded groups in name
+ * order), RotFriction shear histories, wall positions; time steps continue from
+ * the checkpoint's. */
+int esp_load_checkpoint(esp_engine* e, const char* info_file);
 /* Stress sums of the last step, 9 doubles per particle (row-major), ascending
  * id order: CParticle::m_sigma (Particle.cpp:391-400) before the division by
  * the particle volume.  ESP_ERR_STATE unless a BrittleBeamSC group exists. */

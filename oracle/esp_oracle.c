@@ -1680,6 +1680,20 @@ int eso_get_particle_data(eso_engine* e, int field, double* out)
   return ESP_OK;
 }
 
+/* Restart checkpoints are a file format, not part of the per-step path this
+ * restatement checks; the engine's writer and reader are tested on their own. */
+int eso_save_checkpoint(eso_engine* e, const char* prefix, int num_time_steps, int precision)
+{
+  (void)prefix; (void)num_time_steps; (void)precision;
+  return fail(e, ESP_ERR_STATE, "checkpoints are not part of the oracle");
+}
+
+int eso_load_checkpoint(eso_engine* e, const char* info_file)
+{
+  (void)info_file;
+  return fail(e, ESP_ERR_STATE, "checkpoints are not part of the oracle");
+}
+
 int eso_get_particle_stress(eso_engine* e, double* out)
 {
   for (int k = 0; k < e->nidkey; k++) {

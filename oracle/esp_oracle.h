@@ -67,6 +67,8 @@ int eso_get_kernel_times(eso_engine* e, double* ms, long long* launches);
 /* Particle stress tensors (9 doubles, row-major, unscaled sigma of
  * CParticle::applyForce, Particle.cpp:391-400), ascending id order. */
 int eso_get_particle_stress(eso_engine* e, double* out);
+int eso_save_checkpoint(eso_engine* e, const char* prefix, int num_time_steps, int precision);
+int eso_load_checkpoint(eso_engine* e, const char* info_file);
 int eso_add_brittle_beam_sc_group(eso_engine* e, const char* name,
                                   const esp_brittle_beam_sc_prms* sc);
 /* Particle-id pairs of a group in storage order, i.e. the std::list order of
