@@ -127,7 +127,7 @@ SIGNATURES = {
     "get_kernel_times": (_i, [_vp, _pd, ctypes.POINTER(ctypes.c_longlong)]),
 }
 
-KERNEL_CLASSES = ("forces", "gather_integrate", "update")
+KERNEL_CLASSES = ("forces", "gather_integrate")
 
 
 class NativeLib:

@@ -90,8 +90,9 @@ struct DynGroup {
   int n = 0;  // main list sorted by slot key; the appendix (device count napp) follows it
   DBuf<int2> ij;
   DBuf<long long> key;
-  DBuf<double4> ff;
-  DBuf<int> napp;
+  DBuf<double4> ff;   // shear histories the next step reads
+  DBuf<double4> ff2;  // the other history buffer (the next step writes it)
+  DBuf<int> napp;     // [0] pairs appended, [1] pairs the force blocks cover
   DBuf<unsigned long long> hs;  // stored pairs (history groups)
   unsigned hs_mask = 0;
   long long update_ts = 0;
@@ -573,8 +574,8 @@ void upload(esp_engine* e)
     std::vector<double4>().swap(G.h_D4);
   }
   for (auto& G : e->dg) {
-    G.napp.reserve(1);
-    zero(e, G.napp.p, 1);
+    G.napp.reserve(2);
+    zero(e, G.napp.p, 2);
     // pairs loaded from a checkpoint form the stored list the first update
     // starts from (their histories persist while touching)
     std::vector<std::pair<long long, int>> pk;
@@ -608,7 +609,7 @@ void upload(esp_engine* e)
     rebuild_presence(e, G);
   }
   e->broken_cap = std::max(total_bonds, 1);
-  e->broken_list.reserve(e->broken_cap);
+  e->broken_list.reserve(2 * (size_t)e->broken_cap);  // one list per step parity
   e->evt.reserve(e->broken_cap);
   e->ring_size = e->K + 2;
   e->ring.reserve(e->ring_size);
@@ -806,7 +807,7 @@ void fold(esp_engine* e)
     G.key.swap(e->tmp_key);
     if (hist) G.ff.swap(e->tmp_a);
     G.n = tot;
-    zero(e, G.napp.p, 1);
+    zero(e, G.napp.p, 2);
     reserve_group(G, G.n, hist);  // keeps content when the capacity suffices
     rebuild_presence(e, G);       // room for the next appendix
   }
@@ -965,16 +966,23 @@ BondSet bond_set(esp_engine* e)
 // one step's kernels
 // ---------------------------------------------------------------------------
 // segments of the fused force-law launch: bonded groups, then dynamic groups
-ForceLaunch force_launch(esp_engine* e)
+int* broken_list_of(esp_engine* e, long long step)
+{
+  return e->broken_list.p + (size_t)(step & 1) * e->broken_cap;
+}
+
+// flip: this step reads the second history buffer (odd steps of a batch)
+ForceLaunch force_launch(esp_engine* e, long long step, bool flip)
 {
   ForceLaunch fl;
   memset(&fl, 0, sizeof fl);
   fl.do2d = e->do2d;
   fl.ss = e->ss;
   fl.dt = e->dt;
+  fl.step = (double)step;
   fl.slots = e->slots.p;
   fl.slot_of = e->slot_of.p;
-  fl.broken_list = e->broken_list.p;
+  fl.broken_list = broken_list_of(e, step);
   int blk = 0;
   for (int gi : e->bord) {
     BondGroup& G = e->bg[gi];
@@ -1004,9 +1012,12 @@ ForceLaunch force_launch(esp_engine* e)
     s.n = G.n;
     s.rec_off = G.rec_off;
     s.app_rec_off = G.app_rec_off;
-    s.n_app = G.napp.p;
+    s.n_app = G.napp.p + 1;
     s.ij = G.ij.p;
-    s.ff = G.kind == 3 ? G.ff.p : nullptr;
+    if (G.kind == 3) {
+      s.ff = flip ? G.ff2.p : G.ff.p;
+      s.ff_out = flip ? G.ff.p : G.ff2.p;
+    }
     s.k = G.k;
     s.ks = G.ks;
     s.mu_d = G.mu_d;
@@ -1019,12 +1030,13 @@ ForceLaunch force_launch(esp_engine* e)
   return fl;
 }
 
-ScLaunch sc_launch(esp_engine* e)
+ScLaunch sc_launch(esp_engine* e, long long step)
 {
   ScLaunch sl;
   memset(&sl, 0, sizeof sl);
   sl.sig = e->sig.p;
-  sl.broken_list = e->broken_list.p;
+  sl.broken_list = broken_list_of(e, step);
+  sl.step = (double)step;
   int blk = 0;
   for (int gi : e->bord) {
     BondGroup& G = e->bg[gi];
@@ -1034,6 +1046,7 @@ ScLaunch sc_launch(esp_engine* e)
     sl.n[k] = G.n;
     sl.bond_off[k] = G.bond_off;
     sl.ij[k] = G.ij.p;
+    sl.D4[k] = G.D4.p;
     sl.sigma3[k] = G.sc_sigma3;
     sl.cohesion[k] = G.sc_cohesion;
     sl.tan_angle[k] = G.sc_tan;
@@ -1042,18 +1055,32 @@ ScLaunch sc_launch(esp_engine* e)
   sl.nblocks = blk;
   return sl;
 }
-void launch_step(esp_engine* e, const StepCtl& ctl, long long step)
+void launch_step(esp_engine* e, const StepCtl& ctl, long long step, bool flip)
 {
   PView p = e->view();
-  if (ctl.guarded && !e->bg.empty()) {
+  const ForceLaunch fl = force_launch(e, step, flip);
+  UpdParams up;
+  const bool dev_update = ctl.guarded && !e->bg.empty();
+  if (dev_update) {
     // bookkeeping of the previous step's bond breaks, on the device
-    DynAppend da;
-    memset(&da, 0, sizeof da);
+    memset(&up, 0, sizeof up);
+    up.bs = bond_set(e);
+    up.broken_prev = broken_list_of(e, step - 1);
+    up.evt = e->evt.p;
+    up.step = step;
+    up.g = e->grid;
+    up.cell = e->cell.p;
+    up.cell_start = e->cell_start.p;
+    up.cell_items = e->cell_items.p;
+    up.st = e->upd.p;
+    up.flist = e->flist.p;
+    up.xcnt = e->xcnt.p;
     if (!e->dg.empty()) {
       DynGroup& G = e->dg[e->dord[0]];
+      DynAppend& da = up.da;
       da.ij = G.ij.p;
       da.key = G.key.p;
-      da.ff = G.kind == 3 ? G.ff.p : nullptr;
+      da.ff = G.kind == 3 ? (flip ? G.ff2.p : G.ff.p) : nullptr;
       da.n_app = G.napp.p;
       da.n_main = G.n;
       da.rec_off = G.rec_off;
@@ -1063,15 +1090,14 @@ void launch_step(esp_engine* e, const StepCtl& ctl, long long step)
       da.ex = excl_set(e, G);
       da.slot_of = e->slot_of.p;
       da.xbase = e->xbase;
+      up.has_group = 1;
+      up.useg = 0;
+      while (up.useg < fl.count && fl.seg[up.useg].kind == ESP_SEG_ROT_BOND) up.useg++;
     }
-    LaunchTimer lt(e, ESP_KCLASS_UPDATE, ctl.step_id);
-    launch_incremental_update(e->stream, ctl, p, bond_set(e), e->broken_list.p, e->evt.p, step,
-                              e->grid, da, e->cell.p, e->cell_start.p, e->cell_items.p, e->upd.p,
-                              e->flist.p, e->xcnt.p, e->dg.empty() ? 0 : 1);
   }
   {
     LaunchTimer lt(e, ESP_KCLASS_FORCES, ctl.step_id);
-    launch_forces(e->stream, ctl, p, force_launch(e));
+    launch_forces(e->stream, ctl, p, fl, dev_update ? &up : nullptr);
   }
   GatherParams gp;
   memset(&gp, 0, sizeof gp);
@@ -1090,11 +1116,12 @@ void launch_step(esp_engine* e, const StepCtl& ctl, long long step)
     for (int c = 0; c < 3; c++) gp.g[k][c] = e->grav[e->gord[k]].g[c];
   gp.wall_part = e->wall_part.p + (size_t)(ctl.step_id - 1) * gather_blocks(e->n) * gp.nwg * 3;
   gp.sig = e->ss == 9 ? e->sig.p : nullptr;
+  gp.n_app = e->dg.empty() ? nullptr : e->dg[e->dord[0]].napp.p;
   {
     LaunchTimer lt(e, ESP_KCLASS_GATHER, ctl.step_id);
     launch_gather_integrate(e->stream, ctl, p, gp);
   }
-  if (e->ss == 9) launch_sc_break(e->stream, ctl, p, sc_launch(e));
+  if (e->ss == 9) launch_sc_break(e->stream, ctl, p, sc_launch(e, step));
   HIPCHK(hipGetLastError());
 }
 
@@ -1144,6 +1171,26 @@ StepCtl make_ctl(esp_engine* e, long long step, int step_id, int guarded)
   return ctl;
 }
 
+// Shear histories: a step reads G.ff and writes G.ff2, then the two swap.
+// After host bookkeeping rewrote G.ff both buffers are made equal, so flag bit
+// 2 ("zero in both buffers") holds for the next step.
+void sync_history_buffers(esp_engine* e)
+{
+  for (auto& G : e->dg) {
+    if (G.kind != 3) continue;
+    G.ff2.reserve(G.ff.cap);
+    if (G.n > 0)
+      HIPCHK(hipMemcpyAsync(G.ff2.p, G.ff.p, (size_t)G.n * sizeof(double4),
+                            hipMemcpyDeviceToDevice, e->stream));
+  }
+}
+
+void swap_history_buffers(esp_engine* e)
+{
+  for (auto& G : e->dg)
+    if (G.kind == 3) G.ff.swap(G.ff2);
+}
+
 // one step with host bookkeeping: the first step, a rebuild, or an
 // incremental update that did not fit the device appendix
 void host_step(esp_engine* e, int reason)
@@ -1160,8 +1207,8 @@ void host_step(esp_engine* e, int reason)
   // device when the queue stopped in the incremental update)
   if (prev.nbroken > 0 && reason != 2 && !e->bg.empty()) {
     StepCtl c0 = make_ctl(e, step, 1, 0);
-    launch_break_apply(e->stream, c0, e->view(), bond_set(e), e->broken_list.p, e->upd.p,
-                       e->flist.p, e->evt.p, step);
+    launch_break_apply(e->stream, c0, e->view(), bond_set(e), broken_list_of(e, step - 1),
+                       e->upd.p, e->flist.p, e->evt.p, step);
     HIPCHK(hipGetLastError());
   }
   fold(e);
@@ -1190,8 +1237,10 @@ void host_step(esp_engine* e, int reason)
     e->last_ns = e->timestamp;
   }
   rebuild_csr(e);
+  sync_history_buffers(e);
   launch_clear_status(e->stream, e->ring.p, e->ring_size, step, 1, e->halted.p);
-  launch_step(e, make_ctl(e, step, 1, 0), step);
+  launch_step(e, make_ctl(e, step, 1, 0), step, false);
+  swap_history_buffers(e);
   const int nwg = (int)e->wg.size();
   launch_wall_finalize(e->stream, e->wall_part.p, gather_blocks(e->n), nwg, 1, e->wall_hist.p);
   std::vector<double> h((size_t)nwg * 3);
@@ -1216,7 +1265,7 @@ int fast_batch(esp_engine* e, int k, int* reason)
     for (auto& W : e->walls) snap.insert(snap.end(), W.o, W.o + 3);
     wall_o.push_back(snap);
     const long long step = step0 + s;
-    launch_step(e, make_ctl(e, step, s + 1, 1), step);
+    launch_step(e, make_ctl(e, step, s + 1, 1), step, (s & 1) != 0);
   }
   launch_wall_finalize(e->stream, e->wall_part.p, gather_blocks(e->n), nwg, k, e->wall_hist.p);
   int halted = 0;
@@ -1235,6 +1284,7 @@ int fast_batch(esp_engine* e, int k, int* reason)
       for (int c = 0; c < 3; c++) e->walls[w].o[c] = o[3 * w + c];
   }
   if (nwg) record_walls(e, h, done);
+  if (done & 1) swap_history_buffers(e);
   e->t += done;
   e->pending_host = done < k;
   return done;
@@ -2331,7 +2381,7 @@ int esp_get_friction_state(esp_engine* e, const char* name, double* ffric, int* 
     ffric[3 * i] = f[i].x;
     ffric[3 * i + 1] = f[i].y;
     ffric[3 * i + 2] = f[i].z;
-    flg[i] = (int)f[i].w;
+    flg[i] = (int)f[i].w & 3;  // bit 0 slipping, bit 1 touching
   }
   return (int)f.size();
   API_END

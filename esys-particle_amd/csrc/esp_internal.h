@@ -18,6 +18,9 @@
 // tracked also the contact point relative to the particle: 9 doubles) into that
 // particle's slot, and each particle sums its contiguous run of slots in order -
 // the deterministic, atomic-free force reduction.
+// Shear histories are double-buffered: a step reads one buffer and writes the
+// other, so a queued step that has to be redone after host bookkeeping still
+// finds its input.  Flag bit 2 of a history marks "zero in both buffers".
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -91,6 +94,7 @@ struct GatherParams {
   double g[ESP_MAX_GRAVITY][3];
   double* wall_part;  // this step's slot: [block][nwg][3]
   double* sig;        // stress sums [particle][9] (slots of 9 doubles only)
+  int* n_app;         // appendix counters of the incrementally updated group, or nullptr
 };
 
 // Open-addressing hash sets of pair keys (a << 32 | b): live bonds of a bonded
@@ -142,8 +146,8 @@ struct DynAppend {
   int xbase;         // first appendix slot
   int2* ij;          // whole list: main [0, n_main), appendix after it
   long long* key;
-  double4* ff;       // shear history (history groups) or nullptr
-  int* n_app;        // device counter
+  double4* ff;       // shear history read by this step (history groups) or nullptr
+  int* n_app;        // device counters: [0] pairs appended, [1] pairs the force blocks cover
   int n_main;
   int rec_off;       // records of the main list
   int app_rec_off;   // records of the appendix
@@ -168,49 +172,68 @@ struct ForceSeg {
   int bond_off;     // global index of the first bond (break list entries)
   int scaling, meanR;
   int sc;            // bonded group broken by the stress criterion (no force test)
-  const int* n_app;  // appended pairs, device count (dynamic groups)
+  const int* n_app;  // appended pairs the force blocks cover, device count (dynamic groups)
   const int2* ij;
-  double4* ff;       // shear history (RotFriction)
+  double4* ff;       // shear history read by this step (RotFriction)
+  double4* ff_out;   // shear history written by this step
   const double4* k4;
   const double4* mx4;
-  const double4* D4;
+  double4* D4;       // (D0, step of the break or 0)
   double k, ks, mu_d, mu_s;
   double truncated, beta1, beta2;
 };
 struct ForceLaunch {
   int count, nblocks, do2d;
   int ss;  // doubles per slot (6, or 9 with the contact point)
+  int ublk;  // 1: block 0 runs the incremental update (queued steps), segments follow
   double dt;
+  double step;  // this step's number (bond break stamps)
   double* slots;
   const int2* slot_of;
-  int* broken_list;
+  int* broken_list;  // this step's break list
   ForceSeg seg[ESP_MAX_FORCE_SEGS];
+};
+// TSubLattice::updateInteractions after bond breaks, for a queued step: the
+// previous step's breaks are applied and the first dynamic group takes the
+// new pairs of flagged particles; its segment in the force launch is useg.
+struct UpdParams {
+  BondSet bs;
+  const int* broken_prev;  // the previous step's break list
+  int4* evt;
+  long long step;
+  GridParams g;
+  DynAppend da;
+  const int* cell;
+  const int* cell_start;
+  const int* cell_items;
+  UpdState* st;
+  int* flist;
+  int* xcnt;
+  int has_group;
+  int useg;
 };
 // blocks a segment of n interactions takes (dynamic groups: n + ESP_APP_CAP)
 int force_blocks(int n);
-void launch_forces(hipStream_t s, StepCtl ctl, PView p, const ForceLaunch& fl);
+// up: the incremental update of a queued step (fl.ublk = 1), or nullptr
+void launch_forces(hipStream_t s, StepCtl ctl, PView p, const ForceLaunch& fl,
+                   const UpdParams* up);
 // BrittleBeamSC groups: the stress test after the step's integration, one
 // segment of blocks per group
 struct ScLaunch {
   int count, nblocks;
   int blk0[ESP_MAX_BOND_GROUPS], n[ESP_MAX_BOND_GROUPS], bond_off[ESP_MAX_BOND_GROUPS];
   const int2* ij[ESP_MAX_BOND_GROUPS];
+  double4* D4[ESP_MAX_BOND_GROUPS];
   double sigma3[ESP_MAX_BOND_GROUPS], cohesion[ESP_MAX_BOND_GROUPS], tan_angle[ESP_MAX_BOND_GROUPS];
   const double* sig;
-  int* broken_list;
+  int* broken_list;  // this step's break list
+  double step;
 };
 void launch_sc_break(hipStream_t s, StepCtl ctl, PView p, const ScLaunch& sl);
 // PIS_EB::update on the device: bonds of the previous step's break list die,
 // their particles are flagged, events are logged
 void launch_break_apply(hipStream_t s, StepCtl ctl, PView p, BondSet bs, const int* broken_list,
                         UpdState* st, int* flist, int4* evt, long long step);
-// queued steps: the same break application, then getNewList + PIS_NE/PIS_ED
-// incremental update of the first dynamic group (has_group), in one launch
-void launch_incremental_update(hipStream_t s, StepCtl ctl, PView p, BondSet bs,
-                               const int* broken_list, int4* evt, long long step, GridParams g,
-                               DynAppend da, const int* cell, const int* cell_start,
-                               const int* cell_items, UpdState* st, int* flist, int* xcnt,
-                               int has_group);
 void launch_hash_clear(hipStream_t s, HashRef h);
 void launch_hash_insert(hipStream_t s, HashRef h, const long long* keys, int n);
 void launch_bond_keys(hipStream_t s, int nb, const int2* ij, const int* id, long long* keys);

@@ -3,8 +3,10 @@
 // Per step (launched back to back on one stream, no host synchronisation):
 //   k_forces
 //       the force laws of every interaction group in one launch (one block
-//       segment per group), one thread per interaction; each writes its force
-//       record (F1, M1, M2)
+//       segment per group), one thread per interaction; each writes its share
+//       into both particles' slots.  In a queued step block 0 meanwhile applies
+//       the previous step's bond breaks and appends the new pairs of the
+//       flagged particles (and computes their forces)
 //   k_gather_integrate
 //       one thread per particle: wall and body forces, then the records of its
 //       interactions in a fixed order (CSR), then the CParticle/CRotParticle
@@ -22,6 +24,8 @@
 //   the 27-cell neighbourhood emitting pairs sorted by (id1, id2), interaction
 //   storage updates (history carried over by key lookup), bond removal,
 //   wall candidate refresh and the per-particle CSR of force records.
+#include <cstring>
+
 #include "esp_internal.h"
 #include "esp_math.h"
 
@@ -140,6 +144,9 @@ __device__ __forceinline__ void rot_bond_item(const StepCtl& ctl, const PView& p
   const double4 P1 = p.pos[e.x], P2 = p.pos[e.y];
   const double4 Q1 = p.quat[e.x], Q2 = p.quat[e.y];
   const double4 K = g.k4[b], D = g.D4[b], M = g.mx4[b];
+  // broken in an earlier step (D.w holds that step): the incremental update
+  // that runs beside this launch removes the bond and clears its slots
+  if (D.w != 0.0 && D.w < fl.step) return;
   BondResult o;
   esp_rot_bond_law(xyz(P1), dquat{Q1.x, mk3(Q1.y, Q1.z, Q1.w)}, P1.w, xyz(P2),
                    dquat{Q2.x, mk3(Q2.y, Q2.z, Q2.w)}, P2.w, K.x, K.y, K.z, K.w, xyz(D), o);
@@ -153,6 +160,7 @@ __device__ __forceinline__ void rot_bond_item(const StepCtl& ctl, const PView& p
   if (g.sc) return;  // broken by the particle stresses after the integration
   if (esp_bond_broken(o.nForce, o.shForce, o.tMoment, o.bMoment, M.x, M.y, M.z, M.w, g.truncated,
                       g.beta1, g.beta2)) {
+    g.D4[b].w = fl.step;
     const int slot = atomicAdd(&ctl.cur->nbroken, 1);
     fl.broken_list[slot] = g.bond_off + b;
   }
@@ -168,23 +176,26 @@ __device__ __forceinline__ void rot_friction_item(const PView& p, int c, const F
                                                   const ForceLaunch& fl)
 {
   const int nc = g.n;
-  double4* __restrict__ ff = g.ff;
-  if (c >= nc + *g.n_app) return;
   const int2 e = g.ij[c];
   const int2 so = fl.slot_of[record_of(c, nc, g.rec_off, g.app_rec_off)];
   const double4 P1 = p.pos[e.x], P2 = p.pos[e.y];
-  const double4 st = ff[c];
+  const double4 st = g.ff[c];
   double* s0 = fl.slots + (size_t)fl.ss * so.x;
   double* s1 = fl.slots + (size_t)fl.ss * so.y;
   const dv3 Dq = sub3(xyz(P1), xyz(P2));
   const double eq = P1.w + P2.w;
   if (!(dot3(Dq, Dq) < (eq * eq))) {
     // not touching: history cleared, no force.  A pair that did not touch in
-    // the last step already holds a zero history and zero slots.
-    if (st.w != 0.0 || st.x != 0.0 || st.y != 0.0 || st.z != 0.0) {
-      ff[c] = make_double4(0.0, 0.0, 0.0, 0.0);
+    // the last step already holds zero slots; once both history buffers hold
+    // zero (flag bit 2) nothing is written.
+    const int fw = (int)st.w;
+    if ((fw & 3) != 0 || st.x != 0.0 || st.y != 0.0 || st.z != 0.0) {
+      g.ff_out[c] = make_double4(0.0, 0.0, 0.0, 0.0);
       zero_slot(s0, fl.ss);
       zero_slot(s1, fl.ss);
+    } else if (!(fw & 4)) {
+      g.ff[c] = make_double4(0.0, 0.0, 0.0, 4.0);
+      g.ff_out[c] = make_double4(0.0, 0.0, 0.0, 4.0);
     }
     return;
   }
@@ -196,7 +207,7 @@ __device__ __forceinline__ void rot_friction_item(const PView& p, int c, const F
   dv3 f1, m1, m2;
   esp_rot_friction_law(xyz(P1), xyz(V1), xyz(W1), P1.w, xyz(P2), xyz(V2), xyz(W2), P2.w, kk, kks,
                        g.mu_d, g.mu_s, fl.dt, ffr, slipping, f1, m1, m2);
-  ff[c] = make_double4(ffr.x, ffr.y, ffr.z, (double)((slipping ? 1 : 0) | 2));
+  g.ff_out[c] = make_double4(ffr.x, ffr.y, ffr.z, (double)((slipping ? 1 : 0) | 2));
   put3(s0, f1);
   put3(s0 + 3, m1);
   put3(s1, neg3(f1));
@@ -210,7 +221,6 @@ __device__ __forceinline__ void nrot_elastic_item(const PView& p, int c, const F
                                                   const ForceLaunch& fl)
 {
   const int nc = g.n;
-  if (c >= nc + *g.n_app) return;
   const int2 e = g.ij[c];
   const int2 so = fl.slot_of[record_of(c, nc, g.rec_off, g.app_rec_off)];
   const double4 P1 = p.pos[e.x], P2 = p.pos[e.y];
@@ -237,7 +247,6 @@ __device__ __forceinline__ void rot_elastic_item(const PView& p, int c, const Fo
                                                  const ForceLaunch& fl)
 {
   const int nc = g.n;
-  if (c >= nc + *g.n_app) return;
   const int2 e = g.ij[c];
   const int2 so = fl.slot_of[record_of(c, nc, g.rec_off, g.app_rec_off)];
   const double4 P1 = p.pos[e.x], P2 = p.pos[e.y];
@@ -255,26 +264,6 @@ __device__ __forceinline__ void rot_elastic_item(const PView& p, int c, const Fo
   } else {
     put3(s0, mk3(0.0, 0.0, 0.0));
     put3(s1, mk3(0.0, 0.0, 0.0));
-  }
-}
-
-// The force laws of every interaction group in one launch: each segment of
-// blocks covers one group (bonded groups first, their law is the longest), so
-// the groups' memory latencies overlap.  Every interaction writes its slots.
-__global__ __launch_bounds__(BS) void k_forces(StepCtl ctl, PView p, ForceLaunch fl)
-{
-  if (step_skip(ctl)) return;
-  const int vb = (int)blockIdx.x;
-  int s = 0;
-  while (s + 1 < fl.count && vb >= fl.seg[s + 1].blk0) s++;
-  const ForceSeg& g = fl.seg[s];
-  const int c = (vb - g.blk0) * BS + (int)threadIdx.x;
-  switch (g.kind) {
-    case ESP_SEG_ROT_BOND: rot_bond_item(ctl, p, c, g, fl); break;
-    case ESP_SEG_ROT_FRICTION: rot_friction_item(p, c, g, fl); break;
-    case ESP_SEG_NROT_ELASTIC: nrot_elastic_item(p, c, g, fl); break;
-    case ESP_SEG_ROT_ELASTIC: rot_elastic_item(p, c, g, fl); break;
-    default: break;
   }
 }
 
@@ -379,6 +368,9 @@ __global__ __launch_bounds__(BS) void k_gather_integrate(StepCtl ctl, PView p, G
   if (step_skip(ctl)) return;
   __shared__ double lds3[3 * (BS / 64)];
   __shared__ unsigned long long mred[BS / 64];
+  // pairs appended by this step's incremental update are covered by the force
+  // blocks from the next step on
+  if (gp.n_app && blockIdx.x == 0 && threadIdx.x == 0) gp.n_app[1] = gp.n_app[0];
   const int i = blockIdx.x * BS + threadIdx.x;
   const bool act = i < p.n;
   const int ii = act ? i : 0;
@@ -919,29 +911,44 @@ __global__ void k_break_apply(StepCtl ctl, PView p, BondSet bs, const int* broke
     break_one(p, bs, broken_list[t], st, flist, evt, step);
 }
 
-// TSubLattice::updateInteractions after bond breaks, in one block: the previous
-// step's breaks are applied (break_one), then getNewList (ntable.hpp:628-664)
-// and the first dynamic group's incremental update (pi_storage_ed.hpp:137-156 /
-// pi_storage_ne.hpp:79-96): pairs of flagged particles in neighbouring cells of
-// the last build, without a distance test, minus excluded (and, for history
-// groups, stored) pairs, are appended in key order; then every flag is
-// cleared.  When the pairs do not fit the appendix, the queue halts and the
-// host does the update.
-#define U2_BS 1024
+// TSubLattice::updateInteractions after bond breaks, for a queued step, run by
+// block 0 of the step's force launch while the other blocks compute the stored
+// interactions: the previous step's breaks are applied (break_one), then
+// getNewList (ntable.hpp:628-664) and the first dynamic group's incremental
+// update (pi_storage_ed.hpp:137-156 / pi_storage_ne.hpp:79-96): pairs of
+// flagged particles in neighbouring cells of the last build, without a distance
+// test, minus excluded (and, for history groups, stored) pairs, are appended in
+// key order and their forces of this step are computed here; then every flag
+// is cleared.  The force blocks cover the appendix as it was before this step
+// (n_app[1]; the gather advances it to n_app[0]).  Bonds broken in the
+// previous step carry that step in D4.w, so the force blocks skip them without
+// waiting for this block.  When the pairs do not fit the appendix, the queue
+// halts and the host redoes the step (its input history buffer is intact).
 #define NEW_SENTINEL 0x7fffffffffffffffll
-__global__ __launch_bounds__(U2_BS) void k_incremental_update(
-    StepCtl ctl, PView p, BondSet bs, const int* broken_list, int4* evt, long long step,
-    GridParams g, DynAppend da, const int* cell, const int* cell_start, const int* cell_items,
-    UpdState* st, int* flist, int* xcnt, int has_group)
+__device__ __forceinline__ void dyn_item(const PView& p, int c, const ForceSeg& g,
+                                         const ForceLaunch& fl)
 {
-  if (step_skip(ctl)) return;
+  switch (g.kind) {
+    case ESP_SEG_ROT_FRICTION: rot_friction_item(p, c, g, fl); break;
+    case ESP_SEG_NROT_ELASTIC: nrot_elastic_item(p, c, g, fl); break;
+    case ESP_SEG_ROT_ELASTIC: rot_elastic_item(p, c, g, fl); break;
+    default: break;
+  }
+}
+
+__device__ void incremental_update(const StepCtl& ctl, const PView& p, const UpdParams& up,
+                                   const ForceLaunch& fl)
+{
+  __shared__ long long s_key[ESP_NEW_CAP];
+  __shared__ int s_cnt, s_over, s_keep;
   const int nbk = ctl.prev->nbroken;
   if (nbk <= 0) return;
   const int tid = threadIdx.x;
-  for (int t = tid; t < nbk; t += U2_BS) break_one(p, bs, broken_list[t], st, flist, evt, step);
-  if (!has_group) return;
-  __shared__ long long s_key[ESP_NEW_CAP];
-  __shared__ int s_cnt, s_over, s_keep;
+  for (int t = tid; t < nbk; t += BS)
+    break_one(p, up.bs, up.broken_prev[t], up.st, up.flist, up.evt, up.step);
+  if (!up.has_group) return;
+  const DynAppend& da = up.da;
+  const GridParams& g = up.g;
   // flags, flagged list and its count complete: within one workgroup a barrier
   // orders global memory (the waves share one vector L1)
   __syncthreads();
@@ -951,13 +958,13 @@ __global__ __launch_bounds__(U2_BS) void k_incremental_update(
     s_keep = 0;
   }
   __syncthreads();
-  const int nf = st->fcount;
+  const int nf = up.st->fcount;
   // one wave per flagged particle; lane l scans half (l / 27) of the items of
   // neighbour cell l % 27, so the dependent loads of a cell scan run in parallel
   const int lane = tid & 63;
-  for (int f = tid >> 6; f < nf; f += U2_BS / 64) {
-    const int pi = flist[f];
-    const int c = cell[pi];
+  for (int f = tid >> 6; f < nf; f += BS / 64) {
+    const int pi = up.flist[f];
+    const int c = up.cell[pi];
     const int cx = c / (g.ny * g.nz);
     const int rem = c - cx * (g.ny * g.nz);
     const int cy = rem / g.nz;
@@ -968,14 +975,14 @@ __global__ __launch_bounds__(U2_BS) void k_incremental_update(
     const int x = cx + nc / 9 - 1, y = cy + (nc / 3) % 3 - 1, z = cz + nc % 3 - 1;
     if (x < 0 || x >= g.nx || y < 0 || y >= g.ny || z < 0 || z >= g.nz) continue;
     const int c2 = g.ny * g.nz * x + g.nz * y + z;
-    const int t0 = cell_start[c2], t1 = cell_start[c2 + 1];
+    const int t0 = up.cell_start[c2], t1 = up.cell_start[c2 + 1];
     // this lane's items u = t0 + half + 2 j, four at a time with their loads in flight together
     for (int u = t0 + half; u < t1; u += 8) {
       int q[4];
       unsigned fq[4];
       int idq[4];
 #pragma unroll
-      for (int j = 0; j < 4; j++) q[j] = u + 2 * j < t1 ? cell_items[u + 2 * j] : -1;
+      for (int j = 0; j < 4; j++) q[j] = u + 2 * j < t1 ? up.cell_items[u + 2 * j] : -1;
 #pragma unroll
       for (int j = 0; j < 4; j++) {
         fq[j] = q[j] >= 0 ? p.flags[q[j]] : 0u;
@@ -999,7 +1006,7 @@ __global__ __launch_bounds__(U2_BS) void k_incremental_update(
   // A key holds the pair's slots (lower-id particle first).
   int m = 1;
   while (m < n) m <<= 1;
-  for (int t = tid; t < m; t += U2_BS) {
+  for (int t = tid; t < m; t += BS) {
     if (t < n) {
       const long long key = s_key[t];
       const int a = (int)(key >> 32), b = (int)(key & 0xffffffffll);
@@ -1012,7 +1019,7 @@ __global__ __launch_bounds__(U2_BS) void k_incremental_update(
     }
   }
   __syncthreads();
-  if (n <= U2_BS) {
+  if (n <= BS) {
     // few candidates: each kept key moves to its rank among the kept keys (keys
     // are distinct, the sentinel is larger than any), one pass
     long long mine = NEW_SENTINEL;
@@ -1028,12 +1035,12 @@ __global__ __launch_bounds__(U2_BS) void k_incremental_update(
   } else {
     for (int k = 2; k <= m; k <<= 1)
       for (int j = k >> 1; j > 0; j >>= 1) {
-        for (int i = tid; i < m; i += U2_BS) {
+        for (int i = tid; i < m; i += BS) {
           const int ixj = i ^ j;
           if (ixj > i) {
-            const bool up = (i & k) == 0;
+            const bool up_ = (i & k) == 0;
             const long long a = s_key[i], b = s_key[ixj];
-            if ((a > b) == up) {
+            if ((a > b) == up_) {
               s_key[i] = b;
               s_key[ixj] = a;
             }
@@ -1043,30 +1050,20 @@ __global__ __launch_bounds__(U2_BS) void k_incremental_update(
       }
   }
   const int kept = s_keep;
-  const int napp = *da.n_app;
+  const int napp = da.n_app[0];
   if (napp + kept > ESP_APP_CAP) s_over = 1;
   // each new record takes the next appendix slot of both its particles, pairs
   // in key order: the slot is the count of earlier kept pairs on that particle
-  int pa[ESP_NEW_CAP / U2_BS], pb[ESP_NEW_CAP / U2_BS];
-#pragma unroll
-  for (int r = 0; r < ESP_NEW_CAP / U2_BS; r++) {
-    const int t = tid + r * U2_BS;
-    pa[r] = pb[r] = 0;
-    if (t >= kept) continue;
+  for (int t = tid; t < kept; t += BS) {
     const int a = (int)(s_key[t] >> 32), b = (int)(s_key[t] & 0xffffffffll);
     int da_ = 0, db_ = 0;
     for (int u = 0; u < kept; u++) {
       const long long ku = s_key[u];
       const int a2 = (int)(ku >> 32), b2 = (int)(ku & 0xffffffffll);
-      const int ia = (a2 == a) | (b2 == a), ib = (a2 == b) | (b2 == b);
-      if (u < t) {
-        pa[r] += ia;
-        pb[r] += ib;
-      }
-      da_ += ia;
-      db_ += ib;
+      da_ += (a2 == a) | (b2 == a);
+      db_ += (a2 == b) | (b2 == b);
     }
-    if (xcnt[a] + da_ > ESP_XSLOTS || xcnt[b] + db_ > ESP_XSLOTS) s_over = 1;
+    if (up.xcnt[a] + da_ > ESP_XSLOTS || up.xcnt[b] + db_ > ESP_XSLOTS) s_over = 1;
   }
   __syncthreads();
   if (s_over) {
@@ -1076,12 +1073,16 @@ __global__ __launch_bounds__(U2_BS) void k_incremental_update(
     }
     return;
   }
-#pragma unroll
-  for (int r = 0; r < ESP_NEW_CAP / U2_BS; r++) {
-    const int t = tid + r * U2_BS;
-    if (t >= kept) continue;
+  for (int t = tid; t < kept; t += BS) {
     const long long key = s_key[t];
     const int a = (int)(key >> 32), b = (int)(key & 0xffffffffll);
+    int pa = 0, pb = 0;
+    for (int u = 0; u < t; u++) {
+      const long long ku = s_key[u];
+      const int a2 = (int)(ku >> 32), b2 = (int)(ku & 0xffffffffll);
+      pa += (a2 == a) | (b2 == a);
+      pb += (a2 == b) | (b2 == b);
+    }
     const int w = da.n_main + napp + t;
     da.ij[w] = make_int2(a, b);
     da.key[w] = key;
@@ -1089,20 +1090,44 @@ __global__ __launch_bounds__(U2_BS) void k_incremental_update(
     if (da.is_hist) hs_insert(da.present, key);
     // appendix slots start at zero (cleared at the last rebuild, used once)
     da.slot_of[da.app_rec_off + napp + t] =
-        make_int2(da.xbase + ESP_XSLOTS * a + xcnt[a] + pa[r], da.xbase + ESP_XSLOTS * b + xcnt[b] + pb[r]);
+        make_int2(da.xbase + ESP_XSLOTS * a + up.xcnt[a] + pa, da.xbase + ESP_XSLOTS * b + up.xcnt[b] + pb);
   }
   __syncthreads();  // every xcnt read above precedes the increments
-#pragma unroll
-  for (int r = 0; r < ESP_NEW_CAP / U2_BS; r++) {
-    const int t = tid + r * U2_BS;
-    if (t >= kept) continue;
+  for (int t = tid; t < kept; t += BS) {
     const long long key = s_key[t];
-    atomicAdd(&xcnt[(int)(key >> 32)], 1);
-    atomicAdd(&xcnt[(int)(key & 0xffffffffll)], 1);
+    atomicAdd(&up.xcnt[(int)(key >> 32)], 1);
+    atomicAdd(&up.xcnt[(int)(key & 0xffffffffll)], 1);
   }
-  if (tid == 0) *da.n_app = napp + kept;
-  for (int t = tid; t < nf; t += U2_BS) p.flags[flist[t]] &= ~PF_FLAG;
-  if (tid == 0) st->fcount = 0;
+  // this step's forces of the new pairs (each entry was written by the same thread)
+  const ForceSeg& us = fl.seg[up.useg];
+  const int c0 = da.n_main + napp;
+  for (int t = tid; t < kept; t += BS) dyn_item(p, c0 + t, us, fl);
+  if (tid == 0) da.n_app[0] = napp + kept;
+  for (int t = tid; t < nf; t += BS) p.flags[up.flist[t]] &= ~PF_FLAG;
+  if (tid == 0) up.st->fcount = 0;
+}
+
+// The force laws of every interaction group in one launch: each segment of
+// blocks covers one group (bonded groups first, their law is the longest), so
+// the groups' memory latencies overlap.  Every interaction writes its slots.
+// In a queued step block 0 runs the incremental update (fl.ublk).
+__global__ __launch_bounds__(BS, 4) void k_forces(StepCtl ctl, PView p, ForceLaunch fl, UpdParams up)
+{
+  if (step_skip(ctl)) return;
+  if (fl.ublk && blockIdx.x == 0) {
+    incremental_update(ctl, p, up, fl);
+    return;
+  }
+  const int vb = (int)blockIdx.x - fl.ublk;
+  int s = 0;
+  while (s + 1 < fl.count && vb >= fl.seg[s + 1].blk0) s++;
+  const ForceSeg& g = fl.seg[s];
+  const int c = (vb - g.blk0) * BS + (int)threadIdx.x;
+  if (g.kind == ESP_SEG_ROT_BOND) {
+    rot_bond_item(ctl, p, c, g, fl);
+  } else if (c < g.n + *g.n_app) {
+    dyn_item(p, c, g, fl);
+  }
 }
 
 __global__ void k_hash_clear(HashRef h)
@@ -1290,10 +1315,19 @@ int gather_blocks(int n) { return n > 0 ? nblk(n) : 0; }
 
 int force_blocks(int n) { return nblk(n); }
 
-void launch_forces(hipStream_t s, StepCtl ctl, PView p, const ForceLaunch& fl)
+void launch_forces(hipStream_t s, StepCtl ctl, PView p, const ForceLaunch& fl,
+                   const UpdParams* up)
 {
-  if (fl.nblocks <= 0) return;
-  hipLaunchKernelGGL(k_forces, dim3(fl.nblocks), dim3(BS), 0, s, ctl, p, fl);
+  UpdParams u;
+  if (up) {
+    u = *up;
+  } else {
+    memset(&u, 0, sizeof u);
+  }
+  ForceLaunch f = fl;
+  f.ublk = up ? 1 : 0;
+  if (f.nblocks + f.ublk <= 0) return;
+  hipLaunchKernelGGL(k_forces, dim3(f.nblocks + f.ublk), dim3(BS), 0, s, ctl, p, f, u);
 }
 
 void launch_break_apply(hipStream_t s, StepCtl ctl, PView p, BondSet bs, const int* broken_list,
@@ -1302,16 +1336,6 @@ void launch_break_apply(hipStream_t s, StepCtl ctl, PView p, BondSet bs, const i
   if (bs.count <= 0) return;
   hipLaunchKernelGGL(k_break_apply, dim3(16), dim3(BS), 0, s, ctl, p, bs, broken_list, st, flist,
                      evt, step);
-}
-
-void launch_incremental_update(hipStream_t s, StepCtl ctl, PView p, BondSet bs,
-                               const int* broken_list, int4* evt, long long step, GridParams g,
-                               DynAppend da, const int* cell, const int* cell_start,
-                               const int* cell_items, UpdState* st, int* flist, int* xcnt,
-                               int has_group)
-{
-  hipLaunchKernelGGL(k_incremental_update, dim3(1), dim3(U2_BS), 0, s, ctl, p, bs, broken_list,
-                     evt, step, g, da, cell, cell_start, cell_items, st, flist, xcnt, has_group);
 }
 
 void launch_hash_clear(hipStream_t s, HashRef h)
@@ -1361,6 +1385,7 @@ __global__ __launch_bounds__(BS) void k_sc_break(StepCtl ctl, PView p, ScLaunch 
   const double4 P1 = p.pos[e.x], P2 = p.pos[e.y];
   if (esp_sc_broken(xyz(P1), P1.w, sl.sig + 9 * (size_t)e.x, xyz(P2), P2.w,
                     sl.sig + 9 * (size_t)e.y, sl.sigma3[g], sl.cohesion[g], sl.tan_angle[g])) {
+    sl.D4[g][b].w = sl.step;
     const int slot = atomicAdd(&ctl.cur->nbroken, 1);
     sl.broken_list[slot] = sl.bond_off[g] + b;
   }

@@ -251,10 +251,10 @@ int esp_get_rebuild_steps(esp_engine* e, int* steps);
 
 /* ---- measurement (no counterpart in the reference) ----------------------- */
 /* Kernel classes timed by esp_get_kernel_times. */
-#define ESP_KCLASS_FORCES 0 /* force laws of every interaction group (+ break test), per step */
+#define ESP_KCLASS_FORCES 0 /* force laws of every interaction group (+ break test), with a
+                              queued step's bond-break bookkeeping in its block 0 */
 #define ESP_KCLASS_GATHER 1 /* force gather + integration, per step */
-#define ESP_KCLASS_UPDATE 2 /* bond-break bookkeeping of a queued step (break list, new pairs) */
-#define ESP_KCLASS_COUNT 3
+#define ESP_KCLASS_COUNT 2
 /* Enable (1) or disable (0) HIP-event timing of every per-step kernel launch on
  * the engine's stream; timing adds an event pair around each launch. */
 int esp_set_kernel_timing(esp_engine* e, int enable);
