@@ -1055,7 +1055,10 @@ ScLaunch sc_launch(esp_engine* e, long long step)
   sl.nblocks = blk;
   return sl;
 }
-void launch_step(esp_engine* e, const StepCtl& ctl, long long step, bool flip)
+// last: the host may read the particles' total forces and moments after this
+// step (the last step of a queued batch, every host step); other steps keep
+// them in registers only
+void launch_step(esp_engine* e, const StepCtl& ctl, long long step, bool flip, bool last)
 {
   PView p = e->view();
   const ForceLaunch fl = force_launch(e, step, flip);
@@ -1117,6 +1120,7 @@ void launch_step(esp_engine* e, const StepCtl& ctl, long long step, bool flip)
   gp.wall_part = e->wall_part.p + (size_t)(ctl.step_id - 1) * gather_blocks(e->n) * gp.nwg * 3;
   gp.sig = e->ss == 9 ? e->sig.p : nullptr;
   gp.n_app = e->dg.empty() ? nullptr : e->dg[e->dord[0]].napp.p;
+  gp.store_fm = last ? 1 : 0;
   {
     LaunchTimer lt(e, ESP_KCLASS_GATHER, ctl.step_id);
     launch_gather_integrate(e->stream, ctl, p, gp);
@@ -1239,7 +1243,7 @@ void host_step(esp_engine* e, int reason)
   rebuild_csr(e);
   sync_history_buffers(e);
   launch_clear_status(e->stream, e->ring.p, e->ring_size, step, 1, e->halted.p);
-  launch_step(e, make_ctl(e, step, 1, 0), step, false);
+  launch_step(e, make_ctl(e, step, 1, 0), step, false, true);
   swap_history_buffers(e);
   const int nwg = (int)e->wg.size();
   launch_wall_finalize(e->stream, e->wall_part.p, gather_blocks(e->n), nwg, 1, e->wall_hist.p);
@@ -1265,7 +1269,7 @@ int fast_batch(esp_engine* e, int k, int* reason)
     for (auto& W : e->walls) snap.insert(snap.end(), W.o, W.o + 3);
     wall_o.push_back(snap);
     const long long step = step0 + s;
-    launch_step(e, make_ctl(e, step, s + 1, 1), step, (s & 1) != 0);
+    launch_step(e, make_ctl(e, step, s + 1, 1), step, (s & 1) != 0, s == k - 1);
   }
   launch_wall_finalize(e->stream, e->wall_part.p, gather_blocks(e->n), nwg, k, e->wall_hist.p);
   int halted = 0;

@@ -95,6 +95,7 @@ struct GatherParams {
   double* wall_part;  // this step's slot: [block][nwg][3]
   double* sig;        // stress sums [particle][9] (slots of 9 doubles only)
   int* n_app;         // appendix counters of the incrementally updated group, or nullptr
+  int store_fm;       // store the total force and moment (the last step before the host reads them)
 };
 
 // Open-addressing hash sets of pair keys (a << 32 | b): live bonds of a bonded

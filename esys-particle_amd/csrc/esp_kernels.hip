@@ -362,6 +362,7 @@ template <bool STRESS>
 __global__ __launch_bounds__(BS) void k_gather_integrate(StepCtl ctl, PView p, GatherParams gp)
 {
   constexpr int GATHER_CHUNK = STRESS ? 2 : 4;
+  constexpr int GATHER_CHUNK6 = 4;  // slots fetched together (6-double slots)
   double sg[3][3];
   for (int i = 0; i < 3; i++)
     for (int j = 0; j < 3; j++) sg[i][j] = 0.0;
@@ -430,7 +431,30 @@ __global__ __launch_bounds__(BS) void k_gather_integrate(StepCtl ctl, PView p, G
     const int nmain = e1 - e0;
     const int cnt = nmain + xcnt;
     const int ss = STRESS ? 9 : 6;
-    for (int b = 0; b < cnt; b += GATHER_CHUNK) {
+    if (!STRESS) {
+      // 6-double slots are 16-byte aligned: three 16-byte loads each
+      for (int b = 0; b < cnt; b += GATHER_CHUNK6) {
+        double2 qa[GATHER_CHUNK6], qb[GATHER_CHUNK6], qc[GATHER_CHUNK6];
+#pragma unroll
+        for (int k = 0; k < GATHER_CHUNK6; k++) {
+          const int t = b + k;
+          if (t < cnt) {
+            const int slot = t < nmain ? e0 + t : gp.xbase + ESP_XSLOTS * i + (t - nmain);
+            const double2* q = reinterpret_cast<const double2*>(gp.slots + 6 * (size_t)slot);
+            qa[k] = q[0];
+            qb[k] = q[1];
+            qc[k] = q[2];
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < GATHER_CHUNK6; k++) {
+          if (b + k >= cnt) break;
+          F = add3(F, mk3(qa[k].x, qa[k].y, qb[k].x));
+          M = add3(M, mk3(qb[k].y, qc[k].x, qc[k].y));
+        }
+      }
+    }
+    for (int b = 0; STRESS && b < cnt; b += GATHER_CHUNK) {
       dv3 f[GATHER_CHUNK], m[GATHER_CHUNK], rl[GATHER_CHUNK];
 #pragma unroll
       for (int k = 0; k < GATHER_CHUNK; k++) {
@@ -479,7 +503,7 @@ __global__ __launch_bounds__(BS) void k_gather_integrate(StepCtl ctl, PView p, G
       }
       p.ang[i] = make_double4(w.x, w.y, w.z, A.w);
       p.quat[i] = make_double4(q.s, q.v.x, q.v.y, q.v.z);
-      p.mom[i] = make_double4(M.x, M.y, M.z, 0.0);
+      if (gp.store_fm) p.mom[i] = make_double4(M.x, M.y, M.z, 0.0);
     } else if (fl & PF_DYN) {
       if (gp.do2d) F = mk3(F.x, F.y, 0);
       v = add3(v, mul3(mul3(F, invm), dt));
@@ -487,7 +511,7 @@ __global__ __launch_bounds__(BS) void k_gather_integrate(StepCtl ctl, PView p, G
     }
     p.pos[i] = make_double4(x.x, x.y, x.z, r);
     p.vel[i] = make_double4(v.x, v.y, v.z, invm);
-    p.frc[i] = make_double4(F.x, F.y, F.z, 0.0);
+    if (gp.store_fm) p.frc[i] = make_double4(F.x, F.y, F.z, 0.0);
     if (STRESS) {
       double* o = gp.sig + 9 * (size_t)i;
       for (int a = 0; a < 3; a++)
