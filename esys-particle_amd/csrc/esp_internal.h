@@ -39,7 +39,9 @@
 
 // Status of one time step, written by that step's kernels.
 struct StepStatus {
-  unsigned long long maxdisp2;  // bits of max |x - x_old|^2 (non-negative doubles order as integers)
+  // bits of max |x - x_old|^2 when it exceeds (verletDist/2)^2, else 0
+  // (non-negative doubles order as integers)
+  unsigned long long maxdisp2;
   int nbroken;                  // bonds whose break criterion held in this step
   int pad;
 };

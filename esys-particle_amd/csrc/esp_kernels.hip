@@ -527,11 +527,22 @@ __global__ __launch_bounds__(BS) void k_gather_integrate(StepCtl ctl, PView p, G
   if (threadIdx.x == 0) {
     unsigned long long m = mred[0];
     for (int w = 1; w < BS / 64; w++) m = mred[w] > m ? mred[w] : m;
-    if (m) atomicMax(&ctl.cur->maxdisp2, m);
+    // only "beyond verletDist/2" is ever read, so smaller maxima need no atomic
+    if (m > ctl.thresh) atomicMax(&ctl.cur->maxdisp2, m);
   }
   // this block's partial of each wall force: the forces on inner particles,
-  // recomputed from the same values as above, in a fixed-order block sum
+  // recomputed from the same values as above, in a fixed-order block sum;
+  // a block without wall candidates writes zeros
+  const unsigned wall_bits = act ? (flw >> PF_WALL_SHIFT) & 0xFFu : 0u;
+  const int any_wall = gp.nwg > 0 ? __syncthreads_or(wall_bits != 0u) : 0;
   for (int w = 0; w < gp.nwg; w++) {
+    if (!any_wall) {
+      if (threadIdx.x == 0) {
+        double* q = gp.wall_part + ((size_t)blockIdx.x * gp.nwg + w) * 3;
+        q[0] = 0.0; q[1] = 0.0; q[2] = 0.0;
+      }
+      continue;
+    }
     double wx = 0.0, wy = 0.0, wz = 0.0;
     if (act && ((flw >> (PF_WALL_SHIFT + w)) & 1u) && ((flw >> (PF_INNER_SHIFT + w)) & 1u)) {
       const WallParam& W = gp.wall[w];
@@ -1146,7 +1157,16 @@ __global__ __launch_bounds__(BS, 4) void k_forces(StepCtl ctl, PView p, ForceLau
   int s = 0;
   while (s + 1 < fl.count && vb >= fl.seg[s + 1].blk0) s++;
   const ForceSeg& g = fl.seg[s];
-  const int c = (vb - g.blk0) * BS + (int)threadIdx.x;
+  // blocks go round-robin to the 8 XCDs: within a segment each XCD takes a
+  // contiguous range of blocks, so the state of particles shared by
+  // neighbouring interactions is fetched into one L2 (a bijection on the
+  // segment's blocks)
+  const int nseg = (s + 1 < fl.count ? fl.seg[s + 1].blk0 : fl.nblocks) - g.blk0;
+  const int b0 = g.blk0 + fl.ublk;  // launch index of the segment's first block
+  const int d = ((int)blockIdx.x - b0) & 7;
+  const int q8 = nseg >> 3, r8 = nseg & 7;
+  const int lbk = d * q8 + min(d, r8) + ((int)blockIdx.x - b0 - d) / 8;
+  const int c = lbk * BS + (int)threadIdx.x;
   if (g.kind == ESP_SEG_ROT_BOND) {
     rot_bond_item(ctl, p, c, g, fl);
   } else if (c < g.n + *g.n_app) {
