@@ -5,7 +5,8 @@ integration) is hand-written HIP in ``csrc/`` compiled into ``libesp.so``; this
 package exposes it through the reference's scripting interface (``lsm.py``).
 """
 from .lsm import (  # noqa: F401
-    BoundingBox, BrittleBeamPrms, BrittleBeamSCPrms, GravityPrms, LsmMpi, NRotElasticPrms,
+    BoundingBox, BrittleBeamPrms, BrittleBeamSCPrms, DampingPrms, GravityPrms, LinDampingPrms,
+    LocalDampingPrms, LsmMpi, NRotElasticPrms, RotDampingPrms, RotLocalDampingPrms,
     NRotElasticWallPrms, NRotSphere, RotBondPrms, RotElasticPrms, RotFrictionPrms,
     RestartCheckPointPrms, RotSphere, Runnable, TaggedIdConnection, Vec3,
 )

@@ -104,6 +104,9 @@ SIGNATURES = {
     "add_wall": (_i, [_vp, _s, _pd, _pd]),
     "add_elastic_wall_group": (_i, [_vp, _s, _s, _d]),
     "add_gravity": (_i, [_vp, _s, _pd]),
+    "add_damping_group": (_i, [_vp, _i, _s, ctypes.c_double, _i]),
+    "set_particle_vel": (_i, [_vp, _i, _pd]),
+    "set_particle_ang_vel": (_i, [_vp, _i, _pd]),
     "move_wall_by": (_i, [_vp, _s, _pd]),
     "set_wall_step_motion": (_i, [_vp, _s, _pd]),
     "run_time_steps": (_i, [_vp, _i]),
@@ -310,6 +313,21 @@ class Engine:
     def add_gravity(self, name, g):
         g = np.asarray(g, np.float64)
         self._check(self.lib.add_gravity(self.h, _b(name), _dp(g)), "createInteractionGroup(GravityPrms)")
+
+    def set_particle_vel(self, pid, v):
+        v = np.asarray(v, np.float64)
+        self._check(self.lib.set_particle_vel(self.h, int(pid), _dp(v)), "setParticleVelocity")
+
+    def set_particle_ang_vel(self, pid, w):
+        w = np.asarray(w, np.float64)
+        self._check(self.lib.set_particle_ang_vel(self.h, int(pid), _dp(w)),
+                    "setParticleAngularVelocity")
+
+    def add_damping_group(self, kind, name, viscosity, max_iterations):
+        """kind: 0 Damping, 1 RotDamping, 2 LocalDamping, 3 RotLocalDamping."""
+        self._check(self.lib.add_damping_group(self.h, int(kind), _b(name), float(viscosity),
+                                               int(max_iterations)),
+                    "createInteractionGroup(DampingPrms)")
 
     def move_wall_by(self, name, d):
         d = np.asarray(d, np.float64)

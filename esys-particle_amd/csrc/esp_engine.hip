@@ -142,6 +142,15 @@ struct Gravity {
   double g[3];
 };
 
+// a damping group; key: its place in the reference's damping map (the type
+// name for Damping / RotDamping, the group name for the local forms)
+struct Damping {
+  int type;
+  std::string key;
+  double visc;
+  int maxit;
+};
+
 inline long long pkey(int a, int b)
 {
   return (long long)(((unsigned long long)(unsigned)a << 32) | (unsigned long long)(unsigned)b);
@@ -210,6 +219,8 @@ struct esp_engine {
   std::vector<Wall> walls;
   std::vector<WallGroup> wg;
   std::vector<Gravity> grav;
+  std::vector<Damping> damp;  // sorted by key
+  DBuf<double> inert;         // moments of inertia, uploaded when a rotational damping exists
   std::vector<int> dord, bord, word, gord;
 
   long long t = 0, timestamp = 0, last_ns = -1;
@@ -530,6 +541,17 @@ void upload(esp_engine* e)
   }
   h2d(e, e->flags.p, e->h_flags.data(), n);
   h2d(e, e->id.p, e->h_id.data(), n);
+  bool rot_damping = false;
+  for (auto& D : e->damp) rot_damping |= D.type == ESP_ROT_DAMPING;
+  if (rot_damping) {
+    std::vector<double> I(n);
+    for (int i = 0; i < n; i++) {
+      const double m = e->h_mass[i], r = e->h_rad[i];
+      I[i] = e->h_inert[i] != 0.0 ? e->h_inert[i] : (e->do2d ? 0.5 * m * r * r : 0.4 * m * r * r);
+    }
+    e->inert.reserve(cap);
+    h2d(e, e->inert.p, I.data(), n);
+  }
 
   int total_bonds = 0;
   for (int gi : e->bord) {
@@ -1121,6 +1143,13 @@ void launch_step(esp_engine* e, const StepCtl& ctl, long long step, bool flip, b
   gp.sig = e->ss == 9 ? e->sig.p : nullptr;
   gp.n_app = e->dg.empty() ? nullptr : e->dg[e->dord[0]].napp.p;
   gp.store_fm = last ? 1 : 0;
+  gp.ndamp = (int)e->damp.size();
+  for (int d = 0; d < gp.ndamp; d++) {
+    gp.damp_type[d] = e->damp[d].type;
+    gp.damp_visc[d] = e->damp[d].visc;
+    gp.damp_maxit[d] = e->damp[d].maxit;
+  }
+  gp.inert = e->inert.p;
   {
     LaunchTimer lt(e, ESP_KCLASS_GATHER, ctl.step_id);
     launch_gather_integrate(e->stream, ctl, p, gp);
@@ -2023,6 +2052,55 @@ int esp_set_particle_non_rotational(esp_engine* e, int tag)
   API_END
 }
 
+namespace {
+// particle id -> slot (arrival order before the first step, spatial order after)
+int slot_of_id(esp_engine* e, int id)
+{
+  if (e->started) {
+    auto it = e->id2slot.find(id);
+    return it == e->id2slot.end() ? -1 : it->second;
+  }
+  for (int i = 0; i < e->n; i++)
+    if (e->h_id[i] == id) return i;
+  return -1;
+}
+
+// writes the xyz part of one particle's double4 field on the device
+void set_xyz(esp_engine* e, double4* field, int slot, const double v[3])
+{
+  HIPCHK(hipMemcpyAsync(reinterpret_cast<double*>(field + slot), v, 3 * sizeof(double),
+                        hipMemcpyHostToDevice, e->stream));
+  sync(e);
+}
+}  // namespace
+
+int esp_set_particle_vel(esp_engine* e, int id, const double v[3])
+{
+  API_BEGIN
+  if (!v) throw EspError(ESP_ERR_ARG, "no velocity");
+  const int s = slot_of_id(e, id);
+  if (s < 0) throw EspError(ESP_ERR_ARG, "unknown particle id " + std::to_string(id));
+  if (e->started) set_xyz(e, e->vel.p, s, v);
+  else
+    for (int c = 0; c < 3; c++) e->h_vel[3 * s + c] = v[c];
+  return ESP_OK;
+  API_END
+}
+
+int esp_set_particle_ang_vel(esp_engine* e, int id, const double w[3])
+{
+  API_BEGIN
+  if (!w) throw EspError(ESP_ERR_ARG, "no angular velocity");
+  if (!e->rot) throw EspError(ESP_ERR_ARG, "setParticleAngularVelocity needs rotational particles");
+  const int s = slot_of_id(e, id);
+  if (s < 0) throw EspError(ESP_ERR_ARG, "unknown particle id " + std::to_string(id));
+  if (e->started) set_xyz(e, e->ang.p, s, w);
+  else
+    for (int c = 0; c < 3; c++) e->h_ang[3 * s + c] = w[c];
+  return ESP_OK;
+  API_END
+}
+
 int esp_add_connections(esp_engine* e, int n, const int* id1, const int* id2, const int* tag)
 {
   API_BEGIN
@@ -2172,6 +2250,34 @@ int esp_add_gravity(esp_engine* e, const char* name, const double g[3])
   G.name = name;
   for (int c = 0; c < 3; c++) G.g[c] = g[c];
   e->grav.push_back(G);
+  return ESP_OK;
+  API_END
+}
+
+int esp_add_damping_group(esp_engine* e, int type, const char* name, double viscosity,
+                          int max_iterations)
+{
+  API_BEGIN
+  require_setup(e, "createInteractionGroup");
+  static const char* kTypeName[4] = {"Damping", "RotDamping", "LocalDamping", "RotLocalDamping"};
+  if (type < 0 || type > 3) throw EspError(ESP_ERR_ARG, "unknown damping type");
+  if ((type == ESP_ROT_DAMPING || type == ESP_ROT_LOCAL_DAMPING) && !e->rot)
+    throw EspError(ESP_ERR_ARG,
+                   std::string("Trying to setup Damping of unknown type: ") + kTypeName[type]);
+  if (!name) throw EspError(ESP_ERR_ARG, "no damping group name");
+  const std::string key =
+      (type == ESP_DAMPING || type == ESP_ROT_DAMPING) ? std::string(kTypeName[type]) : std::string(name);
+  for (auto& D : e->damp)
+    if (D.key == key) return ESP_OK;  // the map keeps its first entry
+  if (e->damp.size() >= ESP_MAX_DAMPING) throw EspError(ESP_ERR_CAPACITY, "more than 4 damping groups");
+  Damping D;
+  D.type = type;
+  D.key = key;
+  D.visc = viscosity;
+  D.maxit = max_iterations;
+  e->damp.push_back(D);
+  std::sort(e->damp.begin(), e->damp.end(),
+            [](const Damping& a, const Damping& b) { return a.key < b.key; });
   return ESP_OK;
   API_END
 }

@@ -27,8 +27,11 @@
 
 #include <cstdint>
 
+#include "../../include/esp.h"
+
 #define ESP_MAX_WALL_GROUPS 8
 #define ESP_MAX_GRAVITY 4
+#define ESP_MAX_DAMPING 4
 #define ESP_MAX_EXCL 4
 
 #define PF_DYN 1u
@@ -98,6 +101,12 @@ struct GatherParams {
   double* sig;        // stress sums [particle][9] (slots of 9 doubles only)
   int* n_app;         // appendix counters of the incrementally updated group, or nullptr
   int store_fm;       // store the total force and moment (the last step before the host reads them)
+  // damping groups, applied last in the reference's map order of their keys
+  int ndamp;
+  int damp_type[ESP_MAX_DAMPING];  // ESP_DAMPING, ESP_ROT_DAMPING, ...
+  double damp_visc[ESP_MAX_DAMPING];
+  int damp_maxit[ESP_MAX_DAMPING];
+  const double* inert;  // moment of inertia per particle (rotational damping)
 };
 
 // Open-addressing hash sets of pair keys (a << 32 | b): live bonds of a bonded

@@ -474,6 +474,36 @@ __global__ __launch_bounds__(BS) void k_gather_integrate(StepCtl ctl, PView p, G
       }
     }
     const double dt = gp.dt;
+    // damping groups, last (SubLattice.hpp:2143-2147), in their map order
+    for (int d = 0; d < gp.ndamp; d++) {
+      const int ty = gp.damp_type[d];
+      const double visc = gp.damp_visc[d];
+      if (ty == ESP_DAMPING || ty == ESP_ROT_DAMPING) {
+        const bool rd = ty == ESP_ROT_DAMPING;
+        const double mi = rd ? gp.inert[i] : mass;
+        const dv3 vr = esp_damping_vrel(rd ? xyz(A) : v, rd ? M : F, rd ? A.w : invm, mi, visc,
+                                        dt, gp.damp_maxit[d]);
+        const dv3 fd = mul3(mul3(vr, -1.0 * visc), mi);
+        if (rd) {
+          M = add3(M, fd);
+        } else {
+          F = add3(F, fd);
+          if (STRESS) stress_add(sg, fd, sub3(x, x));
+        }
+      } else {
+        const bool rd = ty == ESP_ROT_LOCAL_DAMPING;
+        const dv3 vv = rd ? xyz(A) : v, ff = rd ? M : F;
+        const dv3 fd = mul3(mk3(esp_local_damping_c(ff.x, vv.x), esp_local_damping_c(ff.y, vv.y),
+                                esp_local_damping_c(ff.z, vv.z)),
+                            -1.0 * visc);
+        if (rd) {
+          M = add3(M, fd);
+        } else {
+          F = add3(F, fd);
+          if (STRESS) stress_add(sg, fd, sub3(x, x));
+        }
+      }
+    }
     if (gp.rot) {
       dv3 w = xyz(A);
       dquat q = dquat{Q.x, mk3(Q.y, Q.z, Q.w)};

@@ -188,6 +188,39 @@ ESP_HD void esp_rot_bond_law(dv3 pos1, dquat q1, double r1, dv3 pos2, dquat q2, 
   o.nForce = ((dist < r_0Norm) ? -1.0 : 1.0) * norm3(Fr);
 }
 
+// CDamping::calcForces / CRotDamping::calcForces (Damping.hpp:87-107,
+// RotDamping.hpp:58-84): the damped relative velocity solves
+// v_rel = v - v_ref + s dt (f - v_rel visc m) by fixed-point iteration from
+// zero while the squared norm of the last change, squared, exceeds 1e-12, at
+// most maxit times; a run that reaches maxit gives v_rel = 0.  The reference
+// velocity is zero (DampingPrms), so v - v_ref is v.  For the rotational form
+// v, f, s, m are the angular velocity, moment, 1/I and I.
+ESP_HD dv3 esp_damping_vrel(dv3 v, dv3 f, double s, double m, double visc, double dt, int maxit)
+{
+  dv3 v_rel = mk3(0.0, 0.0, 0.0);
+  double error = 1.0;
+  int count = 0;
+  while ((error * error > 1e-12) & (count < maxit)) {
+    count++;
+    const dv3 v_old = v_rel;
+    v_rel = add3(v, mul3(sub3(f, mul3(mul3(v_rel, visc), m)), s * dt));
+    error = dot3(sub3(v_rel, v_old), sub3(v_rel, v_old));
+  }
+  if (count >= maxit) v_rel = mk3(0.0, 0.0, 0.0);
+  return v_rel;
+}
+
+// CLocalDamping / CRotLocalDamping::calcForces (LocalDamping.hpp:82-117,
+// RotLocalDamping.hpp:82-117): each component of the force (moment) against
+// the sign of the velocity (angular velocity) component, zero where it is zero
+ESP_HD double esp_local_damping_c(double f, double v)
+{
+  double d = fabs(f);
+  if (v < 0) d *= -1.;
+  if (v == 0) d = 0.0;
+  return d;
+}
+
 // CRotBondedInteraction::broken (RotBondedInteraction.cpp:175-219), evaluated
 // on the values calcForces cached for the same bond.
 ESP_HD bool esp_bond_broken(double nForce, double shForce, double tMoment, double bMoment,

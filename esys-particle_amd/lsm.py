@@ -281,6 +281,51 @@ class GravityPrms(InteractionPrms):
         self.name, self.acceleration = name, Vec3(acceleration)
 
 
+class DampingPrms(InteractionPrms):
+    """DampingPrms(type, name, viscosity, maxIterations=100) with type "Damping" or
+    "RotDamping" (InteractionParamsPy.cpp:36-62, 977-997): velocity damping solved
+    by fixed-point iteration (Damping.hpp:87-107, RotDamping.hpp:58-84)."""
+
+    _KIND = {"Damping": 0, "RotDamping": 1}
+
+    def __init__(self, type, name, viscosity, maxIterations=100):
+        if type not in self._KIND:
+            raise RuntimeError("Trying to setup Damping of unknown type: %s" % type)
+        self.type, self.name = type, name
+        self.viscosity, self.maxIterations = float(viscosity), int(maxIterations)
+
+
+class LinDampingPrms(DampingPrms):
+    """LinDampingPrms(name, viscosity, maxIterations=100) = DampingPrms("Damping", ...)."""
+
+    def __init__(self, name, viscosity, maxIterations=100):
+        DampingPrms.__init__(self, "Damping", name, viscosity, maxIterations)
+
+
+class RotDampingPrms(DampingPrms):
+    """RotDampingPrms(name, viscosity, maxIterations=100) = DampingPrms("RotDamping", ...)."""
+
+    def __init__(self, name, viscosity, maxIterations=100):
+        DampingPrms.__init__(self, "RotDamping", name, viscosity, maxIterations)
+
+
+class LocalDampingPrms(InteractionPrms):
+    """LocalDampingPrms(name, viscosity): each force component against the sign
+    of the velocity component (LocalDamping.hpp:82-117)."""
+
+    type = "LocalDamping"
+
+    def __init__(self, name, viscosity):
+        self.name, self.viscosity = name, float(viscosity)
+
+
+class RotLocalDampingPrms(LocalDampingPrms):
+    """RotLocalDampingPrms(name, viscosity): the same on moment and angular
+    velocity (RotLocalDamping.hpp:82-117)."""
+
+    type = "RotLocalDamping"
+
+
 class NRotElasticWallPrms(InteractionPrms):
     """NRotElasticWallPrms(name, wallName, normalK) (WallPrmsPy.cpp)."""
 
@@ -409,6 +454,12 @@ class LsmMpi:
     def setParticleNonRotational(self, tag):
         self._e().set_particle_non_rotational(tag)
 
+    def setParticleVelocity(self, id, Velocity):
+        self._e().set_particle_vel(id, list(Vec3(Velocity)))
+
+    def setParticleAngularVelocity(self, id, angularVelocity):
+        self._e().set_particle_ang_vel(id, list(Vec3(angularVelocity)))
+
     # ---- connections / groups ----
     def createConnections(self, connections):
         cs = list(connections)
@@ -438,12 +489,27 @@ class LsmMpi:
                 "meanR_scaling": int(prms.meanR_scaling)})
         elif isinstance(prms, GravityPrms):
             e.add_gravity(prms.name, list(prms.acceleration))
+        elif isinstance(prms, DampingPrms):
+            e.add_damping_group(DampingPrms._KIND[prms.type], prms.name, prms.viscosity,
+                                prms.maxIterations)
+        elif isinstance(prms, LocalDampingPrms):
+            e.add_damping_group(3 if prms.type == "RotLocalDamping" else 2, prms.name,
+                                prms.viscosity, 0)
         elif isinstance(prms, NRotElasticWallPrms):
             e.add_elastic_wall_group(prms.name, prms.wallName, prms.normalK)
         else:
             raise TypeError("interaction parameters %r are not supported" % type(prms).__name__)
 
     def createGravity(self, prms):
+        self.createInteractionGroup(prms)
+
+    def createDamping(self, prms):
+        self.createInteractionGroup(prms)
+
+    def createLocalDamping(self, prms):
+        self.createInteractionGroup(prms)
+
+    def createRotLocalDamping(self, prms):
         self.createInteractionGroup(prms)
 
     def createExclusion(self, interactionName1, interactionName2):

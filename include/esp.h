@@ -139,6 +139,12 @@ int esp_set_particle_density(esp_engine* e, int tag, int mask, double rho);
  * RotParticle.h:204-206): applied to particles with the given tag. */
 int esp_set_particle_non_dynamic(esp_engine* e, int tag);
 int esp_set_particle_non_rotational(esp_engine* e, int tag);
+/* setParticleVelocity / setParticleAngularVelocity (LsmMpiPy.cpp:1234-1242 ->
+ * CLatticeMaster::setParticleVel / setParticleAngVel, LatticeMaster.cpp:1499):
+ * the particle with the given id, before or between time steps; the angular
+ * form needs rotational particles. */
+int esp_set_particle_vel(esp_engine* e, int id, const double v[3]);
+int esp_set_particle_ang_vel(esp_engine* e, int id, const double w[3]);
 
 /* ---- bonds -------------------------------------------------------------- */
 /* createConnections (LsmMpiPy.cpp:284-288, SubLattice.hpp:600-620). */
@@ -189,6 +195,21 @@ int esp_add_elastic_wall_group(esp_engine* e, const char* name, const char* wall
                                double k);
 /* createInteractionGroup(GravityPrms): BodyForceGroup (BodyForceGroup.hpp:41-65). */
 int esp_add_gravity(esp_engine* e, const char* name, const double g[3]);
+/* createInteractionGroup(DampingPrms / LinDampingPrms / RotDampingPrms /
+ * LocalDampingPrms / RotLocalDampingPrms) -> TSubLattice::doAddDamping
+ * (SubLattice.hpp:1747-1783, RotSubLattice.hpp:78-112).  Damping groups act
+ * after every other force (SubLattice.hpp:2143-2147) in the order of their map
+ * keys: the type name for ESP_DAMPING ("Damping") and ESP_ROT_DAMPING
+ * ("RotDamping"), the group name for the local forms; a second group with an
+ * existing key is ignored, as the reference's map insert ignores it.  The
+ * rotational forms need rotational particles.  max_iterations is used by
+ * ESP_DAMPING and ESP_ROT_DAMPING only (the reference velocity is zero). */
+#define ESP_DAMPING 0           /* CDamping (Damping.hpp:87-107) */
+#define ESP_ROT_DAMPING 1       /* CRotDamping (RotDamping.hpp:58-84) */
+#define ESP_LOCAL_DAMPING 2     /* CLocalDamping (LocalDamping.hpp:82-117) */
+#define ESP_ROT_LOCAL_DAMPING 3 /* CRotLocalDamping (RotLocalDamping.hpp:82-117) */
+int esp_add_damping_group(esp_engine* e, int type, const char* name, double viscosity,
+                          int max_iterations);
 /* moveWallBy (LsmMpiPy.cpp:1161, SubLattice.hpp:2878-2893): immediate. */
 int esp_move_wall_by(esp_engine* e, const char* name, const double d[3]);
 /* A pre-time-step runnable that calls moveWallBy(name, d) at the start of every

@@ -205,6 +205,16 @@ typedef struct {
   v3 g;
 } ograv;
 
+/* a damping group (TSubLattice::doAddDamping, SubLattice.hpp:1747-1783,
+ * RotSubLattice.hpp:78-112): key = its key in the damping map, "Damping" /
+ * "RotDamping" for those types, the group name for the local forms */
+typedef struct {
+  char key[128];
+  int type;
+  double visc;
+  int maxit;
+} odamp;
+
 struct eso_engine {
   esp_config cfg;
   int rot, do2d;
@@ -234,6 +244,8 @@ struct eso_engine {
   int nwg, capwg;
   ograv* gv;
   int ngv, capgv;
+  odamp dp[4];
+  int ndp;
   /* step bookkeeping */
   long long t, timestamp, last_ns;
   int first_time;
@@ -848,6 +860,38 @@ static const char* dg_name(const eso_engine* e, int i) { return e->dg[i].name; }
 static const char* bg_name(const eso_engine* e, int i) { return e->bg[i].name; }
 static const char* wg_name(const eso_engine* e, int i) { return e->wg[i].name; }
 static const char* gv_name(const eso_engine* e, int i) { return e->gv[i].name; }
+static const char* dp_key(const eso_engine* e, int i) { return e->dp[i].key; }
+
+/* CDamping::calcForces / CRotDamping::calcForces (Damping.hpp:87-107,
+ * RotDamping.hpp:58-84): v_rel = v - v_ref + s*dt*(f - v_rel*visc*m) iterated
+ * from zero while error*error > s_limit2 (1e-12, error = norm2 of the change)
+ * and count < maxit; reaching maxit gives zero.  v_ref = 0 (DampingPrms). */
+static v3 damping_vrel(v3 v, v3 frc, double s, double mass, double visc, double dt, int maxit)
+{
+  v3 v_rel = V(0.0, 0.0, 0.0);
+  double error = 1.0;
+  int count = 0;
+  while ((error * error > 1e-12) & (count < maxit)) {
+    count++;
+    v3 v_old = v_rel;
+    double sdt = s * dt;
+    v3 inner = vsub(frc, vmul(vmul(v_rel, visc), mass));
+    v_rel = vadd(vsub(v, V(0.0, 0.0, 0.0)), V(sdt * inner.x, sdt * inner.y, sdt * inner.z));
+    error = vnorm2(vsub(v_rel, v_old));
+  }
+  if (count >= maxit) v_rel = V(0.0, 0.0, 0.0);
+  return v_rel;
+}
+
+/* CLocalDamping / CRotLocalDamping::calcForces (LocalDamping.hpp:82-117,
+ * RotLocalDamping.hpp:82-117), one component */
+static double local_damping_c(double f, double v)
+{
+  double d = fabs(f);
+  if (v < 0) d *= -1.;
+  if (v == 0) d = 0.0;
+  return d;
+}
 
 static void bg_keys(obgroup* G)
 {
@@ -1222,6 +1266,35 @@ static void calc_and_integrate(eso_engine* e)
   }
   free(bo);
 
+  /* last, but not least - damping (m_damping, key order, SubLattice.hpp:2143-2147) */
+  int* dpo = name_order(e->ndp, dp_key, e);
+  for (int k = 0; k < e->ndp; k++) {
+    const odamp* D = &e->dp[dpo[k]];
+    for (int i = 0; i < e->np; i++) {
+      opart* p = &e->P[i];
+      if (D->type == ESP_DAMPING) {
+        v3 vr = damping_vrel(p->vel, p->force, p->inv_mass, p->mass, D->visc, e->dt, D->maxit);
+        double c = -1.0 * D->visc;
+        v3 f = vmul(V(c * vr.x, c * vr.y, c * vr.z), p->mass);
+        apply_force(p, f, p->pos);
+      } else if (D->type == ESP_ROT_DAMPING) {
+        double s = 1.0 / p->inert, in = p->inert;
+        v3 vr = damping_vrel(p->angvel, p->moment, s, in, D->visc, e->dt, D->maxit);
+        double c = -1.0 * D->visc;
+        apply_moment(p, vmul(V(c * vr.x, c * vr.y, c * vr.z), in));
+      } else {
+        int rd = D->type == ESP_ROT_LOCAL_DAMPING;
+        v3 v = rd ? p->angvel : p->vel, f = rd ? p->moment : p->force;
+        v3 d = V(local_damping_c(f.x, v.x), local_damping_c(f.y, v.y), local_damping_c(f.z, v.z));
+        double c = -1.0 * D->visc;
+        v3 fd = V(c * d.x, c * d.y, c * d.z);
+        if (rd) apply_moment(p, fd);
+        else apply_force(p, fd, p->pos);
+      }
+    }
+  }
+  free(dpo);
+
   for (int i = 0; i < e->np; i++) integrate_particle(&e->P[i], e->dt, e->rot, e->do2d);
   for (int w = 0; w < e->nwl; w++) {
     owall* W = &e->wl[w];
@@ -1407,6 +1480,28 @@ int eso_set_particle_non_rotational(eso_engine* e, int tag)
   for (int i = 0; i < e->np; i++)
     if (e->P[i].tag == tag) e->P[i].is_rot = 0;
   return ESP_OK;
+}
+
+/* CLatticeMaster::setParticleVel / setParticleAngVel (LatticeMaster.cpp:1499) */
+int eso_set_particle_vel(eso_engine* e, int id, const double v[3])
+{
+  for (int i = 0; i < e->np; i++)
+    if (e->P[i].id == id) {
+      e->P[i].vel = V(v[0], v[1], v[2]);
+      return ESP_OK;
+    }
+  return fail(e, ESP_ERR_ARG, "unknown particle id %d", id);
+}
+
+int eso_set_particle_ang_vel(eso_engine* e, int id, const double w[3])
+{
+  if (!e->rot) return fail(e, ESP_ERR_ARG, "setParticleAngularVelocity needs rotational particles");
+  for (int i = 0; i < e->np; i++)
+    if (e->P[i].id == id) {
+      e->P[i].angvel = V(w[0], w[1], w[2]);
+      return ESP_OK;
+    }
+  return fail(e, ESP_ERR_ARG, "unknown particle id %d", id);
 }
 
 int eso_add_connections(eso_engine* e, int n, const int* id1, const int* id2, const int* tag)
@@ -1611,6 +1706,26 @@ int eso_add_gravity(eso_engine* e, const char* name, const double g[3])
   ograv* G = &e->gv[e->ngv++];
   strcpy(G->name, name);
   G->g = V(g[0], g[1], g[2]);
+  return ESP_OK;
+}
+
+int eso_add_damping_group(eso_engine* e, int type, const char* name, double viscosity,
+                          int max_iterations)
+{
+  static const char* tname[4] = {"Damping", "RotDamping", "LocalDamping", "RotLocalDamping"};
+  if (type < 0 || type > 3) return fail(e, ESP_ERR_ARG, "unknown damping type");
+  if ((type == ESP_ROT_DAMPING || type == ESP_ROT_LOCAL_DAMPING) && !e->rot)
+    return fail(e, ESP_ERR_ARG, "Trying to setup Damping of unknown type: %s", tname[type]);
+  if (!name || strlen(name) >= 128) return fail(e, ESP_ERR_ARG, "bad group name");
+  const char* key = (type == ESP_DAMPING || type == ESP_ROT_DAMPING) ? tname[type] : name;
+  for (int k = 0; k < e->ndp; k++)
+    if (!strcmp(e->dp[k].key, key)) return ESP_OK; /* map insert keeps the first */
+  if (e->ndp >= 4) return fail(e, ESP_ERR_CAPACITY, "more than 4 damping groups");
+  odamp* D = &e->dp[e->ndp++];
+  strcpy(D->key, key);
+  D->type = type;
+  D->visc = viscosity;
+  D->maxit = max_iterations;
   return ESP_OK;
 }
 

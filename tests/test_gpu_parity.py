@@ -212,3 +212,44 @@ def test_brittle_beam_sc_stress_and_events(pkg, orc, spm):
     r = float(np.mean(spec["rad"]))
     assert np.max(np.abs(g.getParticleData("pos") - o.getParticleData("pos"))) <= TRAJ_TOL * r
     assert rel_field(g.getParticleStress().reshape(-1, 9), o.getParticleStress().reshape(-1, 9)) <= TRAJ_TOL
+
+
+def test_damping_groups(pkg, orc, spm):
+    """Damping and RotDamping (the groups of the reference's compression
+    benchmark, compbench.py:52-58) on the compressed bonded lattice, and
+    LocalDamping on the settling cloud: events bit-exact, trajectories within
+    1e-9 after 200 steps."""
+    spec = lattice_spec(spm)
+    sims = []
+    for sim in (pkg.LsmMpi(stepsPerBatch=32), orc.oracle_lsm_class(pkg)()):
+        spm.apply(sim, spec)
+        sim.createInteractionGroup(pkg.DampingPrms("Damping", "damping1", 50.0, 50))
+        sim.createInteractionGroup(pkg.DampingPrms("RotDamping", "damping2", 50.0, 50))
+        sims.append(sim)
+    g, o = sims
+    print("case: damped bonded lattice n=%d" % len(spec["ids"]), flush=True)
+    g.runTimeSteps(200)
+    o.runTimeSteps(200)
+    assert len(o.getBondBreaks()) > 50
+    assert_events_equal(g, o, ["bonded", "friction"])
+    r = float(np.mean(spec["rad"]))
+    assert np.max(np.abs(g.getParticleData("pos") - o.getParticleData("pos"))) <= TRAJ_TOL * r
+    assert rel_field(g.getParticleData("vel"), o.getParticleData("vel")) <= TRAJ_TOL
+    assert rel_field(g.getParticleData("angvel"), o.getParticleData("angvel")) <= TRAJ_TOL
+    for w in ("bottom", "top"):
+        assert rel_field(g.engine.wall_force_history(w, 200), o.engine.wall_force_history(w, 200)) <= TRAJ_TOL
+
+    spec = spm.gravity_settling(n=600, seed=5, box=(8.0, 16.0, 8.0), dt=2e-3)
+    sims = []
+    for sim in (pkg.LsmMpi(), orc.oracle_lsm_class(pkg)()):
+        spm.apply(sim, spec)
+        sim.createInteractionGroup(pkg.LocalDampingPrms("ld", 0.2))
+        sims.append(sim)
+    g, o = sims
+    print("case: local damping settling n=600", flush=True)
+    g.runTimeSteps(200)
+    o.runTimeSteps(200)
+    assert_events_equal(g, o, ["elastic"])
+    r = float(np.mean(spec["rad"]))
+    assert np.max(np.abs(g.getParticleData("pos") - o.getParticleData("pos"))) <= TRAJ_TOL * r
+    assert rel_field(g.getParticleData("vel"), o.getParticleData("vel")) <= TRAJ_TOL
