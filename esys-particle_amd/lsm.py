@@ -438,6 +438,22 @@ class LsmMpi:
     def createParticle(self, particle):
         self.createParticles([particle])
 
+    def readGeometry(self, fileName):
+        """CLatticeMaster::readGeometry (LatticeMaster.hpp:29-78): the header's
+        bounding box becomes the spatial domain unless one was set, then the
+        particles (mass pi r^2, times 4/3 r in 3-D) and the connections of the
+        .geo file are created (format: geometry.read_geo)."""
+        from .geometry import read_geo
+        g = read_geo(fileName)
+        if g.info.hasAnyPeriodicDimensions():
+            raise NotImplementedError("periodic boundaries are not supported")
+        if self._engine is None:
+            self.setSpatialDomain(BoundingBox(Vec3(g.info.bbox_min), Vec3(g.info.bbox_max)))
+        if len(g.ids):
+            self.createParticleArrays(g.ids, g.pos, g.rad, mass=g.mass, tag=g.tags)
+        if len(g.conn):
+            self.createConnectionArrays(g.conn[:, 0], g.conn[:, 1], g.conn[:, 2])
+
     def createParticleArrays(self, ids, posn, radius, mass=None, tag=None, vel=None):
         """Bulk particle creation from arrays (same semantics as createParticles)."""
         self._e().add_particles(ids, posn, radius, mass, tag, vel)

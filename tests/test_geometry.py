@@ -1,0 +1,129 @@
+"""The .geo reader (CLatticeMaster::readGeometry, LatticeMaster.hpp:29-78;
+GeometryInfo::Impl::read, GeometryInfo.cpp:157-233; GeometryReader.cpp:33-160)
+and the DistConnections bond finder (DistConnections.hpp:139-206).
+
+tests/golden/seepage_input.geo is the geometry file of the reference's
+ESyS-Darcy seepage example (Doc/ESyS-Darcy/examples/seepage/input.geo), kept
+as a data fixture."""
+import importlib
+import itertools
+import math
+import os
+
+import numpy as np
+import pytest
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+@pytest.fixture(scope="module")
+def geom(pkg):
+    return importlib.import_module("esys_particle_amd.geometry")
+
+
+def write_geo(path, header, parts, conns):
+    with open(path, "w") as f:
+        f.write(header)
+        f.write("BeginParticles\nSimple\n%d\n" % len(parts))
+        for p in parts:
+            f.write("%r %r %r %r %d %d\n" % p)
+        f.write("EndParticles\nBeginConnect\n%d\n" % len(conns))
+        for c in conns:
+            f.write("%d %d %d\n" % c)
+        f.write("EndConnect\n")
+
+
+def test_reference_fixture(geom):
+    g = geom.read_geo(os.path.join(GOLDEN, "seepage_input.geo"))
+    assert g.info.version == 1.2 and not g.info.is2d
+    assert g.info.bbox_min == (0.0, 0.0, -0.01) and g.info.bbox_max == (0.04, 0.02, 0.07)
+    assert not g.info.hasAnyPeriodicDimensions()
+    assert len(g.ids) == 511 and len(g.conn) == 0
+    np.testing.assert_array_equal(g.ids, np.arange(511))
+    np.testing.assert_array_equal(g.pos[1], [0.0171073, 0.00808333, 0.0162018])
+    assert g.rad[1] == 0.00331158 and g.tags[1] == 0
+    # GeometryReader.h:52-62: mass = M_PI r^2, then *= (4/3) r in 3-D
+    r = g.rad[1]
+    assert g.mass[1] == (math.pi * r * r) * ((4.0 / 3.0) * r)
+
+
+def test_round_trip_versions(geom, tmp_path):
+    parts = [(0.5, 1.25, -2.0, 0.75, 10, 3), (1.5, 1.0, 0.0, 0.5, 4, 0), (2.0, 2.0, 2.0, 0.25, 7, 1)]
+    conns = [(10, 4, 2), (7, 4, 0)]
+    hdr = ("LSMGeometry 1.3\nBoundingBox -1 -2 -3 4 5 6\nPeriodicBoundaries 0 0 0\n"
+           "Dimension 3D\nGengeoVersion 1.4 999\n")
+    write_geo(tmp_path / "a.geo", hdr, parts, conns)
+    g = geom.read_geo(str(tmp_path / "a.geo"))
+    np.testing.assert_array_equal(g.ids, [10, 4, 7])
+    np.testing.assert_array_equal(g.tags, [3, 0, 1])
+    np.testing.assert_array_equal(g.pos, [p[:3] for p in parts])
+    np.testing.assert_array_equal(g.conn, conns)
+    # version 1.1 has no Dimension line and is 2-D: mass = pi r^2
+    write_geo(tmp_path / "b.geo", "LSMGeometry 1.1\nBoundingBox 0 0 0 1 1 1\nPeriodicBoundaries 0 0 0\n",
+              parts, [])
+    g = geom.read_geo(str(tmp_path / "b.geo"))
+    assert g.info.is2d and len(g.conn) == 0
+    np.testing.assert_array_equal(g.mass, [math.pi * p[3] * p[3] for p in parts])
+    # an "ESyS..." header is read as version 1.2
+    write_geo(tmp_path / "c.geo", "ESyS-Particle 2.3\nBoundingBox 0 0 0 1 1 1\n"
+              "PeriodicBoundaries 0 1 0\nDimension 2d\n", parts, [])
+    g = geom.read_geo(str(tmp_path / "c.geo"))
+    assert g.info.version == 1.2 and g.info.is2d and g.info.periodic == (False, True, False)
+
+
+@pytest.mark.parametrize("header,match", [
+    ("LSMGeometry 1.4\nBoundingBox 0 0 0 1 1 1\nPeriodicBoundaries 0 0 0\nDimension 3D\n", "version"),
+    ("Geometry 1.2\n", "Unrecognised file type"),
+    ("LSMGeometry 1.2\nBox 0 0 0 1 1 1\n", "BoundingBox"),
+])
+def test_header_errors(geom, tmp_path, header, match):
+    write_geo(tmp_path / "e.geo", header, [], [])
+    with pytest.raises(RuntimeError, match=match):
+        geom.read_geo(str(tmp_path / "e.geo"))
+
+
+def test_read_geometry_into_model(pkg, orc, tmp_path):
+    parts = [(1.0, 1.0, 1.0, 0.5, 0, 0), (2.0, 1.0, 1.0, 0.5, 1, 0), (1.0, 2.0, 1.0, 0.4, 2, 5)]
+    write_geo(tmp_path / "m.geo", "LSMGeometry 1.2\nBoundingBox 0 0 0 4 4 4\nPeriodicBoundaries 0 0 0\n"
+              "Dimension 3D\n", parts, [(0, 1, 0), (0, 2, 0)])
+    sim = orc.oracle_lsm_class(pkg)()
+    sim.initNeighbourSearch("RotSphere", 1.2, 0.1)
+    sim.setTimeStepSize(1e-4)
+    sim.readGeometry(str(tmp_path / "m.geo"))  # the header's box becomes the domain
+    sim.createInteractionGroup(pkg.RotBondPrms("b", 10.0, 5.0, 5.0, 5.0, 1e9, 1e9, 1e9, 1e9, 0))
+    sim.runTimeStep()
+    assert sim.getNumParticles() == 3
+    r = np.array([p[3] for p in parts])
+    np.testing.assert_array_equal(sim.getParticleData("mass"), (math.pi * r * r) * ((4.0 / 3.0) * r))
+    assert sim.getNumInteractions("b") == 2
+    # a periodic geometry is refused (periodic boundaries are not built)
+    write_geo(tmp_path / "p.geo", "LSMGeometry 1.2\nBoundingBox 0 0 0 4 4 4\nPeriodicBoundaries 1 0 0\n"
+              "Dimension 3D\n", parts, [])
+    s2 = orc.oracle_lsm_class(pkg)()
+    s2.initNeighbourSearch("RotSphere", 1.2, 0.1)
+    with pytest.raises(NotImplementedError):
+        s2.readGeometry(str(tmp_path / "p.geo"))
+
+
+def test_dist_connections_brute_force(geom):
+    rng = np.random.default_rng(11)
+    n = 400
+    rad = rng.uniform(0.2, 0.5, n)
+    pos = rng.uniform(0.0, 8.0, (n, 3))
+    ids = rng.permutation(np.arange(1000, 1000 + n))
+    dc = geom.DistConnections(0.1, 4, (ids, pos, rad))
+    expect = []
+    for a, b in itertools.combinations(range(n), 2):
+        d = pos[a] - pos[b]
+        rs = 0.1 + rad[a] + rad[b]
+        if d[0] * d[0] + d[1] * d[1] + d[2] * d[2] <= rs * rs:
+            expect.append((min(ids[a], ids[b]), max(ids[a], ids[b])))
+    expect.sort()
+    assert len(expect) > 50
+    np.testing.assert_array_equal(dc.getIdPairs(), np.array(expect).reshape(-1, 2))
+    conns = list(dc)
+    assert all(c.getTag() == 4 for c in conns) and len(dc) == len(expect)
+    # the distance test includes equality (<=): surfaces exactly maxDist apart bond
+    edge = geom.DistConnections(0.25, 0, ([0, 1, 2], [[0, 0, 0], [1.0, 0, 0], [2.5, 0, 0]],
+                                          [0.5, 0.25, 0.5]))
+    np.testing.assert_array_equal(edge.getIdPairs(), [[0, 1]])
