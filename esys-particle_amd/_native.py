@@ -126,6 +126,8 @@ SIGNATURES = {
     "load_checkpoint": (_i, [_vp, _s]),
     "get_bond_breaks": (_i, [_vp, _pi, _pi, _pi]),
     "get_rebuild_steps": (_i, [_vp, _pi]),
+    "track_bond_criteria": (_i, [_vp]),
+    "get_bond_criteria": (_i, [_vp, _s, _pi, _pi, _pd]),
     "set_kernel_timing": (_i, [_vp, _i]),
     "get_kernel_times": (_i, [_vp, _pd, ctypes.POINTER(ctypes.c_longlong)]),
 }
@@ -392,6 +394,20 @@ class Engine:
         m = self._check(self.lib.get_interactions(self.h, _b(name), _ip(a), _ip(b)),
                         "getInteractions")
         return np.stack([a[:m], b[:m]], axis=1)
+
+    def track_bond_criteria(self):
+        self._check(self.lib.track_bond_criteria(self.h), "createFieldSaver(breaking_criterion)")
+
+    def bond_criteria(self, name):
+        """(ids (n, 2), criterion (n,)) of the live bonds in list order."""
+        n = self._check(self.lib.get_bond_criteria(self.h, _b(name), None, None, None),
+                        "getBondCriteria")
+        a = np.zeros(max(n, 1), np.int32)
+        b = np.zeros(max(n, 1), np.int32)
+        c = np.zeros(max(n, 1))
+        m = self._check(self.lib.get_bond_criteria(self.h, _b(name), _ip(a), _ip(b), _dp(c)),
+                        "getBondCriteria")
+        return np.stack([a[:m], b[:m]], 1), c[:m]
 
     def friction_state(self, name):
         n = self.interaction_count(name)

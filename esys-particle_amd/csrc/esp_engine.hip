@@ -220,6 +220,8 @@ struct esp_engine {
   std::vector<WallGroup> wg;
   std::vector<Gravity> grav;
   std::vector<Damping> damp;  // sorted by key
+  bool track_crit = false;    // bonds store their breaking criterion each step
+  DBuf<double> crit;          // per global bond index
   DBuf<double> inert;         // moments of inertia, uploaded when a rotational damping exists
   std::vector<int> dord, bord, word, gord;
 
@@ -631,6 +633,10 @@ void upload(esp_engine* e)
     rebuild_presence(e, G);
   }
   e->broken_cap = std::max(total_bonds, 1);
+  if (e->track_crit) {
+    e->crit.reserve((size_t)e->broken_cap);
+    zero(e, e->crit.p, (size_t)e->broken_cap);
+  }
   e->broken_list.reserve(2 * (size_t)e->broken_cap);  // one list per step parity
   e->evt.reserve(e->broken_cap);
   e->ring_size = e->K + 2;
@@ -1005,6 +1011,7 @@ ForceLaunch force_launch(esp_engine* e, long long step, bool flip)
   fl.slots = e->slots.p;
   fl.slot_of = e->slot_of.p;
   fl.broken_list = broken_list_of(e, step);
+  fl.crit = e->track_crit ? e->crit.p : nullptr;
   int blk = 0;
   for (int gi : e->bord) {
     BondGroup& G = e->bg[gi];
@@ -2476,6 +2483,46 @@ int esp_get_interactions(esp_engine* e, const char* name, int* id1, int* id2)
     id2[i] = (int)(k[i] & 0xffffffffLL);
   }
   return (int)k.size();
+  API_END
+}
+
+int esp_track_bond_criteria(esp_engine* e)
+{
+  API_BEGIN
+  require_setup(e, "createFieldSaver(breaking_criterion)");
+  e->track_crit = true;
+  return ESP_OK;
+  API_END
+}
+
+int esp_get_bond_criteria(esp_engine* e, const char* name, int* id1, int* id2, double* crit)
+{
+  API_BEGIN
+  const int b = find_bg(e, name ? name : "");
+  if (b < 0) throw EspError(ESP_ERR_ARG, "not a bonded interaction group");
+  if (!e->track_crit) throw EspError(ESP_ERR_STATE, "bond criteria are not tracked");
+  BondGroup& G = e->bg[b];
+  const std::vector<char> alive = bond_alive(e, G);
+  std::vector<double> c(std::max(G.n, 1), 0.0);
+  std::vector<int> rank(std::max(G.n, 1));
+  if (e->started && G.n > 0) {
+    d2h(e, c.data(), e->crit.p + G.bond_off, G.n);
+    d2h(e, rank.data(), G.rank.p, G.n);
+    sync(e);
+  } else {
+    for (int i = 0; i < G.n; i++) rank[i] = i;
+  }
+  std::vector<int> order;
+  for (int i = 0; i < G.n; i++)
+    if (alive[i]) order.push_back(i);
+  std::sort(order.begin(), order.end(), [&](int x, int y) { return rank[x] < rank[y]; });
+  if (id1 && id2 && crit)
+    for (size_t k = 0; k < order.size(); k++) {
+      id1[k] = G.hid1[order[k]];
+      id2[k] = G.hid2[order[k]];
+      crit[k] = c[order[k]];
+    }
+  return (int)order.size();
   API_END
 }
 

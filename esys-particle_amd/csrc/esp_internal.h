@@ -203,6 +203,7 @@ struct ForceLaunch {
   double* slots;
   const int2* slot_of;
   int* broken_list;  // this step's break list
+  double* crit;      // per global bond index: breaking criterion of this step, or nullptr
   ForceSeg seg[ESP_MAX_FORCE_SEGS];
 };
 // TSubLattice::updateInteractions after bond breaks, for a queued step: the

@@ -157,6 +157,9 @@ __device__ __forceinline__ void rot_bond_item(const StepCtl& ctl, const PView& p
   put3(s1, neg3(o.f1));
   put3(s1 + 3, o.m2);
   if (fl.ss == 9) put_rel(s0, s1, P1, P2);
+  // CRotBondedInteraction::getCriterion (RotBondedInteraction.cpp:221-227)
+  if (fl.crit)
+    fl.crit[g.bond_off + b] = o.nForce / M.x + o.shForce / M.y + o.tMoment / M.z + o.bMoment / M.w;
   if (g.sc) return;  // broken by the particle stresses after the integration
   if (esp_bond_broken(o.nForce, o.shForce, o.tMoment, o.bMoment, M.x, M.y, M.z, M.w, g.truncated,
                       g.beta1, g.beta2)) {

@@ -274,6 +274,78 @@ class RestartCheckPointPrms:
         return min(b + ((t - b) // i + 1) * i, e)
 
 
+class FieldSaverPrms:
+    """Common part of the field saver parameters (FieldSaverPrmsPy.cpp:46-70):
+    a field is saved after every step t with begin <= t <= end and
+    (t - begin) % incr == 0 (AFieldMaster::needSave, FieldMaster.cpp:97-100);
+    step 0 is the state before the first step."""
+
+    def __init__(self, fieldName, fileName, fileFormat, beginTimeStep, endTimeStep, timeStepIncr):
+        self.fieldName, self.fileName, self.fileFormat = fieldName, fileName, fileFormat
+        self.beginTimeStep, self.endTimeStep = int(beginTimeStep), int(endTimeStep)
+        self.timeStepIncr = int(timeStepIncr)
+        if self.timeStepIncr <= 0:
+            raise ValueError("timeStepIncr must be positive")
+
+    def needSave(self, t):
+        b, e, i = self.beginTimeStep, self.endTimeStep, self.timeStepIncr
+        return (t - b) % i == 0 and b <= t <= e
+
+    def nextSave(self, t):
+        """First save step after step t, or None."""
+        b, e, i = self.beginTimeStep, self.endTimeStep, self.timeStepIncr
+        nxt = b if t < b else b + ((t - b) // i + 1) * i
+        return nxt if nxt <= e else None
+
+
+class WallVectorFieldSaverPrms(FieldSaverPrms):
+    """WallVectorFieldSaverPrms(wallName, fieldName, fileName, fileFormat,
+    beginTimeStep, endTimeStep, timeStepIncr) (WallFieldSaverPrmsPy.cpp:287-330):
+    fieldName "Force" or "Position" (CWall::getVectorFieldFunction, Wall.cpp:72-87)
+    of one wall or a list of walls; fileFormat "RAW_SERIES" appends one line per
+    save, each wall's vector in scientific notation with 10 digits followed by two
+    spaces (VectorWallFieldMaster::writeAsRAW_SERIES, WallFieldMaster.cpp:24-44)."""
+
+    def __init__(self, wallName, fieldName, fileName, fileFormat, beginTimeStep, endTimeStep,
+                 timeStepIncr):
+        FieldSaverPrms.__init__(self, fieldName, fileName, fileFormat, beginTimeStep, endTimeStep,
+                                timeStepIncr)
+        self.wallNames = [wallName] if isinstance(wallName, str) else list(wallName)
+        if fieldName not in ("Force", "Position"):
+            raise ValueError("invalid name [ %s ] for wall vector field access function" % fieldName)
+        if fileFormat != "RAW_SERIES":
+            raise ValueError("wall fields are saved as RAW_SERIES only")
+
+    def getWallNameList(self):
+        return list(self.wallNames)
+
+
+class InteractionScalarFieldSaverPrms(FieldSaverPrms):
+    """InteractionScalarFieldSaverPrms(interactionName, fieldName, fileName,
+    fileFormat, beginTimeStep, endTimeStep, timeStepIncr)
+    (InteractionFieldSaverPrmsPy.cpp:32-52): fieldName "count" (bonds: every
+    bond; RotFriction: touching pairs, RotFricInteraction.cpp:447-452;
+    RotElastic: every pair) or "breaking_criterion" of a bonded group
+    (CRotBondedInteraction::getCriterion, RotBondedInteraction.cpp:221-227);
+    fileFormat "SUM" appends the sum over the group's interactions in list order,
+    scientific notation with 10 digits (ScalarInteractionFieldMaster::writeAsSUM,
+    InteractionFieldMaster.cpp:460-487)."""
+
+    def __init__(self, interactionName, fieldName, fileName, fileFormat, beginTimeStep,
+                 endTimeStep, timeStepIncr):
+        FieldSaverPrms.__init__(self, fieldName, fileName, fileFormat, beginTimeStep, endTimeStep,
+                                timeStepIncr)
+        self.interactionName = interactionName
+        if fieldName not in ("count", "breaking_criterion"):
+            raise ValueError("interaction field %r is not supported" % fieldName)
+        if fileFormat != "SUM":
+            raise ValueError("interaction scalar fields are saved as SUM only")
+
+
+def _sci10(x):
+    return "%.10e" % x
+
+
 class GravityPrms(InteractionPrms):
     """GravityPrms(name, acceleration) (InteractionParamsPy.cpp:917-925)."""
 
@@ -356,6 +428,9 @@ class LsmMpi:
         self._pre = []
         self._post = []
         self._checkpointer = None
+        self._savers = []
+        self._started_saving = False
+        self._group_kinds = {}
 
     # the library this facade binds; the product path is the HIP engine
     def _open_native(self):
@@ -496,9 +571,12 @@ class LsmMpi:
             e.add_brittle_beam_sc_group(prms.name, prms._c())
         elif isinstance(prms, NRotElasticPrms):
             e.add_nrot_elastic_group(prms.name, prms.normalK, prms.scaling)
+            self._group_kinds[prms.name] = "nrot_elastic"
         elif isinstance(prms, RotElasticPrms):
             e.add_rot_elastic_group(prms.name, prms.normalK, prms.scaling)
+            self._group_kinds[prms.name] = "rot_elastic"
         elif isinstance(prms, RotFrictionPrms):
+            self._group_kinds[prms.name] = "friction"
             e.add_rot_friction_group(prms.name, {
                 "k_n": prms.normalK, "mu_d": prms.dynamicMu, "mu_s": prms.staticMu,
                 "k_s": prms.shearK, "scaling": int(prms.scaling), "rigid": int(prms.rigid),
@@ -567,14 +645,77 @@ class LsmMpi:
         model whose groups and walls the script has created, before the first step."""
         self._e().load_checkpoint(fileName)
 
+    # ---- field savers ----
+    def createFieldSaver(self, prms):
+        """LsmMpiPy::createFieldSaver (LsmMpiPy.cpp): wall vector fields and
+        interaction scalar fields, written after the steps the parameters name
+        (CLatticeMaster::runOneStep, LatticeMaster.cpp:2611-2632, 2672-2693)."""
+        e = self._e()
+        if isinstance(prms, WallVectorFieldSaverPrms):
+            for w in prms.wallNames:
+                e.wall_position(w)  # raises for an unknown wall
+        elif isinstance(prms, InteractionScalarFieldSaverPrms):
+            kind = self._group_kind(prms.interactionName)
+            if prms.fieldName == "breaking_criterion":
+                if kind != "bond":
+                    raise ValueError("breaking_criterion needs a bonded interaction group")
+                e.track_bond_criteria()
+            elif kind == "nrot_elastic":
+                raise NotImplementedError("count of an NRotElastic group is not supported")
+        else:
+            raise TypeError("field saver parameters %r are not supported" % type(prms).__name__)
+        self._savers.append(prms)
+
+    def _group_kind(self, name):
+        e = self._e()
+        if e.interaction_count(name) < 0:
+            raise ValueError("unknown interaction group %r" % name)
+        return self._group_kinds.get(name, "bond")
+
+    def _write_field(self, s, t):
+        e = self._e()
+        if isinstance(s, WallVectorFieldSaverPrms):
+            vals = [e.wall_force(w) if s.fieldName == "Force" else e.wall_position(w)
+                    for w in s.wallNames]
+            line = "".join(" ".join(_sci10(c) for c in v) + "  " for v in vals) + "\n"
+        else:
+            kind = self._group_kind(s.interactionName)
+            if s.fieldName == "breaking_criterion":
+                total = 0.0
+                for c in e.bond_criteria(s.interactionName)[1]:
+                    total += float(c)
+            elif kind == "friction":
+                total = float(np.count_nonzero(e.friction_state(s.interactionName)[1] & 2))
+            else:
+                total = float(e.interaction_count(s.interactionName))
+            line = _sci10(0.0 + total) + "\n"
+        with open(s.fileName, "a") as f:
+            f.write(line)
+
+    def _save_fields(self, t):
+        for s in self._savers:
+            if s.needSave(t):
+                self._write_field(s, t)
+
+    def _next_event(self, t):
+        steps = []
+        if self._checkpointer is not None:
+            steps.append(self._checkpointer.nextCheckPoint(t))
+        steps += [s.nextSave(t) for s in self._savers]
+        steps = [x for x in steps if x is not None]
+        return min(steps) if steps else None
+
     def _save_if_due(self, t):
+        self._save_fields(t)
         cp = self._checkpointer
         if cp is not None and cp.isCheckPoint(t):
             self._e().save_checkpoint(cp.fileNamePrefix, self._nts, cp.precision)
 
     def _first_step_checkpoint(self):
-        # CLatticeMaster::runOneStep saves step 0 after the first search (LatticeMaster.cpp:2635-2640)
-        if self._checkpointer is not None and self._e().stats()["time_step"] == 0:
+        # CLatticeMaster::runOneStep saves the fields of step 0 before the first
+        # search and the step-0 checkpoint after it (LatticeMaster.cpp:2611-2640)
+        if self._e().stats()["time_step"] == 0 and not self._started_saving:
+            self._started_saving = True
             self._save_if_due(0)
 
     def runTimeStep(self):
@@ -594,13 +735,13 @@ class LsmMpi:
             for _ in range(n):
                 self.runTimeStep()
             return
-        if self._checkpointer is None:
+        if self._checkpointer is None and not self._savers:
             self._e().run(n)
             return
         self._first_step_checkpoint()
         while n > 0:
             t = self.getTimeStep()
-            nxt = self._checkpointer.nextCheckPoint(t)
+            nxt = self._next_event(t)
             k = n if nxt is None else min(n, nxt - t)
             self._e().run(k)
             n -= k

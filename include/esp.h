@@ -267,6 +267,16 @@ int esp_get_particle_stress(esp_engine* e, double* out);
 /* Bond-break events (step, id1, id2) in (step, id1, id2) order; returns the count
  * when the pointers are NULL. */
 int esp_get_bond_breaks(esp_engine* e, int* step, int* id1, int* id2);
+/* For InteractionScalarFieldSaverPrms(bondGroup, "breaking_criterion", ...):
+ * from the first time step on every bond stores
+ * CRotBondedInteraction::getCriterion (RotBondedInteraction.cpp:221-227) of
+ * the forces of its last step.  Before the first time step only. */
+int esp_track_bond_criteria(esp_engine* e);
+/* The live bonds of a bonded group in the group's list (connection) order
+ * with their criterion of the last step (0 before the first step, as the
+ * reference's zero-initialised cache); returns the count.  With NULL arrays
+ * only the count is returned.  Needs esp_track_bond_criteria. */
+int esp_get_bond_criteria(esp_engine* e, const char* name, int* id1, int* id2, double* crit);
 /* Steps at which the neighbour table was rebuilt; returns the count when NULL. */
 int esp_get_rebuild_steps(esp_engine* e, int* steps);
 

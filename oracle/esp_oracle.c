@@ -1896,6 +1896,30 @@ int eso_get_interactions(eso_engine* e, const char* name, int* id1, int* id2)
   return n;
 }
 
+int eso_track_bond_criteria(eso_engine* e)
+{
+  (void)e;
+  return ESP_OK;
+}
+
+/* CRotBondedInteraction::getCriterion (RotBondedInteraction.cpp:221-227) on the
+ * values the last calcForces cached, bonds in list order */
+int eso_get_bond_criteria(eso_engine* e, const char* name, int* id1, int* id2, double* crit)
+{
+  int g = find_bg(e, name);
+  if (g < 0) return fail(e, ESP_ERR_ARG, "not a bonded interaction group");
+  const obgroup* G = &e->bg[g];
+  if (id1 && id2 && crit)
+    for (int i = 0; i < G->n; i++) {
+      const obond* b = &G->v[i];
+      id1[i] = b->id1;
+      id2[i] = b->id2;
+      crit[i] = b->nForce / b->max_n + b->shForce / b->max_s + b->tMoment / b->max_t +
+                b->bMoment / b->max_b;
+    }
+  return G->n;
+}
+
 int eso_get_group_list_order(eso_engine* e, const char* name, int* id1, int* id2)
 {
   int g = find_dg(e, name);
