@@ -9,6 +9,6 @@ from .lsm import (  # noqa: F401
     LocalDampingPrms, LsmMpi, NRotElasticPrms, RotDampingPrms, RotLocalDampingPrms,
     NRotElasticWallPrms, NRotSphere, RotBondPrms, RotElasticPrms, RotFrictionPrms,
     RestartCheckPointPrms, RotSphere, Runnable, TaggedIdConnection, Vec3,
-    InteractionScalarFieldSaverPrms, WallVectorFieldSaverPrms,
+    InteractionScalarFieldSaverPrms, WallVectorFieldSaverPrms, NRotBondedWallPrms,
 )
 from ._native import Engine, load_hip_library, LIB_PATH  # noqa: F401

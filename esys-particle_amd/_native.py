@@ -105,6 +105,7 @@ SIGNATURES = {
     "add_elastic_wall_group": (_i, [_vp, _s, _s, _d]),
     "add_gravity": (_i, [_vp, _s, _pd]),
     "add_damping_group": (_i, [_vp, _i, _s, ctypes.c_double, _i]),
+    "add_bonded_wall_group": (_i, [_vp, _s, _s, ctypes.c_double, _i, _i]),
     "set_particle_vel": (_i, [_vp, _i, _pd]),
     "set_particle_ang_vel": (_i, [_vp, _i, _pd]),
     "move_wall_by": (_i, [_vp, _s, _pd]),
@@ -311,6 +312,11 @@ class Engine:
     def add_elastic_wall_group(self, name, wall, k):
         self._check(self.lib.add_elastic_wall_group(self.h, _b(name), _b(wall), float(k)),
                     "createInteractionGroup(NRotElasticWallPrms)")
+
+    def add_bonded_wall_group(self, name, wall, k, tag, mask):
+        self._check(self.lib.add_bonded_wall_group(self.h, _b(name), _b(wall), float(k), int(tag),
+                                                   int(mask)),
+                    "createInteractionGroup(NRotBondedWallPrms)")
 
     def add_gravity(self, name, g):
         g = np.asarray(g, np.float64)

@@ -135,6 +135,7 @@ struct WallGroup {
   std::string name;
   int wall;
   double k;
+  int bonded = 0, tag = 0, mask = 0;  // CBWallInteractionGroup (tagged particles bonded)
 };
 
 struct Gravity {
@@ -223,6 +224,7 @@ struct esp_engine {
   bool track_crit = false;    // bonds store their breaking criterion each step
   DBuf<double> crit;          // per global bond index
   DBuf<double> inert;         // moments of inertia, uploaded when a rotational damping exists
+  DBuf<double4> init;         // positions at creation, uploaded when a bonded wall exists
   std::vector<int> dord, bord, word, gord;
 
   long long t = 0, timestamp = 0, last_ns = -1;
@@ -540,6 +542,22 @@ void upload(esp_engine* e)
     h2d(e, e->ang.p, A.data(), n);
     h2d(e, e->quat.p, Q.data(), n);
     h2d(e, e->mom.p, Z.data(), n);
+  }
+  // tags matching a bonded wall group (static: tags do not change)
+  bool bonded_wall = false;
+  for (size_t k = 0; k < e->word.size(); k++) {
+    const WallGroup& G = e->wg[e->word[k]];
+    if (!G.bonded) continue;
+    bonded_wall = true;
+    for (int i = 0; i < n; i++)
+      if ((e->h_tag[i] & G.mask) == (G.tag & G.mask)) e->h_flags[i] |= 1u << (PF_BTAG_SHIFT + k);
+  }
+  if (bonded_wall) {
+    std::vector<double4> I0(n);
+    for (int i = 0; i < n; i++)
+      I0[i] = make_double4(e->h_init[3 * i], e->h_init[3 * i + 1], e->h_init[3 * i + 2], 0.0);
+    e->init.reserve(cap);
+    h2d(e, e->init.p, I0.data(), n);
   }
   h2d(e, e->flags.p, e->h_flags.data(), n);
   h2d(e, e->id.p, e->h_id.data(), n);
@@ -920,8 +938,10 @@ void wall_params(esp_engine* e, WallParam* out)
     for (int c = 0; c < 3; c++) {
       out[k].o[c] = W.o[c];
       out[k].nrm[c] = W.n[c];
+      out[k].disp[c] = W.o[c] - W.old[c];
     }
     out[k].k = G.k;
+    out[k].bonded = G.bonded;
   }
 }
 
@@ -1157,6 +1177,7 @@ void launch_step(esp_engine* e, const StepCtl& ctl, long long step, bool flip, b
     gp.damp_maxit[d] = e->damp[d].maxit;
   }
   gp.inert = e->inert.p;
+  gp.init = e->init.p;
   {
     LaunchTimer lt(e, ESP_KCLASS_GATHER, ctl.step_id);
     launch_gather_integrate(e->stream, ctl, p, gp);
@@ -2243,6 +2264,24 @@ int esp_add_elastic_wall_group(esp_engine* e, const char* name, const char* wall
   if (!name || name_taken(e, name)) throw EspError(ESP_ERR_ARG, "bad or duplicate group name");
   if (e->wg.size() >= ESP_MAX_WALL_GROUPS) throw EspError(ESP_ERR_CAPACITY, "more than 8 wall groups");
   e->wg.push_back(WallGroup{name, w, k});
+  return ESP_OK;
+  API_END
+}
+
+int esp_add_bonded_wall_group(esp_engine* e, const char* name, const char* wall, double k,
+                              int tag, int mask)
+{
+  API_BEGIN
+  require_setup(e, "createInteractionGroup");
+  const int w = find_wall(e, wall ? wall : "");
+  if (w < 0) throw EspError(ESP_ERR_ARG, std::string("unknown wall '") + (wall ? wall : "") + "'");
+  if (!name || name_taken(e, name)) throw EspError(ESP_ERR_ARG, "bad or duplicate group name");
+  if (e->wg.size() >= ESP_MAX_WALL_GROUPS) throw EspError(ESP_ERR_CAPACITY, "more than 8 wall groups");
+  WallGroup G{name, w, k};
+  G.bonded = 1;
+  G.tag = tag;
+  G.mask = mask;
+  e->wg.push_back(G);
   return ESP_OK;
   API_END
 }

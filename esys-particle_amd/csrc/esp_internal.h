@@ -9,7 +9,8 @@
 //   old[i]    = (x, y, z, mass)                position at the last neighbour search
 //   frc[i], mom[i]                             total force / moment of the last step
 //   flags[i]  = dynamic | rotational<<1 | bond-broken flag<<2 |
-//               wall-group membership<<8 (8 bits) | wall-group inner<<16 (8 bits)
+//               wall-group membership<<8 (8 bits) | wall-group inner<<16 (8 bits) |
+//               tag matches bonded wall group<<24 (8 bits)
 // Interactions are structure-of-arrays per group (pair slots int2, history
 // double4).  Every interaction has a record index; at each rebuild a
 // per-particle CSR orders a particle's interactions for summation and slot_of
@@ -39,6 +40,7 @@
 #define PF_FLAG 4u
 #define PF_WALL_SHIFT 8
 #define PF_INNER_SHIFT 16
+#define PF_BTAG_SHIFT 24  // bit 24 + w: tag matches bonded wall group w (static)
 
 // Status of one time step, written by that step's kernels.
 struct StepStatus {
@@ -83,6 +85,8 @@ struct WallParam {
   double o[3];
   double nrm[3];
   double k;
+  int bonded;      // CBWallInteractionGroup: tagged candidates are bonded to the wall
+  double disp[3];  // CWall::getTotalDisplacement: origin - origin at creation
 };
 
 struct GatherParams {
@@ -107,6 +111,7 @@ struct GatherParams {
   double damp_visc[ESP_MAX_DAMPING];
   int damp_maxit[ESP_MAX_DAMPING];
   const double* inert;  // moment of inertia per particle (rotational damping)
+  const double4* init;  // position at creation per particle (bonded walls)
 };
 
 // Open-addressing hash sets of pair keys (a << 32 | b): live bonds of a bonded

@@ -288,6 +288,18 @@ __device__ __forceinline__ bool cell_coords(const GridParams& g, double x, doubl
   return true;
 }
 
+// CBondedWallInteraction::calcForces (BWallInteraction.hpp:35-52): the force on
+// the particle, -(total displacement of the particle - that of the wall) k',
+// k' = k pi r (3-D; the constant of BWallInteraction.hpp:25-31) or k (2-D)
+__device__ __forceinline__ dv3 bonded_wall_force(const WallParam& W, dv3 x, dv3 x_init, double r,
+                                                 int do2d)
+{
+  const double scale = do2d ? 1.0 : 3.1415926536 * r;
+  const double kb = W.k * scale;
+  const dv3 D = sub3(sub3(x, x_init), mk3(W.disp[0], W.disp[1], W.disp[2]));
+  return mul3(mul3(D, kb), -1.0);
+}
+
 // Wall-group membership and inner flags of one particle (CEWallInteractionGroup::
 // Update, EWallInteractionGroup.hpp:57-82, with isInInner, ntable.hpp:207-222).
 __device__ __forceinline__ unsigned wall_membership(const GridParams& g, int nwg,
@@ -402,10 +414,17 @@ __global__ __launch_bounds__(BS) void k_gather_integrate(StepCtl ctl, PView p, G
     flw = wall_membership(gp.grid, gp.nwg, gp.wall, P, fl);
     p.flags[i] = flw;
   }
-  // wall interaction groups, name order (EWallInteraction.hpp:42-78)
+  // wall interaction groups, name order (EWallInteraction.hpp:42-78); in a
+  // bonded group the tagged candidates are bonded instead
   for (int w = 0; w < gp.nwg; w++) {
     if (!act || !((flw >> (PF_WALL_SHIFT + w)) & 1u)) continue;
     const WallParam& W = gp.wall[w];
+    if (W.bonded && ((flw >> (PF_BTAG_SHIFT + w)) & 1u)) {
+      const dv3 fb = bonded_wall_force(W, x, xyz(gp.init[i]), r, gp.do2d);
+      F = add3(F, fb);
+      if (STRESS) stress_add(sg, fb, sub3(x, x));
+      continue;
+    }
     const dv3 n = mk3(W.nrm[0], W.nrm[1], W.nrm[2]);
     const double dist = dot3(sub3(x, mk3(W.o[0], W.o[1], W.o[2])), n);
     if (dist < r) {
@@ -581,7 +600,11 @@ __global__ __launch_bounds__(BS) void k_gather_integrate(StepCtl ctl, PView p, G
       const WallParam& W = gp.wall[w];
       const dv3 n = mk3(W.nrm[0], W.nrm[1], W.nrm[2]);
       const double dist = dot3(sub3(x0, mk3(W.o[0], W.o[1], W.o[2])), n);
-      if (dist < r) {
+      if (W.bonded && ((flw >> (PF_BTAG_SHIFT + w)) & 1u)) {
+        // CWall::addForce(force) subtracts the bonded force: the particle's share
+        const dv3 fb = bonded_wall_force(W, x0, xyz(gp.init[i]), r, gp.do2d);
+        wx = fb.x; wy = fb.y; wz = fb.z;
+      } else if (dist < r) {
         double f = 1.0;
         if (!gp.do2d) f *= 3.141592654 * r;
         const dv3 force = mul3(n, (f * W.k) * (r - dist));

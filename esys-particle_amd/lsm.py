@@ -398,6 +398,17 @@ class RotLocalDampingPrms(LocalDampingPrms):
     type = "RotLocalDamping"
 
 
+class NRotBondedWallPrms(InteractionPrms):
+    """NRotBondedWallPrms(name, wallName, normalK, particleTag, tagMask=-1)
+    (WallPrmsPy.cpp:40-75, 130-166): the wall's candidate particles with
+    (tag & tagMask) == (particleTag & tagMask) are bonded to it, the others
+    interact elastically (BWallInteractionGroup.hpp)."""
+
+    def __init__(self, name, wallName, normalK, particleTag, tagMask=-1):
+        self.name, self.wallName, self.normalK = name, wallName, float(normalK)
+        self.particleTag, self.tagMask = int(particleTag), int(tagMask)
+
+
 class NRotElasticWallPrms(InteractionPrms):
     """NRotElasticWallPrms(name, wallName, normalK) (WallPrmsPy.cpp)."""
 
@@ -591,6 +602,9 @@ class LsmMpi:
                                 prms.viscosity, 0)
         elif isinstance(prms, NRotElasticWallPrms):
             e.add_elastic_wall_group(prms.name, prms.wallName, prms.normalK)
+        elif isinstance(prms, NRotBondedWallPrms):
+            e.add_bonded_wall_group(prms.name, prms.wallName, prms.normalK, prms.particleTag,
+                                    prms.tagMask)
         else:
             raise TypeError("interaction parameters %r are not supported" % type(prms).__name__)
 

@@ -193,6 +193,14 @@ int esp_add_wall(esp_engine* e, const char* name, const double origin[3],
  * (EWallInteractionGroup.hpp:42-82, EWallInteraction.hpp:26-78). */
 int esp_add_elastic_wall_group(esp_engine* e, const char* name, const char* wall,
                                double k);
+/* createInteractionGroup(NRotBondedWallPrms(name, wallName, normalK,
+ * particleTag, tagMask=-1)): CBWallInteractionGroup (BWallInteractionGroup.hpp,
+ * BWallInteraction.hpp:22-52): of the wall's candidate particles those with
+ * (tag & mask) == (particleTag & mask) are bonded to the wall (force
+ * -(particle displacement since creation - wall displacement) k pi r), the
+ * others interact elastically as with esp_add_elastic_wall_group. */
+int esp_add_bonded_wall_group(esp_engine* e, const char* name, const char* wall, double k,
+                              int tag, int mask);
 /* createInteractionGroup(GravityPrms): BodyForceGroup (BodyForceGroup.hpp:41-65). */
 int esp_add_gravity(esp_engine* e, const char* name, const double g[3]);
 /* createInteractionGroup(DampingPrms / LinDampingPrms / RotDampingPrms /

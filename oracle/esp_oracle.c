@@ -185,6 +185,7 @@ typedef struct {
 typedef struct {
   char name[128];
   v3 origin, normal, force, step_motion;
+  v3 oldpos; /* CWall::m_oldpos: the origin at creation (Wall.cpp:38) */
   int has_motion;
   double* hist; /* force after each step */
   int nhist, caphist;
@@ -194,9 +195,11 @@ typedef struct {
   char name[128];
   int wall;
   double k;
+  int bonded, tag, mask; /* CBWallInteractionGroup (BWallInteractionGroup.hpp) */
   int* pidx; /* candidate particles (list order) */
   int* inner;
   double* keff;
+  int* tagged; /* bonded groups: the candidate's tag matches (bonded, not elastic) */
   int n, cap;
 } owgroup;
 
@@ -1000,11 +1003,13 @@ static void wg_update(eso_engine* e, owgroup* G)
       GROW(G->pidx, G->n, G->cap);
       G->inner = realloc(G->inner, sizeof(int) * (size_t)G->cap);
       G->keff = realloc(G->keff, sizeof(double) * (size_t)G->cap);
+      G->tagged = realloc(G->tagged, sizeof(int) * (size_t)G->cap);
       double f = 1.0;
       if (!e->do2d) f *= 3.141592654 * e->P[i].rad;
       G->pidx[G->n] = i;
       G->inner[G->n] = nt_inner(e, e->P[i].pos);
       G->keff[G->n] = f * G->k;
+      G->tagged[G->n] = G->bonded && ((e->P[i].tag & G->mask) == (G->tag & G->mask));
       G->n++;
     }
   }
@@ -1165,7 +1170,20 @@ static void calc_and_integrate(eso_engine* e)
   for (int k = 0; k < e->nwg; k++) {
     owgroup* G = &e->wg[word[k]];
     owall* W = &e->wl[G->wall];
+    /* CBWallInteractionGroup::calcForces: bonded interactions first
+     * (CBondedWallInteraction::calcForces, BWallInteraction.hpp:35-52) */
+    for (int c = 0; G->bonded && c < G->n; c++) {
+      if (!G->tagged[c]) continue;
+      opart* p = &e->P[G->pidx[c]];
+      double scale = e->do2d ? 1.0 : 3.1415926536 * p->rad;
+      double kb = G->k * scale;
+      v3 D = vsub(vsub(p->pos, p->initpos), vsub(W->origin, W->oldpos));
+      v3 force = vmul(D, kb);
+      apply_force(p, vmul(force, -1.0), p->pos);
+      if (G->inner[c]) W->force = vsub(W->force, force);
+    }
     for (int c = 0; c < G->n; c++) {
+      if (G->bonded && G->tagged[c]) continue;
       opart* p = &e->P[G->pidx[c]];
       double dist = vdot(vsub(p->pos, W->origin), W->normal);
       if (dist < p->rad) {
@@ -1679,6 +1697,7 @@ int eso_add_wall(eso_engine* e, const char* name, const double o[3], const doubl
   memset(W, 0, sizeof *W);
   strcpy(W->name, name);
   W->origin = V(o[0], o[1], o[2]);
+  W->oldpos = W->origin;
   W->normal = V(n[0], n[1], n[2]);
   return ESP_OK;
 }
@@ -1695,6 +1714,18 @@ int eso_add_elastic_wall_group(eso_engine* e, const char* name, const char* wall
   strcpy(G->name, name);
   G->wall = w;
   G->k = k;
+  return ESP_OK;
+}
+
+int eso_add_bonded_wall_group(eso_engine* e, const char* name, const char* wall, double k,
+                              int tag, int mask)
+{
+  int rc = eso_add_elastic_wall_group(e, name, wall, k);
+  if (rc) return rc;
+  owgroup* G = &e->wg[e->nwg - 1];
+  G->bonded = 1;
+  G->tag = tag;
+  G->mask = mask;
   return ESP_OK;
 }
 

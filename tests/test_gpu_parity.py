@@ -253,3 +253,33 @@ def test_damping_groups(pkg, orc, spm):
     r = float(np.mean(spec["rad"]))
     assert np.max(np.abs(g.getParticleData("pos") - o.getParticleData("pos"))) <= TRAJ_TOL * r
     assert rel_field(g.getParticleData("vel"), o.getParticleData("vel")) <= TRAJ_TOL
+
+
+def test_bonded_wall_shear(pkg, orc, spm):
+    """NRotBondedWallPrms: the particles at the floor of the bonded lattice are
+    bonded to it while it slides sideways, an elastic top wall compresses:
+    events bit-exact, trajectories and wall forces within 1e-9 after 200 steps."""
+    spec = lattice_spec(spm)
+    (bn, bo, bnrm, bk, _), (tn, to, tnrm, tk, tstep) = spec.pop("walls")
+    sims = []
+    for sim in (pkg.LsmMpi(stepsPerBatch=32), orc.oracle_lsm_class(pkg)()):
+        spm.apply(sim, spec)
+        sim.createWall(bn, pkg.Vec3(*bo), pkg.Vec3(*bnrm))
+        sim.createInteractionGroup(pkg.NRotBondedWallPrms("floor_bond", bn, bk, 0))
+        sim.setWallStepMotion(bn, pkg.Vec3(2e-4, 0.0, 0.0))
+        sim.createWall(tn, pkg.Vec3(*to), pkg.Vec3(*tnrm))
+        sim.createInteractionGroup(pkg.NRotElasticWallPrms(tn + "_wall", tn, tk))
+        sim.setWallStepMotion(tn, pkg.Vec3(*tstep))
+        sims.append(sim)
+    g, o = sims
+    print("case: bonded wall shear n=%d" % len(spec["ids"]), flush=True)
+    g.runTimeSteps(200)
+    o.runTimeSteps(200)
+    assert_events_equal(g, o, ["bonded", "friction"])
+    r = float(np.mean(spec["rad"]))
+    assert np.max(np.abs(g.getParticleData("pos") - o.getParticleData("pos"))) <= TRAJ_TOL * r
+    assert rel_field(g.getParticleData("vel"), o.getParticleData("vel")) <= TRAJ_TOL
+    ho = o.engine.wall_force_history(bn, 200)
+    assert np.abs(ho[:, 0]).max() > 0  # the floor drags the bonded particles
+    for w in (bn, tn):
+        assert rel_field(g.engine.wall_force_history(w, 200), o.engine.wall_force_history(w, 200)) <= TRAJ_TOL
