@@ -106,6 +106,7 @@ SIGNATURES = {
     "add_gravity": (_i, [_vp, _s, _pd]),
     "add_damping_group": (_i, [_vp, _i, _s, ctypes.c_double, _i]),
     "add_bonded_wall_group": (_i, [_vp, _s, _s, ctypes.c_double, _i, _i]),
+    "set_interaction_group_tags": (_i, [_vp, _s, _i, _i, _i, _i]),
     "set_particle_vel": (_i, [_vp, _i, _pd]),
     "set_particle_ang_vel": (_i, [_vp, _i, _pd]),
     "move_wall_by": (_i, [_vp, _s, _pd]),
@@ -312,6 +313,11 @@ class Engine:
     def add_elastic_wall_group(self, name, wall, k):
         self._check(self.lib.add_elastic_wall_group(self.h, _b(name), _b(wall), float(k)),
                     "createInteractionGroup(NRotElasticWallPrms)")
+
+    def set_interaction_group_tags(self, name, tag1, mask1, tag2, mask2):
+        self._check(self.lib.set_interaction_group_tags(self.h, _b(name), int(tag1), int(mask1),
+                                                        int(tag2), int(mask2)),
+                    "createInteractionGroupTagged")
 
     def add_bonded_wall_group(self, name, wall, k, tag, mask):
         self._check(self.lib.add_bonded_wall_group(self.h, _b(name), _b(wall), float(k), int(tag),

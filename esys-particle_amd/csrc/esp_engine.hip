@@ -97,6 +97,7 @@ struct DynGroup {
   unsigned hs_mask = 0;
   long long update_ts = 0;
   std::vector<int> ex;
+  int tagged = 0, tag1 = 0, mask1 = 0, tag2 = 0, mask2 = 0;  // createInteractionGroupTagged
   int rec_off = 0, app_rec_off = 0;
   // pairs and shear histories loaded from a restart checkpoint, stored at the
   // first step (PIS_ED::loadCheckPointData, pi_storage_ed.hpp:225-255)
@@ -225,6 +226,7 @@ struct esp_engine {
   DBuf<double> crit;          // per global bond index
   DBuf<double> inert;         // moments of inertia, uploaded when a rotational damping exists
   DBuf<double4> init;         // positions at creation, uploaded when a bonded wall exists
+  DBuf<int> tag;              // particle tags, uploaded when a tagged group exists
   std::vector<int> dord, bord, word, gord;
 
   long long t = 0, timestamp = 0, last_ns = -1;
@@ -405,6 +407,12 @@ ExclSet excl_set(esp_engine* e, const DynGroup& G)
     s.h[s.count].mask = e->bg[b].hs_mask;
     s.count++;
   }
+  s.tagged = G.tagged;
+  s.tag1 = G.tag1;
+  s.mask1 = G.mask1;
+  s.tag2 = G.tag2;
+  s.mask2 = G.mask2;
+  s.tag = e->tag.p;
   return s;
 }
 
@@ -561,6 +569,12 @@ void upload(esp_engine* e)
   }
   h2d(e, e->flags.p, e->h_flags.data(), n);
   h2d(e, e->id.p, e->h_id.data(), n);
+  bool tagged_group = false;
+  for (auto& G : e->dg) tagged_group |= G.tagged != 0;
+  if (tagged_group) {
+    e->tag.reserve(cap);
+    h2d(e, e->tag.p, e->h_tag.data(), n);
+  }
   bool rot_damping = false;
   for (auto& D : e->damp) rot_damping |= D.type == ESP_ROT_DAMPING;
   if (rot_damping) {
@@ -2522,6 +2536,24 @@ int esp_get_interactions(esp_engine* e, const char* name, int* id1, int* id2)
     id2[i] = (int)(k[i] & 0xffffffffLL);
   }
   return (int)k.size();
+  API_END
+}
+
+int esp_set_interaction_group_tags(esp_engine* e, const char* name, int tag1, int mask1,
+                                   int tag2, int mask2)
+{
+  API_BEGIN
+  require_setup(e, "createInteractionGroupTagged");
+  const int d = find_dg(e, name ? name : "");
+  if (d < 0) throw EspError(ESP_ERR_ARG, "not a dynamic interaction group");
+  DynGroup& G = e->dg[d];
+  G.tagged = 1;
+  if (tag1 <= tag2) {
+    G.tag1 = tag1; G.mask1 = mask1; G.tag2 = tag2; G.mask2 = mask2;
+  } else {
+    G.tag1 = tag2; G.mask1 = mask2; G.tag2 = tag1; G.mask2 = mask1;
+  }
+  return ESP_OK;
   API_END
 }
 

@@ -125,9 +125,15 @@ struct HashRef {
   unsigned mask;
 };
 
+// what a dynamic group refuses besides stored pairs: pairs of an excluding
+// bonded group (isExcluded) and, for groups created with
+// createInteractionGroupTagged, pairs whose tags do not match
+// (pi_storage_ed_t.hpp:55-70; tag1 <= tag2)
 struct ExclSet {
   HashRef h[ESP_MAX_EXCL];
   int count;
+  int tagged, tag1, mask1, tag2, mask2;
+  const int* tag;  // per particle slot
 };
 
 #define ESP_MAX_BOND_GROUPS 8

@@ -102,6 +102,22 @@ __device__ __forceinline__ bool excluded(const ExclSet& ex, long long key)
   return false;
 }
 
+// a candidate pair (particle slots a, b; id key) the group does not take
+__device__ __forceinline__ bool rejected(const ExclSet& ex, long long key, int a, int b)
+{
+  if (ex.tagged) {
+    int t1 = ex.tag[a], t2 = ex.tag[b];
+    if (t1 > t2) {
+      const int th = t1;
+      t1 = t2;
+      t2 = th;
+    }
+    if (!(((t1 & ex.mask1) == (ex.tag1 & ex.mask1)) && ((t2 & ex.mask2) == (ex.tag2 & ex.mask2))))
+      return true;
+  }
+  return excluded(ex, key);
+}
+
 // ---------------------------------------------------------------------------
 // per-step interaction kernels
 // ---------------------------------------------------------------------------
@@ -831,7 +847,7 @@ __global__ void k_ed_mark(int nc, const int* cp, const int* cq, const long long*
   const int c = blockIdx.x * BS + threadIdx.x;
   if (c >= nc) return;
   const long long key = ckey[c];
-  if (excluded(ex, pair_key(p.id[cp[c]], p.id[cq[c]]))) {
+  if (rejected(ex, pair_key(p.id[cp[c]], p.id[cq[c]]), cp[c], cq[c])) {
     keep[c] = 0;
     src[c] = -1;
     return;
@@ -853,7 +869,7 @@ __global__ void k_ed_mark(int nc, const int* cp, const int* cq, const long long*
 __global__ void k_ne_mark(int nc, const int* cp, const int* cq, const int* id, ExclSet ex, int* keep)
 {
   const int c = blockIdx.x * BS + threadIdx.x;
-  if (c < nc) keep[c] = excluded(ex, pair_key(id[cp[c]], id[cq[c]])) ? 0 : 1;
+  if (c < nc) keep[c] = rejected(ex, pair_key(id[cp[c]], id[cq[c]]), cp[c], cq[c]) ? 0 : 1;
 }
 
 __global__ void k_pair_emit(int nc, const int* keep, const int* off, const int* cp, const int* cq,
@@ -879,7 +895,7 @@ __global__ void k_filter_new(int nc, const int* cp, const int* cq, const int* id
 {
   const int c = blockIdx.x * BS + threadIdx.x;
   if (c >= nc) return;
-  int k = excluded(ex, pair_key(id[cp[c]], id[cq[c]])) ? 0 : 1;
+  int k = rejected(ex, pair_key(id[cp[c]], id[cq[c]]), cp[c], cq[c]) ? 0 : 1;
   if (k && check_present && hs_find(present, ckey[c]) >= 0) k = 0;
   keep[c] = k;
 }
@@ -1101,7 +1117,7 @@ __device__ void incremental_update(const StepCtl& ctl, const PView& p, const Upd
     if (t < n) {
       const long long key = s_key[t];
       const int a = (int)(key >> 32), b = (int)(key & 0xffffffffll);
-      const bool keep = !excluded(da.ex, pair_key(p.id[a], p.id[b])) &&
+      const bool keep = !rejected(da.ex, pair_key(p.id[a], p.id[b]), a, b) &&
                         !(da.is_hist && hs_find(da.present, key) >= 0);
       if (keep) atomicAdd(&s_keep, 1);
       else s_key[t] = NEW_SENTINEL;

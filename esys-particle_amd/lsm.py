@@ -608,6 +608,15 @@ class LsmMpi:
         else:
             raise TypeError("interaction parameters %r are not supported" % type(prms).__name__)
 
+    def createInteractionGroupTagged(self, prms, tag1, mask1, tag2, mask2):
+        """createInteractionGroupTagged (LsmMpiPy.cpp:750-835): a RotFriction,
+        NRotElastic or RotElastic group that takes only pairs whose tags match
+        (pi_storage_ed_t.hpp, pi_storage_ne_t.hpp)."""
+        if not isinstance(prms, (RotFrictionPrms, NRotElasticPrms, RotElasticPrms)):
+            raise TypeError("tagged groups are RotFriction, NRotElastic or RotElastic groups")
+        self.createInteractionGroup(prms)
+        self._e().set_interaction_group_tags(prms.name, tag1, mask1, tag2, mask2)
+
     def createGravity(self, prms):
         self.createInteractionGroup(prms)
 

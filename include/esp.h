@@ -184,6 +184,13 @@ int esp_add_rot_friction_group(esp_engine* e, const char* name,
 /* createExclusion(bonded, dynamic): TSubLattice::setExIG (SubLattice.hpp:1980-2009)
  * -> PIS_ED::addExIG / PIS_NE::addExIG (pi_storage_ed.hpp:20-40). */
 int esp_add_exclusion(esp_engine* e, const char* excluding, const char* excluded);
+/* createInteractionGroupTagged(prms, tag1, mask1, tag2, mask2) (LsmMpiPy.cpp:
+ * 750-835 -> PIS_ED_T / PIS_NE_T, pi_storage_ed_t.hpp, pi_storage_ne_t.hpp):
+ * the dynamic group just created takes only pairs whose smaller tag t1 and
+ * larger tag t2 satisfy (t1 & m1) == (T1 & m1) and (t2 & m2) == (T2 & m2),
+ * (T1, m1), (T2, m2) the given pairs sorted by tag.  Before the first step. */
+int esp_set_interaction_group_tags(esp_engine* e, const char* name, int tag1, int mask1,
+                                   int tag2, int mask2);
 
 /* ---- walls and body forces ---------------------------------------------- */
 /* createWall (LsmMpiPy.cpp:1145-1151, Wall.cpp:35-43). */

@@ -159,6 +159,9 @@ typedef struct {
   long long update_ts;
   int ex[16]; /* indices of excluding bonded groups */
   int nex;
+  /* createInteractionGroupTagged: tags sorted so that tag1 <= tag2
+   * (pi_storage_ed_t.hpp:15-30, pi_storage_ne_t.hpp:15-30) */
+  int tagged, tag1, mask1, tag2, mask2;
 } ogroup;
 
 typedef struct {
@@ -922,6 +925,20 @@ static int dg_excluded(eso_engine* e, const ogroup* G, int id1, int id2)
   return 0;
 }
 
+/* the tag test of PIS_ED_T / PIS_NE_T::update (pi_storage_ed_t.hpp:55-70):
+ * the pair's smaller tag against (tag1, mask1), the larger against (tag2, mask2) */
+static int dg_tags_match(const ogroup* G, int ta, int tb)
+{
+  if (!G->tagged) return 1;
+  int t1 = ta, t2 = tb;
+  if (t1 > t2) {
+    int th = t1;
+    t1 = t2;
+    t2 = th;
+  }
+  return ((t1 & G->mask1) == (G->tag1 & G->mask1)) && ((t2 & G->mask2) == (G->tag2 & G->mask2));
+}
+
 static void dg_push(eso_engine* e, ogroup* G, int i1, int i2)
 {
   GROW(G->v, G->n, G->cap);
@@ -979,6 +996,7 @@ static void dg_update(eso_engine* e, ogroup* G)
   for (int p = 0; p < L.n; p++) {
     int i1 = L.a[p], i2 = L.b[p];
     int id1 = e->P[i1].id, id2 = e->P[i2].id;
+    if (!dg_tags_match(G, e->P[i1].tag, e->P[i2].tag)) continue;
     if (dg_excluded(e, G, id1, id2)) continue;
     if (G->kind == G_ED_ROTFRICTION && bsearch_ll(present, npresent, pkey(id1, id2)) >= 0)
       continue;
@@ -1925,6 +1943,21 @@ int eso_get_interactions(eso_engine* e, const char* name, int* id1, int* id2)
   free(keys);
   free(perm);
   return n;
+}
+
+int eso_set_interaction_group_tags(eso_engine* e, const char* name, int tag1, int mask1, int tag2,
+                                   int mask2)
+{
+  int g = find_dg(e, name);
+  if (g < 0) return fail(e, ESP_ERR_ARG, "not a dynamic interaction group");
+  ogroup* G = &e->dg[g];
+  G->tagged = 1;
+  if (tag1 <= tag2) {
+    G->tag1 = tag1; G->mask1 = mask1; G->tag2 = tag2; G->mask2 = mask2;
+  } else {
+    G->tag1 = tag2; G->mask1 = mask2; G->tag2 = tag1; G->mask2 = mask1;
+  }
+  return ESP_OK;
 }
 
 int eso_track_bond_criteria(eso_engine* e)

@@ -50,6 +50,8 @@ int eso_add_damping_group(eso_engine* e, int type, const char* name, double visc
                           int max_iterations);
 int eso_set_particle_vel(eso_engine* e, int id, const double v[3]);
 int eso_track_bond_criteria(eso_engine* e);
+int eso_set_interaction_group_tags(eso_engine* e, const char* name, int tag1, int mask1, int tag2,
+                                   int mask2);
 int eso_get_bond_criteria(eso_engine* e, const char* name, int* id1, int* id2, double* crit);
 int eso_set_particle_ang_vel(eso_engine* e, int id, const double w[3]);
 int eso_move_wall_by(eso_engine* e, const char* name, const double d[3]);

@@ -283,3 +283,41 @@ def test_bonded_wall_shear(pkg, orc, spm):
     assert np.abs(ho[:, 0]).max() > 0  # the floor drags the bonded particles
     for w in (bn, tn):
         assert rel_field(g.engine.wall_force_history(w, 200), o.engine.wall_force_history(w, 200)) <= TRAJ_TOL
+
+
+def test_tagged_groups(pkg, orc, spm):
+    """createInteractionGroupTagged: a RotFriction group between tags 0 and 1
+    and a RotElastic group among tag-2 particles in a settling cloud with
+    rebuilds: pair lists and rebuild steps bit-exact, trajectories within 1e-9
+    after 200 steps."""
+    spec = spm.gravity_settling(n=600, seed=8, box=(8.0, 16.0, 8.0), dt=2e-3)
+    tags = np.random.default_rng(2).integers(0, 3, len(spec["ids"])).astype(np.int32)
+    vel = np.random.default_rng(5).normal(0, 2.0, (len(tags), 3))  # collisions from the start
+    sims = []
+    for sim in (pkg.LsmMpi(), orc.oracle_lsm_class(pkg)()):
+        sim.initNeighbourSearch("RotSphere", spec["grid"], spec["alpha"])
+        sim.setTimeStepSize(spec["dt"])
+        lo, hi = spec["domain"]
+        sim.setSpatialDomain(pkg.BoundingBox(pkg.Vec3(*lo), pkg.Vec3(*hi)))
+        sim.createParticleArrays(spec["ids"], spec["pos"], spec["rad"], mass=spec["rad"] ** 3,
+                                 tag=tags, vel=vel)
+        sim.createInteractionGroupTagged(pkg.RotFrictionPrms("f01", 100.0, 0.5, 0.5, 50.0), 1, -1, 0, -1)
+        sim.createInteractionGroupTagged(pkg.RotElasticPrms("e2", 100.0), 2, -1, 2, -1)
+        sim.createInteractionGroup(pkg.GravityPrms("g", pkg.Vec3(*spec["gravity"])))
+        for name, o, nrm, k, step in spec["walls"]:
+            sim.createWall(name, pkg.Vec3(*o), pkg.Vec3(*nrm))
+            sim.createInteractionGroup(pkg.NRotElasticWallPrms(name + "_wall", name, k))
+        sims.append(sim)
+    g, o = sims
+    print("case: tagged groups n=%d" % len(spec["ids"]), flush=True)
+    g.runTimeSteps(200)
+    o.runTimeSteps(200)
+    assert len(o.engine.rebuild_steps()) >= 10
+    assert_events_equal(g, o, ["f01", "e2"])
+    gf, gfl = g.engine.friction_state("f01")
+    of, ofl = o.engine.friction_state("f01")
+    np.testing.assert_array_equal(gfl, ofl)
+    assert len(o.getInteractionPairs("f01")) > 50 and len(o.getInteractionPairs("e2")) > 20
+    r = float(np.mean(spec["rad"]))
+    assert np.max(np.abs(g.getParticleData("pos") - o.getParticleData("pos"))) <= TRAJ_TOL * r
+    assert rel_field(g.getParticleData("vel"), o.getParticleData("vel")) <= TRAJ_TOL
