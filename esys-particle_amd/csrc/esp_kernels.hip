@@ -389,7 +389,8 @@ __device__ __forceinline__ void stress_add(double (&sg)[3][3], dv3 f, dv3 rel)
 
 // STRESS: records carry contact points and every applied force also updates
 // the particle's stress sum, in the order the forces are summed
-template <bool STRESS>
+// EXTRA: damping groups or bonded walls exist (kept out of the plain kernel)
+template <bool STRESS, bool EXTRA>
 __global__ __launch_bounds__(BS) void k_gather_integrate(StepCtl ctl, PView p, GatherParams gp)
 {
   constexpr int GATHER_CHUNK = STRESS ? 2 : 4;
@@ -435,7 +436,7 @@ __global__ __launch_bounds__(BS) void k_gather_integrate(StepCtl ctl, PView p, G
   for (int w = 0; w < gp.nwg; w++) {
     if (!act || !((flw >> (PF_WALL_SHIFT + w)) & 1u)) continue;
     const WallParam& W = gp.wall[w];
-    if (W.bonded && ((flw >> (PF_BTAG_SHIFT + w)) & 1u)) {
+    if (EXTRA && W.bonded && ((flw >> (PF_BTAG_SHIFT + w)) & 1u)) {
       const dv3 fb = bonded_wall_force(W, x, xyz(gp.init[i]), r, gp.do2d);
       F = add3(F, fb);
       if (STRESS) stress_add(sg, fb, sub3(x, x));
@@ -513,7 +514,7 @@ __global__ __launch_bounds__(BS) void k_gather_integrate(StepCtl ctl, PView p, G
     }
     const double dt = gp.dt;
     // damping groups, last (SubLattice.hpp:2143-2147), in their map order
-    for (int d = 0; d < gp.ndamp; d++) {
+    for (int d = 0; EXTRA && d < gp.ndamp; d++) {
       const int ty = gp.damp_type[d];
       const double visc = gp.damp_visc[d];
       if (ty == ESP_DAMPING || ty == ESP_ROT_DAMPING) {
@@ -616,7 +617,7 @@ __global__ __launch_bounds__(BS) void k_gather_integrate(StepCtl ctl, PView p, G
       const WallParam& W = gp.wall[w];
       const dv3 n = mk3(W.nrm[0], W.nrm[1], W.nrm[2]);
       const double dist = dot3(sub3(x0, mk3(W.o[0], W.o[1], W.o[2])), n);
-      if (W.bonded && ((flw >> (PF_BTAG_SHIFT + w)) & 1u)) {
+      if (EXTRA && W.bonded && ((flw >> (PF_BTAG_SHIFT + w)) & 1u)) {
         // CWall::addForce(force) subtracts the bonded force: the particle's share
         const dv3 fb = bonded_wall_force(W, x0, xyz(gp.init[i]), r, gp.do2d);
         wx = fb.x; wy = fb.y; wz = fb.z;
@@ -1480,10 +1481,16 @@ void launch_alive_flags(hipStream_t s, int nb, const int2* ij, int* keep)
 void launch_gather_integrate(hipStream_t s, StepCtl ctl, PView p, const GatherParams& gp)
 {
   if (p.n <= 0) return;
-  if (gp.sig)
-    hipLaunchKernelGGL(k_gather_integrate<true>, dim3(nblk(p.n)), dim3(BS), 0, s, ctl, p, gp);
-  else
-    hipLaunchKernelGGL(k_gather_integrate<false>, dim3(nblk(p.n)), dim3(BS), 0, s, ctl, p, gp);
+  bool extra = gp.ndamp > 0;
+  for (int w = 0; w < gp.nwg; w++) extra |= gp.wall[w].bonded != 0;
+  const dim3 g(nblk(p.n)), b(BS);
+  if (gp.sig) {
+    if (extra) hipLaunchKernelGGL((k_gather_integrate<true, true>), g, b, 0, s, ctl, p, gp);
+    else hipLaunchKernelGGL((k_gather_integrate<true, false>), g, b, 0, s, ctl, p, gp);
+  } else {
+    if (extra) hipLaunchKernelGGL((k_gather_integrate<false, true>), g, b, 0, s, ctl, p, gp);
+    else hipLaunchKernelGGL((k_gather_integrate<false, false>), g, b, 0, s, ctl, p, gp);
+  }
 }
 
 // BrittleBeamSCInteraction::broken for every live bond of the stress-criterion
