@@ -107,6 +107,7 @@ SIGNATURES = {
     "add_damping_group": (_i, [_vp, _i, _s, ctypes.c_double, _i]),
     "add_bonded_wall_group": (_i, [_vp, _s, _s, ctypes.c_double, _i, _i]),
     "set_interaction_group_tags": (_i, [_vp, _s, _i, _i, _i, _i]),
+    "set_wall_group_tags": (_i, [_vp, _s, _i, _i]),
     "set_particle_vel": (_i, [_vp, _i, _pd]),
     "set_particle_ang_vel": (_i, [_vp, _i, _pd]),
     "move_wall_by": (_i, [_vp, _s, _pd]),
@@ -318,6 +319,10 @@ class Engine:
         self._check(self.lib.set_interaction_group_tags(self.h, _b(name), int(tag1), int(mask1),
                                                         int(tag2), int(mask2)),
                     "createInteractionGroupTagged")
+
+    def set_wall_group_tags(self, name, tag, mask):
+        self._check(self.lib.set_wall_group_tags(self.h, _b(name), int(tag), int(mask)),
+                    "createInteractionGroupTagged(NRotElasticWallPrms)")
 
     def add_bonded_wall_group(self, name, wall, k, tag, mask):
         self._check(self.lib.add_bonded_wall_group(self.h, _b(name), _b(wall), float(k), int(tag),

@@ -137,6 +137,7 @@ struct WallGroup {
   int wall;
   double k;
   int bonded = 0, tag = 0, mask = 0;  // CBWallInteractionGroup (tagged particles bonded)
+  int tagged = 0;  // CTaggedEWallInteractionGroup (only tagged particles interact)
 };
 
 struct Gravity {
@@ -555,8 +556,8 @@ void upload(esp_engine* e)
   bool bonded_wall = false;
   for (size_t k = 0; k < e->word.size(); k++) {
     const WallGroup& G = e->wg[e->word[k]];
-    if (!G.bonded) continue;
-    bonded_wall = true;
+    if (!G.bonded && !G.tagged) continue;
+    bonded_wall |= G.bonded != 0;
     for (int i = 0; i < n; i++)
       if ((e->h_tag[i] & G.mask) == (G.tag & G.mask)) e->h_flags[i] |= 1u << (PF_BTAG_SHIFT + k);
   }
@@ -956,6 +957,7 @@ void wall_params(esp_engine* e, WallParam* out)
     }
     out[k].k = G.k;
     out[k].bonded = G.bonded;
+    out[k].tagged = G.tagged;
   }
 }
 
@@ -2297,6 +2299,22 @@ int esp_add_bonded_wall_group(esp_engine* e, const char* name, const char* wall,
   G.mask = mask;
   e->wg.push_back(G);
   return ESP_OK;
+  API_END
+}
+
+int esp_set_wall_group_tags(esp_engine* e, const char* name, int tag, int mask)
+{
+  API_BEGIN
+  require_setup(e, "createInteractionGroupTagged");
+  for (auto& G : e->wg)
+    if (G.name == (name ? name : "")) {
+      if (G.bonded) throw EspError(ESP_ERR_ARG, "a bonded wall group selects its tags itself");
+      G.tagged = 1;
+      G.tag = tag;
+      G.mask = mask;
+      return ESP_OK;
+    }
+  throw EspError(ESP_ERR_ARG, "not an elastic wall group");
   API_END
 }
 

@@ -86,6 +86,7 @@ struct WallParam {
   double nrm[3];
   double k;
   int bonded;      // CBWallInteractionGroup: tagged candidates are bonded to the wall
+  int tagged;      // CTaggedEWallInteractionGroup: only tagged candidates interact
   double disp[3];  // CWall::getTotalDisplacement: origin - origin at creation
 };
 

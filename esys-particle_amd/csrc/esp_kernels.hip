@@ -436,6 +436,7 @@ __global__ __launch_bounds__(BS) void k_gather_integrate(StepCtl ctl, PView p, G
   for (int w = 0; w < gp.nwg; w++) {
     if (!act || !((flw >> (PF_WALL_SHIFT + w)) & 1u)) continue;
     const WallParam& W = gp.wall[w];
+    if (EXTRA && W.tagged && !((flw >> (PF_BTAG_SHIFT + w)) & 1u)) continue;
     if (EXTRA && W.bonded && ((flw >> (PF_BTAG_SHIFT + w)) & 1u)) {
       const dv3 fb = bonded_wall_force(W, x, xyz(gp.init[i]), r, gp.do2d);
       F = add3(F, fb);
@@ -617,7 +618,9 @@ __global__ __launch_bounds__(BS) void k_gather_integrate(StepCtl ctl, PView p, G
       const WallParam& W = gp.wall[w];
       const dv3 n = mk3(W.nrm[0], W.nrm[1], W.nrm[2]);
       const double dist = dot3(sub3(x0, mk3(W.o[0], W.o[1], W.o[2])), n);
-      if (EXTRA && W.bonded && ((flw >> (PF_BTAG_SHIFT + w)) & 1u)) {
+      if (EXTRA && W.tagged && !((flw >> (PF_BTAG_SHIFT + w)) & 1u)) {
+        // untagged candidates of a tagged elastic wall: no interaction
+      } else if (EXTRA && W.bonded && ((flw >> (PF_BTAG_SHIFT + w)) & 1u)) {
         // CWall::addForce(force) subtracts the bonded force: the particle's share
         const dv3 fb = bonded_wall_force(W, x0, xyz(gp.init[i]), r, gp.do2d);
         wx = fb.x; wy = fb.y; wz = fb.z;
@@ -1482,7 +1485,7 @@ void launch_gather_integrate(hipStream_t s, StepCtl ctl, PView p, const GatherPa
 {
   if (p.n <= 0) return;
   bool extra = gp.ndamp > 0;
-  for (int w = 0; w < gp.nwg; w++) extra |= gp.wall[w].bonded != 0;
+  for (int w = 0; w < gp.nwg; w++) extra |= gp.wall[w].bonded != 0 || gp.wall[w].tagged != 0;
   const dim3 g(nblk(p.n)), b(BS);
   if (gp.sig) {
     if (extra) hipLaunchKernelGGL((k_gather_integrate<true, true>), g, b, 0, s, ctl, p, gp);

@@ -608,12 +608,22 @@ class LsmMpi:
         else:
             raise TypeError("interaction parameters %r are not supported" % type(prms).__name__)
 
-    def createInteractionGroupTagged(self, prms, tag1, mask1, tag2, mask2):
-        """createInteractionGroupTagged (LsmMpiPy.cpp:750-835): a RotFriction,
-        NRotElastic or RotElastic group that takes only pairs whose tags match
-        (pi_storage_ed_t.hpp, pi_storage_ne_t.hpp)."""
+    def createInteractionGroupTagged(self, prms, *tags):
+        """createInteractionGroupTagged (LsmMpiPy.cpp:750-835, 1203-1206):
+        (prms, tag1, mask1, tag2, mask2) for a RotFriction, NRotElastic or
+        RotElastic group that takes only pairs whose tags match
+        (pi_storage_ed_t.hpp, pi_storage_ne_t.hpp); (NRotElasticWallPrms, tag,
+        mask) for an elastic wall acting only on particles with that tag
+        (TaggedEWallInteractionGroup.hpp)."""
+        if isinstance(prms, NRotElasticWallPrms):
+            tag, mask = tags
+            self.createInteractionGroup(prms)
+            self._e().set_wall_group_tags(prms.name, tag, mask)
+            return
         if not isinstance(prms, (RotFrictionPrms, NRotElasticPrms, RotElasticPrms)):
-            raise TypeError("tagged groups are RotFriction, NRotElastic or RotElastic groups")
+            raise TypeError("tagged groups are RotFriction, NRotElastic, RotElastic or "
+                            "NRotElasticWall groups")
+        tag1, mask1, tag2, mask2 = tags
         self.createInteractionGroup(prms)
         self._e().set_interaction_group_tags(prms.name, tag1, mask1, tag2, mask2)
 

@@ -209,7 +209,7 @@ def apply(sim, spec, steps_per_batch=None):
     sim.setSpatialDomain(lsm.BoundingBox(lsm.Vec3(*lo), lsm.Vec3(*hi)))
     n = len(spec["ids"])
     sim.createParticleArrays(spec["ids"], spec["pos"], spec["rad"],
-                             mass=spec["rad"] ** 3, tag=np.zeros(n, np.int32),
+                             mass=spec["rad"] ** 3, tag=spec.get("tags", np.zeros(n, np.int32)),
                              vel=spec.get("vel"))
     if spec.get("density") is not None:
         sim.setParticleDensity(0, 0, spec["density"])

@@ -208,6 +208,11 @@ int esp_add_elastic_wall_group(esp_engine* e, const char* name, const char* wall
  * others interact elastically as with esp_add_elastic_wall_group. */
 int esp_add_bonded_wall_group(esp_engine* e, const char* name, const char* wall, double k,
                               int tag, int mask);
+/* createInteractionGroupTagged(NRotElasticWallPrms, tag, mask)
+ * (LsmMpiPy.cpp:1203-1206 -> CTaggedEWallInteractionGroup,
+ * TaggedEWallInteractionGroup.hpp:28-52): the elastic wall group just created
+ * acts only on candidates with (tag & mask) == (particleTag & mask). */
+int esp_set_wall_group_tags(esp_engine* e, const char* name, int tag, int mask);
 /* createInteractionGroup(GravityPrms): BodyForceGroup (BodyForceGroup.hpp:41-65). */
 int esp_add_gravity(esp_engine* e, const char* name, const double g[3]);
 /* createInteractionGroup(DampingPrms / LinDampingPrms / RotDampingPrms /

@@ -199,6 +199,7 @@ typedef struct {
   int wall;
   double k;
   int bonded, tag, mask; /* CBWallInteractionGroup (BWallInteractionGroup.hpp) */
+  int tagged_only;       /* CTaggedEWallInteractionGroup (TaggedEWallInteractionGroup.hpp) */
   int* pidx; /* candidate particles (list order) */
   int* inner;
   double* keff;
@@ -1017,6 +1018,8 @@ static void wg_update(eso_engine* e, owgroup* G)
   G->n = 0;
   for (int i = 0; i < e->np; i++) {
     double dist = vdot(vsub(e->P[i].pos, W->origin), W->normal);
+    /* the tagged group keeps only candidates with the right tag (TaggedEWallInteractionGroup.hpp:42-44) */
+    if (G->tagged_only && (e->P[i].tag & G->mask) != (G->tag & G->mask)) continue;
     if (fabs(dist) < e->dim) {
       GROW(G->pidx, G->n, G->cap);
       G->inner = realloc(G->inner, sizeof(int) * (size_t)G->cap);
@@ -1745,6 +1748,19 @@ int eso_add_bonded_wall_group(eso_engine* e, const char* name, const char* wall,
   G->tag = tag;
   G->mask = mask;
   return ESP_OK;
+}
+
+int eso_set_wall_group_tags(eso_engine* e, const char* name, int tag, int mask)
+{
+  for (int k = 0; k < e->nwg; k++)
+    if (!strcmp(e->wg[k].name, name ? name : "")) {
+      if (e->wg[k].bonded) return fail(e, ESP_ERR_ARG, "a bonded wall group selects its tags itself");
+      e->wg[k].tagged_only = 1;
+      e->wg[k].tag = tag;
+      e->wg[k].mask = mask;
+      return ESP_OK;
+    }
+  return fail(e, ESP_ERR_ARG, "not an elastic wall group");
 }
 
 int eso_add_gravity(eso_engine* e, const char* name, const double g[3])

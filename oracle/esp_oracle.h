@@ -46,6 +46,7 @@ int eso_add_elastic_wall_group(eso_engine* e, const char* name, const char* wall
 int eso_add_gravity(eso_engine* e, const char* name, const double g[3]);
 int eso_add_bonded_wall_group(eso_engine* e, const char* name, const char* wall, double k,
                               int tag, int mask);
+int eso_set_wall_group_tags(eso_engine* e, const char* name, int tag, int mask);
 int eso_add_damping_group(eso_engine* e, int type, const char* name, double viscosity,
                           int max_iterations);
 int eso_set_particle_vel(eso_engine* e, int id, const double v[3]);

@@ -67,3 +67,24 @@ def test_tag_mask(pkg, orc):
     np.testing.assert_allclose(f[0], [10 * 0.2e-3 * kp, 0, 0], rtol=1e-12)
     np.testing.assert_allclose(f[1], [10 * 0.2e-3 * kp, 0, 0], rtol=1e-12)
     np.testing.assert_array_equal(f[2], [0.0, 0.0, 0.0])
+
+
+def test_tagged_elastic_wall(pkg, orc):
+    """createInteractionGroupTagged(NRotElasticWallPrms, tag, mask): only
+    candidates with the tag feel the wall (TaggedEWallInteractionGroup.hpp:28-52);
+    the force of the elastic law is k 3.141592654 r (r - dist) along the normal
+    (EWallInteraction.hpp:26-67)."""
+    sim = orc.oracle_lsm_class(pkg)()
+    sim.initNeighbourSearch("NRotSphere", 2.0, 0.2)
+    sim.setTimeStepSize(1e-4)
+    sim.setSpatialDomain(pkg.BoundingBox(pkg.Vec3(-10, -10, -10), pkg.Vec3(10, 10, 10)))
+    sim.createParticleArrays([0, 1, 2], [[0.0, 0.4, 0.0], [3.0, 0.4, 0.0], [6.0, 0.4, 0.0]],
+                             [0.5] * 3, mass=[1.0] * 3, tag=[5, 4, 13])
+    sim.createWall("floor", pkg.Vec3(0, 0, 0), pkg.Vec3(0, 1, 0))
+    sim.createInteractionGroupTagged(pkg.NRotElasticWallPrms("w", "floor", 20.0), 5, 7)
+    sim.runTimeStep()
+    f = sim.getParticleData("force")
+    push = 20.0 * 3.141592654 * 0.5 * (0.5 - 0.4)
+    for k, hit in enumerate([True, False, True]):  # 5 & 7 == 5, 4 & 7 != 5, 13 & 7 == 5
+        np.testing.assert_allclose(f[k], [0.0, push if hit else 0.0, 0.0], rtol=1e-12)
+    np.testing.assert_allclose(list(sim.getWallForce("floor")), [0.0, 2 * push, 0.0], rtol=1e-12)
