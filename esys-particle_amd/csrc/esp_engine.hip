@@ -207,6 +207,7 @@ struct esp_engine {
   DBuf<UpdState> upd;
   int nrec = 0;
   DBuf<double> slots;     // per-particle slots: CSR runs, then ESP_XSLOTS appendix slots each
+  DBuf<unsigned char> sact;  // per slot: 1 while it holds a nonzero share
   DBuf<int2> slot_of;     // record -> its two slots
   int ss = 6;             // doubles per slot (9 when particle stresses are tracked)
   int xbase = 0;          // first appendix slot
@@ -1003,6 +1004,8 @@ void rebuild_csr(esp_engine* e)
   const size_t nslot = (size_t)total + (size_t)std::max(e->n, 1) * ESP_XSLOTS;
   e->slots.reserve(nslot * e->ss);
   zero(e, e->slots.p, nslot * e->ss);
+  e->sact.reserve(nslot);
+  zero(e, e->sact.p, nslot);
   zero(e, e->xcnt.p, std::max(e->n, 1));
   HIPCHK(hipGetLastError());
 }
@@ -1013,6 +1016,7 @@ BondSet bond_set(esp_engine* e)
   memset(&bs, 0, sizeof bs);
   bs.ss = e->ss;
   bs.slots = e->slots.p;
+  bs.sact = e->sact.p;
   bs.slot_of = e->slot_of.p;
   for (int gi : e->bord) {
     BondGroup& G = e->bg[gi];
@@ -1045,6 +1049,7 @@ ForceLaunch force_launch(esp_engine* e, long long step, bool flip)
   fl.dt = e->dt;
   fl.step = (double)step;
   fl.slots = e->slots.p;
+  fl.sact = e->sact.p;
   fl.slot_of = e->slot_of.p;
   fl.broken_list = broken_list_of(e, step);
   fl.crit = e->track_crit ? e->crit.p : nullptr;
@@ -1174,6 +1179,7 @@ void launch_step(esp_engine* e, const StepCtl& ctl, long long step, bool flip, b
   gp.xbase = e->xbase;
   gp.grid = e->grid;
   gp.slots = e->slots.p;
+  gp.sact = e->sact.p;
   gp.dt = e->dt;
   gp.rot = e->rot;
   gp.do2d = e->do2d;

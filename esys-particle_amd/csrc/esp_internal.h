@@ -96,6 +96,7 @@ struct GatherParams {
   int xbase;
   GridParams grid;     // wall candidates are refreshed after bond breaks
   const double* slots;
+  const unsigned char* sact;  // per slot: nonzero contents
   double dt;
   int rot, do2d;
   int nwg;
@@ -143,6 +144,7 @@ struct BondSet {
   int count;
   int ss;               // doubles per slot
   double* slots;
+  unsigned char* sact;  // per slot: nonzero contents
   const int2* slot_of;  // record -> its two slots
   int off[ESP_MAX_BOND_GROUPS];  // global bond index of the group's first bond
   int n[ESP_MAX_BOND_GROUPS];
@@ -213,6 +215,7 @@ struct ForceLaunch {
   double dt;
   double step;  // this step's number (bond break stamps)
   double* slots;
+  unsigned char* sact;  // per slot: 1 while it holds a nonzero share (zero slots are not read)
   const int2* slot_of;
   int* broken_list;  // this step's break list
   double* crit;      // per global bond index: breaking criterion of this step, or nullptr

@@ -173,6 +173,8 @@ __device__ __forceinline__ void rot_bond_item(const StepCtl& ctl, const PView& p
   put3(s1, neg3(o.f1));
   put3(s1 + 3, o.m2);
   if (fl.ss == 9) put_rel(s0, s1, P1, P2);
+  fl.sact[so.x] = 1;
+  fl.sact[so.y] = 1;
   // CRotBondedInteraction::getCriterion (RotBondedInteraction.cpp:221-227)
   if (fl.crit)
     fl.crit[g.bond_off + b] = o.nForce / M.x + o.shForce / M.y + o.tMoment / M.z + o.bMoment / M.w;
@@ -212,6 +214,8 @@ __device__ __forceinline__ void rot_friction_item(const PView& p, int c, const F
       g.ff_out[c] = make_double4(0.0, 0.0, 0.0, 0.0);
       zero_slot(s0, fl.ss);
       zero_slot(s1, fl.ss);
+      fl.sact[so.x] = 0;
+      fl.sact[so.y] = 0;
     } else if (!(fw & 4)) {
       g.ff[c] = make_double4(0.0, 0.0, 0.0, 4.0);
       g.ff_out[c] = make_double4(0.0, 0.0, 0.0, 4.0);
@@ -232,6 +236,8 @@ __device__ __forceinline__ void rot_friction_item(const PView& p, int c, const F
   put3(s1, neg3(f1));
   put3(s1 + 3, m2);
   if (fl.ss == 9) put_rel(s0, s1, P1, P2);
+  fl.sact[so.x] = 1;
+  fl.sact[so.y] = 1;
 }
 
 // CElasticInteraction::calcForces (ElasticInteraction.cpp:54-71); no moments
@@ -255,9 +261,13 @@ __device__ __forceinline__ void nrot_elastic_item(const PView& p, int c, const F
     put3(s0, neg3(force));
     put3(s1, force);
     if (fl.ss == 9) put_rel(s0, s1, P1, P2);
+    fl.sact[so.x] = 1;
+    fl.sact[so.y] = 1;
   } else {
     put3(s0, mk3(0.0, 0.0, 0.0));
     put3(s1, mk3(0.0, 0.0, 0.0));
+    fl.sact[so.x] = 0;
+    fl.sact[so.y] = 0;
   }
 }
 
@@ -280,9 +290,13 @@ __device__ __forceinline__ void rot_elastic_item(const PView& p, int c, const Fo
     put3(s0, neg3(force));
     put3(s1, force);
     if (fl.ss == 9) put_rel(s0, s1, P1, P2);
+    fl.sact[so.x] = 1;
+    fl.sact[so.y] = 1;
   } else {
     put3(s0, mk3(0.0, 0.0, 0.0));
     put3(s1, mk3(0.0, 0.0, 0.0));
+    fl.sact[so.x] = 0;
+    fl.sact[so.y] = 0;
   }
 }
 
@@ -473,14 +487,21 @@ __global__ __launch_bounds__(BS) void k_gather_integrate(StepCtl ctl, PView p, G
     const int ss = STRESS ? 9 : 6;
     if (!STRESS) {
       // 6-double slots are 16-byte aligned: three 16-byte loads each
+      // a zero slot (a contact apart) is skipped: adding +0 changes no sum
       for (int b = 0; b < cnt; b += GATHER_CHUNK6) {
         double2 qa[GATHER_CHUNK6], qb[GATHER_CHUNK6], qc[GATHER_CHUNK6];
+        int sl[GATHER_CHUNK6];
+        bool on[GATHER_CHUNK6];
 #pragma unroll
         for (int k = 0; k < GATHER_CHUNK6; k++) {
           const int t = b + k;
-          if (t < cnt) {
-            const int slot = t < nmain ? e0 + t : gp.xbase + ESP_XSLOTS * i + (t - nmain);
-            const double2* q = reinterpret_cast<const double2*>(gp.slots + 6 * (size_t)slot);
+          sl[k] = t < nmain ? e0 + t : gp.xbase + ESP_XSLOTS * i + (t - nmain);
+          on[k] = t < cnt && gp.sact[sl[k]] != 0;
+        }
+#pragma unroll
+        for (int k = 0; k < GATHER_CHUNK6; k++) {
+          if (on[k]) {
+            const double2* q = reinterpret_cast<const double2*>(gp.slots + 6 * (size_t)sl[k]);
             qa[k] = q[0];
             qb[k] = q[1];
             qc[k] = q[2];
@@ -488,7 +509,7 @@ __global__ __launch_bounds__(BS) void k_gather_integrate(StepCtl ctl, PView p, G
         }
 #pragma unroll
         for (int k = 0; k < GATHER_CHUNK6; k++) {
-          if (b + k >= cnt) break;
+          if (!on[k]) continue;
           F = add3(F, mk3(qa[k].x, qa[k].y, qb[k].x));
           M = add3(M, mk3(qb[k].y, qc[k].x, qc[k].y));
         }
@@ -1008,6 +1029,8 @@ __device__ __forceinline__ void break_one(PView p, const BondSet& bs, int gb, Up
     s0[k] = 0.0;
     s1[k] = 0.0;
   }
+  bs.sact[so.x] = 0;
+  bs.sact[so.y] = 0;
   evt[ev] = make_int4((int)step, id1, id2, 0);
   hs_erase(bs.hs[g], pair_key(id1, id2));
 }
