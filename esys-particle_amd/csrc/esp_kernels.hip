@@ -198,22 +198,21 @@ __device__ __forceinline__ void rot_friction_item(const PView& p, int c, const F
 {
   const int nc = g.n;
   const int2 e = g.ij[c];
-  const int2 so = fl.slot_of[record_of(c, nc, g.rec_off, g.app_rec_off)];
+  const int rec = record_of(c, nc, g.rec_off, g.app_rec_off);
   const double4 P1 = p.pos[e.x], P2 = p.pos[e.y];
   const double4 st = g.ff[c];
-  double* s0 = fl.slots + (size_t)fl.ss * so.x;
-  double* s1 = fl.slots + (size_t)fl.ss * so.y;
   const dv3 Dq = sub3(xyz(P1), xyz(P2));
   const double eq = P1.w + P2.w;
   if (!(dot3(Dq, Dq) < (eq * eq))) {
     // not touching: history cleared, no force.  A pair that did not touch in
     // the last step already holds zero slots; once both history buffers hold
-    // zero (flag bit 2) nothing is written.
+    // zero (flag bit 2) nothing is written (nor are its slots looked up).
     const int fw = (int)st.w;
     if ((fw & 3) != 0 || st.x != 0.0 || st.y != 0.0 || st.z != 0.0) {
+      const int2 so = fl.slot_of[rec];
       g.ff_out[c] = make_double4(0.0, 0.0, 0.0, 0.0);
-      zero_slot(s0, fl.ss);
-      zero_slot(s1, fl.ss);
+      zero_slot(fl.slots + (size_t)fl.ss * so.x, fl.ss);
+      zero_slot(fl.slots + (size_t)fl.ss * so.y, fl.ss);
       fl.sact[so.x] = 0;
       fl.sact[so.y] = 0;
     } else if (!(fw & 4)) {
@@ -222,6 +221,10 @@ __device__ __forceinline__ void rot_friction_item(const PView& p, int c, const F
     }
     return;
   }
+  // touching: the slot lookup goes out with the velocity loads
+  const int2 so = fl.slot_of[rec];
+  double* s0 = fl.slots + (size_t)fl.ss * so.x;
+  double* s1 = fl.slots + (size_t)fl.ss * so.y;
   const double4 V1 = p.vel[e.x], V2 = p.vel[e.y], W1 = p.ang[e.x], W2 = p.ang[e.y];
   double kk, kks;
   esp_friction_k(g.k, g.ks, g.scaling, g.meanR, P1.w, P2.w, fl.do2d, kk, kks);
