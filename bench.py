@@ -28,6 +28,7 @@ HBM_PEAK_GBS = 8000.0
 
 # algorithmic bytes (compulsory state traffic) per unit, FP64 structure of arrays
 B_PARTICLE_ROT = 256  # read pos3 vel3 w3 q4 r 1/m 1/I old3 (19 doubles), write pos vel w q (13)
+B_PARTICLE_NROT = 136  # read pos3 vel3 r 1/m old3 (11 doubles), write pos vel (6)
 B_BOND = 96           # i, j (2 x int32) + kr ks kt kb + 4 strengths + D0 (3 doubles)
 B_FRICTION = 64       # i, j + shear spring (3) + flags, read and write
 B_ELASTIC = 8         # i, j
@@ -85,21 +86,23 @@ def load_specimen(path):
         return pickle.load(f)
 
 
-def cpu_baseline(pkg, spm, spec, budget_s):
+def cpu_baseline(pkg, spm, spec, budget_s, max_steps):
     """The oracle (C restatement of the reference path, one thread) stepping the
-    same specimen from its initial state until ~budget_s seconds have passed."""
+    same specimen from its initial state until ~budget_s seconds have passed or
+    max_steps steps (the device run's length) are done."""
     orc = _load("esp_oracle_driver", os.path.join(ROOT, "oracle", "oracle.py"))
     sim = orc.oracle_lsm_class(pkg)()
     spm.apply(sim, spec)
-    n = len(spec["ids"])
     sim.runTimeSteps(1)  # first step: initial neighbour search, excluded
-    steps, t0 = 0, time.perf_counter()
-    while time.perf_counter() - t0 < budget_s:
+    steps, psteps, t0 = 0, 0, time.perf_counter()
+    while time.perf_counter() - t0 < budget_s and steps < max_steps:
         sim.runTimeSteps(10)
         steps += 10
+        psteps += 10 * sim.getNumParticles()
     el = time.perf_counter() - t0
-    return {"value": steps * n / el, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-            "sample": "steps 2-%d of the same 100k bonded specimen (%.1f s)" % (steps + 1, el)}
+    return {"value": psteps / el, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+            "sample": "steps 2-%d of the same specimen, %d particles at the end (%.1f s)" % (
+                steps + 1, sim.getNumParticles(), el)}
 
 
 def main():
@@ -115,6 +118,10 @@ def main():
                     help="npz file: load the specimen from it if present, else build and save it")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device 10 times (counter calibration)")
+    ap.add_argument("--workload", default="compression", choices=["compression", "settling"],
+                    help="compression: the bonded 100k-particle compression (default); "
+                         "settling: 10,000 NRotSphere settling under gravity on an elastic "
+                         "floor (NRotElastic k=1, dt=1e-4, the reference's CPU-sized case)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the engine holds one subdomain on one device "
@@ -127,14 +134,21 @@ def main():
     pkg = load_pkg()
     spm = importlib.import_module("esys_particle_amd.specimens")
     t0 = time.perf_counter()
+    settling = args.workload == "settling"
     spec = load_specimen(args.specimen) if args.specimen and os.path.exists(args.specimen) else None
     if spec is None:
-        spec = spm.dense_bonded_compression(n=args.particles, seed=3)
+        if settling:
+            spec = spm.gravity_settling(n=10000, seed=1, box=(20.0, 40.0, 20.0), dt=1e-4)
+            # the soft floor (k=1) lets the lowest particles sink through it; the
+            # reference deletes particles that leave the neighbour table, this
+            # engine does not, so the domain reaches further down
+            spec["domain"] = ((-1.0, -10.0, -1.0), (21.0, 41.0, 21.0))
+        else:
+            spec = spm.dense_bonded_compression(n=args.particles, seed=3)
         if args.specimen:
             save_specimen(args.specimen, spec)
-    log("specimen: %d particles, %d bonds, box %s (%.1f s)" % (
-        len(spec["ids"]), len(spec["bonds"]), tuple(round(b, 2) for b in spec["box"]),
-        time.perf_counter() - t0))
+    log("specimen: %d particles, %d bonds (%.1f s)" % (
+        len(spec["ids"]), len(spec.get("bonds", [])), time.perf_counter() - t0))
 
     sim = pkg.LsmMpi(device=0, stepsPerBatch=args.batch)
     spm.apply(sim, spec)
@@ -164,14 +178,17 @@ def main():
 
     nb = 0.5 * (s1["num_bonds"] + s2["num_bonds"])
     nd = 0.5 * (s1["num_dynamic"] + s2["num_dynamic"])
-    per_launch = {"forces": nb * B_BOND + nd * B_FRICTION,
-                  "gather_integrate": n * B_PARTICLE_ROT}
-    traffic = {"forces": (nb + nd) * TRAFFIC_FORCES_PER_INTERACTION,
-               "gather_integrate": n * TRAFFIC_GATHER_PER_PARTICLE}
+    b_part = B_PARTICLE_NROT if settling else B_PARTICLE_ROT
+    b_dyn = B_ELASTIC if settling else B_FRICTION
+    per_launch = {"forces": nb * B_BOND + nd * b_dyn,
+                  "gather_integrate": n * b_part}
+    # measured bytes per unit exist for the compression workload only
+    traffic = {"forces": None if settling else (nb + nd) * TRAFFIC_FORCES_PER_INTERACTION,
+               "gather_integrate": None if settling else n * TRAFFIC_GATHER_PER_PARTICLE}
     dom = max((k for k in per_launch if kt[k][1] > 0), key=lambda k: kt[k][0])
     avg_s = kt[dom][0] / kt[dom][1] * 1e-3
     achieved = per_launch[dom] / avg_s / 1e9
-    step_bytes = n * B_PARTICLE_ROT + nb * B_BOND + nd * B_FRICTION
+    step_bytes = n * b_part + nb * B_BOND + nd * b_dyn
     ms_per_step = el / args.steps * 1e3
     out = {
         "metric": "particle-steps/s",
@@ -186,7 +203,15 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {
+        "config": ({
+            "workload": "gravity settling, %d NRotSphere, radii U[0.2, 0.5], density 1, "
+                        "NRotElastic k=1, elastic floor k=1, g=9.81, dt=1e-4 (domain down "
+                        "to y=-10)" % n,
+            "particles": n,
+            "elastic_pairs": int(s1["num_dynamic"]),
+            "box": [20.0, 40.0, 20.0],
+            "steps_per_sync": args.batch,
+        } if settling else {
             "workload": "bonded rotational uniaxial compression, %d RotSphere, radii "
                         "log-uniform [0.2, 1.0], RotBond kr=1e5 ks=kt=kb=3e4, cohesion "
                         "N(50,5), RotFriction mu=0.6, walls 0.05/time, dt=1e-5" % n,
@@ -195,7 +220,7 @@ def main():
             "friction_pairs": int(s1["num_dynamic"]),
             "prism": [round(b, 3) for b in spec["box"]],
             "steps_per_sync": args.batch,
-        },
+        }),
         "roofline": {
             "bound": "hbm",
             "kernel": dom,
@@ -204,7 +229,7 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic[dom],
-            "traffic_GBs": traffic[dom] / avg_s / 1e9,
+            "traffic_GBs": None if traffic[dom] is None else traffic[dom] / avg_s / 1e9,
             "algorithmic_bytes_per_launch": per_launch[dom],
             "avg_launch_us": avg_s * 1e6,
         },
@@ -219,7 +244,8 @@ def main():
     }
     if not args.no_cpu_baseline:
         t0 = time.perf_counter()
-        out["cpu_baseline"] = cpu_baseline(pkg, spm, spec, args.cpu_budget)
+        out["cpu_baseline"] = cpu_baseline(pkg, spm, spec, args.cpu_budget,
+                                           args.warmup + args.steps)
         log("cpu baseline: %.1f s" % (time.perf_counter() - t0))
     print(json.dumps(out), flush=True)
 
