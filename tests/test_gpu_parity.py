@@ -350,3 +350,49 @@ def test_tagged_wall(pkg, orc, spm):
     for w in (bn, tn):
         assert rel_field(g.engine.wall_force_history(w, 200), o.engine.wall_force_history(w, 200)) <= TRAJ_TOL
     assert np.abs(o.engine.wall_force_history(tn, 200)).max() > 0
+
+
+def test_2d_bonded_lattice(pkg, orc, spm):
+    """force2dComputations(True) on a planar (one layer, all z equal) bonded
+    lattice squeezed by the walls: the 2-D branches of the bond, friction and
+    rotational integration laws.  Forces and moments of step 2 within 1e-12,
+    events bit-exact and trajectories within 1e-9 after 200 steps; the motion
+    stays in the plane (z, angvel x/y and the quaternion's x/y parts remain
+    exactly zero on both sides)."""
+    spec = spm.lattice_block(nx=12, ny=12, nz=1, dt=1e-4, vel_sd=0.05, wall_speed=8.0)
+    spec["vel"][:, 2] = 0.0
+    spec["pos"][:, 2] = 1.5
+    top = float(spec["pos"][:, 1].max() + spec["rad"].max()) - 0.01
+    bot = float(spec["pos"][:, 1].min() - spec["rad"].max()) + 0.01
+    w = spec["walls"]
+    spec["walls"] = [(w[0][0], (0.0, bot, 0.0)) + w[0][2:], (w[1][0], (0.0, top, 0.0)) + w[1][2:]]
+    sims = []
+    for sim in (pkg.LsmMpi(stepsPerBatch=32), orc.oracle_lsm_class(pkg)()):
+        sim.force2dComputations(True)
+        spm.apply(sim, spec)
+        sims.append(sim)
+    g, o = sims
+    print("case: 2-D lattice n=%d" % len(spec["ids"]), flush=True)
+    g.runTimeSteps(2)
+    o.runTimeSteps(2)
+    assert rel_field(g.getParticleData("force"), o.getParticleData("force")) <= FORCE_TOL
+    assert rel_field(g.getParticleData("moment"), o.getParticleData("moment")) <= FORCE_TOL
+    g.runTimeSteps(198)
+    o.runTimeSteps(198)
+    assert len(o.getBondBreaks()) > 20 and len(o.engine.rebuild_steps()) >= 2
+    assert_events_equal(g, o, ["bonded", "friction"])
+    r = float(np.mean(spec["rad"]))
+    assert np.max(np.abs(g.getParticleData("pos") - o.getParticleData("pos"))) <= TRAJ_TOL * r
+    assert rel_field(g.getParticleData("vel"), o.getParticleData("vel")) <= TRAJ_TOL
+    assert rel_field(g.getParticleData("angvel"), o.getParticleData("angvel")) <= TRAJ_TOL
+    gf, gfl = g.engine.friction_state("friction")
+    of, ofl = o.engine.friction_state("friction")
+    np.testing.assert_array_equal(gfl, ofl)
+    assert (ofl & 2).sum() > 10
+    for sim in (g, o):
+        assert np.all(sim.getParticleData("pos")[:, 2] == 1.5)
+        assert np.all(sim.getParticleData("angvel")[:, :2] == 0.0)
+        assert np.all(sim.getParticleData("quat")[:, 1:3] == 0.0)
+    for name in ("bottom", "top"):
+        assert rel_field(g.engine.wall_force_history(name, 200),
+                         o.engine.wall_force_history(name, 200)) <= TRAJ_TOL
