@@ -411,7 +411,8 @@ template <bool STRESS, bool EXTRA>
 __global__ __launch_bounds__(BS) void k_gather_integrate(StepCtl ctl, PView p, GatherParams gp)
 {
   constexpr int GATHER_CHUNK = STRESS ? 2 : 4;
-  constexpr int GATHER_CHUNK6 = 4;  // slots fetched together (6-double slots)
+  constexpr int GATHER_CHUNK6 = 8;  // active slots fetched together (6-double slots)
+  constexpr int GATHER_WIN = 28;    // slots per activity-mask window (8 words with any offset)
   double sg[3][3];
   for (int i = 0; i < 3; i++)
     for (int j = 0; j < 3; j++) sg[i][j] = 0.0;
@@ -489,33 +490,63 @@ __global__ __launch_bounds__(BS) void k_gather_integrate(StepCtl ctl, PView p, G
     const int cnt = nmain + xcnt;
     const int ss = STRESS ? 9 : 6;
     if (!STRESS) {
-      // 6-double slots are 16-byte aligned: three 16-byte loads each
-      // a zero slot (a contact apart) is skipped: adding +0 changes no sum
-      for (int b = 0; b < cnt; b += GATHER_CHUNK6) {
-        double2 qa[GATHER_CHUNK6], qb[GATHER_CHUNK6], qc[GATHER_CHUNK6];
-        int sl[GATHER_CHUNK6];
-        bool on[GATHER_CHUNK6];
+      // A zero slot (a contact apart) is skipped: adding +0 changes no sum.
+      // The activity bytes (0 or 1) of up to GATHER_WIN slots of the run come
+      // in as aligned 32-bit words in one round of loads and become a bit
+      // mask; the active slots then load GATHER_CHUNK6 at a time, in run order.
+      // 6-double slots are 16-byte aligned: three 16-byte loads each.
+      const unsigned* sw = reinterpret_cast<const unsigned*>(gp.sact);
+      for (int wb = 0; wb < cnt; wb += GATHER_WIN) {
+        unsigned mask = 0;
+        int base = 0;  // slot index of mask bit 0
+        if (wb < nmain) {
+          const int s0 = e0 + wb, nw = min(GATHER_WIN, nmain - wb), sh = s0 & 3;
+          unsigned wv[GATHER_WIN / 4 + 1];
 #pragma unroll
-        for (int k = 0; k < GATHER_CHUNK6; k++) {
-          const int t = b + k;
-          sl[k] = t < nmain ? e0 + t : gp.xbase + ESP_XSLOTS * i + (t - nmain);
-          on[k] = t < cnt && gp.sact[sl[k]] != 0;
+          for (int k = 0; k <= GATHER_WIN / 4; k++) wv[k] = 4 * k < sh + nw ? sw[(s0 >> 2) + k] : 0u;
+          unsigned long long m = 0;
+#pragma unroll
+          for (int k = 0; k <= GATHER_WIN / 4; k++) {
+            const unsigned v = wv[k];
+            const unsigned long long m4 = (v & 1u) | ((v >> 7) & 2u) | ((v >> 14) & 4u) | ((v >> 21) & 8u);
+            m |= m4 << (4 * k);
+          }
+          mask = (unsigned)(m >> sh) & (nw == 32 ? 0xFFFFFFFFu : ((1u << nw) - 1u));
+          base = s0;
+        } else {
+          // the appendix slots follow the run (at most ESP_XSLOTS of them)
+          for (int t = 0; t < xcnt; t++)
+            if (gp.sact[gp.xbase + ESP_XSLOTS * i + t]) mask |= 1u << t;
+          base = gp.xbase + ESP_XSLOTS * i;
         }
+        while (mask) {
+          double2 qa[GATHER_CHUNK6], qb[GATHER_CHUNK6], qc[GATHER_CHUNK6];
+          int sl[GATHER_CHUNK6];
+          bool on[GATHER_CHUNK6];
 #pragma unroll
-        for (int k = 0; k < GATHER_CHUNK6; k++) {
-          if (on[k]) {
-            const double2* q = reinterpret_cast<const double2*>(gp.slots + 6 * (size_t)sl[k]);
-            qa[k] = q[0];
-            qb[k] = q[1];
-            qc[k] = q[2];
+          for (int k = 0; k < GATHER_CHUNK6; k++) {
+            on[k] = mask != 0;
+            sl[k] = base + (on[k] ? __builtin_ctz(mask) : 0);
+            mask &= mask - 1u;
+          }
+#pragma unroll
+          for (int k = 0; k < GATHER_CHUNK6; k++) {
+            if (on[k]) {
+              const double2* q = reinterpret_cast<const double2*>(gp.slots + 6 * (size_t)sl[k]);
+              qa[k] = q[0];
+              qb[k] = q[1];
+              qc[k] = q[2];
+            }
+          }
+#pragma unroll
+          for (int k = 0; k < GATHER_CHUNK6; k++) {
+            if (!on[k]) continue;
+            F = add3(F, mk3(qa[k].x, qa[k].y, qb[k].x));
+            M = add3(M, mk3(qb[k].y, qc[k].x, qc[k].y));
           }
         }
-#pragma unroll
-        for (int k = 0; k < GATHER_CHUNK6; k++) {
-          if (!on[k]) continue;
-          F = add3(F, mk3(qa[k].x, qa[k].y, qb[k].x));
-          M = add3(M, mk3(qb[k].y, qc[k].x, qc[k].y));
-        }
+        // the appendix starts its own window right after the run
+        if (wb < nmain && wb + GATHER_WIN >= nmain) wb = nmain - GATHER_WIN;
       }
     }
     for (int b = 0; STRESS && b < cnt; b += GATHER_CHUNK) {
