@@ -37,8 +37,8 @@ B_ELASTIC = 8         # i, j
 # FETCH_SIZE and WRITE_SIZE over steady-state steps, each scaled by the same
 # counter's reading for a plain 256 MiB device copy (FETCH_SIZE x2.000,
 # WRITE_SIZE x1.000); profiles/step_kernels_bonded100k.md
-TRAFFIC_GATHER_PER_PARTICLE = 667.0
-TRAFFIC_FORCES_PER_INTERACTION = 179.0
+TRAFFIC_GATHER_PER_PARTICLE = 582.0
+TRAFFIC_FORCES_PER_INTERACTION = 178.0
 
 
 def _load(name, path, pkgdir=None):
