@@ -396,3 +396,49 @@ def test_2d_bonded_lattice(pkg, orc, spm):
     for name in ("bottom", "top"):
         assert rel_field(g.engine.wall_force_history(name, 200),
                          o.engine.wall_force_history(name, 200)) <= TRAJ_TOL
+
+
+def _lattice_100k(spm):
+    spec = spm.lattice_block(nx=40, ny=50, nz=50, dt=1e-4, vel_sd=0.02, wall_speed=5.0,
+                             coh_mean=200.0, coh_sd=40.0)
+    top = float(spec["pos"][:, 1].max() + spec["rad"].max()) - 0.01
+    bot = float(spec["pos"][:, 1].min() - spec["rad"].max()) + 0.01
+    w = spec["walls"]
+    spec["walls"] = [(w[0][0], (0.0, bot, 0.0)) + w[0][2:], (w[1][0], (0.0, top, 0.0)) + w[1][2:]]
+    return spec
+
+
+def test_full_size_bonded_lattice_100k(pkg, orc, spm):
+    """100,000 bonded RotSpheres (about 297,000 bonds, cohesion N(200, 40)) under
+    the walls for 150 steps, queued 64 per synchronisation: the bonds that break
+    do so on the device's queued path.  Events bit-exact, trajectories within
+    1e-9, at the size of the compression benchmark."""
+    spec = _lattice_100k(spm)
+    g, o = run_pair(pkg, orc, spm, spec, 150, batch=64)
+    assert len(o.getBondBreaks()) > 50
+    assert_events_equal(g, o, ["bonded", "friction"])
+    r = float(np.mean(spec["rad"]))
+    assert np.max(np.abs(g.getParticleData("pos") - o.getParticleData("pos"))) <= TRAJ_TOL * r
+    assert rel_field(g.getParticleData("vel"), o.getParticleData("vel")) <= TRAJ_TOL
+    assert rel_field(g.getParticleData("angvel"), o.getParticleData("angvel")) <= TRAJ_TOL
+    for name in ("bottom", "top"):
+        assert rel_field(g.engine.wall_force_history(name, 150),
+                         o.engine.wall_force_history(name, 150)) <= TRAJ_TOL
+
+
+def test_compression_specimen_first_steps(pkg, orc, spm):
+    """The benchmark's own specimen (100,000 RotSpheres, radii log-uniform
+    [0.2, 1.0], 210,507 bonds with cohesion N(50, 5)): about 23,000 bonds break
+    in step 1, and the flagged-pair list of step 2 (pairs of flagged particles in
+    neighbouring cells, no distance test) takes the friction group to about 1.2
+    million pairs.  After 5 steps: events and lists bit-exact, forces and moments
+    of step 5 within 1e-12."""
+    spec = spm.dense_bonded_compression(n=100000, seed=3)
+    g, o = run_pair(pkg, orc, spm, spec, 5, batch=64)
+    assert len(o.getBondBreaks()) > 20000
+    assert o.getNumInteractions("friction") > 1000000
+    assert_events_equal(g, o, ["bonded", "friction"])
+    assert rel_field(g.getParticleData("force"), o.getParticleData("force")) <= FORCE_TOL
+    assert rel_field(g.getParticleData("moment"), o.getParticleData("moment")) <= FORCE_TOL
+    r = float(np.mean(spec["rad"]))
+    assert np.max(np.abs(g.getParticleData("pos") - o.getParticleData("pos"))) <= TRAJ_TOL * r
