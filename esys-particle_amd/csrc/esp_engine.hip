@@ -276,24 +276,24 @@ hipEvent_t next_event(esp_engine* e)
   return e->ev_pool[e->ev_used++];
 }
 
-// event pair around one kernel launch (no-op unless timing is on)
-struct LaunchTimer {
-  esp_engine* e;
-  hipEvent_t b;
-  int cls, step_id;
-  LaunchTimer(esp_engine* e_, int cls_, int step_id_) : e(e_), cls(cls_), step_id(step_id_)
-  {
-    if (!e->timing) return;
-    hipEvent_t a = next_event(e);
-    b = next_event(e);
-    HIPCHK(hipEventRecord(a, e->stream));
-    e->tl.push_back({a, b, cls, step_id});
-  }
-  ~LaunchTimer() noexcept(false)
-  {
-    if (e->timing) HIPCHK(hipEventRecord(b, e->stream));
-  }
+// (start, stop) events for one launch of a queued step when timing is on,
+// else two nulls: the launch stamps them at the kernel's own start and end
+struct LaunchEvents {
+  hipEvent_t a = nullptr, b = nullptr;
 };
+LaunchEvents launch_events(esp_engine* e)
+{
+  LaunchEvents t;
+  if (!e->timing) return t;
+  t.a = next_event(e);
+  t.b = next_event(e);
+  return t;
+}
+// keeps the record of a launch that happened (a skipped launch stamps nothing)
+void note_launch(esp_engine* e, const LaunchEvents& t, bool launched, int cls, int step_id)
+{
+  if (t.a && launched) e->tl.push_back({t.a, t.b, cls, step_id});
+}
 
 // after a synchronisation: add the launches of steps 1..done of the batch
 void collect_timing(esp_engine* e, int done)
@@ -1169,8 +1169,9 @@ void launch_step(esp_engine* e, const StepCtl& ctl, long long step, bool flip, b
     }
   }
   {
-    LaunchTimer lt(e, ESP_KCLASS_FORCES, ctl.step_id);
-    launch_forces(e->stream, ctl, p, fl, dev_update ? &up : nullptr);
+    const LaunchEvents t = launch_events(e);
+    note_launch(e, t, launch_forces(e->stream, ctl, p, fl, dev_update ? &up : nullptr, t.a, t.b),
+                ESP_KCLASS_FORCES, ctl.step_id);
   }
   GatherParams gp;
   memset(&gp, 0, sizeof gp);
@@ -1201,8 +1202,9 @@ void launch_step(esp_engine* e, const StepCtl& ctl, long long step, bool flip, b
   gp.inert = e->inert.p;
   gp.init = e->init.p;
   {
-    LaunchTimer lt(e, ESP_KCLASS_GATHER, ctl.step_id);
-    launch_gather_integrate(e->stream, ctl, p, gp);
+    const LaunchEvents t = launch_events(e);
+    note_launch(e, t, launch_gather_integrate(e->stream, ctl, p, gp, t.a, t.b), ESP_KCLASS_GATHER,
+                ctl.step_id);
   }
   if (e->ss == 9) launch_sc_break(e->stream, ctl, p, sc_launch(e, step));
   HIPCHK(hipGetLastError());

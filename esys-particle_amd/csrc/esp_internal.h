@@ -242,9 +242,10 @@ struct UpdParams {
 };
 // blocks a segment of n interactions takes (dynamic groups: n + ESP_APP_CAP)
 int force_blocks(int n);
-// up: the incremental update of a queued step (fl.ublk = 1), or nullptr
-void launch_forces(hipStream_t s, StepCtl ctl, PView p, const ForceLaunch& fl,
-                   const UpdParams* up);
+// up: the incremental update of a queued step (fl.ublk = 1), or nullptr;
+// ev_a/ev_b: events the launch stamps at the kernel's start and end
+bool launch_forces(hipStream_t s, StepCtl ctl, PView p, const ForceLaunch& fl,
+                   const UpdParams* up, hipEvent_t ev_a = nullptr, hipEvent_t ev_b = nullptr);
 // BrittleBeamSC groups: the stress test after the step's integration, one
 // segment of blocks per group
 struct ScLaunch {
@@ -266,7 +267,8 @@ void launch_hash_clear(hipStream_t s, HashRef h);
 void launch_hash_insert(hipStream_t s, HashRef h, const long long* keys, int n);
 void launch_bond_keys(hipStream_t s, int nb, const int2* ij, const int* id, long long* keys);
 void launch_alive_flags(hipStream_t s, int nb, const int2* ij, int* keep);
-void launch_gather_integrate(hipStream_t s, StepCtl ctl, PView p, const GatherParams& gp);
+bool launch_gather_integrate(hipStream_t s, StepCtl ctl, PView p, const GatherParams& gp,
+                             hipEvent_t ev_a = nullptr, hipEvent_t ev_b = nullptr);
 // wall forces [step][nwg][3] from the partials [step][block][nwg][3]
 void launch_wall_finalize(hipStream_t s, const double* part, int nblocks, int nwg, int nsteps,
                           double* out);

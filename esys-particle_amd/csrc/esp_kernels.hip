@@ -25,6 +25,7 @@
 //   storage updates (history carried over by key lookup), bond removal,
 //   wall candidate refresh and the per-particle CSR of force records.
 #include <cstring>
+#include <hip/hip_ext.h>
 
 #include "esp_internal.h"
 #include "esp_math.h"
@@ -1492,8 +1493,11 @@ int gather_blocks(int n) { return n > 0 ? nblk(n) : 0; }
 
 int force_blocks(int n) { return nblk(n); }
 
-void launch_forces(hipStream_t s, StepCtl ctl, PView p, const ForceLaunch& fl,
-                   const UpdParams* up)
+// With a pair of events (ev_a, ev_b both set) the launch stamps them itself at
+// the kernel's start and end (hipExtLaunchKernelGGL), so their elapsed time is
+// the kernel's own execution time; false when nothing was launched.
+bool launch_forces(hipStream_t s, StepCtl ctl, PView p, const ForceLaunch& fl,
+                   const UpdParams* up, hipEvent_t ev_a, hipEvent_t ev_b)
 {
   UpdParams u;
   if (up) {
@@ -1503,8 +1507,13 @@ void launch_forces(hipStream_t s, StepCtl ctl, PView p, const ForceLaunch& fl,
   }
   ForceLaunch f = fl;
   f.ublk = up ? 1 : 0;
-  if (f.nblocks + f.ublk <= 0) return;
-  hipLaunchKernelGGL(k_forces, dim3(f.nblocks + f.ublk), dim3(BS), 0, s, ctl, p, f, u);
+  if (f.nblocks + f.ublk <= 0) return false;
+  if (ev_a && ev_b)
+    hipExtLaunchKernelGGL(k_forces, dim3(f.nblocks + f.ublk), dim3(BS), 0, s, ev_a, ev_b, 0, ctl, p,
+                          f, u);
+  else
+    hipLaunchKernelGGL(k_forces, dim3(f.nblocks + f.ublk), dim3(BS), 0, s, ctl, p, f, u);
+  return true;
 }
 
 void launch_break_apply(hipStream_t s, StepCtl ctl, PView p, BondSet bs, const int* broken_list,
@@ -1538,19 +1547,31 @@ void launch_alive_flags(hipStream_t s, int nb, const int2* ij, int* keep)
   hipLaunchKernelGGL(k_alive_flags, dim3(nblk(nb)), dim3(BS), 0, s, nb, ij, keep);
 }
 
-void launch_gather_integrate(hipStream_t s, StepCtl ctl, PView p, const GatherParams& gp)
+template <bool STRESS, bool EXTRA>
+static void gather_launch(hipStream_t s, StepCtl ctl, PView p, const GatherParams& gp,
+                          hipEvent_t ev_a, hipEvent_t ev_b)
 {
-  if (p.n <= 0) return;
+  const dim3 g(nblk(p.n)), b(BS);
+  if (ev_a && ev_b)
+    hipExtLaunchKernelGGL((k_gather_integrate<STRESS, EXTRA>), g, b, 0, s, ev_a, ev_b, 0, ctl, p, gp);
+  else
+    hipLaunchKernelGGL((k_gather_integrate<STRESS, EXTRA>), g, b, 0, s, ctl, p, gp);
+}
+
+bool launch_gather_integrate(hipStream_t s, StepCtl ctl, PView p, const GatherParams& gp,
+                             hipEvent_t ev_a, hipEvent_t ev_b)
+{
+  if (p.n <= 0) return false;
   bool extra = gp.ndamp > 0;
   for (int w = 0; w < gp.nwg; w++) extra |= gp.wall[w].bonded != 0 || gp.wall[w].tagged != 0;
-  const dim3 g(nblk(p.n)), b(BS);
   if (gp.sig) {
-    if (extra) hipLaunchKernelGGL((k_gather_integrate<true, true>), g, b, 0, s, ctl, p, gp);
-    else hipLaunchKernelGGL((k_gather_integrate<true, false>), g, b, 0, s, ctl, p, gp);
+    if (extra) gather_launch<true, true>(s, ctl, p, gp, ev_a, ev_b);
+    else gather_launch<true, false>(s, ctl, p, gp, ev_a, ev_b);
   } else {
-    if (extra) hipLaunchKernelGGL((k_gather_integrate<false, true>), g, b, 0, s, ctl, p, gp);
-    else hipLaunchKernelGGL((k_gather_integrate<false, false>), g, b, 0, s, ctl, p, gp);
+    if (extra) gather_launch<false, true>(s, ctl, p, gp, ev_a, ev_b);
+    else gather_launch<false, false>(s, ctl, p, gp, ev_a, ev_b);
   }
+  return true;
 }
 
 // BrittleBeamSCInteraction::broken for every live bond of the stress-criterion

@@ -307,7 +307,8 @@ int esp_get_rebuild_steps(esp_engine* e, int* steps);
 #define ESP_KCLASS_GATHER 1 /* force gather + integration, per step */
 #define ESP_KCLASS_COUNT 2
 /* Enable (1) or disable (0) HIP-event timing of every per-step kernel launch on
- * the engine's stream; timing adds an event pair around each launch. */
+ * the engine's stream: each launch stamps a (start, stop) event pair at the
+ * kernel's own start and end (hipExtLaunchKernelGGL). */
 int esp_set_kernel_timing(esp_engine* e, int enable);
 /* Accumulated device time (ms) and launch count per kernel class, over the
  * steps that completed since timing was enabled; resets the accumulators. */
