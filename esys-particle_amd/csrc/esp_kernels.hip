@@ -414,6 +414,7 @@ __global__ __launch_bounds__(BS) void k_gather_integrate(StepCtl ctl, PView p, G
   constexpr int GATHER_CHUNK = STRESS ? 2 : 4;
   constexpr int GATHER_CHUNK6 = 8;  // active slots fetched together (6-double slots)
   constexpr int GATHER_WIN = 28;    // slots per activity-mask window (8 words with any offset)
+  static_assert(GATHER_WIN < 32 && GATHER_WIN % 4 == 0, "a window's mask fits 32 bits");
   double sg[3][3];
   for (int i = 0; i < 3; i++)
     for (int j = 0; j < 3; j++) sg[i][j] = 0.0;
@@ -512,7 +513,7 @@ __global__ __launch_bounds__(BS) void k_gather_integrate(StepCtl ctl, PView p, G
             const unsigned long long m4 = (v & 1u) | ((v >> 7) & 2u) | ((v >> 14) & 4u) | ((v >> 21) & 8u);
             m |= m4 << (4 * k);
           }
-          mask = (unsigned)(m >> sh) & (nw == 32 ? 0xFFFFFFFFu : ((1u << nw) - 1u));
+          mask = (unsigned)(m >> sh) & ((1u << nw) - 1u);  // nw <= GATHER_WIN < 32
           base = s0;
         } else {
           // the appendix slots follow the run (at most ESP_XSLOTS of them)
