@@ -111,7 +111,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20000)
     ap.add_argument("--warmup", type=int, default=500)
     ap.add_argument("--particles", type=int, default=100000)
-    ap.add_argument("--batch", type=int, default=64, help="steps queued per host sync")
+    ap.add_argument("--batch", type=int, default=128, help="steps queued per host sync")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--specimen", default=None,
