@@ -125,7 +125,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the engine holds one subdomain on one device "
-              "(multi-GPU runs are not part of this build)", file=sys.stderr)
+              "(runs over several GPUs are not supported)", file=sys.stderr)
         sys.exit(2)
     if args.copy_probe:
         copy_probe(args.copy_probe)
