@@ -442,3 +442,42 @@ def test_compression_specimen_first_steps(pkg, orc, spm):
     assert rel_field(g.getParticleData("moment"), o.getParticleData("moment")) <= FORCE_TOL
     r = float(np.mean(spec["rad"]))
     assert np.max(np.abs(g.getParticleData("pos") - o.getParticleData("pos"))) <= TRAJ_TOL * r
+
+
+def test_2d_nrot_settling(pkg, orc, spm):
+    """force2dComputations(True) with NRotSphere: a planar cloud (jittered grid,
+    random in-plane velocities) falling onto an elastic floor under gravity,
+    NRotElastic contacts (their 2-D stiffness form) and CParticle's 2-D
+    integration.  Over 300 steps (about 14 rebuilds): rebuild steps, neighbour
+    sets and pair lists bit-exact, trajectories and the floor's force history
+    within 1e-9, z exactly zero on both sides."""
+    spec = spm.gravity_settling(n=10, seed=4, box=(40.0, 30.0, 0.0), dt=1e-3, k=1000.0,
+                                wall_k=1000.0)
+    rng = np.random.default_rng(9)
+    gx, gy = np.meshgrid(np.arange(30), np.arange(20), indexing="ij")
+    pos = np.zeros((600, 3))
+    pos[:, 0] = 1.0 + 1.2 * gx.ravel() + rng.uniform(-0.05, 0.05, 600)
+    pos[:, 1] = 1.0 + 1.2 * gy.ravel() + rng.uniform(-0.05, 0.05, 600)
+    vel = rng.normal(0.0, 1.0, (600, 3))
+    vel[:, 2] = 0.0
+    spec.update(ids=np.arange(600, dtype=np.int32), pos=pos, rad=rng.uniform(0.2, 0.5, 600),
+                vel=vel, domain=((-1.0, -1.0, -1.0), (40.0, 30.0, 1.0)))
+    sims = []
+    for sim in (pkg.LsmMpi(stepsPerBatch=32), orc.oracle_lsm_class(pkg)()):
+        sim.force2dComputations(True)
+        spm.apply(sim, spec)
+        sims.append(sim)
+    g, o = sims
+    print("case: 2-D NRot settling n=600", flush=True)
+    g.runTimeSteps(300)
+    o.runTimeSteps(300)
+    assert len(o.engine.rebuild_steps()) >= 5 and len(o.getInteractionPairs("elastic")) > 100
+    assert_events_equal(g, o, ["elastic"])
+    r = float(np.mean(spec["rad"]))
+    assert np.max(np.abs(g.getParticleData("pos") - o.getParticleData("pos"))) <= TRAJ_TOL * r
+    assert rel_field(g.getParticleData("vel"), o.getParticleData("vel")) <= TRAJ_TOL
+    for sim in (g, o):
+        assert np.all(sim.getParticleData("pos")[:, 2] == 0.0)
+        assert np.all(sim.getParticleData("vel")[:, 2] == 0.0)
+    assert rel_field(g.engine.wall_force_history("bottom", 300),
+                     o.engine.wall_force_history("bottom", 300)) <= TRAJ_TOL
