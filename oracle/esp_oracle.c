@@ -135,6 +135,9 @@ typedef struct {
   double rad, mass, inv_mass, inert, inv_inert;
   quat q;
   double sigma[3][3];
+  /* sums of the norms of the forces / moments applied in the last step: the
+   * per-particle scale of the parity tolerance (test instrumentation) */
+  double fabs_sum, mabs_sum;
   int flag, is_dyn, is_rot;
 } opart;
 
@@ -531,12 +534,17 @@ static void nt_pairs(eso_engine* e, pairlist* L, int flagged)
 static inline void apply_force(opart* p, v3 f, v3 pos)
 {
   p->force = vadd(p->force, f);
+  p->fabs_sum += vnorm(f);
   v3 rel = vsub(pos, p->pos);
   for (int i = 0; i < 3; i++)
     for (int j = 0; j < 3; j++) p->sigma[i][j] -= vget(f, i) * vget(rel, j);
 }
 /* CRotParticle::applyMoment (RotParticle.cpp:329-332). */
-static inline void apply_moment(opart* p, v3 m) { p->moment = vadd(p->moment, m); }
+static inline void apply_moment(opart* p, v3 m)
+{
+  p->moment = vadd(p->moment, m);
+  p->mabs_sum += vnorm(m);
+}
 
 /* CRotParticle::integrate + CParticle::integrate + CRotParticle::rescale
  * (RotParticle.cpp:277-321, Particle.cpp:257-274). */
@@ -1182,6 +1190,8 @@ static void calc_and_integrate(eso_engine* e)
     opart* p = &e->P[i];
     p->force = V(0.0, 0.0, 0.0);
     p->moment = V(0.0, 0.0, 0.0);
+    p->fabs_sum = 0.0;
+    p->mabs_sum = 0.0;
     memset(p->sigma, 0, sizeof p->sigma);
   }
   for (int w = 0; w < e->nwl; w++) e->wl[w].force = V(0.0, 0.0, 0.0);
@@ -1862,6 +1872,20 @@ int eso_get_particle_data(eso_engine* e, int field, double* out)
 
 /* Restart checkpoints are a file format, not part of the per-step path this
  * restatement checks; the engine's writer and reader are tested on their own. */
+/* Test instrumentation: per particle (ascending id) the sum of the norms of
+ * the forces and the sum of the norms of the moments applied in the last step,
+ * sum_k |f_ik| and sum_k |m_ik| (the scale of a normwise per-particle
+ * tolerance). */
+int eso_get_force_scale(eso_engine* e, double* out)
+{
+  for (int k = 0; k < e->nidkey; k++) {
+    const opart* p = &e->P[e->idkey[k] & 0xffffffffLL];
+    out[2 * k] = p->fabs_sum;
+    out[2 * k + 1] = p->mabs_sum;
+  }
+  return ESP_OK;
+}
+
 int eso_save_checkpoint(eso_engine* e, const char* prefix, int num_time_steps, int precision)
 {
   (void)prefix; (void)num_time_steps; (void)precision;

@@ -17,6 +17,7 @@ _vp = ctypes.c_void_p
 
 ORACLE_EXTRA = {
     "get_group_list_order": (ctypes.c_int, [_vp, ctypes.c_char_p, _pi, _pi]),
+    "get_force_scale": (ctypes.c_int, [_vp, _pd]),
     "law_rot_bond": (None, [_pd, _pd, _pd]),
     "law_rot_friction": (None, [_pd, _pd, _pd, _pd]),
     "law_nrot_elastic": (None, [_pd, ctypes.c_double, _pd]),
