@@ -11,8 +11,8 @@ Workload (one step = one runTimeStep of the whole specimen on one GPU):
 
 Prints one JSON line: throughput in particle-steps/s over the timed steps, the
 roofline of the dominant kernel (algorithmic bytes / HIP-event-measured launch
-time against 8 TB/s), and the oracle (CPU restatement, 1 core) timed on a
-bounded sample of the same specimen.
+time against 8 TB/s), and the oracle (CPU restatement) timed on a bounded sample
+of the same specimen, on one core and as one process per core of the CPU share.
 """
 import argparse
 import importlib
@@ -30,15 +30,9 @@ HBM_PEAK_GBS = 8000.0
 B_PARTICLE_ROT = 256  # read pos3 vel3 w3 q4 r 1/m 1/I old3 (19 doubles), write pos vel w q (13)
 B_PARTICLE_NROT = 136  # read pos3 vel3 r 1/m old3 (11 doubles), write pos vel (6)
 B_BOND = 96           # i, j (2 x int32) + kr ks kt kb + 4 strengths + D0 (3 doubles)
-B_FRICTION = 64       # i, j + shear spring (3) + flags, read and write
+B_FRICTION = 64       # touching contact: i, j + shear spring (3) + flags, read and write
+B_FRICTION_IDLE = 12  # stored pair apart: i, j + the flag word saying its history is zero
 B_ELASTIC = 8         # i, j
-
-# HBM bytes one launch moves on this workload, per unit: rocprofv3 --pmc
-# FETCH_SIZE and WRITE_SIZE over steady-state steps, each scaled by the same
-# counter's reading for a plain 256 MiB device copy (FETCH_SIZE x2.000,
-# WRITE_SIZE x1.000); profiles/step_kernels_bonded100k.md
-TRAFFIC_GATHER_PER_PARTICLE = 582.0
-TRAFFIC_FORCES_PER_INTERACTION = 178.0
 
 
 def _load(name, path, pkgdir=None):
@@ -86,23 +80,57 @@ def load_specimen(path):
         return pickle.load(f)
 
 
-def cpu_baseline(pkg, spm, spec, budget_s, max_steps):
+def cpu_baseline(pkg, spm, spec, budget_s):
     """The oracle (C restatement of the reference path, one thread) stepping the
-    same specimen from its initial state until ~budget_s seconds have passed or
-    max_steps steps (the device run's length) are done."""
+    same specimen from its initial state until ~budget_s seconds have passed."""
     orc = _load("esp_oracle_driver", os.path.join(ROOT, "oracle", "oracle.py"))
     sim = orc.oracle_lsm_class(pkg)()
     spm.apply(sim, spec)
     sim.runTimeSteps(1)  # first step: initial neighbour search, excluded
     steps, psteps, t0 = 0, 0, time.perf_counter()
-    while time.perf_counter() - t0 < budget_s and steps < max_steps:
+    while time.perf_counter() - t0 < budget_s:
         sim.runTimeSteps(10)
         steps += 10
         psteps += 10 * sim.getNumParticles()
     el = time.perf_counter() - t0
     return {"value": psteps / el, "unit": "particle-steps/s", "cores": 1, "kind": "port",
             "sample": "steps 2-%d of the same specimen, %d particles at the end (%.1f s)" % (
-                steps + 1, sim.getNumParticles(), el)}
+                steps + 1, sim.getNumParticles(), el)}, steps
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline_all_cores(spec_path, budget_s, workers):
+    """`workers` single-thread oracle processes, each stepping its own copy of
+    the same specimen for ~budget_s seconds: aggregate particle-steps/s, the
+    throughput of the host cores on independent copies (an upper bound on a
+    domain-decomposed run of the reference over the same cores).  Child
+    processes, so nothing here shares the GPU process's state."""
+    import subprocess
+    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-worker", spec_path,
+           "--cpu-budget", str(budget_s)]
+    procs = [subprocess.Popen(cmd, stdout=subprocess.PIPE, text=True) for _ in range(workers)]
+    res = []
+    for p in procs:
+        out, _ = p.communicate()
+        if p.returncode != 0:
+            raise RuntimeError("cpu worker failed (%d)" % p.returncode)
+        res.append(json.loads(out.strip().splitlines()[-1]))
+    return {"value": sum(r["value"] for r in res), "unit": "particle-steps/s",
+            "cores": workers, "kind": "port", "nproc": os.cpu_count(),
+            "cpu_model": cpu_model(),
+            "sample": "%d concurrent single-thread oracle processes, each steps 2-~%d of its "
+                      "own copy of the same specimen (%.0f s each)" % (
+                          workers, min(r["steps"] for r in res) + 1, budget_s)}
 
 
 def main():
@@ -114,6 +142,10 @@ def main():
     ap.add_argument("--batch", type=int, default=128, help="steps queued per host sync")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--cpu-workers", type=int, default=0,
+                    help="all-cores CPU leg: this many oracle processes (0: the CPU share, "
+                         "at most 16)")
+    ap.add_argument("--cpu-worker", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--specimen", default=None,
                     help="npz file: load the specimen from it if present, else build and save it")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -130,6 +162,13 @@ def main():
     if args.copy_probe:
         copy_probe(args.copy_probe)
         return
+    if args.cpu_worker:
+        pkg = load_pkg()
+        spm = importlib.import_module("esys_particle_amd.specimens")
+        r, steps = cpu_baseline(pkg, spm, load_specimen(args.cpu_worker), args.cpu_budget)
+        r["steps"] = steps
+        print(json.dumps(r), flush=True)
+        return
 
     pkg = load_pkg()
     spm = importlib.import_module("esys_particle_amd.specimens")
@@ -138,11 +177,9 @@ def main():
     spec = load_specimen(args.specimen) if args.specimen and os.path.exists(args.specimen) else None
     if spec is None:
         if settling:
+            # the soft floor (k=1) lets the lowest particles sink through it and
+            # out of the neighbour table, which erases them, as the reference does
             spec = spm.gravity_settling(n=10000, seed=1, box=(20.0, 40.0, 20.0), dt=1e-4)
-            # the soft floor (k=1) lets the lowest particles sink through it; the
-            # reference deletes particles that leave the neighbour table, this
-            # engine does not, so the domain reaches further down
-            spec["domain"] = ((-1.0, -10.0, -1.0), (21.0, 41.0, 21.0))
         else:
             spec = spm.dense_bonded_compression(n=args.particles, seed=3)
         if args.specimen:
@@ -166,29 +203,40 @@ def main():
     value = n * args.steps / el
     log("timed %d steps: %.3f s, stats %s" % (args.steps, el, s1))
 
-    # per-kernel device time, HIP events on the engine's stream, same step count
+    # per-kernel device time, HIP events on the engine's stream, same step count;
+    # the units the kernels process are counted at the start and end of this pass
+    def units():
+        st = eng.stats()
+        touch = None
+        if not settling:
+            _, fl = eng.friction_state("friction")
+            touch = int(((fl >> 1) & 1).sum())
+        return st, touch
+    s1b, t_a = units()
     eng.set_kernel_timing(True)
     t1 = time.perf_counter()
     sim.runTimeSteps(args.steps)
     el_t = time.perf_counter() - t1
     kt = eng.kernel_times()
     eng.set_kernel_timing(False)
-    s2 = eng.stats()
+    s2, t_b = units()
     log("instrumented %d steps: %.3f s, kernels %s" % (args.steps, el_t, kt))
 
-    nb = 0.5 * (s1["num_bonds"] + s2["num_bonds"])
-    nd = 0.5 * (s1["num_dynamic"] + s2["num_dynamic"])
+    nb = 0.5 * (s1b["num_bonds"] + s2["num_bonds"])
+    nd = 0.5 * (s1b["num_dynamic"] + s2["num_dynamic"])
     b_part = B_PARTICLE_NROT if settling else B_PARTICLE_ROT
-    b_dyn = B_ELASTIC if settling else B_FRICTION
-    per_launch = {"forces": nb * B_BOND + nd * b_dyn,
+    if settling:
+        dyn_bytes = nd * B_ELASTIC
+        ntouch = None
+    else:
+        ntouch = 0.5 * (t_a + t_b)
+        dyn_bytes = ntouch * B_FRICTION + (nd - ntouch) * B_FRICTION_IDLE
+    per_launch = {"forces": nb * B_BOND + dyn_bytes,
                   "gather_integrate": n * b_part}
-    # measured bytes per unit exist for the compression workload only
-    traffic = {"forces": None if settling else (nb + nd) * TRAFFIC_FORCES_PER_INTERACTION,
-               "gather_integrate": None if settling else n * TRAFFIC_GATHER_PER_PARTICLE}
     dom = max((k for k in per_launch if kt[k][1] > 0), key=lambda k: kt[k][0])
     avg_s = kt[dom][0] / kt[dom][1] * 1e-3
     achieved = per_launch[dom] / avg_s / 1e9
-    step_bytes = n * b_part + nb * B_BOND + nd * b_dyn
+    step_bytes = n * b_part + nb * B_BOND + dyn_bytes
     ms_per_step = el / args.steps * 1e3
     out = {
         "metric": "particle-steps/s",
@@ -205,8 +253,8 @@ def main():
         "data": "synthetic",
         "config": ({
             "workload": "gravity settling, %d NRotSphere, radii U[0.2, 0.5], density 1, "
-                        "NRotElastic k=1, elastic floor k=1, g=9.81, dt=1e-4 (domain down "
-                        "to y=-10)" % n,
+                        "NRotElastic k=1, elastic floor k=1, g=9.81, dt=1e-4, box "
+                        "20x40x20" % n,
             "particles": n,
             "elastic_pairs": int(s1["num_dynamic"]),
             "box": [20.0, 40.0, 20.0],
@@ -218,6 +266,7 @@ def main():
             "particles": n,
             "bonds": int(s1["num_bonds"]),
             "friction_pairs": int(s1["num_dynamic"]),
+            "touching_pairs": None if ntouch is None else int(ntouch),
             "prism": [round(b, 3) for b in spec["box"]],
             "steps_per_sync": args.batch,
         }),
@@ -228,8 +277,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic[dom],
-            "traffic_GBs": None if traffic[dom] is None else traffic[dom] / avg_s / 1e9,
+            "traffic": None,
             "algorithmic_bytes_per_launch": per_launch[dom],
             "avg_launch_us": avg_s * 1e6,
         },
@@ -244,9 +292,20 @@ def main():
     }
     if not args.no_cpu_baseline:
         t0 = time.perf_counter()
-        out["cpu_baseline"] = cpu_baseline(pkg, spm, spec, args.cpu_budget,
-                                           args.warmup + args.steps)
+        out["cpu_baseline"] = cpu_baseline(pkg, spm, spec, args.cpu_budget)[0]
         log("cpu baseline: %.1f s" % (time.perf_counter() - t0))
+        import tempfile
+        try:
+            share = len(os.sched_getaffinity(0))
+        except AttributeError:
+            share = os.cpu_count() or 1
+        workers = args.cpu_workers or max(1, min(share, 16))
+        with tempfile.TemporaryDirectory() as td:
+            path = os.path.join(td, "specimen.pkl")
+            save_specimen(path, spec)
+            t0 = time.perf_counter()
+            out["cpu_baseline_all_cores"] = cpu_baseline_all_cores(path, args.cpu_budget, workers)
+        log("cpu baseline, %d processes: %.1f s" % (workers, time.perf_counter() - t0))
     print(json.dumps(out), flush=True)
 
 

@@ -34,6 +34,7 @@ class EspConfig(ctypes.Structure):
         ("do_2d", ctypes.c_int),
         ("device", ctypes.c_int),
         ("steps_per_batch", ctypes.c_int),
+        ("circular_x", ctypes.c_int),
     ]
 
 
@@ -111,6 +112,7 @@ SIGNATURES = {
     "set_particle_vel": (_i, [_vp, _i, _pd]),
     "set_particle_ang_vel": (_i, [_vp, _i, _pd]),
     "move_wall_by": (_i, [_vp, _s, _pd]),
+    "apply_force_to_wall": (_i, [_vp, _s, _pd]),
     "set_wall_step_motion": (_i, [_vp, _s, _pd]),
     "run_time_steps": (_i, [_vp, _i]),
     "get_num_particles": (_i, [_vp]),
@@ -185,7 +187,8 @@ class Engine:
     """One engine handle (one subdomain) over a NativeLib."""
 
     def __init__(self, lib, particle_type, grid_spacing, verlet_dist, domain_min,
-                 domain_max, dt=0.0, do_2d=False, device=0, steps_per_batch=32):
+                 domain_max, dt=0.0, do_2d=False, device=0, steps_per_batch=32,
+                 circular_x=False):
         self.lib = lib
         cfg = EspConfig()
         cfg.particle_type = particle_type
@@ -198,6 +201,7 @@ class Engine:
         cfg.do_2d = int(bool(do_2d))
         cfg.device = int(device)
         cfg.steps_per_batch = int(steps_per_batch)
+        cfg.circular_x = int(bool(circular_x))
         h = ctypes.c_void_p()
         rc = lib.create(ctypes.byref(cfg), ctypes.byref(h))
         if rc != 0:
@@ -351,6 +355,10 @@ class Engine:
     def move_wall_by(self, name, d):
         d = np.asarray(d, np.float64)
         self._check(self.lib.move_wall_by(self.h, _b(name), _dp(d)), "moveWallBy")
+
+    def apply_force_to_wall(self, name, f):
+        f = np.ascontiguousarray(f, np.float64)
+        self._check(self.lib.apply_force_to_wall(self.h, _b(name), _dp(f)), "applyForceToWall")
 
     def set_wall_step_motion(self, name, d):
         d = np.asarray(d, np.float64)

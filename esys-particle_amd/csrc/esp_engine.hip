@@ -180,6 +180,10 @@ struct esp_engine {
   // state a restart checkpoint may carry beyond createParticles (per slot)
   std::vector<double> h_quat, h_ang, h_inert;  // h_inert 0: 0.4 m r^2 (0.5 m r^2 in 2-D)
   std::vector<double> h_init;                  // CParticle::m_initpos
+  // creation rank per slot: the particles' order in the reference's particle
+  // list (insertion order; erasure keeps the order of the rest)
+  std::vector<long long> h_order;
+  long long next_order = 0;
   bool forced_search = false;  // a checkpoint forced the neighbour search after the last step
   std::vector<int> h_conn;
   std::unordered_map<int, int> id2slot;
@@ -395,6 +399,7 @@ int host_cell(esp_engine* e, double x, double y, double z)
   for (double v : f)
     if (!(v >= -2147483648.0 && v < 2147483648.0)) return -1;
   int ix = (int)f[0] - g.gx, iy = (int)f[1] - g.gy, iz = (int)f[2] - g.gz;
+  if (g.circ_x && x >= g.x0 && x < g.x0 + g.period) ix = ix < 0 ? 0 : (ix >= g.nx ? g.nx - 1 : ix);
   if (ix >= 0 && ix < g.nx && iy >= 0 && iy < g.ny && iz >= 0 && iz < g.nz)
     return g.ny * g.nz * ix + g.nz * iy + iz;
   return -1;
@@ -513,6 +518,7 @@ void spatial_order(esp_engine* e)
   permute(e->h_quat, 4);
   permute(e->h_ang, 3);
   permute(e->h_inert, 1);
+  permute(e->h_order, 1);
   e->id2slot.clear();
   for (int i = 0; i < n; i++) e->id2slot[e->h_id[i]] = i;
 }
@@ -709,19 +715,170 @@ void upload(esp_engine* e)
 // neighbour search: ParallelParticleArray::rebuild + rebuildInteractions +
 // resetDisplacements (pp_array.hpp:433-554, SubLattice.hpp:2360-2410)
 // ---------------------------------------------------------------------------
+// NeighborTable::build erases the particles outside the padded grid
+// (ntable.hpp:281-286) and the interaction storages then drop every
+// interaction that lost a particle (PIS_E::rebuild, pi_storage_e.hpp:111-155;
+// PIS_NE / PIS_ED rebuild their lists from the new full list).  On the device:
+// a stable compaction of every per-particle array in slot order, the pairs of
+// every group remapped to the new slots (slots are renumbered monotonically, so
+// key-sorted lists stay sorted and bonds keep their summation ranks), the hash
+// sets rebuilt, the flagged-particle list remapped; the host mirrors follow
+// with one copy of the keep flags.  Called on the host bookkeeping path after
+// fold() (no dead bonds, no appendices), with e->cell holding -1 for the
+// particles to erase.
+template <class T>
+void compact_buf(esp_engine* e, DBuf<T>& b, int n, int nk, int width)
+{
+  if (!b.p) return;
+  DBuf<T> t;
+  t.reserve((size_t)std::max(nk, 1) * width);
+  launch_compact_field(e->stream, n, width, e->keep.p, e->keep_off.p, b.p, t.p);
+  HIPCHK(hipGetLastError());
+  sync(e);
+  b.swap(t);
+}
+
+template <class T>
+void compact_host(std::vector<T>& v, const std::vector<int>& keep, int width)
+{
+  size_t w = 0;
+  for (size_t i = 0; i < keep.size(); i++)
+    if (keep[i])
+      for (int c = 0; c < width; c++) v[w++] = v[(size_t)width * i + c];
+  v.resize(w);
+}
+
+void erase_outside(esp_engine* e)
+{
+  const int n = e->n;
+  e->keep.reserve((size_t)n + 1);
+  e->keep_off.reserve((size_t)n + 1);
+  launch_keep_cells(e->stream, n, e->cell.p, e->keep.p);
+  const int nk = scan_total(e, e->keep.p, e->keep_off.p, n);
+  std::vector<int> hkeep(n), hslot(n);
+  d2h(e, hkeep.data(), e->keep.p, n);
+  DBuf<int> nslot;
+  nslot.reserve((size_t)std::max(n, 1));
+  launch_new_slot(e->stream, n, e->keep.p, e->keep_off.p, nslot.p);
+  d2h(e, hslot.data(), nslot.p, n);
+  sync(e);
+  // interactions first (they read the old slots through nslot)
+  DBuf<int> pkeep, poff;
+  DBuf<int2> rij;
+  for (auto& G : e->bg) {
+    if (G.n <= 0) continue;
+    pkeep.reserve((size_t)G.n + 1);
+    poff.reserve((size_t)G.n + 1);
+    rij.reserve(G.n);
+    launch_remap_pairs(e->stream, G.n, G.ij.p, nslot.p, rij.p, pkeep.p);
+    const int nb2 = scan_total(e, pkeep.p, poff.p, G.n);
+    std::vector<int> bk(G.n);
+    d2h(e, bk.data(), pkeep.p, G.n);
+    const size_t c2 = std::max(nb2, 1);
+    e->tmp_ij.reserve(c2); e->tmp_a.reserve(c2); e->tmp_b.reserve(c2); e->tmp_c.reserve(c2);
+    e->tmp_rank.reserve(c2);
+    launch_compact_bonds(e->stream, G.n, pkeep.p, poff.p, rij.p, G.k4.p, G.mx4.p, G.D4.p, G.rank.p,
+                         e->tmp_ij.p, e->tmp_a.p, e->tmp_b.p, e->tmp_c.p, e->tmp_rank.p);
+    HIPCHK(hipGetLastError());
+    sync(e);
+    G.ij.swap(e->tmp_ij); G.k4.swap(e->tmp_a); G.mx4.swap(e->tmp_b); G.D4.swap(e->tmp_c);
+    G.rank.swap(e->tmp_rank);
+    int w = 0;
+    for (int b = 0; b < G.n; b++)
+      if (bk[b]) { G.hid1[w] = G.hid1[b]; G.hid2[w] = G.hid2[b]; w++; }
+    G.hid1.resize(w);
+    G.hid2.resize(w);
+    G.n = nb2;
+  }
+  for (auto& G : e->dg) {
+    if (G.n <= 0) continue;
+    const bool hist = G.kind == 3;
+    pkeep.reserve((size_t)G.n + 1);
+    poff.reserve((size_t)G.n + 1);
+    rij.reserve(G.n);
+    launch_remap_pairs(e->stream, G.n, G.ij.p, nslot.p, rij.p, pkeep.p);
+    const int nd = scan_total(e, pkeep.p, poff.p, G.n);
+    e->tmp_ij.reserve((size_t)nd + ESP_APP_CAP);
+    e->tmp_key.reserve((size_t)nd + ESP_APP_CAP);
+    if (hist) e->tmp_ff.reserve((size_t)nd + ESP_APP_CAP);
+    launch_pack_dyn(e->stream, G.n, pkeep.p, poff.p, rij.p, hist ? G.ff.p : nullptr, e->tmp_ij.p,
+                    e->tmp_key.p, hist ? e->tmp_ff.p : nullptr);
+    HIPCHK(hipGetLastError());
+    sync(e);
+    G.ij.swap(e->tmp_ij);
+    G.key.swap(e->tmp_key);
+    if (hist) G.ff.swap(e->tmp_ff);
+    G.n = nd;
+  }
+  // the flagged-particle list of the device bookkeeping
+  UpdState st;
+  d2h(e, &st, e->upd.p, 1);
+  sync(e);
+  if (st.fcount > 0) {
+    std::vector<int> fl(st.fcount);
+    d2h(e, fl.data(), e->flist.p, st.fcount);
+    sync(e);
+    int w = 0;
+    for (int v : fl)
+      if (v >= 0 && v < n && hslot[v] >= 0) fl[w++] = hslot[v];
+    h2d(e, e->flist.p, fl.data(), w);
+    h2d(e, &e->upd.p->fcount, &w, 1);
+    sync(e);
+  }
+  // per-particle arrays
+  compact_buf(e, e->pos, n, nk, 1);
+  compact_buf(e, e->vel, n, nk, 1);
+  compact_buf(e, e->ang, n, nk, 1);
+  compact_buf(e, e->quat, n, nk, 1);
+  compact_buf(e, e->old, n, nk, 1);
+  compact_buf(e, e->frc, n, nk, 1);
+  compact_buf(e, e->mom, n, nk, 1);
+  compact_buf(e, e->flags, n, nk, 1);
+  compact_buf(e, e->id, n, nk, 1);
+  compact_buf(e, e->inert, n, nk, 1);
+  compact_buf(e, e->init, n, nk, 1);
+  compact_buf(e, e->tag, n, nk, 1);
+  // host mirrors (per slot)
+  compact_host(e->h_id, hkeep, 1);
+  compact_host(e->h_tag, hkeep, 1);
+  compact_host(e->h_pos, hkeep, 3);
+  compact_host(e->h_vel, hkeep, 3);
+  compact_host(e->h_rad, hkeep, 1);
+  compact_host(e->h_mass, hkeep, 1);
+  compact_host(e->h_flags, hkeep, 1);
+  compact_host(e->h_init, hkeep, 3);
+  compact_host(e->h_quat, hkeep, 4);
+  compact_host(e->h_ang, hkeep, 3);
+  compact_host(e->h_inert, hkeep, 1);
+  compact_host(e->h_order, hkeep, 1);
+  e->id2slot.clear();
+  for (int i = 0; i < nk; i++) e->id2slot[e->h_id[i]] = i;
+  e->n = nk;
+  for (auto& G : e->bg) rebuild_bond_hash(e, G);
+  for (auto& G : e->dg) rebuild_presence(e, G);
+  sync(e);
+}
+
 void rebuild(esp_engine* e, long long step)
 {
   PView p = e->view();
+  // circular x: particles that crossed an x end take their image position
+  // (CParticle::setCircular when the ghost copy takes over at the table rebuild)
+  if (e->grid.circ_x) launch_wrap_x(e->stream, p, e->grid.x0, e->grid.period, e->init.p);
   zero(e, e->cell_count.p, e->ncell + 1);
   zero(e, e->dev_int.p, 1);
   launch_cells_count(e->stream, p, e->grid, e->cell.p, e->cell_count.p, e->dev_int.p);
   int nout = 0;
   d2h(e, &nout, e->dev_int.p, 1);
   sync(e);
-  if (nout > 0)
-    throw EspError(ESP_ERR_STATE,
-                   "particles left the neighbour-table domain; removing particles "
-                   "(NeighborTable::build) is not supported by the device engine");
+  if (nout > 0) {
+    erase_outside(e);
+    p = e->view();
+    zero(e, e->cell_count.p, e->ncell + 1);
+    zero(e, e->dev_int.p, 1);
+    launch_cells_count(e->stream, p, e->grid, e->cell.p, e->cell_count.p, e->dev_int.p);
+    HIPCHK(hipGetLastError());
+  }
   scan_total(e, e->cell_count.p, e->cell_start.p, e->ncell);
   zero(e, e->cell_fill.p, e->ncell);
   launch_cells_fill(e->stream, e->n, e->cell.p, e->cell_start.p, e->cell_fill.p, e->cell_items.p);
@@ -890,7 +1047,7 @@ void dyn_update(esp_engine* e, DynGroup& G, bool& new_list_taken)
     if (hist) {
       e->src.reserve(std::max(nc, 1));
       launch_ed_mark(e->stream, nc, e->cand_p.p, e->cand_q.p, e->cand_key.p, ex, G.key.p, G.n,
-                     G.ij.p, p, e->keep.p, e->src.p);
+                     G.ij.p, p, e->grid.period, e->keep.p, e->src.p);
     } else {
       launch_ne_mark(e->stream, nc, e->cand_p.p, e->cand_q.p, e->id.p, ex, e->keep.p);
     }
@@ -1045,6 +1202,7 @@ ForceLaunch force_launch(esp_engine* e, long long step, bool flip)
   ForceLaunch fl;
   memset(&fl, 0, sizeof fl);
   fl.do2d = e->do2d;
+  fl.period = e->grid.period;
   fl.ss = e->ss;
   fl.dt = e->dt;
   fl.step = (double)step;
@@ -1105,6 +1263,7 @@ ScLaunch sc_launch(esp_engine* e, long long step)
   ScLaunch sl;
   memset(&sl, 0, sizeof sl);
   sl.sig = e->sig.p;
+  sl.period = e->grid.period;
   sl.broken_list = broken_list_of(e, step);
   sl.step = (double)step;
   int blk = 0;
@@ -1533,9 +1692,16 @@ BondGroup& add_bond_group(esp_engine* e, const char* name, const esp_rot_bond_pr
     const int s1 = i1->second, s2 = i2->second;
     double maxes[4] = {prm->max_n, prm->max_s, prm->max_t, prm->max_b};
     if (per_bond_max) memcpy(maxes, per_bond_max + 4 * j, sizeof maxes);
+    // circular x: the initial bond vector runs to the image of particle 2
+    double x2 = e->h_pos[3 * s2];
+    if (e->grid.circ_x) {
+      const double d = e->h_pos[3 * s1] - x2, L = e->grid.period;
+      if (d > 0.5 * L) x2 = x2 + L;
+      else if (d < -0.5 * L) x2 = x2 - L;
+    }
     const BondConst bc = esp_bond_constants(
         mk3(e->h_pos[3 * s1], e->h_pos[3 * s1 + 1], e->h_pos[3 * s1 + 2]), e->h_rad[s1],
-        mk3(e->h_pos[3 * s2], e->h_pos[3 * s2 + 1], e->h_pos[3 * s2 + 2]), e->h_rad[s2], prm->kr,
+        mk3(x2, e->h_pos[3 * s2 + 1], e->h_pos[3 * s2 + 2]), e->h_rad[s2], prm->kr,
         prm->ks, prm->kt, prm->kb, maxes, prm->scaling, prm->meanR_scaling, e->do2d);
     G.hid1.push_back(id1);
     G.hid2.push_back(id2);
@@ -1606,7 +1772,7 @@ void cp_write(esp_engine* e, const std::string& prefix, int num_time_steps, int 
     o << "BoundingBox " << e->cfg.domain_min[0] << " " << e->cfg.domain_min[1] << " "
       << e->cfg.domain_min[2] << " " << e->cfg.domain_max[0] << " " << e->cfg.domain_max[1]
       << " " << e->cfg.domain_max[2] << "\n";
-    o << "PeriodicBoundaries 0 0 0\nDimension " << (e->do2d ? "2D" : "3D") << "\n" << data_name;
+    o << "PeriodicBoundaries " << (e->grid.circ_x ? 1 : 0) << " 0 0\nDimension " << (e->do2d ? "2D" : "3D") << "\n" << data_name;
   }
   std::ofstream o(data_name);
   if (!o) throw EspError(ESP_ERR_ARG, "cannot write " + data_name);
@@ -1776,7 +1942,8 @@ void cp_load(esp_engine* e, const std::string& info_path)
   for (int c = 0; c < 6; c++) in.d();
   in.expect("PeriodicBoundaries");
   for (int c = 0; c < 3; c++)
-    if (in.i() != 0) throw EspError(ESP_ERR_ARG, "periodic boundaries are not supported");
+    if (in.i() != (c == 0 ? e->grid.circ_x : 0))
+      throw EspError(ESP_ERR_ARG, "the checkpoint's periodic dimensions differ from the domain's");
   in.expect("Dimension");
   in.next();
   std::vector<std::string> files(in.t.begin() + in.k, in.t.end());
@@ -1957,13 +2124,19 @@ int esp_create(const esp_config* cfg, esp_engine** out)
   }
   // TSubLattice::initNeighborTable (SubLattice.hpp:455-478) and the 1x1x1
   // ParallelParticleArray table (pp_array.hpp:61-103)
-  const double nr = cfg->grid_spacing;
+  double nr = cfg->grid_spacing;
   double mn[3], mx[3], nmin[3], nmax[3];
   for (int c = 0; c < 3; c++) { mn[c] = cfg->domain_min[c]; mx[c] = cfg->domain_max[c]; }
   double sz[3];
+  if (cfg->circular_x) {
+    // TSubLattice::initNeighborTable(min, max, circ) (SubLattice.hpp:502-515): the
+    // range is adjusted so that whole cells span the circular x size
+    const double xs = mx[0] - mn[0];
+    if (std::fabs(xs / nr - (double)lrint(xs / nr)) > 1e-6) nr = xs / std::floor(xs / nr);
+  }
   for (int c = 0; c < 3; c++) {
     sz[c] = mx[c] - mn[c];
-    sz[c] = nr * std::ceil(sz[c] / nr);
+    if (!(c == 0 && cfg->circular_x)) sz[c] = nr * std::ceil(sz[c] / nr);
   }
   for (int c = 0; c < 3; c++) {
     const double grow = sz[c] - (mx[c] - mn[c]);
@@ -1982,6 +2155,10 @@ int esp_create(const esp_config* cfg, esp_engine** out)
   if (ncells > 2.0e8) {
     g_create_err = "neighbour table too large";
     return ESP_ERR_CAPACITY;
+  }
+  if (cfg->circular_x && ng[0] < 3) {
+    g_create_err = "a circular x dimension needs at least 3 cells";
+    return ESP_ERR_ARG;
   }
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
@@ -2007,6 +2184,16 @@ int esp_create(const esp_config* cfg, esp_engine** out)
   e->grid.dim = nr;
   e->grid.gx = e->grid.gy = e->grid.gz = -1;
   e->grid.nx = (int)ng[0] + 2; e->grid.ny = (int)ng[1] + 2; e->grid.nz = (int)ng[2] + 2;
+  e->grid.circ_x = 0;
+  e->grid.period = 0.0;
+  if (cfg->circular_x) {
+    // one table over the whole circular x range: no padding cells, the
+    // stencil wraps, interactions see the nearest image
+    e->grid.circ_x = 1;
+    e->grid.gx = 0;
+    e->grid.nx = (int)ng[0];
+    e->grid.period = nmax[0] - nmin[0];
+  }
   e->ncell = e->grid.nx * e->grid.ny * e->grid.nz;
   if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
     g_create_err = "hipStreamCreate failed";
@@ -2065,6 +2252,7 @@ int esp_add_particles(esp_engine* e, int n, const int* id, const int* tag, const
     e->h_quat.insert(e->h_quat.end(), {1.0, 0.0, 0.0, 0.0});
     e->h_ang.insert(e->h_ang.end(), {0.0, 0.0, 0.0});
     e->h_inert.push_back(0.0);
+    e->h_order.push_back(e->next_order++);
   }
   return ESP_OK;
   API_END
@@ -2374,6 +2562,148 @@ int esp_move_wall_by(esp_engine* e, const char* name, const double d[3])
   const int w = find_wall(e, name ? name : "");
   if (w < 0) throw EspError(ESP_ERR_ARG, std::string("unknown wall '") + (name ? name : "") + "'");
   for (int c = 0; c < 3; c++) e->walls[w].o[c] = e->walls[w].o[c] + d[c];
+  return ESP_OK;
+  API_END
+}
+
+namespace {
+// applyForceToWall (LsmMpiPy.cpp:1208 -> TSubLattice::applyForceToWall,
+// SubLattice.hpp:2935-2950) for wall interaction group k (name order):
+// CEWallInteractionGroup::applyForce (EWallInteractionGroup.hpp:93-164) or
+// CBWallInteractionGroup::applyForce (BWallInteractionGroup.hpp:71-145).  The
+// candidates come from the device; the iteration runs here over them in the
+// reference's list order (particle creation order), with its sums from zero.
+void apply_force_to_wall(esp_engine* e, int k, const double Fa[3])
+{
+  const WallGroup& G = e->wg[e->word[k]];
+  Wall& W = e->walls[G.wall];
+  struct Cand {
+    long long order;
+    unsigned flags;
+    double4 pos, init;
+  };
+  std::vector<Cand> cs;
+  if (e->started && e->n > 0) {
+    const int n = e->n;
+    DBuf<int> cnt, slot;
+    DBuf<unsigned> flg;
+    DBuf<double4> pos, ipos;
+    cnt.reserve(1);
+    slot.reserve(n);
+    flg.reserve(n);
+    pos.reserve(n);
+    if (G.bonded) ipos.reserve(n);
+    zero(e, cnt.p, 1);
+    launch_wall_cands(e->stream, e->view(), k, G.bonded ? e->init.p : nullptr, cnt.p, slot.p, flg.p,
+                      pos.p, ipos.p);
+    HIPCHK(hipGetLastError());
+    int m = 0;
+    d2h(e, &m, cnt.p, 1);
+    sync(e);
+    std::vector<int> hs(m);
+    std::vector<unsigned> hf(m);
+    std::vector<double4> hp(m), hi(G.bonded ? m : 0);
+    d2h(e, hs.data(), slot.p, m);
+    d2h(e, hf.data(), flg.p, m);
+    d2h(e, hp.data(), pos.p, m);
+    if (G.bonded) d2h(e, hi.data(), ipos.p, m);
+    sync(e);
+    cs.resize(m);
+    for (int c = 0; c < m; c++)
+      cs[c] = Cand{e->h_order[hs[c]], hf[c], hp[c], G.bonded ? hi[c] : make_double4(0, 0, 0, 0)};
+    std::sort(cs.begin(), cs.end(), [](const Cand& a, const Cand& b) { return a.order < b.order; });
+  }
+  auto inner = [&](const Cand& c) { return ((c.flags >> (PF_INNER_SHIFT + k)) & 1u) != 0; };
+  auto tagged = [&](const Cand& c) {
+    return (G.bonded || G.tagged) && ((c.flags >> (PF_BTAG_SHIFT + k)) & 1u) != 0;
+  };
+  const dv3 n = mk3(W.n[0], W.n[1], W.n[2]);
+  auto dist = [&](const Cand& c) {
+    return dot3(sub3(mk3(c.pos.x, c.pos.y, c.pos.z), mk3(W.o[0], W.o[1], W.o[2])), n);
+  };
+  // CElasticWallInteraction: k pi r (3-D, EWallInteraction.hpp:26-36); the
+  // bonded form k pi r with its own constant (BWallInteraction.hpp:20-31)
+  auto keff = [&](const Cand& c) { return (e->do2d ? 1.0 : 3.141592654 * c.pos.w) * G.k; };
+  auto kbond = [&](const Cand& c) { return G.k * (e->do2d ? 1.0 : 3.1415926536 * c.pos.w); };
+  auto elastic_force = [&](const Cand& c) {  // CElasticWallInteraction::getForce
+    dv3 f = mk3(0.0, 0.0, 0.0);
+    const double d = dist(c);
+    if (d < c.pos.w) f = mul3(n, keff(c) * (c.pos.w - d));
+    return mul3(f, -1.0);
+  };
+  const dv3 F = mk3(Fa[0], Fa[1], Fa[2]);
+  const dv3 O_f = div3(F, norm3(F));
+  auto move = [&](double d) {
+    const dv3 v = mul3(O_f, d);
+    W.o[0] = W.o[0] + v.x;
+    W.o[1] = W.o[1] + v.y;
+    W.o[2] = W.o[2] + v.z;
+  };
+  if (!G.bonded) {
+    int it = 0;
+    double d, df, ef;
+    do {
+      double kk = 0.0;
+      for (const Cand& c : cs)
+        if ((!G.tagged || tagged(c)) && dist(c) < c.pos.w) kk += keff(c);
+      if (kk > 0) {
+        dv3 Fl = mk3(0.0, 0.0, 0.0);
+        for (const Cand& c : cs) {
+          if ((G.tagged && !tagged(c)) || !inner(c)) continue;
+          const dv3 f = elastic_force(c);
+          Fl = add3(Fl, mul3(O_f, dot3(f, O_f)));
+        }
+        df = dot3(add3(F, Fl), O_f);
+        d = df / kk;
+        ef = df / norm3(F);
+      } else {
+        d = 1e-5;
+        ef = 1;
+      }
+      move(d);
+      it++;
+    } while ((it < 50) && (ef > 1e-3));
+    return;
+  }
+  double K = 0.0;
+  for (const Cand& c : cs)
+    if (tagged(c) && inner(c)) K += kbond(c);
+  for (const Cand& c : cs)
+    if (!tagged(c) && inner(c) && dist(c) < c.pos.w) K += keff(c);
+  if (!(K > 0)) throw EspError(ESP_ERR_NUMERIC, "wall group '" + G.name + "' has no stiffness to servo");
+  int it = 0;
+  double d;
+  do {
+    const dv3 wdi = mk3(W.o[0] - W.old[0], W.o[1] - W.old[1], W.o[2] - W.old[2]);
+    dv3 Fl = mk3(0.0, 0.0, 0.0);
+    for (const Cand& c : cs) {
+      if (!tagged(c) || !inner(c)) continue;
+      const dv3 D = sub3(sub3(mk3(c.pos.x, c.pos.y, c.pos.z), mk3(c.init.x, c.init.y, c.init.z)), wdi);
+      const dv3 f = mul3(D, kbond(c));
+      Fl = add3(Fl, mul3(O_f, dot3(f, O_f)));
+    }
+    for (const Cand& c : cs) {
+      if (tagged(c) || !inner(c)) continue;
+      const dv3 f = elastic_force(c);
+      Fl = add3(Fl, mul3(O_f, dot3(f, O_f)));
+    }
+    d = dot3(add3(F, Fl), O_f) / K;
+    move(d);
+    it++;
+  } while ((it < 10) && (std::fabs(d) > 10e-6));
+}
+}  // namespace
+
+int esp_apply_force_to_wall(esp_engine* e, const char* name, const double F[3])
+{
+  API_BEGIN
+  if (!name || !F) throw EspError(ESP_ERR_ARG, "bad arguments");
+  refresh_orders(e);
+  int k = -1;
+  for (size_t i = 0; i < e->word.size(); i++)
+    if (e->wg[e->word[i]].name == name) k = (int)i;
+  if (k < 0) throw EspError(ESP_ERR_ARG, std::string("unknown wall interaction group '") + name + "'");
+  apply_force_to_wall(e, k, F);
   return ESP_OK;
   API_END
 }

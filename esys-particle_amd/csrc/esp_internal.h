@@ -66,6 +66,10 @@ struct GridParams {
   double x0, y0, z0, dim;
   int gx, gy, gz;  // global index of the table's minimum corner (-1)
   int nx, ny, nz;
+  // circular x: no padding cells in x (gx = 0), the stencil wraps and the
+  // period is L = nx * dim (0: not circular)
+  int circ_x;
+  double period;
 };
 
 struct PView {
@@ -210,6 +214,7 @@ struct ForceSeg {
 };
 struct ForceLaunch {
   int count, nblocks, do2d;
+  double period;  // circular x: particle 2 is imaged next to particle 1 (0: not circular)
   int ss;  // doubles per slot (6, or 9 with the contact point)
   int ublk;  // 1: block 0 runs the incremental update (queued steps), segments follow
   double dt;
@@ -250,6 +255,7 @@ bool launch_forces(hipStream_t s, StepCtl ctl, PView p, const ForceLaunch& fl,
 // segment of blocks per group
 struct ScLaunch {
   int count, nblocks;
+  double period;  // circular x period (0: not circular)
   int blk0[ESP_MAX_BOND_GROUPS], n[ESP_MAX_BOND_GROUPS], bond_off[ESP_MAX_BOND_GROUPS];
   const int2* ij[ESP_MAX_BOND_GROUPS];
   double4* D4[ESP_MAX_BOND_GROUPS];
@@ -288,9 +294,33 @@ void launch_pairs(hipStream_t s, int mode_emit, int flagged, PView p, GridParams
                   const int* cell, const int* cell_start, const int* cell_items, int* cnt,
                   const int* off, int* out_p, int* out_q, long long* out_key);
 void launch_clear_flag_bit(hipStream_t s, PView p, UpdState* st);
+// candidates of wall group w (any order): slots, flags, positions, initial positions
+void launch_wall_cands(hipStream_t s, PView p, int w, const double4* init, int* cnt, int* slot,
+                       unsigned* flg, double4* pos, double4* ipos);
+// erasure of particles outside the padded grid (NeighborTable::build):
+// keep[i] = cell[i] >= 0; nslot[i] = new slot or -1; stable compaction of a
+// per-particle field of `width` elements; interaction pairs remapped to the new
+// slots (keep: both particles survive); a dynamic group's kept pairs with keys
+void launch_keep_cells(hipStream_t s, int n, const int* cell, int* keep);
+void launch_new_slot(hipStream_t s, int n, const int* keep, const int* off, int* nslot);
+void launch_compact_field(hipStream_t s, int n, int width, const int* keep, const int* off,
+                          const double4* src, double4* dst);
+void launch_compact_field(hipStream_t s, int n, int width, const int* keep, const int* off,
+                          const double* src, double* dst);
+void launch_compact_field(hipStream_t s, int n, int width, const int* keep, const int* off,
+                          const int* src, int* dst);
+void launch_compact_field(hipStream_t s, int n, int width, const int* keep, const int* off,
+                          const unsigned* src, unsigned* dst);
+void launch_remap_pairs(hipStream_t s, int nc, const int2* ij, const int* nslot, int2* out,
+                        int* keep);
+void launch_pack_dyn(hipStream_t s, int nc, const int* keep, const int* off, const int2* ij,
+                     const double4* ff, int2* oij, long long* okey, double4* off_);
 void launch_ed_mark(hipStream_t s, int nc, const int* cp, const int* cq, const long long* ckey,
                     ExclSet ex, const long long* okey, int nold, const int2* oij, PView p,
-                    int* keep, int* src);
+                    double period, int* keep, int* src);
+// circular x: particles beyond an x end move to their image (position, old
+// position and, when kept, initial position)
+void launch_wrap_x(hipStream_t s, PView p, double x0, double period, double4* init);
 void launch_ne_mark(hipStream_t s, int nc, const int* cp, const int* cq, const int* id, ExclSet ex,
                     int* keep);
 void launch_pair_emit(hipStream_t s, int nc, const int* keep, const int* off, const int* cp,

@@ -52,6 +52,20 @@ __device__ __forceinline__ bool step_skip(const StepCtl& c)
   return false;
 }
 
+// The image of particle 2 nearest to particle 1 in a circular x dimension:
+// shifted by the period when the two are more than half a period apart, one
+// addition, as a ghost copy carries its shifted position (CParticle::setCircular,
+// Particle.cpp:336-342); period 0: not circular.
+__device__ __forceinline__ double4 x_image(double4 P1, double4 P2, double period)
+{
+  if (period > 0.0) {
+    const double d = P1.x - P2.x;
+    if (d > 0.5 * period) P2.x = P2.x + period;
+    else if (d < -0.5 * period) P2.x = P2.x - period;
+  }
+  return P2;
+}
+
 __device__ __forceinline__ long long pair_key(int a, int b)
 {
   return (long long)(((unsigned long long)(unsigned)a << 32) | (unsigned long long)(unsigned)b);
@@ -158,7 +172,7 @@ __device__ __forceinline__ void rot_bond_item(const StepCtl& ctl, const PView& p
   const int2 e = ij[b];
   const int2 so = fl.slot_of[g.rec_off + b];
   if (e.x < 0) return;  // broken earlier: removed from the group, slots left at zero
-  const double4 P1 = p.pos[e.x], P2 = p.pos[e.y];
+  const double4 P1 = p.pos[e.x], P2 = x_image(P1, p.pos[e.y], fl.period);
   const double4 Q1 = p.quat[e.x], Q2 = p.quat[e.y];
   const double4 K = g.k4[b], D = g.D4[b], M = g.mx4[b];
   // broken in an earlier step (D.w holds that step): the incremental update
@@ -200,7 +214,7 @@ __device__ __forceinline__ void rot_friction_item(const PView& p, int c, const F
   const int nc = g.n;
   const int2 e = g.ij[c];
   const int rec = record_of(c, nc, g.rec_off, g.app_rec_off);
-  const double4 P1 = p.pos[e.x], P2 = p.pos[e.y];
+  const double4 P1 = p.pos[e.x], P2 = x_image(P1, p.pos[e.y], fl.period);
   const double4 st = g.ff[c];
   const dv3 Dq = sub3(xyz(P1), xyz(P2));
   const double eq = P1.w + P2.w;
@@ -252,7 +266,7 @@ __device__ __forceinline__ void nrot_elastic_item(const PView& p, int c, const F
   const int nc = g.n;
   const int2 e = g.ij[c];
   const int2 so = fl.slot_of[record_of(c, nc, g.rec_off, g.app_rec_off)];
-  const double4 P1 = p.pos[e.x], P2 = p.pos[e.y];
+  const double4 P1 = p.pos[e.x], P2 = x_image(P1, p.pos[e.y], fl.period);
   double* s0 = fl.slots + (size_t)fl.ss * so.x;
   double* s1 = fl.slots + (size_t)fl.ss * so.y;
   const dv3 D = sub3(xyz(P1), xyz(P2));
@@ -282,7 +296,7 @@ __device__ __forceinline__ void rot_elastic_item(const PView& p, int c, const Fo
   const int nc = g.n;
   const int2 e = g.ij[c];
   const int2 so = fl.slot_of[record_of(c, nc, g.rec_off, g.app_rec_off)];
-  const double4 P1 = p.pos[e.x], P2 = p.pos[e.y];
+  const double4 P1 = p.pos[e.x], P2 = x_image(P1, p.pos[e.y], fl.period);
   double* s0 = fl.slots + (size_t)fl.ss * so.x;
   double* s1 = fl.slots + (size_t)fl.ss * so.y;
   const dv3 rb = sub3(xyz(P1), xyz(P2));
@@ -319,6 +333,9 @@ __device__ __forceinline__ bool cell_coords(const GridParams& g, double x, doubl
   ix = (int)fx - g.gx;
   iy = (int)fy - g.gy;
   iz = (int)fz - g.gz;
+  // circular x: positions are wrapped into [x0, x0 + L) at each rebuild; a
+  // rounding of (x - x0)/dim onto the end of the range stays in the last cell
+  if (g.circ_x && x >= g.x0 && x < g.x0 + g.period) ix = ix < 0 ? 0 : (ix >= g.nx ? g.nx - 1 : ix);
   return true;
 }
 
@@ -343,10 +360,10 @@ __device__ __forceinline__ unsigned wall_membership(const GridParams& g, int nwg
   int ix = 0, iy = 0, iz = 0;
   bool inner = false;
   if (cell_coords(g, P.x, P.y, P.z, ix, iy, iz)) {
-    if (g.nz < 3) inner = (ix > 0) && (ix < g.nx - 1) && (iy > 0) && (iy < g.ny - 1);
-    else
-      inner = (ix > 0) && (ix < g.nx - 1) && (iy > 0) && (iy < g.ny - 1) && (iz > 0) &&
-              (iz < g.nz - 1);
+    // circular x: no padding cells in x, every x is inner
+    const bool xin = g.circ_x || ((ix > 0) && (ix < g.nx - 1));
+    if (g.nz < 3) inner = xin && (iy > 0) && (iy < g.ny - 1);
+    else inner = xin && (iy > 0) && (iy < g.ny - 1) && (iz > 0) && (iz < g.nz - 1);
   }
   for (int w = 0; w < nwg; w++) {
     const dv3 n = mk3(W[w].nrm[0], W[w].nrm[1], W[w].nrm[2]);
@@ -810,12 +827,15 @@ __global__ void k_pairs(int emit, int flagged, PView p, GridParams g, double alp
   const int idp = p.id[pi];
   const int base = emit ? off[k] : 0;
   int count = 0;
-  const int x0 = cx > 0 ? cx - 1 : cx, x1 = cx < g.nx - 1 ? cx + 1 : cx;
+  // circular x: the stencil wraps (nx >= 3, so its cells are distinct)
+  const int x0 = g.circ_x ? cx - 1 : (cx > 0 ? cx - 1 : cx);
+  const int x1 = g.circ_x ? cx + 1 : (cx < g.nx - 1 ? cx + 1 : cx);
   const int y0 = cy > 0 ? cy - 1 : cy, y1 = cy < g.ny - 1 ? cy + 1 : cy;
   const int z0 = cz > 0 ? cz - 1 : cz, z1 = cz < g.nz - 1 ? cz + 1 : cz;
-  for (int x = x0; x <= x1; x++)
+  for (int xx = x0; xx <= x1; xx++)
     for (int y = y0; y <= y1; y++)
       for (int z = z0; z <= z1; z++) {
+        const int x = g.circ_x ? (xx + g.nx) % g.nx : xx;
         const int c2 = g.ny * g.nz * x + g.nz * y + z;
         const int t1 = cell_start[c2 + 1];
         for (int t = cell_start[c2]; t < t1; t++) {
@@ -825,7 +845,7 @@ __global__ void k_pairs(int emit, int flagged, PView p, GridParams g, double alp
           if (flagged) {
             if (!(p.flags[q] & PF_FLAG)) continue;
           } else {
-            const double4 Q = p.pos[q];
+            const double4 Q = x_image(P, p.pos[q], g.period);
             const dv3 d = sub3(xyz(P), xyz(Q));
             const double dist2 = dot3(d, d);
             const double dmax = P.w + Q.w + alpha;
@@ -855,6 +875,74 @@ __global__ void k_pairs(int emit, int flagged, PView p, GridParams g, double alp
     out_key[j + 1] = kv;
     out_q[j + 1] = qv;
   }
+}
+
+// ---------------------------------------------------------------------------
+// erasure of particles outside the padded grid (NeighborTable::build,
+// ntable.hpp:281-286): stable compaction in slot order
+// ---------------------------------------------------------------------------
+__global__ void k_keep_cells(int n, const int* cell, int* keep)
+{
+  const int i = blockIdx.x * BS + threadIdx.x;
+  if (i < n) keep[i] = cell[i] >= 0 ? 1 : 0;
+}
+
+// new slot of every particle, -1 for an erased one
+__global__ void k_new_slot(int n, const int* keep, const int* off, int* nslot)
+{
+  const int i = blockIdx.x * BS + threadIdx.x;
+  if (i < n) nslot[i] = keep[i] ? off[i] : -1;
+}
+
+template <class T>
+__global__ void k_compact_field(int n, int width, const int* keep, const int* off, const T* src,
+                                T* dst)
+{
+  const int i = blockIdx.x * BS + threadIdx.x;
+  if (i >= n || !keep[i]) return;
+  for (int c = 0; c < width; c++) dst[(size_t)width * off[i] + c] = src[(size_t)width * i + c];
+}
+
+// an interaction's particles at their new slots; kept while both survive
+// (PIS_E::rebuild, pi_storage_e.hpp:111-155)
+__global__ void k_remap_pairs(int nc, const int2* ij, const int* nslot, int2* out, int* keep)
+{
+  const int c = blockIdx.x * BS + threadIdx.x;
+  if (c >= nc) return;
+  const int2 e = ij[c];
+  const int a = e.x >= 0 ? nslot[e.x] : -1, b = e.y >= 0 ? nslot[e.y] : -1;
+  out[c] = make_int2(a, b);
+  keep[c] = (a >= 0 && b >= 0) ? 1 : 0;
+}
+
+// the kept pairs of a dynamic group with their slot keys (order kept: slots
+// are renumbered monotonically, so the list stays sorted by key)
+__global__ void k_pack_dyn(int nc, const int* keep, const int* off, const int2* ij,
+                           const double4* ff, int2* oij, long long* okey, double4* off_)
+{
+  const int c = blockIdx.x * BS + threadIdx.x;
+  if (c >= nc || !keep[c]) return;
+  const int w = off[c];
+  const int2 e = ij[c];
+  oij[w] = e;
+  okey[w] = pair_key(e.x, e.y);
+  if (off_) off_[w] = ff[c];
+}
+
+// the candidate particles of wall group w (flag bit PF_WALL_SHIFT + w): slot,
+// flags, position and (bonded walls) initial position, in any order
+__global__ void k_wall_cands(PView p, int w, const double4* init, int* cnt, int* slot,
+                             unsigned* flg, double4* pos, double4* ipos)
+{
+  const int i = blockIdx.x * BS + threadIdx.x;
+  if (i >= p.n) return;
+  const unsigned f = p.flags[i];
+  if (!((f >> (PF_WALL_SHIFT + w)) & 1u)) return;
+  const int k = atomicAdd(cnt, 1);
+  slot[k] = i;
+  flg[k] = f;
+  pos[k] = p.pos[i];
+  if (init) ipos[k] = init[i];
 }
 
 __global__ void k_clear_flag_bit(PView p, UpdState* st)
@@ -902,8 +990,8 @@ __device__ __forceinline__ int upper_bound_key(const long long* a, int n, long l
 // kept unless excluded; it inherits the history of the stored interaction with
 // the same key if that one is persistent (RotFricInteraction.cpp:345-351).
 __global__ void k_ed_mark(int nc, const int* cp, const int* cq, const long long* ckey, ExclSet ex,
-                          const long long* okey, int nold, const int2* oij, PView p, int* keep,
-                          int* src)
+                          const long long* okey, int nold, const int2* oij, PView p, double period,
+                          int* keep, int* src)
 {
   const int c = blockIdx.x * BS + threadIdx.x;
   if (c >= nc) return;
@@ -918,7 +1006,7 @@ __global__ void k_ed_mark(int nc, const int* cp, const int* cq, const long long*
   const int j = bsearch_key(okey, nold, key);
   if (j >= 0) {
     const int2 e = oij[j];
-    const double4 P1 = p.pos[e.x], P2 = p.pos[e.y];
+    const double4 P1 = p.pos[e.x], P2 = x_image(P1, p.pos[e.y], period);
     const dv3 D = sub3(xyz(P1), xyz(P2));
     const double dist = dot3(D, D);
     const double eq = P1.w + P2.w;
@@ -1142,7 +1230,9 @@ __device__ void incremental_update(const StepCtl& ctl, const PView& p, const Upd
     const int idp = p.id[pi];
     if (lane >= 54) continue;
     const int nc = lane % 27, half = lane / 27;
-    const int x = cx + nc / 9 - 1, y = cy + (nc / 3) % 3 - 1, z = cz + nc % 3 - 1;
+    int x = cx + nc / 9 - 1;
+    const int y = cy + (nc / 3) % 3 - 1, z = cz + nc % 3 - 1;
+    if (g.circ_x) x = (x + g.nx) % g.nx;  // the stencil wraps in a circular x
     if (x < 0 || x >= g.nx || y < 0 || y >= g.ny || z < 0 || z >= g.nz) continue;
     const int c2 = g.ny * g.nz * x + g.nz * y + z;
     const int t0 = up.cell_start[c2], t1 = up.cell_start[c2 + 1];
@@ -1587,7 +1677,7 @@ __global__ __launch_bounds__(BS) void k_sc_break(StepCtl ctl, PView p, ScLaunch 
   if (b >= sl.n[g]) return;
   const int2 e = sl.ij[g][b];
   if (e.x < 0) return;
-  const double4 P1 = p.pos[e.x], P2 = p.pos[e.y];
+  const double4 P1 = p.pos[e.x], P2 = x_image(P1, p.pos[e.y], sl.period);
   if (esp_sc_broken(xyz(P1), P1.w, sl.sig + 9 * (size_t)e.x, xyz(P2), P2.w,
                     sl.sig + 9 * (size_t)e.y, sl.sigma3[g], sl.cohesion[g], sl.tan_angle[g])) {
     sl.D4[g][b].w = sl.step;
@@ -1647,6 +1737,70 @@ void launch_pairs(hipStream_t s, int mode_emit, int flagged, PView p, GridParams
                      cell, cell_start, cell_items, cnt, off, out_p, out_q, out_key);
 }
 
+void launch_keep_cells(hipStream_t s, int n, const int* cell, int* keep)
+{
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_keep_cells, dim3(nblk(n)), dim3(BS), 0, s, n, cell, keep);
+}
+
+void launch_new_slot(hipStream_t s, int n, const int* keep, const int* off, int* nslot)
+{
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_new_slot, dim3(nblk(n)), dim3(BS), 0, s, n, keep, off, nslot);
+}
+
+template <class T>
+static void compact_field(hipStream_t s, int n, int width, const int* keep, const int* off,
+                          const T* src, T* dst)
+{
+  if (n <= 0) return;
+  hipLaunchKernelGGL((k_compact_field<T>), dim3(nblk(n)), dim3(BS), 0, s, n, width, keep, off, src,
+                     dst);
+}
+void launch_compact_field(hipStream_t s, int n, int width, const int* keep, const int* off,
+                          const double4* src, double4* dst)
+{
+  compact_field(s, n, width, keep, off, src, dst);
+}
+void launch_compact_field(hipStream_t s, int n, int width, const int* keep, const int* off,
+                          const double* src, double* dst)
+{
+  compact_field(s, n, width, keep, off, src, dst);
+}
+void launch_compact_field(hipStream_t s, int n, int width, const int* keep, const int* off,
+                          const int* src, int* dst)
+{
+  compact_field(s, n, width, keep, off, src, dst);
+}
+void launch_compact_field(hipStream_t s, int n, int width, const int* keep, const int* off,
+                          const unsigned* src, unsigned* dst)
+{
+  compact_field(s, n, width, keep, off, src, dst);
+}
+
+void launch_remap_pairs(hipStream_t s, int nc, const int2* ij, const int* nslot, int2* out,
+                        int* keep)
+{
+  if (nc <= 0) return;
+  hipLaunchKernelGGL(k_remap_pairs, dim3(nblk(nc)), dim3(BS), 0, s, nc, ij, nslot, out, keep);
+}
+
+void launch_pack_dyn(hipStream_t s, int nc, const int* keep, const int* off, const int2* ij,
+                     const double4* ff, int2* oij, long long* okey, double4* off_)
+{
+  if (nc <= 0) return;
+  hipLaunchKernelGGL(k_pack_dyn, dim3(nblk(nc)), dim3(BS), 0, s, nc, keep, off, ij, ff, oij, okey,
+                     off_);
+}
+
+void launch_wall_cands(hipStream_t s, PView p, int w, const double4* init, int* cnt, int* slot,
+                       unsigned* flg, double4* pos, double4* ipos)
+{
+  if (p.n <= 0) return;
+  hipLaunchKernelGGL(k_wall_cands, dim3(nblk(p.n)), dim3(BS), 0, s, p, w, init, cnt, slot, flg,
+                     pos, ipos);
+}
+
 void launch_clear_flag_bit(hipStream_t s, PView p, UpdState* st)
 {
   hipLaunchKernelGGL(k_clear_flag_bit, dim3(nblk(p.n > 0 ? p.n : 1)), dim3(BS), 0, s, p, st);
@@ -1654,11 +1808,38 @@ void launch_clear_flag_bit(hipStream_t s, PView p, UpdState* st)
 
 void launch_ed_mark(hipStream_t s, int nc, const int* cp, const int* cq, const long long* ckey,
                     ExclSet ex, const long long* okey, int nold, const int2* oij, PView p,
-                    int* keep, int* src)
+                    double period, int* keep, int* src)
 {
   if (nc <= 0) return;
   hipLaunchKernelGGL(k_ed_mark, dim3(nblk(nc)), dim3(BS), 0, s, nc, cp, cq, ckey, ex, okey, nold,
-                     oij, p, keep, src);
+                     oij, p, period, keep, src);
+}
+
+__global__ void k_wrap_x(PView p, double x0, double period, double4* init)
+{
+  const int i = blockIdx.x * BS + threadIdx.x;
+  if (i >= p.n) return;
+  double4 P = p.pos[i];
+  double sh = 0.0;
+  if (P.x < x0) sh = period;
+  else if (P.x >= x0 + period) sh = -period;
+  if (sh == 0.0) return;
+  P.x = P.x + sh;
+  p.pos[i] = P;
+  double4 O = p.old[i];
+  O.x = O.x + sh;
+  p.old[i] = O;
+  if (init) {
+    double4 I = init[i];
+    I.x = I.x + sh;
+    init[i] = I;
+  }
+}
+
+void launch_wrap_x(hipStream_t s, PView p, double x0, double period, double4* init)
+{
+  if (p.n <= 0) return;
+  hipLaunchKernelGGL(k_wrap_x, dim3(nblk(p.n)), dim3(BS), 0, s, p, x0, period, init);
 }
 
 void launch_ne_mark(hipStream_t s, int nc, const int* cp, const int* cq, const int* id, ExclSet ex,

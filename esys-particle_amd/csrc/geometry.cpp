@@ -210,4 +210,76 @@ double esp_geom_snap(int n, double* pos, const double* rad, int nb, const int* b
   return maxres;
 }
 
+// Random sequential insertion: sphere k (radii in the given order) is placed
+// at a uniformly random centre in [r, box - r] (x in [0, box_x) when
+// periodic_x, distances then taken to the nearest x image) where it overlaps no
+// earlier sphere; up to max_tries attempts each.  A hash grid of cell 2 r_max
+// holds the placed spheres.  Returns the number placed (pos holds them).
+int esp_geom_rsa(int n, const double* rad, const double* box, int periodic_x, unsigned long long seed,
+                 long long max_tries, double* pos)
+{
+  SplitMix rng(seed);
+  double rmax = 0.0;
+  for (int i = 0; i < n; i++) rmax = std::max(rmax, rad[i]);
+  const double cs = 2.0 * rmax;
+  const int nx = std::max(1, (int)(box[0] / cs)), ny = std::max(1, (int)(box[1] / cs)),
+            nz = std::max(1, (int)(box[2] / cs));
+  const double cx = box[0] / nx, cy = box[1] / ny, cz = box[2] / nz;
+  std::vector<std::vector<int>> cell((size_t)nx * ny * nz);
+  auto cidx = [&](double x, double y, double z, int& ix, int& iy, int& iz) {
+    ix = std::min(nx - 1, std::max(0, (int)(x / cx)));
+    iy = std::min(ny - 1, std::max(0, (int)(y / cy)));
+    iz = std::min(nz - 1, std::max(0, (int)(z / cz)));
+  };
+  int placed = 0;
+  for (int k = 0; k < n; k++) {
+    const double r = rad[k];
+    bool ok = false;
+    for (long long t = 0; t < max_tries && !ok; t++) {
+      const double x = periodic_x ? rng.uniform() * box[0] : r + rng.uniform() * (box[0] - 2 * r);
+      const double y = r + rng.uniform() * (box[1] - 2 * r);
+      const double z = r + rng.uniform() * (box[2] - 2 * r);
+      int ix, iy, iz;
+      cidx(x, y, z, ix, iy, iz);
+      ok = true;
+      for (int dx = -1; dx <= 1 && ok; dx++) {
+        int jx = ix + dx;
+        if (periodic_x) jx = (jx + nx) % nx;
+        else if (jx < 0 || jx >= nx) continue;
+        if (periodic_x && nx < 3 && dx != 0 && (ix + dx + nx) % nx == ix) continue;
+        for (int dy = -1; dy <= 1 && ok; dy++) {
+          const int jy = iy + dy;
+          if (jy < 0 || jy >= ny) continue;
+          for (int dz = -1; dz <= 1 && ok; dz++) {
+            const int jz = iz + dz;
+            if (jz < 0 || jz >= nz) continue;
+            for (int j : cell[((size_t)jx * ny + jy) * nz + jz]) {
+              double ddx = x - pos[3 * j];
+              if (periodic_x) {
+                if (ddx > 0.5 * box[0]) ddx -= box[0];
+                else if (ddx < -0.5 * box[0]) ddx += box[0];
+              }
+              const double ddy = y - pos[3 * j + 1], ddz = z - pos[3 * j + 2];
+              const double rr = r + rad[j];
+              if (ddx * ddx + ddy * ddy + ddz * ddz < rr * rr) {
+                ok = false;
+                break;
+              }
+            }
+          }
+        }
+      }
+      if (ok) {
+        pos[3 * k] = x;
+        pos[3 * k + 1] = y;
+        pos[3 * k + 2] = z;
+        cell[((size_t)ix * ny + iy) * nz + iz].push_back(k);
+        placed = k + 1;
+      }
+    }
+    if (!ok) break;
+  }
+  return placed;
+}
+
 }  // extern "C"

@@ -18,6 +18,9 @@ def _load():
         pd = ctypes.POINTER(ctypes.c_double)
         _lib.esp_geom_pack.restype = ctypes.c_double
         _lib.esp_geom_pack.argtypes = [ctypes.c_int, pd, pd, ctypes.c_ulonglong, ctypes.c_int, pd]
+        _lib.esp_geom_rsa.restype = ctypes.c_int
+        _lib.esp_geom_rsa.argtypes = [ctypes.c_int, pd, pd, ctypes.c_int, ctypes.c_ulonglong,
+                                      ctypes.c_longlong, pd]
         _lib.esp_geom_snap.restype = ctypes.c_double
         _lib.esp_geom_snap.argtypes = [ctypes.c_int, pd, pd, ctypes.c_int,
                                        ctypes.POINTER(ctypes.c_int), pd, ctypes.c_int,
@@ -36,6 +39,23 @@ def dense_packing(rad, box, seed, max_sweeps=6000):
     ov = lib.esp_geom_pack(len(rad), rad.ctypes.data_as(pd), box.ctypes.data_as(pd),
                            ctypes.c_ulonglong(seed), int(max_sweeps), pos.ctypes.data_as(pd))
     return pos, ov
+
+
+def rsa_packing(rad, box, seed, periodic_x=False, max_tries=100000):
+    """Random sequential insertion of spheres with radii ``rad`` (in that order)
+    into [0, box] (x periodic when ``periodic_x``): positions, or RuntimeError
+    when a sphere finds no free place in ``max_tries`` attempts."""
+    lib = _load()
+    rad = np.ascontiguousarray(rad, np.float64)
+    box = np.ascontiguousarray(box, np.float64)
+    pos = np.zeros((len(rad), 3))
+    pd = ctypes.POINTER(ctypes.c_double)
+    k = lib.esp_geom_rsa(len(rad), rad.ctypes.data_as(pd), box.ctypes.data_as(pd),
+                         int(bool(periodic_x)), ctypes.c_ulonglong(seed), int(max_tries),
+                         pos.ctypes.data_as(pd))
+    if k != len(rad):
+        raise RuntimeError("random packing too dense: placed %d of %d" % (k, len(rad)))
+    return pos
 
 
 def snap_bonds(pos, rad, bonds, box, max_sweeps=20000, tol=1e-10):

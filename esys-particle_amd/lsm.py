@@ -466,16 +466,27 @@ class LsmMpi:
             raise RuntimeError("force2dComputations must precede setSpatialDomain")
         self._do2d = bool(do2d)
 
-    def setSpatialDomain(self, bbox, maxPt=None):
+    def setSpatialDomain(self, bbox, maxPt=None, circDimList=None):
+        """setSpatialDomain(bbox) / setSpatialDomain(bbox, circDimList)
+        (LsmMpiPy.cpp:1085-1100): circDimList = [x, y, z] circular flags;
+        x only (the reference's ghost exchange is x-only, pp_array.hpp:433-536)."""
+        if isinstance(maxPt, (list, tuple)) and circDimList is None:
+            maxPt, circDimList = None, maxPt
         if maxPt is not None:
             bbox = BoundingBox(bbox, maxPt)
         if self._ptype is None:
             raise RuntimeError("initNeighbourSearch must be called before setSpatialDomain")
         if self._engine is not None:
             raise RuntimeError("spatial domain already set")
+        circ = [bool(c) for c in (circDimList or (False, False, False))]
+        if circ[1] or circ[2]:
+            raise NotImplementedError("circular y or z boundaries are not supported "
+                                      "(x only, as the reference's exchange)")
+        self._circ = circ
         self._engine = _native.Engine(self._open_native(), self._ptype, self._grid, self._verlet,
                                       list(bbox.getMinPt()), list(bbox.getMaxPt()), self._dt,
-                                      self._do2d, self._device, self._steps_per_batch)
+                                      self._do2d, self._device, self._steps_per_batch,
+                                      circular_x=circ[0])
 
     def _e(self):
         if self._engine is None:
@@ -531,10 +542,11 @@ class LsmMpi:
         .geo file are created (format: geometry.read_geo)."""
         from .geometry import read_geo
         g = read_geo(fileName)
-        if g.info.hasAnyPeriodicDimensions():
-            raise NotImplementedError("periodic boundaries are not supported")
         if self._engine is None:
-            self.setSpatialDomain(BoundingBox(Vec3(g.info.bbox_min), Vec3(g.info.bbox_max)))
+            self.setSpatialDomain(BoundingBox(Vec3(g.info.bbox_min), Vec3(g.info.bbox_max)),
+                                  circDimList=list(g.info.periodic))
+        elif list(g.info.periodic) != list(getattr(self, "_circ", [False] * 3)):
+            raise ValueError("the .geo file's periodic dimensions differ from the domain's")
         if len(g.ids):
             self.createParticleArrays(g.ids, g.pos, g.rad, mass=g.mass, tag=g.tags)
         if len(g.conn):
@@ -648,6 +660,12 @@ class LsmMpi:
 
     def moveWallBy(self, name, move):
         self._e().move_wall_by(name, list(Vec3(move)))
+
+    def applyForceToWall(self, interactionName, force):
+        """applyForceToWall (LsmMpiPy.cpp:1208): move the wall of the wall
+        interaction group along the force until the group's wall force
+        balances it (the servo of ServoWallLoaderRunnable)."""
+        self._e().apply_force_to_wall(interactionName, list(force))
 
     def setWallStepMotion(self, name, move):
         """Move wall ``name`` by ``move`` at the start of every time step, as a

@@ -201,12 +201,70 @@ def lattice_block(nx=8, ny=12, nz=8, r=0.5, jitter=1e-3, rad_spread=0.0, seed=7,
     return spec
 
 
+class ServoWallLoader(lsm.Runnable):
+    """Pre-time-step runnable holding a wall group's wall at a constant normal
+    force: applyForceToWall(group, force) every step (the servo of the
+    reference's ServoWallLoaderRunnable in its shear-cell tutorial, without the
+    ramp), from time step ``start`` on."""
+
+    def __init__(self, sim, group, force, start=1):
+        self.sim, self.group, self.force, self.start = sim, group, lsm.Vec3(*force), start
+
+    def run(self):
+        # the wall groups list their candidates at the first step's update
+        if self.sim.getTimeStep() >= self.start:
+            self.sim.applyForceToWall(self.group, self.force)
+
+
+def shear_cell(n=2000000, seed=11, rmin=0.25, rmax=0.75, aspect=(200.0, 40.0, 100.0), phi=0.35,
+               box=None, dt=5e-5, kn=1e4, ks=1e4, mu=0.5, wall_speed=0.1, stress=1.0,
+               wall_k=1e4, alpha=0.2, density=1.0):
+    """Frictional shear cell, periodic in x: RotSphere, radii U[rmin, rmax],
+    random sequential packing at solid fraction ``phi`` in a box of the given
+    aspect (or ``box``); RotFriction kn, ks, mu between all particles; the
+    particles within 2 rmax of the bottom (tag 2) and top (tag 3) are bonded to
+    rough walls (NRotBondedWallPrms) that slide at -/+wall_speed in x; the top
+    wall is servo-loaded to the normal stress ``stress`` over its area."""
+    from . import geometry
+    rng = np.random.default_rng(seed)
+    rad = rng.uniform(rmin, rmax, n)
+    rad = np.sort(rad)[::-1].copy()  # largest first: insertion stays feasible
+    if box is None:
+        vs = float(np.sum(4.0 / 3.0 * math.pi * rad ** 3))
+        s = (vs / phi / (aspect[0] * aspect[1] * aspect[2])) ** (1.0 / 3.0)
+        box = (aspect[0] * s, aspect[1] * s, aspect[2] * s)
+    pos = geometry.rsa_packing(rad, box, seed + 1, periodic_x=True)
+    # the ids follow a random permutation of the insertion order
+    perm = rng.permutation(n)
+    pos, rad = pos[perm], rad[perm]
+    tags = np.ones(n, np.int32)
+    tags[pos[:, 1] < 2 * rmax] = 2
+    tags[pos[:, 1] > box[1] - 2 * rmax] = 3
+    v_step = wall_speed * dt
+    area = box[0] * box[2]
+    return {
+        "ptype": "RotSphere", "grid": 2 * rmax + alpha, "alpha": alpha, "dt": dt,
+        "domain": ((0.0, -1.0, -1.0), (box[0], box[1] + 1.0, box[2] + 1.0)),
+        "circ": [True, False, False], "box": box,
+        "ids": np.arange(n, dtype=np.int32), "pos": pos, "rad": rad, "tags": tags,
+        "density": density,
+        "friction": ("friction", kn, mu, mu, ks),
+        "walls": [],
+        "bonded_walls": [("bottom", (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), wall_k, 2,
+                          (-v_step, 0.0, 0.0)),
+                         ("top", (0.0, box[1], 0.0), (0.0, -1.0, 0.0), wall_k, 3,
+                          (v_step, 0.0, 0.0))],
+        "servo": ("top_bonds", (0.0, -stress * area, 0.0)),
+    }
+
+
 def apply(sim, spec, steps_per_batch=None):
     """Set a specimen up on an LsmMpi-compatible simulation object."""
     sim.initNeighbourSearch(spec["ptype"], spec["grid"], spec["alpha"])
     sim.setTimeStepSize(spec["dt"])
     lo, hi = spec["domain"]
-    sim.setSpatialDomain(lsm.BoundingBox(lsm.Vec3(*lo), lsm.Vec3(*hi)))
+    sim.setSpatialDomain(lsm.BoundingBox(lsm.Vec3(*lo), lsm.Vec3(*hi)),
+                         circDimList=spec.get("circ"))
     n = len(spec["ids"])
     sim.createParticleArrays(spec["ids"], spec["pos"], spec["rad"],
                              mass=spec["rad"] ** 3, tag=spec.get("tags", np.zeros(n, np.int32)),
@@ -238,4 +296,11 @@ def apply(sim, spec, steps_per_batch=None):
         sim.createInteractionGroup(lsm.NRotElasticWallPrms(name + "_wall", name, k))
         if any(step):
             sim.setWallStepMotion(name, lsm.Vec3(*step))
+    for name, o, nrm, k, tag, step in spec.get("bonded_walls", []):
+        sim.createWall(name, lsm.Vec3(*o), lsm.Vec3(*nrm))
+        sim.createInteractionGroup(lsm.NRotBondedWallPrms(name + "_bonds", name, k, tag))
+        if any(step):
+            sim.setWallStepMotion(name, lsm.Vec3(*step))
+    if spec.get("servo") is not None:
+        sim.addPreTimeStepRunnable(ServoWallLoader(sim, *spec["servo"]))
     return sim

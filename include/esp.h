@@ -59,6 +59,14 @@ typedef struct esp_config {
   int do_2d;             /* 1: particles move in the x-y plane only */
   int device;            /* HIP device ordinal */
   int steps_per_batch;   /* steps queued per host synchronisation (>=1) */
+  int circular_x;        /* 1: periodic in x (setSpatialDomain(bbox, circDimList=[1,0,0]),
+                            LsmMpiPy.cpp:1093-1100 -> TSubLattice::initNeighborTable(min, max,
+                            circ), SubLattice.hpp:495-564): the cell size is adjusted so that
+                            an integer number of cells spans x, particles crossing an x end
+                            re-enter at the other (wrapped at each neighbour-table rebuild,
+                            as CParticle::setCircular, Particle.cpp:336-342) and every
+                            interaction sees the image of its higher-id particle nearest to
+                            its lower-id particle.  y and z cannot be circular. */
 } esp_config;
 
 /* RotBondPrms / BrittleBeamPrms after the Python-side mapping
@@ -232,6 +240,17 @@ int esp_add_damping_group(esp_engine* e, int type, const char* name, double visc
                           int max_iterations);
 /* moveWallBy (LsmMpiPy.cpp:1161, SubLattice.hpp:2878-2893): immediate. */
 int esp_move_wall_by(esp_engine* e, const char* name, const double d[3]);
+/* applyForceToWall(interactionName, force) (LsmMpiPy.cpp:1208 ->
+ * TSubLattice::applyForceToWall, SubLattice.hpp:2935-2950): the wall of the
+ * wall interaction group `name` moves along the force so that the group's wall
+ * force balances it: CEWallInteractionGroup::applyForce (elastic groups,
+ * EWallInteractionGroup.hpp:93-164: at most 50 iterations of a stiffness-weighted
+ * step until the relative force error is <= 1e-3) or CBWallInteractionGroup::
+ * applyForce (bonded groups, BWallInteractionGroup.hpp:71-145: at most 10, until
+ * the step is <= 1e-5).  Immediate, like moveWallBy; a servo-loaded wall calls
+ * it from a pre-time-step runnable.  ESP_ERR_NUMERIC for a bonded group without
+ * stiffness (the reference would move its wall by an infinite distance). */
+int esp_apply_force_to_wall(esp_engine* e, const char* name, const double force[3]);
 /* A pre-time-step runnable that calls moveWallBy(name, d) at the start of every
  * step (compbench.py:20-35 pattern, LatticeMaster.cpp:2647).  d = 0 removes it. */
 int esp_set_wall_step_motion(esp_engine* e, const char* name, const double d[3]);

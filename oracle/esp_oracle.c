@@ -232,6 +232,10 @@ struct eso_engine {
   /* neighbour table (ntable.hpp:52-85) */
   v3 p0;
   double dim, alpha;
+  /* circular x (TSubLattice::initNeighborTable with circ, SubLattice.hpp:495-564):
+   * period L = table length in x, cells 0..nx-1 without padding in x */
+  int circ_x;
+  double period;
   int gidx, gidy, gidz, nx, ny, nz, ncell;
   int* cell_start; /* CSR of cell contents in list order */
   int* cell_items;
@@ -342,6 +346,10 @@ static int nt_index(const eso_engine* e, v3 pos)
 {
   int ok = 1;
   int ix = cell_coord(pos.x, e->p0.x, e->dim, &ok) - e->gidx;
+  /* circular x: positions are wrapped into [x0, x0 + L) at each rebuild; a
+   * rounding of (x - x0)/dim onto the end of the range stays in the last cell */
+  if (e->circ_x && ok && pos.x >= e->p0.x && pos.x < e->p0.x + e->period)
+    ix = ix < 0 ? 0 : (ix >= e->nx ? e->nx - 1 : ix);
   int iy = cell_coord(pos.y, e->p0.y, e->dim, &ok) - e->gidy;
   int iz = cell_coord(pos.z, e->p0.z, e->dim, &ok) - e->gidz;
   if (!ok) return -1;
@@ -350,7 +358,8 @@ static int nt_index(const eso_engine* e, v3 pos)
   return -1;
 }
 
-/* NeighborTable::isInInner (ntable.hpp:207-222). */
+/* NeighborTable::isInInner (ntable.hpp:207-222).  With circular x the table
+ * has no padding cells in x, so every x is inner. */
 static int nt_inner(const eso_engine* e, v3 pos)
 {
   int ok = 1;
@@ -358,9 +367,23 @@ static int nt_inner(const eso_engine* e, v3 pos)
   int iy = cell_coord(pos.y, e->p0.y, e->dim, &ok) - e->gidy;
   int iz = cell_coord(pos.z, e->p0.z, e->dim, &ok) - e->gidz;
   if (!ok) return 0;
-  if (e->nz < 3) return (ix > 0) && (ix < e->nx - 1) && (iy > 0) && (iy < e->ny - 1);
-  return (ix > 0) && (ix < e->nx - 1) && (iy > 0) && (iy < e->ny - 1) && (iz > 0) &&
-         (iz < e->nz - 1);
+  int xin = e->circ_x ? 1 : (ix > 0) && (ix < e->nx - 1);
+  if (e->nz < 3) return xin && (iy > 0) && (iy < e->ny - 1);
+  return xin && (iy > 0) && (iy < e->ny - 1) && (iz > 0) && (iz < e->nz - 1);
+}
+
+/* The image of x2 nearest to x1 in a circular x dimension: x2 shifted by the
+ * period L when the two are more than L/2 apart, one addition, as a ghost copy
+ * carries the shifted position (CParticle::setCircular, Particle.cpp:336-342).
+ * The particle of higher id is the one imaged to the other's side. */
+static inline v3 x_image(const eso_engine* e, v3 x1, v3 x2)
+{
+  if (e->circ_x) {
+    double d = x1.x - x2.x;
+    if (d > 0.5 * e->period) x2.x = x2.x + e->period;
+    else if (d < -0.5 * e->period) x2.x = x2.x - e->period;
+  }
+  return x2;
 }
 
 static void rebuild_idmap(eso_engine* e)
@@ -444,6 +467,22 @@ static void repoint_interactions(eso_engine* e)
 static void nt_build(eso_engine* e)
 {
   int w = 0, removed = 0;
+  /* circular x: a particle that crossed an x end moves to its image on the
+   * other side, with its initial and old positions (CParticle::setCircular,
+   * Particle.cpp:336-342, applied when the ghost copy takes the particle over
+   * at the table rebuild, pp_array.hpp:433-493) */
+  if (e->circ_x)
+    for (int i = 0; i < e->np; i++) {
+      opart* p = &e->P[i];
+      double sh = 0.0;
+      if (p->pos.x < e->p0.x) sh = e->period;
+      else if (p->pos.x >= e->p0.x + e->period) sh = -e->period;
+      if (sh != 0.0) {
+        p->pos.x = p->pos.x + sh;
+        p->initpos.x = p->initpos.x + sh;
+        p->oldpos.x = p->oldpos.x + sh;
+      }
+    }
   for (int i = 0; i < e->np; i++) {
     if (nt_index(e, e->P[i].pos) != -1) e->P[w++] = e->P[i];
     else removed++;
@@ -477,7 +516,9 @@ static inline int nt_lin(const eso_engine* e, int x, int y, int z)
 /* addPairsToListLocal / addPairsToList (ntable.hpp:471-521). */
 static inline int near_pair(const eso_engine* e, int i, int j)
 {
-  double dist2 = vnorm2(vsub(e->P[i].pos, e->P[j].pos));
+  /* from the particle of lower id (the same norm without a circular image) */
+  if (e->P[j].id < e->P[i].id) { int t = i; i = j; j = t; }
+  double dist2 = vnorm2(vsub(e->P[i].pos, x_image(e, e->P[i].pos, e->P[j].pos)));
   double dmax = e->P[i].rad + e->P[j].rad + e->alpha;
   return dist2 <= (dmax * dmax);
 }
@@ -507,10 +548,14 @@ static void nt_pairs(eso_engine* e, pairlist* L, int flagged)
         int xmin = (ix > 0) ? ix - 1 : ix;
         int ymin = (iy > 0) ? iy - 1 : iy;
         int zmin = (iz > 0) ? iz - 1 : iz;
+        if (e->circ_x) { /* neighbours across the circular x edge (nx >= 3) */
+          xmin = ix - 1;
+          xmax = ix + 1;
+        }
         for (int x = xmin; x <= xmax; x++)
           for (int y = ymin; y <= ymax; y++)
             for (int z = zmin; z <= zmax; z++) {
-              int idx2 = nt_lin(e, x, y, z);
+              int idx2 = nt_lin(e, e->circ_x ? (x + e->nx) % e->nx : x, y, z);
               if (idx2 <= idx1) continue;
               int s2 = e->cell_start[idx2], n2 = e->cell_start[idx2 + 1] - s2;
               for (int a = 0; a < n1; a++) {
@@ -531,6 +576,16 @@ static void nt_pairs(eso_engine* e, pairlist* L, int flagged)
 /* Particles                                                                   */
 /* ------------------------------------------------------------------------- */
 /* CParticle::applyForce (Particle.cpp:391-400). */
+static inline void apply_force_at(opart* p, v3 f, v3 pos, v3 ppos)
+{
+  p->force = vadd(p->force, f);
+  p->fabs_sum += vnorm(f);
+  v3 rel = vsub(pos, ppos);
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) p->sigma[i][j] -= vget(f, i) * vget(rel, j);
+}
+/* the contact point relative to the particle's own position; an interaction
+ * across the circular edge passes the position of the image it computed with */
 static inline void apply_force(opart* p, v3 f, v3 pos)
 {
   p->force = vadd(p->force, f);
@@ -982,7 +1037,7 @@ static void dg_update(eso_engine* e, ogroup* G)
       for (int i = 0; i < G->n; i++) {
         odyn d = G->v[i];
         /* isPersistent (RotFricInteraction.cpp:345-351) */
-        v3 D = vsub(e->P[d.i1].pos, e->P[d.i2].pos);
+        v3 D = vsub(e->P[d.i1].pos, x_image(e, e->P[d.i1].pos, e->P[d.i2].pos));
         double dist = vdot(D, D);
         double eq = e->P[d.i1].rad + e->P[d.i2].rad;
         if (dist <= (eq * eq)) G->v[w++] = d;
@@ -1114,7 +1169,7 @@ static int sc_broken(const eso_engine* e, const obgroup* G, const obond* b)
 {
   const opart* p1 = &e->P[b->i1];
   const opart* p2 = &e->P[b->i2];
-  v3 d = vsub(p2->pos, p1->pos);
+  v3 d = vsub(x_image(e, p1->pos, p2->pos), p1->pos);
   v3 n = vdiv(d, vnorm(d));
   v3 s;
   if (n.x != 1.0 || n.y != 0.0 || n.z != 0.0) {
@@ -1243,16 +1298,17 @@ static void calc_and_integrate(eso_engine* e)
       odyn* d = &G->v[c];
       opart* p1 = &e->P[d->i1];
       opart* p2 = &e->P[d->i2];
+      const v3 x2 = x_image(e, p1->pos, p2->pos);
       if (G->kind == G_NE_ELASTIC) {
         /* CElasticInteraction::calcForces (ElasticInteraction.cpp:54-71) */
-        v3 D = vsub(p1->pos, p2->pos);
+        v3 D = vsub(p1->pos, x2);
         double dist = vdot(D, D);
         double eq_dist = p1->rad + p2->rad;
         if (dist < (eq_dist * eq_dist)) {
           dist = sqrt(dist);
           v3 force = vmul(D, d->k * (dist - eq_dist) / dist);
-          v3 pos = vadd(p2->pos, vmul(D, p2->rad / eq_dist));
-          apply_force(p2, force, pos);
+          v3 pos = vadd(x2, vmul(D, p2->rad / eq_dist));
+          apply_force_at(p2, force, pos, x2);
           apply_force(p1, vmul(force, -1.0), pos);
           d->touching = 1;
         } else {
@@ -1260,29 +1316,29 @@ static void calc_and_integrate(eso_engine* e)
         }
       } else if (G->kind == G_NE_ROTELASTIC) {
         /* CRotElasticInteraction::calcForces (RotElasticInteraction.cpp:69-88) */
-        v3 rb = vsub(p1->pos, p2->pos);
+        v3 rb = vsub(p1->pos, x2);
         double rbNorm = vnorm(rb);
         double r_0Norm = p1->rad + p2->rad;
         v3 delta_r = vdiv(vmul(rb, rbNorm - r_0Norm), rbNorm);
         if (rbNorm < r_0Norm) {
           v3 force = vmul(delta_r, d->k);
-          v3 pos = vadd(p2->pos, vmul(rb, p2->rad / (p1->rad + p2->rad)));
+          v3 pos = vadd(x2, vmul(rb, p2->rad / (p1->rad + p2->rad)));
           apply_force(p1, vmul(force, -1.0), pos);
-          apply_force(p2, force, pos);
+          apply_force_at(p2, force, pos, x2);
         }
       } else {
         fric_out o;
         v3 fn = V(0, 0, 0);
-        rot_friction_law(p1->pos, p1->vel, p1->angvel, p1->rad, p2->pos, p2->vel, p2->angvel,
+        rot_friction_law(p1->pos, p1->vel, p1->angvel, p1->rad, x2, p2->vel, p2->angvel,
                          p2->rad, d->k, d->ks, G->mu_d, G->mu_s, e->dt, &d->ffric, &d->slipping,
                          &d->ediss, &fn, &o);
         if (o.touching) {
-          apply_force(p2, fn, o.cpos);
+          apply_force_at(p2, fn, o.cpos, x2);
           apply_force(p1, vmul(fn, -1.0), o.cpos);
           apply_moment(p1, o.m1);
           apply_moment(p2, o.m2);
           apply_force(p1, o.f1, o.cpos);
-          apply_force(p2, o.f2, o.cpos);
+          apply_force_at(p2, o.f2, o.cpos, x2);
           d->touching = 1;
         } else {
           d->touching = 0;
@@ -1301,13 +1357,14 @@ static void calc_and_integrate(eso_engine* e)
       opart* p1 = &e->P[b->i1];
       opart* p2 = &e->P[b->i2];
       bond_out o;
-      rot_bond_law(p1->pos, p1->q, p1->rad, p2->pos, p2->q, p2->rad, b->kr, b->ks, b->kt, b->kb,
+      const v3 x2 = x_image(e, p1->pos, p2->pos);
+      rot_bond_law(p1->pos, p1->q, p1->rad, x2, p2->q, p2->rad, b->kr, b->ks, b->kt, b->kb,
                    b->D, &o);
       b->nForce = o.nForce;
       b->shForce = o.shForce;
       b->tMoment = o.tMoment;
       b->bMoment = o.bMoment;
-      apply_force(p2, o.f2, o.cpos);
+      apply_force_at(p2, o.f2, o.cpos, x2);
       apply_force(p1, o.f1, o.cpos);
       apply_moment(p1, o.m1);
       apply_moment(p2, o.m2);
@@ -1399,7 +1456,15 @@ int eso_create(const esp_config* cfg, eso_engine** out)
   v3 mn = V(cfg->domain_min[0], cfg->domain_min[1], cfg->domain_min[2]);
   v3 mx = V(cfg->domain_max[0], cfg->domain_max[1], cfg->domain_max[2]);
   double nr = e->dim;
-  double xs = mx.x - mn.x; xs = nr * ceil(xs / nr);
+  double xs = mx.x - mn.x;
+  if (cfg->circular_x) {
+    /* TSubLattice::initNeighborTable(min, max, circ) (SubLattice.hpp:502-515):
+     * the range is adjusted so that whole cells span the circular x size */
+    if (fabs(xs / nr - lrint(xs / nr)) > 1e-6) nr = xs / floor(xs / nr);
+    e->dim = nr;
+  } else {
+    xs = nr * ceil(xs / nr);
+  }
   double ys = mx.y - mn.y; ys = nr * ceil(ys / nr);
   double zs = mx.z - mn.z; zs = nr * ceil(zs / nr);
   v3 grow = vsub(V(xs, ys, zs), vsub(mx, mn));
@@ -1424,6 +1489,17 @@ int eso_create(const esp_config* cfg, eso_engine** out)
   }
   e->gidx = e->gidy = e->gidz = -1;
   e->nx = (int)nxg + 2;
+  if (cfg->circular_x) {
+    if (nxg < 3) {
+      snprintf(g_create_err, sizeof g_create_err, "a circular x dimension needs at least 3 cells");
+      free(e);
+      return ESP_ERR_ARG;
+    }
+    e->circ_x = 1;
+    e->gidx = 0;
+    e->nx = (int)nxg;
+    e->period = nmax.x - nmin.x;
+  }
   e->ny = (int)nyg + 2;
   e->nz = (int)nzg + 2;
   e->ncell = e->nx * e->ny * e->nz;
@@ -1654,7 +1730,8 @@ static int add_bond_group(eso_engine* e, const char* name, const esp_rot_bond_pr
     bd->id2 = id2;
     bd->i1 = i1;
     bd->i2 = i2;
-    bond_constants(e->P[i1].pos, e->P[i1].rad, e->P[i2].pos, e->P[i2].rad, prm, maxes, e->do2d, bd);
+    bond_constants(e->P[i1].pos, e->P[i1].rad, x_image(e, e->P[i1].pos, e->P[i2].pos),
+                   e->P[i2].rad, prm, maxes, e->do2d, bd);
   }
   free(ck); free(cpos); free(first);
   e->nbg++;
@@ -1809,6 +1886,97 @@ int eso_move_wall_by(eso_engine* e, const char* name, const double d[3])
   int w = find_wall(e, name);
   if (w < 0) return fail(e, ESP_ERR_ARG, "unknown wall '%s'", name ? name : "");
   e->wl[w].origin = vadd(e->wl[w].origin, V(d[0], d[1], d[2]));
+  return ESP_OK;
+}
+
+/* applyForceToWall(interactionName, force) (LsmMpiPy.cpp:1208 ->
+ * TSubLattice::applyForceToWall, SubLattice.hpp:2935-2950): the wall of the
+ * group moves along the force direction until the group's wall force balances
+ * the force.  Elastic groups: CEWallInteractionGroup::applyForce
+ * (EWallInteractionGroup.hpp:93-164); bonded groups:
+ * CBWallInteractionGroup::applyForce (BWallInteractionGroup.hpp:71-145).
+ * Candidates in their list order, sums from zero in that order. */
+int eso_apply_force_to_wall(eso_engine* e, const char* name, const double Fa[3])
+{
+  int g = -1;
+  for (int i = 0; i < e->nwg; i++)
+    if (!strcmp(e->wg[i].name, name ? name : "")) g = i;
+  if (g < 0) return fail(e, ESP_ERR_ARG, "unknown wall interaction group '%s'", name ? name : "");
+  owgroup* G = &e->wg[g];
+  owall* W = &e->wl[G->wall];
+  const v3 F = V(Fa[0], Fa[1], Fa[2]);
+  const v3 O_f = vdiv(F, vnorm(F));
+  if (!G->bonded) {
+    int it = 0;
+    double d, df, ef;
+    do {
+      double k = 0.0;
+      for (int c = 0; c < G->n; c++) {
+        const opart* p = &e->P[G->pidx[c]];
+        double dist = vdot(vsub(p->pos, W->origin), W->normal);
+        if (dist < p->rad) k += G->keff[c];
+      }
+      if (k > 0) {
+        v3 Fl = V(0.0, 0.0, 0.0);
+        for (int c = 0; c < G->n; c++) {
+          if (!G->inner[c]) continue;
+          const opart* p = &e->P[G->pidx[c]];
+          v3 f = V(0.0, 0.0, 0.0);
+          double dist = vdot(vsub(p->pos, W->origin), W->normal);
+          if (dist < p->rad) f = vmul(W->normal, G->keff[c] * (p->rad - dist));
+          f = vmul(f, -1.0);
+          Fl = vadd(Fl, vmul(O_f, vdot(f, O_f)));
+        }
+        df = vdot(vadd(F, Fl), O_f);
+        d = df / k;
+        ef = df / vnorm(F);
+      } else {
+        d = 1e-5;
+        ef = 1;
+      }
+      W->origin = vadd(W->origin, vmul(O_f, d));
+      it++;
+    } while ((it < 50) && (ef > 1e-3));
+    return ESP_OK;
+  }
+  double scale_base = 3.1415926536;
+  double K = 0.0;
+  for (int c = 0; c < G->n; c++) { /* bonded interactions first, then elastic */
+    if (!G->tagged[c] || !G->inner[c]) continue;
+    const opart* p = &e->P[G->pidx[c]];
+    K += G->k * (e->do2d ? 1.0 : scale_base * p->rad);
+  }
+  for (int c = 0; c < G->n; c++) {
+    if (G->tagged[c] || !G->inner[c]) continue;
+    const opart* p = &e->P[G->pidx[c]];
+    double dist = vdot(vsub(p->pos, W->origin), W->normal);
+    if (dist < p->rad) K += G->keff[c];
+  }
+  if (!(K > 0)) return fail(e, ESP_ERR_NUMERIC, "wall group '%s' has no stiffness to servo", name);
+  int it = 0;
+  double d;
+  do {
+    v3 Fl = V(0.0, 0.0, 0.0);
+    for (int c = 0; c < G->n; c++) {
+      if (!G->tagged[c] || !G->inner[c]) continue;
+      const opart* p = &e->P[G->pidx[c]];
+      v3 D = vsub(vsub(p->pos, p->initpos), vsub(W->origin, W->oldpos));
+      v3 f = vmul(D, G->k * (e->do2d ? 1.0 : scale_base * p->rad));
+      Fl = vadd(Fl, vmul(O_f, vdot(f, O_f)));
+    }
+    for (int c = 0; c < G->n; c++) {
+      if (G->tagged[c] || !G->inner[c]) continue;
+      const opart* p = &e->P[G->pidx[c]];
+      v3 f = V(0.0, 0.0, 0.0);
+      double dist = vdot(vsub(p->pos, W->origin), W->normal);
+      if (dist < p->rad) f = vmul(W->normal, G->keff[c] * (p->rad - dist));
+      f = vmul(f, -1.0);
+      Fl = vadd(Fl, vmul(O_f, vdot(f, O_f)));
+    }
+    d = vdot(vadd(F, Fl), O_f) / K;
+    W->origin = vadd(W->origin, vmul(O_f, d));
+    it++;
+  } while ((it < 10) && (fabs(d) > 10e-6));
   return ESP_OK;
 }
 

@@ -86,6 +86,7 @@ int eso_add_brittle_beam_sc_group(eso_engine* e, const char* name,
  * the reference's interaction storage; returns the count. */
 int eso_get_group_list_order(eso_engine* e, const char* name, int* id1, int* id2);
 
+int eso_apply_force_to_wall(eso_engine* e, const char* name, const double F[3]);
 /* Per particle (ascending id): sum of the norms of the forces and sum of the
  * norms of the moments applied in the last step (2 doubles each). */
 int eso_get_force_scale(eso_engine* e, double* out);

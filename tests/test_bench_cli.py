@@ -32,7 +32,27 @@ def test_cpu_baseline_leg(pkg):
     bench = _load("esp_bench", os.path.join(ROOT, "bench.py"))
     spm = importlib.import_module("esys_particle_amd.specimens")
     spec = spm.lattice_block(nx=4, ny=6, nz=4, dt=1e-4, vel_sd=0.05, wall_speed=5.0)
-    out = bench.cpu_baseline(pkg, spm, spec, budget_s=0.5, max_steps=20)
+    out, steps = bench.cpu_baseline(pkg, spm, spec, budget_s=0.3)
     assert out["unit"] == "particle-steps/s" and out["kind"] == "port" and out["cores"] == 1
-    assert out["value"] > 0
-    assert out["sample"].startswith("steps 2-21 of the same specimen, 96 particles")
+    assert out["value"] > 0 and steps >= 10 and steps % 10 == 0
+    assert out["sample"].startswith("steps 2-%d of the same specimen, " % (steps + 1))
+
+
+def test_cpu_baseline_all_cores_leg(pkg, tmp_path):
+    """Independent single-thread oracle processes on copies of the specimen;
+    the aggregate rate, the process count and the CPU model are reported."""
+    bench = _load("esp_bench", os.path.join(ROOT, "bench.py"))
+    spm = importlib.import_module("esys_particle_amd.specimens")
+    spec = spm.lattice_block(nx=4, ny=6, nz=4, dt=1e-4, vel_sd=0.05, wall_speed=5.0)
+    path = str(tmp_path / "spec.pkl")
+    bench.save_specimen(path, spec)
+    out = bench.cpu_baseline_all_cores(path, 0.3, 2)
+    assert out["cores"] == 2 and out["kind"] == "port" and out["value"] > 0
+    assert out["nproc"] >= 1 and isinstance(out["cpu_model"], str)
+
+
+def test_idle_friction_pair_bytes():
+    """A stored friction pair whose particles are apart needs its two ids and
+    the flag word that says its history is zero."""
+    bench = _load("esp_bench", os.path.join(ROOT, "bench.py"))
+    assert bench.B_FRICTION_IDLE == 8 + 4

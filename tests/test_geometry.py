@@ -96,13 +96,19 @@ def test_read_geometry_into_model(pkg, orc, tmp_path):
     r = np.array([p[3] for p in parts])
     np.testing.assert_array_equal(sim.getParticleData("mass"), (math.pi * r * r) * ((4.0 / 3.0) * r))
     assert sim.getNumInteractions("b") == 2
-    # a periodic geometry is refused (periodic boundaries are not built)
+    # a geometry periodic in x sets up a circular x domain; y / z are refused
     write_geo(tmp_path / "p.geo", "LSMGeometry 1.2\nBoundingBox 0 0 0 4 4 4\nPeriodicBoundaries 1 0 0\n"
               "Dimension 3D\n", parts, [])
     s2 = orc.oracle_lsm_class(pkg)()
     s2.initNeighbourSearch("RotSphere", 1.2, 0.1)
+    s2.readGeometry(str(tmp_path / "p.geo"))
+    assert s2.getNumParticles() == 3
+    write_geo(tmp_path / "q.geo", "LSMGeometry 1.2\nBoundingBox 0 0 0 4 4 4\nPeriodicBoundaries 0 1 0\n"
+              "Dimension 3D\n", parts, [])
+    s3 = orc.oracle_lsm_class(pkg)()
+    s3.initNeighbourSearch("RotSphere", 1.2, 0.1)
     with pytest.raises(NotImplementedError):
-        s2.readGeometry(str(tmp_path / "p.geo"))
+        s3.readGeometry(str(tmp_path / "q.geo"))
 
 
 def test_dist_connections_brute_force(geom):

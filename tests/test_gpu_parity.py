@@ -498,3 +498,95 @@ def test_2d_nrot_settling(pkg, orc, spm):
         assert np.all(sim.getParticleData("vel")[:, 2] == 0.0)
     assert rel_field(g.engine.wall_force_history("bottom", 300),
                      o.engine.wall_force_history("bottom", 300)) <= TRAJ_TOL
+
+
+def _assert_same_particles(g, o, tol_pos):
+    assert g.getNumParticles() == o.getNumParticles()
+    np.testing.assert_array_equal(g.getParticleIds(), o.getParticleIds())
+    assert np.max(np.abs(g.getParticleData("pos") - o.getParticleData("pos"))) <= tol_pos
+
+
+def test_settling_particles_erased_outside_grid(pkg, orc, spm):
+    """A cloud sinks through a soft floor and out of the neighbour table's
+    padded grid: NeighborTable::build erases the particles that left
+    (ntable.hpp:281-286) at each rebuild; the survivors, their contacts and the
+    trajectories follow the oracle."""
+    spec = spm.gravity_settling(n=600, seed=5, box=(8.0, 16.0, 8.0), dt=2e-3, wall_k=1e-3)
+    spec["domain"] = ((-1.0, 2.5, -1.0), (9.0, 17.0, 9.0))
+    spec["vel"] = np.tile([0.0, -3.0, 0.0], (600, 1))
+    g, o = run_pair(pkg, orc, spm, spec, 300)
+    assert o.getNumParticles() < 500  # ~150 particles erased on the way
+    assert_events_equal(g, o, ["elastic"])
+    _assert_same_particles(g, o, TRAJ_TOL * float(np.mean(spec["rad"])))
+    assert rel_field(g.getParticleData("vel"), o.getParticleData("vel")) <= TRAJ_TOL
+
+
+def test_bonded_block_erased_outside_grid(pkg, orc, spm):
+    """A bonded rotational block falls out of the grid: erased particles take
+    their bonds and friction contacts with them (PIS_E::rebuild,
+    pi_storage_e.hpp:111-155) without break events; the survivors' bonds,
+    contacts, shear histories and break events follow the oracle."""
+    spec = spm.lattice_block(nx=6, ny=10, nz=6, dt=1e-4, vel_sd=0.05)
+    spec["vel"] = spec["vel"] + np.array([0.0, -20.0, 0.0])
+    spec["walls"] = []
+    spec["gravity"] = (0.0, -9.81, 0.0)
+    hi = spec["domain"][1]
+    spec["domain"] = ((-1.0, 2.4, -1.0), hi)
+    g, o = run_pair(pkg, orc, spm, spec, 300)
+    assert o.getNumParticles() < 340 and len(o.getBondBreaks()) > 100
+    assert_events_equal(g, o, ["bonded", "friction"])
+    _assert_same_particles(g, o, TRAJ_TOL * 0.5)
+    assert np.max(np.abs(g.getParticleData("quat") - o.getParticleData("quat"))) <= TRAJ_TOL
+    gf, gfl = g.engine.friction_state("friction")
+    of, ofl = o.engine.friction_state("friction")
+    np.testing.assert_array_equal(gfl, ofl)
+    assert rel_field(gf, of) <= TRAJ_TOL
+
+
+def shear_spec(spm, n=1500, seed=3):
+    """A reduced periodic shear cell that moves: walls sliding at 10 per unit
+    time, random initial velocities, dt 5e-4 (rebuilds, particles crossing the
+    circular x ends, contacts across them, a servo-loaded top wall)."""
+    spec = spm.shear_cell(n=n, seed=seed, phi=0.35, dt=5e-4, wall_speed=10.0)
+    rng = np.random.default_rng(seed)
+    spec["vel"] = rng.normal(0.0, 2.0, (n, 3))
+    return spec
+
+
+def test_periodic_shear_cell(pkg, orc, spm):
+    """Circular x, rough bonded walls and the servo: neighbour sets, friction
+    lists, rebuild steps bit-exact, trajectories and wall positions within 1e-9
+    after 300 steps, particles carried across the x ends."""
+    spec = shear_spec(spm)
+    g, o = run_pair(pkg, orc, spm, spec, 300)
+    assert len(o.engine.rebuild_steps()) >= 5
+    assert_events_equal(g, o, ["friction"])
+    L = spec["box"][0]
+    x = o.getParticleData("pos")[:, 0]
+    assert np.all((x > -1.0) & (x < L + 1.0))
+    _assert_same_particles(g, o, TRAJ_TOL * 0.5)
+    assert np.max(np.abs(g.getParticleData("quat") - o.getParticleData("quat"))) <= TRAJ_TOL
+    assert rel_field(g.getParticleData("vel"), o.getParticleData("vel")) <= TRAJ_TOL
+    gf, gfl = g.engine.friction_state("friction")
+    of, ofl = o.engine.friction_state("friction")
+    np.testing.assert_array_equal(gfl, ofl)
+    assert rel_field(gf, of) <= TRAJ_TOL
+    for w in ("top", "bottom"):
+        np.testing.assert_allclose(list(g.getWallPosition(w)), list(o.getWallPosition(w)),
+                                   rtol=1e-12, atol=1e-12)
+        np.testing.assert_allclose(g.engine.wall_force_history(w, 300),
+                                   o.engine.wall_force_history(w, 300), rtol=TRAJ_TOL,
+                                   atol=TRAJ_TOL * float(np.abs(o.engine.wall_force_history(w, 300)).max()))
+    # the servo holds the top wall's normal force at the stress times the area
+    area = spec["box"][0] * spec["box"][2]
+    assert abs(o.getWallForce("top")[1] + 1.0 * area) < 1e-2 * area
+
+
+def test_periodic_step_forces(pkg, orc, spm):
+    """Forces and moments of one step with contacts across the circular x
+    ends, within 1e-12 of the sum of the applied norms per particle."""
+    spec = shear_spec(spm, n=1200, seed=5)
+    spec["servo"] = None
+    g, o = run_pair(pkg, orc, spm, spec, 2)
+    assert rel_particle(g, o, "force") <= FORCE_TOL
+    assert rel_particle(g, o, "moment") <= FORCE_TOL
