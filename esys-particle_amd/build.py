@@ -12,7 +12,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libesp.so")
-SOURCES = ["esp_kernels.hip", "esp_engine.hip"]
+SOURCES = ["esp_kernels.hip", "esp_engine.hip", "esp_diff.hip"]
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     "-ffp-contract=off",

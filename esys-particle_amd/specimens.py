@@ -151,6 +151,7 @@ def bonded_compression(n=1000, seed=3, rmin=0.2, rmax=1.0, box=None, phi=0.3, dt
         "ptype": "RotSphere", "grid": 2 * rmax + alpha, "alpha": alpha, "dt": dt,
         "domain": ((-1.0, -1.0, -1.0), (box[0] + 1.0, ly + 1.0, box[2] + 1.0)),
         "ids": np.arange(n, dtype=np.int32), "pos": pos, "rad": rad, "density": 1.0,
+        "box": tuple(box), "cohesion": coh_mean,
         "bonds": pr.astype(np.int32),
         "bond_prms": dict(name="bonded", normalK=kr, shearK=ks, torsionK=kt, bendingK=kb,
                           tag=0, strengths=strengths),
@@ -162,15 +163,21 @@ def bonded_compression(n=1000, seed=3, rmin=0.2, rmax=1.0, box=None, phi=0.3, dt
 
 
 def dense_bonded_compression(n=100000, seed=3, rmin=0.2, rmax=1.0, phi=0.58, max_sweeps=2000,
-                             snap_sweeps=1500, **kw):
+                             snap_sweeps=1500, height=None, **kw):
     """bonded_compression on a dense grow-and-relax packing (geometry.py): radii
-    log-uniform in [rmin, rmax], a 1:2:1 prism sized for solid fraction phi."""
+    log-uniform in [rmin, rmax], a 1:2:1 prism sized for solid fraction phi (or
+    a square prism of the given height, so that specimens of a batch share the
+    walls)."""
     from . import geometry
     rng = np.random.default_rng(seed)
     rad = np.exp(rng.uniform(math.log(rmin), math.log(rmax), n))
     vs = float(np.sum(4.0 / 3.0 * math.pi * rad ** 3))
-    s = (vs / phi / 2.0) ** (1.0 / 3.0)
-    box = (s, 2 * s, s)
+    if height is None:
+        s = (vs / phi / 2.0) ** (1.0 / 3.0)
+        box = (s, 2 * s, s)
+    else:
+        w = math.sqrt(vs / phi / height)
+        box = (w, float(height), w)
     pos, _ = geometry.dense_packing(rad, box, seed + 1, max_sweeps)
     # relax the remaining overlaps so that contacts sit at (near) exact touching
     pos, _ = geometry.snap_bonds(pos, rad, np.zeros((0, 2), np.int32), box, snap_sweeps, 1e-9)

@@ -1,0 +1,84 @@
+// Host harness over esys-particle_amd/csrc/esp_difflaw.h for the CPU tests:
+// the bond and friction laws of the differentiable engine evaluated on doubles
+// (values) and on dual numbers seeded along one input (Jacobian columns).
+#include "esp_difflaw.h"
+
+// an input seeded with a unit tangent (a plain value for doubles)
+template <class T>
+static T seed(double v);
+template <>
+double seed<double>(double v)
+{
+  return v;
+}
+template <>
+esp_dual seed<esp_dual>(double v)
+{
+  return esp_dual(v, 1.0);
+}
+
+template <class T>
+static void bond_eval(const double* in, const double* prm, int k, double* out, bool tangent)
+{
+  // in: x1[3] x2[3] q1[4] q2[4]; prm: r1 r2 kr ks kt kb D0[3]
+  T u[16];
+  for (int t = 0; t < 16; t++) u[t] = T(in[t]);
+  T kr = T(prm[2]), ks = T(prm[3]);
+  if (tangent) {
+    if (k < 14) u[k] = seed<T>(in[k]);
+    if (k == 14) kr = seed<T>(prm[2]);
+    if (k == 15) ks = seed<T>(prm[3]);
+  }
+  TBondResult<T> r;
+  tbond_law<T>(tv3<T>{u[0], u[1], u[2]}, tquat<T>{u[6], {u[7], u[8], u[9]}}, prm[0],
+               tv3<T>{u[3], u[4], u[5]}, tquat<T>{u[10], {u[11], u[12], u[13]}}, prm[1], kr, ks,
+               T(prm[4]), T(prm[5]), tv3<T>{T(prm[6]), T(prm[7]), T(prm[8])}, r);
+  const T o[13] = {r.f1.x, r.f1.y, r.f1.z, r.m1.x, r.m1.y, r.m1.z, r.m2.x,
+                   r.m2.y, r.m2.z, r.nForce, r.shForce, r.tMoment, r.bMoment};
+  for (int t = 0; t < 13; t++) out[t] = tangent ? edl_tan(o[t]) : edl_val(o[t]);
+}
+
+template <class T>
+static int fric_eval(const double* in, const double* prm, int slip, int k, double* out,
+                     bool tangent)
+{
+  // in: x1 x2 v1 v2 w1 w2 ffric[3]; prm: r1 r2 k ks mu dt
+  T u[21];
+  for (int t = 0; t < 21; t++) u[t] = T(in[t]);
+  T kk = T(prm[2]), ks = T(prm[3]), mu = T(prm[4]);
+  if (tangent) {
+    if (k < 21) u[k] = seed<T>(in[k]);
+    if (k == 21) kk = seed<T>(prm[2]);
+    if (k == 22) ks = seed<T>(prm[3]);
+    if (k == 23) mu = seed<T>(prm[4]);
+  }
+  tv3<T> ffr{u[18], u[19], u[20]}, f1, m1, m2;
+  bool s = slip != 0;
+  const bool touching = tfriction_law<T>(tv3<T>{u[0], u[1], u[2]}, tv3<T>{u[6], u[7], u[8]},
+                                         tv3<T>{u[12], u[13], u[14]}, prm[0],
+                                         tv3<T>{u[3], u[4], u[5]}, tv3<T>{u[9], u[10], u[11]},
+                                         tv3<T>{u[15], u[16], u[17]}, prm[1], kk, ks, mu, mu,
+                                         prm[5], ffr, s, f1, m1, m2);
+  const T o[12] = {f1.x, f1.y, f1.z, m1.x, m1.y, m1.z, m2.x, m2.y, m2.z, ffr.x, ffr.y, ffr.z};
+  for (int t = 0; t < 12; t++) out[t] = tangent ? edl_tan(o[t]) : edl_val(o[t]);
+  return (touching ? 1 : 0) | (s ? 2 : 0);
+}
+
+extern "C" {
+void dl_bond(const double* in, const double* prm, double* out)
+{
+  bond_eval<double>(in, prm, -1, out, false);
+}
+void dl_bond_tangent(const double* in, const double* prm, int k, double* out)
+{
+  bond_eval<esp_dual>(in, prm, k, out, true);
+}
+int dl_fric(const double* in, const double* prm, int slip, double* out)
+{
+  return fric_eval<double>(in, prm, slip, -1, out, false);
+}
+int dl_fric_tangent(const double* in, const double* prm, int slip, int k, double* out)
+{
+  return fric_eval<esp_dual>(in, prm, slip, k, out, true);
+}
+}
