@@ -133,6 +133,31 @@ def cpu_baseline_all_cores(spec_path, budget_s, workers):
                           workers, min(r["steps"] for r in res) + 1, budget_s)}
 
 
+def condition(sim, eng, steps, settling):
+    """Specimen preparation before the warm-up: the compression specimen starts
+    with a bond-break cascade (bonds whose initial gap exceeds their failure
+    strain break in the first steps, ~23,000 of them at 100,000 particles) whose
+    flagged-pair list holds ~1.2 million friction pairs until the next neighbour
+    rebuild.  steps < 0: run until that rebuild has happened (the steady state
+    that makes up the rest of a 20,000-step run), at most 5,000 steps; 0: none.
+    Untimed, like the warm-up; returns the number of steps run."""
+    if steps == 0 or settling:
+        return 0
+    t0 = time.perf_counter()
+    if steps > 0:
+        sim.runTimeSteps(steps)
+        done = steps
+    else:
+        sim.runTimeSteps(1)
+        done = 1
+        first = eng.stats()["num_rebuilds"]
+        while done < 5000 and eng.stats()["num_rebuilds"] == first:
+            sim.runTimeSteps(100)
+            done += 100
+    log("conditioning %d steps: %.2f s, stats %s" % (done, time.perf_counter() - t0, eng.stats()))
+    return done
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -140,6 +165,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=500)
     ap.add_argument("--particles", type=int, default=100000)
     ap.add_argument("--batch", type=int, default=128, help="steps queued per host sync")
+    ap.add_argument("--condition", type=int, default=-1,
+                    help="compression: steps run before the warm-up to take the specimen "
+                         "through its initial bond-break cascade (-1: until the first "
+                         "neighbour rebuild after it, at most 5000; 0: none)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--cpu-workers", type=int, default=0,
@@ -190,6 +219,7 @@ def main():
     sim = pkg.LsmMpi(device=0, stepsPerBatch=args.batch)
     spm.apply(sim, spec)
     eng = sim.engine
+    cond = condition(sim, eng, args.condition, settling)
     t0 = time.perf_counter()
     sim.runTimeSteps(args.warmup)
     log("warmup %d steps: %.2f s, stats %s" % (args.warmup, time.perf_counter() - t0, eng.stats()))
@@ -269,6 +299,8 @@ def main():
             "touching_pairs": None if ntouch is None else int(ntouch),
             "prism": [round(b, 3) for b in spec["box"]],
             "steps_per_sync": args.batch,
+            "conditioning_steps": cond,
+            "first_timed_step": cond + args.warmup + 1,
         }),
         "roofline": {
             "bound": "hbm",
