@@ -57,10 +57,13 @@ _SIGS = {
     "esp_diff_set_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     "esp_diff_forward": (ctypes.c_int, [ctypes.c_void_p, _pd, _pd, _pd, _pd]),
     "esp_diff_backward": (ctypes.c_int, [ctypes.c_void_p, _pd]),
+    "esp_diff_forward_tangent": (ctypes.c_int, [ctypes.c_void_p, _pd, _pd, _pd, _pd, _pd, _pd]),
     "esp_diff_get_particle_data": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _pd]),
     "esp_diff_get_bond_breaks": (ctypes.c_int, [ctypes.c_void_p, _pi, _pi, _pi]),
     "esp_diff_get_rebuild_steps": (ctypes.c_int, [ctypes.c_void_p, _pi]),
     "esp_diff_get_times": (ctypes.c_int, [ctypes.c_void_p, _pd, _pd]),
+    "esp_diff_get_branch_digest": (ctypes.c_int, [ctypes.c_void_p,
+                                                  ctypes.POINTER(ctypes.c_uint64)]),
 }
 
 _LIB = None
@@ -169,6 +172,21 @@ class DiffModel:
                     "forward")
         return stress[:, :self.nsamples], loss
 
+    def forward_tangent(self, theta, dtheta, target):
+        """(stress, loss, dloss): a forward run that also carries the state's
+        tangent along the parameter direction dtheta [nspec, 4] (forward mode);
+        dloss[s] = d loss_s / d eps at theta + eps dtheta."""
+        th = np.ascontiguousarray(theta, np.float64).reshape(self.nspec, 4)
+        dth = np.ascontiguousarray(dtheta, np.float64).reshape(self.nspec, 4)
+        tg = np.ascontiguousarray(target, np.float64).reshape(self.nspec, self.nsamples)
+        stress = np.zeros((self.nspec, max(self.nsamples, 1)))
+        loss = np.zeros(self.nspec)
+        dloss = np.zeros(self.nspec)
+        self._check(self.lib.esp_diff_forward_tangent(self.h, _dp(th), _dp(dth), _dp(tg),
+                                                      _dp(stress), _dp(loss), _dp(dloss)),
+                    "forward_tangent")
+        return stress[:, :self.nsamples], loss, dloss
+
     def backward(self):
         """d loss_s / d theta_s of the last forward run, [nspec, 4]."""
         g = np.zeros((self.nspec, 4))
@@ -195,6 +213,15 @@ class DiffModel:
         self._check(self.lib.esp_diff_get_rebuild_steps(self.h, _ip(s)), "rebuild_steps")
         return s[:m]
 
+    def branch_digest(self):
+        """Per-step branch digests of the last forward run (uint64 [nsteps]):
+        contact touching/slipping and wall contacts hashed per step."""
+        m = self._check(self.lib.esp_diff_get_branch_digest(self.h, None), "branch_digest")
+        out = np.zeros(max(m, 1), np.uint64)
+        self._check(self.lib.esp_diff_get_branch_digest(
+            self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))), "branch_digest")
+        return out[:m]
+
     def times_ms(self):
         f, b = ctypes.c_double(), ctypes.c_double()
         self._check(self.lib.esp_diff_get_times(self.h, ctypes.byref(f), ctypes.byref(b)), "times")
@@ -203,21 +230,25 @@ class DiffModel:
     # ---- specimens ----
     @staticmethod
     def from_specimens(specs, nsteps, sample_every, gap=None, checkpoint_every=100, device=0,
-                       block=256, wall_k=None):
+                       block=256, wall_k=None, x_start=0.0):
         """A model of bonded compression specimens (specimens.bonded_compression
         dicts: RotSphere, one RotBond group with per-bond strengths, RotFriction,
         top and bottom walls moving at their steps).  Specimen s is shifted by
-        x_s along x; all must share the walls (same y extent and wall motion)."""
+        x_s along x (the first one's left edge at x_start; model.starts holds
+        every specimen's left edge); all must share the walls (same y extent and
+        wall motion).  Coordinates round differently at different offsets, so a
+        specimen runs bitwise as it would alone only at the same x_s."""
         specs = list(specs)
         s0 = specs[0]
         grid, alpha, dt = s0["grid"], s0["alpha"], s0["dt"]
         gap = 2.0 * grid if gap is None else gap
         pos, rad, mass, sid, bi, bj, rel = [], [], [], [], [], [], []
         kt, kb, area = [], [], []
-        x = 0.0
+        x = float(x_start)
         base = 0
-        offsets = []
+        offsets, starts = [], []
         for s, sp in enumerate(specs):
+            starts.append(x)
             p = np.array(sp["pos"], np.float64)
             lo = float(p[:, 0].min() - sp["rad"].max())
             shift = x - lo
@@ -259,6 +290,7 @@ class DiffModel:
             model.add_wall(o, nrm, kw, step, measure=(name == "top"),
                            area=np.array(area) if name == "top" else None)
         model.offsets = offsets
+        model.starts = starts
         model.specs = specs
         model.host = {"grid": grid, "alpha": alpha, "dt": dt, "domain": (lo, hi), "pos": pos,
                       "rad": rad, "mass": mass, "sid": sid, "bi": bi, "bj": bj, "rel": rel,

@@ -96,6 +96,25 @@ __device__ __forceinline__ long long dpair_key(int a, int b)
   return (long long)(((unsigned long long)(unsigned)a << 32) | (unsigned long long)(unsigned)b);
 }
 
+// Branch digest of a step: each wave hashes (its first index, its lanes'
+// branch bits) and adds the hash into the step's word; the sum is independent
+// of the order the waves finish in.  Two runs with equal digests took the same
+// contact, slip and wall-contact branches (pair and particle indices of the
+// same neighbour-table epochs).
+__device__ __forceinline__ unsigned long long dmix(unsigned long long z)
+{
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ void wave_digest(unsigned long long* dig, unsigned long long tag,
+                                            unsigned long long m0, unsigned long long m1)
+{
+  const int first = __ffsll((long long)__ballot(1)) - 1;
+  if ((int)__lane_id() == first) atomicAdd(dig, dmix(dmix(tag) ^ m0) + dmix(m1 ^ ~tag));
+}
+
 // ---------------------------------------------------------------------------
 // device views
 // ---------------------------------------------------------------------------
@@ -140,6 +159,89 @@ struct DCands {
 __device__ __forceinline__ tv3<double> dx3(double4 a) { return tv3<double>{a.x, a.y, a.z}; }
 
 // ---------------------------------------------------------------------------
+// tangent-linear (forward-mode) kernels: the derivative of one forward run
+// along a parameter direction, on the branch the run takes.  T holds the
+// state's tangent (x, v, w, q), th the shear histories' tangents; the laws run
+// on duals seeded with the state's tangents and the parameters' direction.
+// ---------------------------------------------------------------------------
+struct DTan {
+  double4* x;
+  double4* v;
+  double4* w;
+  double4* q;
+};
+
+__device__ __forceinline__ tv3<esp_dual> dd3(double4 a, double4 t)
+{
+  return tv3<esp_dual>{esp_dual(a.x, t.x), esp_dual(a.y, t.y), esp_dual(a.z, t.z)};
+}
+__device__ __forceinline__ tquat<esp_dual> ddq(double4 a, double4 t)
+{
+  return tquat<esp_dual>{esp_dual(a.x, t.x), {esp_dual(a.y, t.y), esp_dual(a.z, t.z), esp_dual(a.w, t.w)}};
+}
+
+__global__ void k_bond_tan(DState S, DTan T, DBonds B, const double4* theta, const double4* dtheta,
+                           int step, double* bt)
+{
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B.nb) return;
+  double* o = bt + 9 * (size_t)b;
+  if (B.bstep[b] < step) {
+    for (int k = 0; k < 9; k++) o[k] = 0.0;
+    return;
+  }
+  const int2 e = B.ij[b];
+  const double4 D = B.D0[b];
+  const int s = S.spec[e.x];
+  const double4 th = theta[s], dth = dtheta[s];
+  const double2 tk = B.tkb[s];
+  const double4 P1 = S.x[e.x], P2 = S.x[e.y];
+  esp_dual kr, ks;
+  double kt, kb;
+  tbond_k<esp_dual>(esp_dual(th.x, dth.x), esp_dual(th.y, dth.y), tk.x, tk.y, D.w, P1.w, P2.w, 1,
+                    1, 0, kr, ks, kt, kb);
+  TBondResult<esp_dual> r;
+  tbond_law<esp_dual>(dd3(P1, T.x[e.x]), ddq(S.q[e.x], T.q[e.x]), P1.w, dd3(P2, T.x[e.y]),
+                      ddq(S.q[e.y], T.q[e.y]), P2.w, kr, ks, esp_dual(kt), esp_dual(kb),
+                      tv3<esp_dual>{esp_dual(D.x), esp_dual(D.y), esp_dual(D.z)}, r);
+  o[0] = r.f1.x.d; o[1] = r.f1.y.d; o[2] = r.f1.z.d;
+  o[3] = r.m1.x.d; o[4] = r.m1.y.d; o[5] = r.m1.z.d;
+  o[6] = r.m2.x.d; o[7] = r.m2.y.d; o[8] = r.m2.z.d;
+}
+
+__global__ void k_fric_tan(DState S, DTan T, DCands C, const int* bstep, const double4* theta,
+                           const double4* dtheta, const double4* hin, const double4* thin,
+                           double4* thout, int step, double dt, double* ct)
+{
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= C.nc) return;
+  double* o = ct + 9 * (size_t)c;
+  const int bb = C.bond[c];
+  if (bb >= 0 && bstep[bb] >= step) {
+    for (int k = 0; k < 9; k++) o[k] = 0.0;
+    thout[c] = make_double4(0.0, 0.0, 0.0, 0.0);
+    return;
+  }
+  const int2 e = C.ij[c];
+  const double4 P1 = S.x[e.x], P2 = S.x[e.y];
+  const int s = S.spec[e.x];
+  const double4 th = theta[s], dth = dtheta[s];
+  esp_dual kk, kks;
+  tfriction_k<esp_dual>(esp_dual(th.x, dth.x), esp_dual(th.y, dth.y), 1, 1, P1.w, P2.w, 0, kk, kks);
+  const double4 h = hin[c], dh = thin[c];
+  tv3<esp_dual> ffr{esp_dual(h.x, dh.x), esp_dual(h.y, dh.y), esp_dual(h.z, dh.z)}, f1, m1, m2;
+  bool slipping = (((int)h.w) & 1) != 0;
+  const esp_dual mu(th.w, dth.w);
+  tfriction_law<esp_dual>(dd3(P1, T.x[e.x]), dd3(S.v[e.x], T.v[e.x]), dd3(S.w[e.x], T.w[e.x]), P1.w,
+                          dd3(P2, T.x[e.y]), dd3(S.v[e.y], T.v[e.y]), dd3(S.w[e.y], T.w[e.y]), P2.w,
+                          kk, kks, mu, mu, dt, ffr, slipping, f1, m1, m2);
+  thout[c] = make_double4(ffr.x.d, ffr.y.d, ffr.z.d, 0.0);
+  o[0] = f1.x.d; o[1] = f1.y.d; o[2] = f1.z.d;
+  o[3] = m1.x.d; o[4] = m1.y.d; o[5] = m1.z.d;
+  o[6] = m2.x.d; o[7] = m2.y.d; o[8] = m2.z.d;
+}
+
+// ---------------------------------------------------------------------------
 // forward kernels
 // ---------------------------------------------------------------------------
 __global__ void k_bond_fwd(DState S, DBonds B, const double4* theta, int step, double* bout)
@@ -182,7 +284,8 @@ __global__ void k_bond_fwd(DState S, DBonds B, const double4* theta, int step, d
 }
 
 __global__ void k_fric_fwd(DState S, DCands C, const int* bstep, const double4* theta,
-                           const double4* hin, double4* hout, int step, double dt, double* cout)
+                           const double4* hin, double4* hout, int step, double dt, double* cout,
+                           unsigned long long* dig)
 {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= C.nc) return;
@@ -206,6 +309,10 @@ __global__ void k_fric_fwd(DState S, DCands C, const int* bstep, const double4* 
                                               dx3(W2), P2.w, kk, kks, th.w, th.w, dt, ffr,
                                               slipping, f1, m1, m2);
   hout[c] = make_double4(ffr.x, ffr.y, ffr.z, (double)((slipping ? 1 : 0) | (touching ? 2 : 0)));
+  if (dig) {
+    const unsigned long long mt = __ballot(touching), ms = __ballot(touching && slipping);
+    wave_digest(dig, (unsigned long long)(c & ~63), mt, ms);
+  }
   o[0] = f1.x; o[1] = f1.y; o[2] = f1.z;
   o[3] = m1.x; o[4] = m1.y; o[5] = m1.z;
   o[6] = m2.x; o[7] = m2.y; o[8] = m2.z;
@@ -217,7 +324,8 @@ __global__ void k_fric_fwd(DState S, DCands C, const int* bstep, const double4* 
 __global__ void k_gather_fwd(DState S, const double4* mass_old, const int* inc_off,
                              const int* inc_ent, int ncand, const double* cout, const double* bout,
                              DStep sp, double* wfn, unsigned long long* md,
-                             unsigned long long thresh)
+                             unsigned long long thresh, unsigned long long* dig, DTan T,
+                             const double* ct, const double* bt, double* wft)
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= S.n) return;
@@ -225,20 +333,30 @@ __global__ void k_gather_fwd(DState S, const double4* mass_old, const int* inc_o
   const dv3 x0 = mk3(P.x, P.y, P.z);
   const double r = P.w;
   dv3 F = mk3(0.0, 0.0, 0.0), M = mk3(0.0, 0.0, 0.0);
-  double wn = 0.0;
+  dv3 tF = mk3(0.0, 0.0, 0.0), tM = mk3(0.0, 0.0, 0.0);  // tangents (T.x set)
+  double wn = 0.0, twn = 0.0;
   for (int k = 0; k < sp.nw; k++) {  // CElasticWallInteraction::calcForces
     const DWall& W = sp.wall[k];
     const dv3 n = mk3(W.n[0], W.n[1], W.n[2]);
     const double dist = dot3(sub3(x0, mk3(W.o[0], W.o[1], W.o[2])), n);
+    if (dig) wave_digest(dig, ((unsigned long long)(i & ~63) << 3) | (unsigned long long)(k + 1),
+                         __ballot(dist < r), 0ull);
     if (dist < r) {
       double f = 1.0;
       f *= 3.141592654 * r;
       const dv3 force = mul3(n, (f * W.k) * (r - dist));
       F = add3(F, force);
       if (W.measure) wn = dot3(force, n);
+      if (T.x) {  // d force = -k' n (n . dx)
+        const double4 Tx = T.x[i];
+        const dv3 tf = mul3(n, -((f * W.k) * dot3(n, mk3(Tx.x, Tx.y, Tx.z))));
+        tF = add3(tF, tf);
+        if (W.measure) twn = dot3(tf, n);
+      }
     }
   }
   if (wfn) wfn[i] = wn;
+  if (wft) wft[i] = twn;
   const double4 Om = mass_old[i];
   if (sp.has_g) F = add3(F, mul3(mk3(sp.g[0], sp.g[1], sp.g[2]), Om.w));
   const int e0 = inc_off[i], e1 = inc_off[i + 1];
@@ -252,6 +370,16 @@ __global__ void k_gather_fwd(DState S, const double4* mass_old, const int* inc_o
     } else {
       F = add3(F, mk3(-q[0], -q[1], -q[2]));
       M = add3(M, mk3(q[6], q[7], q[8]));
+    }
+    if (T.x) {
+      const double* u = rec < ncand ? ct + 9 * (size_t)rec : bt + 9 * (size_t)(rec - ncand);
+      if (side == 0) {
+        tF = add3(tF, mk3(u[0], u[1], u[2]));
+        tM = add3(tM, mk3(u[3], u[4], u[5]));
+      } else {
+        tF = sub3(tF, mk3(u[0], u[1], u[2]));
+        tM = add3(tM, mk3(u[6], u[7], u[8]));
+      }
     }
   }
   // CRotParticle::integrate + rescale, CParticle::integrate (RotParticle.cpp:
@@ -273,6 +401,26 @@ __global__ void k_gather_fwd(DState S, const double4* mass_old, const int* inc_o
     const double inverse = 1.0 / module;
     qq.s = inverse * qq.s;
     qq.v = mul3(qq.v, inverse);
+  }
+  if (T.x) {
+    // tangents of w1 = w + M dt/I, qt = q + h (0, w1) q, q1 = qt/|qt|,
+    // v1 = v + F dt/m, x1 = x + v1 dt (q0, w: the state before the step)
+    const double4 Tx = T.x[i], Tv = T.v[i], Tw = T.w[i], Tq = T.q[i];
+    const dquat q0 = dquat{Q.x, mk3(Q.y, Q.z, Q.w)}, tq0 = dquat{Tq.x, mk3(Tq.y, Tq.z, Tq.w)};
+    const dv3 tw1 = add3(mk3(Tw.x, Tw.y, Tw.z), mul3(tM, dt * A.w));
+    const dquat a1 = qmul4(dquat{0, tw1}, q0), a2 = qmul4(dquat{0, w}, tq0);
+    const dquat tqt = dquat{tq0.s + h * (a1.s + a2.s), add3(tq0.v, mul3(add3(a1.v, a2.v), h))};
+    dquat tq1 = tqt;
+    if (module != 0.0) {
+      const double pr = qq.s * tqt.s + dot3(qq.v, tqt.v);
+      tq1 = dquat{(tqt.s - qq.s * pr) / module, div3(sub3(tqt.v, mul3(qq.v, pr)), module)};
+    }
+    const dv3 tv1 = add3(mk3(Tv.x, Tv.y, Tv.z), mul3(mul3(tF, Vv.w), dt));
+    const dv3 tx1 = add3(mk3(Tx.x, Tx.y, Tx.z), mul3(tv1, dt));
+    T.x[i] = make_double4(tx1.x, tx1.y, tx1.z, 0.0);
+    T.v[i] = make_double4(tv1.x, tv1.y, tv1.z, 0.0);
+    T.w[i] = make_double4(tw1.x, tw1.y, tw1.z, 0.0);
+    T.q[i] = make_double4(tq1.s, tq1.v.x, tq1.v.y, tq1.v.z);
   }
   S.x[i] = make_double4(x.x, x.y, x.z, r);
   S.v[i] = make_double4(v.x, v.y, v.z, Vv.w);
@@ -372,6 +520,7 @@ __global__ void k_inc_sort2(int n, const int* off, int* ent)
     ent[j + 1] = v;
   }
 }
+
 
 // ---------------------------------------------------------------------------
 // adjoint kernels
@@ -667,6 +816,13 @@ struct esp_diff {
   DevBuf<double4> theta;
   std::vector<double> h_theta, h_target, h_stress;
   DevBuf<unsigned long long> md;
+  DevBuf<unsigned long long> digest;  // per step: branch digest of the last forward run
+  // tangent-linear run (forward with a parameter direction)
+  bool tan_on = false;
+  DevBuf<double4> tx, tv, tw, tq, thA, thB, dtheta;
+  DevBuf<double> bt, ct, wft, stress_t;
+  std::vector<double> h_stress_t;
+  DTan tan() { return DTan{tx.p, tv.p, tw.p, tq.p}; }
   DevBuf<int> cell, cell_count, cell_start, cell_fill, cell_items, ccnt, coff, cp, cq, scan_tmp,
       inc_cnt, inc_fill;
   DevBuf<long long> ckey;
@@ -791,14 +947,30 @@ void rebuild(esp_diff* d, int step)
                        P ? d->hA.p : nullptr, hn.p);
     if (!P) DCHK(hipMemsetAsync(hn.p, 0, sizeof(double4) * nc, d->stream));
   }
+  DevBuf<double4> tn, tn2;
+  if (d->tan_on) {
+    tn.reserve(c1);
+    tn2.reserve(c1);
+    if (nc > 0) {
+      hipLaunchKernelGGL(k_carry_hist, dim3(nblocks(nc, bs)), dim3(bs), 0, d->stream, nc, E->src.p,
+                         P ? d->thA.p : nullptr, tn.p);
+      if (!P) DCHK(hipMemsetAsync(tn.p, 0, sizeof(double4) * nc, d->stream));
+    }
+    d->ct.reserve(c1 * 9);
+  }
   DCHK(hipGetLastError());
   dsync(d);
   d->hA.swap(hn);
   d->hB.swap(hn2);
+  if (d->tan_on) {
+    d->thA.swap(tn);
+    d->thB.swap(tn2);
+  }
   d->epochs.push_back(std::move(E));
   d->rebuild_steps.push_back(step);
   d->bout.reserve((size_t)std::max(d->nb, 1) * 9);
   d->cout.reserve(c1 * 9);
+  if (d->tan_on) d->bt.reserve((size_t)std::max(d->nb, 1) * 9);
 }
 
 template <class T>
@@ -863,35 +1035,57 @@ void upload_initial(esp_diff* d)
 }
 
 // one forward step (no search): forces, update, sample
-void fwd_step(esp_diff* d, const Epoch& E, int step, bool sample, int sample_idx)
+// dig: the step's branch-digest word (nullptr: not recorded)
+void fwd_step(esp_diff* d, const Epoch& E, int step, bool sample, int sample_idx,
+              unsigned long long* dig)
 {
   const int bs = d->bs;
   DState S = d->state();
   DBonds B{d->nb, d->bij.p, d->bD0.p, d->brel.p, d->bstep.p, d->tkb.p};
   DCands C{E.nc, E.ij.p, E.bond.p};
+  const DTan T = d->tan_on ? d->tan() : DTan{nullptr, nullptr, nullptr, nullptr};
+  if (d->tan_on) {  // tangents from the state before the step (before the forward kernels)
+    if (d->nb > 0)
+      hipLaunchKernelGGL(k_bond_tan, dim3(nblocks(d->nb, bs)), dim3(bs), 0, d->stream, S, T, B,
+                         d->theta.p, d->dtheta.p, step, d->bt.p);
+    if (E.nc > 0)
+      hipLaunchKernelGGL(k_fric_tan, dim3(nblocks(E.nc, bs)), dim3(bs), 0, d->stream, S, T, C,
+                         d->bstep.p, d->theta.p, d->dtheta.p, d->hA.p, d->thA.p, d->thB.p, step,
+                         d->cfg.dt, d->ct.p);
+  }
   if (d->nb > 0)
     hipLaunchKernelGGL(k_bond_fwd, dim3(nblocks(d->nb, bs)), dim3(bs), 0, d->stream, S, B,
                        d->theta.p, step, d->bout.p);
   if (E.nc > 0)
     hipLaunchKernelGGL(k_fric_fwd, dim3(nblocks(E.nc, bs)), dim3(bs), 0, d->stream, S, C, d->bstep.p,
-                       d->theta.p, d->hA.p, d->hB.p, step, d->cfg.dt, d->cout.p);
+                       d->theta.p, d->hA.p, d->hB.p, step, d->cfg.dt, d->cout.p, dig);
   const double a = 0.5 * d->cfg.verlet_dist;
   const double sr = a * a;
   unsigned long long thresh;
   memcpy(&thresh, &sr, 8);
   hipLaunchKernelGGL(k_gather_fwd, dim3(nblocks(d->n, bs)), dim3(bs), 0, d->stream, S, d->o.p,
                      E.inc_off.p, E.inc_ent.p, E.nc, d->cout.p, d->bout.p, d->step_params(step),
-                     sample ? d->wfn.p : nullptr, d->md.p, thresh);
-  if (sample)
+                     sample ? d->wfn.p : nullptr, d->md.p, thresh, dig, T, d->ct.p, d->bt.p,
+                     (sample && d->tan_on) ? d->wft.p : nullptr);
+  if (sample) {
     hipLaunchKernelGGL(k_spec_sum, dim3(d->nspec), dim3(256), 0, d->stream, d->wfn.p,
                        d->spec_start.p, d->stress_dev.p + (size_t)sample_idx * d->nspec);
+    if (d->tan_on)
+      hipLaunchKernelGGL(k_spec_sum, dim3(d->nspec), dim3(256), 0, d->stream, d->wft.p,
+                         d->spec_start.p, d->stress_t.p + (size_t)sample_idx * d->nspec);
+  }
   DCHK(hipGetLastError());
   d->hA.swap(d->hB);
+  if (d->tan_on) d->thA.swap(d->thB);
 }
 
 int nsamples(const esp_diff* d) { return d->nsteps / d->every; }
 
-void forward(esp_diff* d, const double* theta, const double* target, double* stress, double* loss)
+// dtheta (nullable): a parameter direction per specimen; the run then also
+// carries the state's tangent along it and dloss gets d loss / d epsilon at
+// theta + epsilon dtheta
+void forward(esp_diff* d, const double* theta, const double* target, double* stress, double* loss,
+             const double* dtheta = nullptr, double* dloss = nullptr)
 {
   if (d->n <= 0 || d->nsteps <= 0) throw DiffError(ESP_ERR_STATE, "particles and run not set");
   hipEvent_t e0, e1;
@@ -908,9 +1102,27 @@ void forward(esp_diff* d, const double* theta, const double* target, double* str
   DCHK(hipMemcpyAsync(d->theta.p, th.data(), sizeof(double4) * d->nspec, hipMemcpyHostToDevice,
                       d->stream));
   d->stress_dev.reserve((size_t)std::max(ns, 1) * d->nspec);
+  d->digest.reserve((size_t)d->nsteps + 1);
+  DCHK(hipMemsetAsync(d->digest.p, 0, sizeof(unsigned long long) * (d->nsteps + 1), d->stream));
   d->epochs.clear();
   d->ckpts.clear();
   d->rebuild_steps.clear();
+  d->tan_on = dtheta != nullptr;
+  if (d->tan_on) {
+    const size_t c = std::max(d->n, 1);
+    d->tx.reserve(c); d->tv.reserve(c); d->tw.reserve(c); d->tq.reserve(c);
+    for (DevBuf<double4>* b : {&d->tx, &d->tv, &d->tw, &d->tq})
+      DCHK(hipMemsetAsync(b->p, 0, sizeof(double4) * c, d->stream));  // the start state is fixed
+    d->wft.reserve(c);
+    d->bt.reserve((size_t)std::max(d->nb, 1) * 9);
+    d->stress_t.reserve((size_t)std::max(ns, 1) * d->nspec);
+    d->dtheta.reserve(d->nspec);
+    std::vector<double4> dt4(d->nspec);
+    for (int s = 0; s < d->nspec; s++)
+      dt4[s] = make_double4(dtheta[4 * s], dtheta[4 * s + 1], dtheta[4 * s + 2], dtheta[4 * s + 3]);
+    DCHK(hipMemcpyAsync(d->dtheta.p, dt4.data(), sizeof(double4) * d->nspec, hipMemcpyHostToDevice,
+                        d->stream));
+  }
   upload_initial(d);
   const int K = std::max(1, d->cfg.checkpoint_every);
   bool need = true;
@@ -919,7 +1131,7 @@ void forward(esp_diff* d, const double* theta, const double* target, double* str
     if ((step - 1) % K == 0) save_checkpoint(d, step);
     const bool sample = step % d->every == 0;
     DCHK(hipMemsetAsync(d->md.p, 0, sizeof(unsigned long long), d->stream));
-    fwd_step(d, *d->epochs.back(), step, sample, step / d->every - 1);
+    fwd_step(d, *d->epochs.back(), step, sample, step / d->every - 1, d->digest.p + step);
     unsigned long long m = 0;
     DCHK(hipMemcpyAsync(&m, d->md.p, sizeof m, hipMemcpyDeviceToHost, d->stream));
     dsync(d);
@@ -928,6 +1140,11 @@ void forward(esp_diff* d, const double* theta, const double* target, double* str
   d->h_stress.assign((size_t)ns * d->nspec, 0.0);
   DCHK(hipMemcpyAsync(d->h_stress.data(), d->stress_dev.p, sizeof(double) * ns * d->nspec,
                       hipMemcpyDeviceToHost, d->stream));
+  if (d->tan_on) {
+    d->h_stress_t.assign((size_t)ns * d->nspec, 0.0);
+    DCHK(hipMemcpyAsync(d->h_stress_t.data(), d->stress_t.p, sizeof(double) * ns * d->nspec,
+                        hipMemcpyDeviceToHost, d->stream));
+  }
   DCHK(hipEventRecord(e1, d->stream));
   dsync(d);
   float ms = 0;
@@ -944,7 +1161,17 @@ void forward(esp_diff* d, const double* theta, const double* target, double* str
       acc = acc + df * df;
     }
     loss[s] = ns > 0 ? acc / ns : 0.0;
+    if (d->tan_on && dloss) {
+      double dacc = 0.0;
+      for (int k = 0; k < ns; k++) {
+        const double st = d->h_stress[(size_t)k * d->nspec + s] / d->h_area[s];
+        const double dst = d->h_stress_t[(size_t)k * d->nspec + s] / d->h_area[s];
+        dacc = dacc + 2.0 * (st - d->h_target[(size_t)s * ns + k]) * dst;
+      }
+      dloss[s] = ns > 0 ? dacc / ns : 0.0;
+    }
   }
+  d->tan_on = false;
   d->have_forward = true;
 }
 
@@ -1019,7 +1246,7 @@ void backward(esp_diff* d, double* grad)
       if (!htape[t]) htape[t] = std::make_unique<DevBuf<double4>>();
       htape[t]->reserve(std::max(nc, 1));
       d2d(d, htape[t]->p, d->hA.p, nc);
-      fwd_step(d, *d->epochs[cur], step, false, 0);
+      fwd_step(d, *d->epochs[cur], step, false, 0, nullptr);
     }
     {
       double4* T = tape.p + (size_t)(s1 - s0) * 4 * c;
@@ -1353,6 +1580,17 @@ int esp_diff_forward(esp_diff* d, const double* theta, const double* target, dou
   DAPI_END
 }
 
+int esp_diff_forward_tangent(esp_diff* d, const double* theta, const double* dtheta,
+                             const double* target, double* stress, double* loss, double* dloss)
+{
+  DAPI_BEGIN
+  if (!theta || !dtheta || !target || !loss || !dloss) throw DiffError(ESP_ERR_ARG, "null argument");
+  if (d->h_area.empty()) throw DiffError(ESP_ERR_STATE, "no measured wall");
+  forward(d, theta, target, stress, loss, dtheta, dloss);
+  return ESP_OK;
+  DAPI_END
+}
+
 int esp_diff_backward(esp_diff* d, double* grad)
 {
   DAPI_BEGIN
@@ -1409,6 +1647,19 @@ int esp_diff_get_rebuild_steps(esp_diff* d, int* steps)
   if (steps)
     for (size_t k = 0; k < d->rebuild_steps.size(); k++) steps[k] = d->rebuild_steps[k];
   return (int)d->rebuild_steps.size();
+  DAPI_END
+}
+
+int esp_diff_get_branch_digest(esp_diff* d, unsigned long long* out)
+{
+  DAPI_BEGIN
+  if (!d->have_forward) throw DiffError(ESP_ERR_STATE, "no forward run");
+  if (out) {
+    DCHK(hipMemcpyAsync(out, d->digest.p + 1, sizeof(unsigned long long) * d->nsteps,
+                        hipMemcpyDeviceToHost, d->stream));
+    dsync(d);
+  }
+  return d->nsteps;
   DAPI_END
 }
 

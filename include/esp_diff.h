@@ -100,6 +100,20 @@ int esp_diff_get_particle_data(esp_diff* d, int field, double* out);
 int esp_diff_get_bond_breaks(esp_diff* d, int* step, int* i, int* j);
 /* Steps at which the neighbour table was rebuilt in the last forward run. */
 int esp_diff_get_rebuild_steps(esp_diff* d, int* steps);
+/* esp_diff_forward along a parameter direction: also carries the tangent of
+ * the state (forward mode, on the branch the run takes) and returns
+ * dloss[s] = d loss_s / d eps at theta + eps * dtheta (dtheta [nspec][4]).
+ * An independent derivative of the same forward run, for checking the
+ * adjoint (grad . dtheta); no reference counterpart. */
+int esp_diff_forward_tangent(esp_diff* d, const double* theta, const double* dtheta,
+                             const double* target, double* stress, double* loss, double* dloss);
+/* Branch digest of each step of the last forward run (steps 1..nsteps into
+ * out[0..nsteps-1]): a hash sum of every contact's touching and slipping state
+ * and every particle's wall contacts.  Runs whose digests and bond breaks are
+ * equal took the same branches, so their loss is differentiable between them.
+ * Returns nsteps (out NULL: count only).  No reference counterpart: it serves
+ * the finite-difference checks of the adjoint. */
+int esp_diff_get_branch_digest(esp_diff* d, unsigned long long* out);
 /* Device time (ms) of the last forward and backward runs. */
 int esp_diff_get_times(esp_diff* d, double* forward_ms, double* backward_ms);
 
