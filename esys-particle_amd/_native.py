@@ -140,6 +140,12 @@ SIGNATURES = {
 KERNEL_CLASSES = ("forces", "gather_integrate")
 
 
+# entry points of the HIP library only (the oracle has no counterpart)
+HIP_ONLY_SIGNATURES = {
+    "apply_force_to_wall_noalloc": (_i, [_vp, _s, _pd]),
+}
+
+
 class NativeLib:
     """Function table over one shared library exporting the engine C-ABI."""
 
@@ -167,7 +173,7 @@ def load_hip_library():
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(
                 "libesp.so is not built (expected %s); run __graft_entry__.build()" % LIB_PATH)
-        _HIP_LIB = NativeLib(LIB_PATH, "esp_")
+        _HIP_LIB = NativeLib(LIB_PATH, "esp_", HIP_ONLY_SIGNATURES)
     return _HIP_LIB
 
 
@@ -355,6 +361,11 @@ class Engine:
     def move_wall_by(self, name, d):
         d = np.asarray(d, np.float64)
         self._check(self.lib.move_wall_by(self.h, _b(name), _dp(d)), "moveWallBy")
+
+    def apply_force_to_wall_noalloc(self, name, f):
+        f = np.ascontiguousarray(f, np.float64)
+        self._check(self.lib.apply_force_to_wall_noalloc(self.h, _b(name), _dp(f)),
+                    "applyForceToWall (no allocation)")
 
     def apply_force_to_wall(self, name, f):
         f = np.ascontiguousarray(f, np.float64)

@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <array>
 #include <fstream>
+#include <mutex>
 #include <sstream>
 #include <functional>
 #include <cmath>
@@ -160,6 +161,8 @@ inline long long pkey(int a, int b)
 }
 
 thread_local std::string g_create_err;
+
+void* scratch_on(int dev);  // the allocation-free servo's staging (defined below)
 
 }  // namespace
 
@@ -2200,6 +2203,12 @@ int esp_create(const esp_config* cfg, esp_engine** out)
     delete e;
     return ESP_ERR_DEVICE;
   }
+  // the allocation-free servo's staging address on this device, looked up
+  // here with the device current (a failure is reported by that entry point)
+  try {
+    (void)scratch_on(cfg->device);
+  } catch (const EspError&) {
+  }
   *out = e;
   return ESP_OK;
 }
@@ -2573,16 +2582,26 @@ namespace {
 // CBWallInteractionGroup::applyForce (BWallInteractionGroup.hpp:71-145).  The
 // candidates come from the device; the iteration runs here over them in the
 // reference's list order (particle creation order), with its sums from zero.
+// a wall candidate: creation rank, flags, position, initial position
+struct WallCand {
+  long long order;
+  unsigned flags;
+  double4 pos, init;
+};
+
+void servo_iterate(esp_engine* e, int k, const double Fa[3], const WallCand* cs, int m);
+
+struct CandRange {
+  const WallCand* p;
+  int m;
+  const WallCand* begin() const { return p; }
+  const WallCand* end() const { return p + m; }
+};
+
 void apply_force_to_wall(esp_engine* e, int k, const double Fa[3])
 {
   const WallGroup& G = e->wg[e->word[k]];
-  Wall& W = e->walls[G.wall];
-  struct Cand {
-    long long order;
-    unsigned flags;
-    double4 pos, init;
-  };
-  std::vector<Cand> cs;
+  std::vector<WallCand> cs;
   if (e->started && e->n > 0) {
     const int n = e->n;
     DBuf<int> cnt, slot;
@@ -2594,8 +2613,8 @@ void apply_force_to_wall(esp_engine* e, int k, const double Fa[3])
     pos.reserve(n);
     if (G.bonded) ipos.reserve(n);
     zero(e, cnt.p, 1);
-    launch_wall_cands(e->stream, e->view(), k, G.bonded ? e->init.p : nullptr, cnt.p, slot.p, flg.p,
-                      pos.p, ipos.p);
+    launch_wall_cands(e->stream, e->view(), k, G.bonded ? e->init.p : nullptr, n, cnt.p, slot.p,
+                      flg.p, pos.p, ipos.p);
     HIPCHK(hipGetLastError());
     int m = 0;
     d2h(e, &m, cnt.p, 1);
@@ -2610,9 +2629,19 @@ void apply_force_to_wall(esp_engine* e, int k, const double Fa[3])
     sync(e);
     cs.resize(m);
     for (int c = 0; c < m; c++)
-      cs[c] = Cand{e->h_order[hs[c]], hf[c], hp[c], G.bonded ? hi[c] : make_double4(0, 0, 0, 0)};
-    std::sort(cs.begin(), cs.end(), [](const Cand& a, const Cand& b) { return a.order < b.order; });
+      cs[c] = WallCand{e->h_order[hs[c]], hf[c], hp[c], G.bonded ? hi[c] : make_double4(0, 0, 0, 0)};
+    std::sort(cs.begin(), cs.end(),
+              [](const WallCand& a, const WallCand& b) { return a.order < b.order; });
   }
+  servo_iterate(e, k, Fa, cs.data(), (int)cs.size());
+}
+
+// the servo's iteration over the candidates in list order
+void servo_iterate(esp_engine* e, int k, const double Fa[3], const WallCand* cs, int m)
+{
+  using Cand = WallCand;
+  const WallGroup& G = e->wg[e->word[k]];
+  Wall& W = e->walls[G.wall];
   auto inner = [&](const Cand& c) { return ((c.flags >> (PF_INNER_SHIFT + k)) & 1u) != 0; };
   auto tagged = [&](const Cand& c) {
     return (G.bonded || G.tagged) && ((c.flags >> (PF_BTAG_SHIFT + k)) & 1u) != 0;
@@ -2644,11 +2673,11 @@ void apply_force_to_wall(esp_engine* e, int k, const double Fa[3])
     double d, df, ef;
     do {
       double kk = 0.0;
-      for (const Cand& c : cs)
+      for (const Cand& c : CandRange{cs, m})
         if ((!G.tagged || tagged(c)) && dist(c) < c.pos.w) kk += keff(c);
       if (kk > 0) {
         dv3 Fl = mk3(0.0, 0.0, 0.0);
-        for (const Cand& c : cs) {
+        for (const Cand& c : CandRange{cs, m}) {
           if ((G.tagged && !tagged(c)) || !inner(c)) continue;
           const dv3 f = elastic_force(c);
           Fl = add3(Fl, mul3(O_f, dot3(f, O_f)));
@@ -2666,9 +2695,9 @@ void apply_force_to_wall(esp_engine* e, int k, const double Fa[3])
     return;
   }
   double K = 0.0;
-  for (const Cand& c : cs)
+  for (const Cand& c : CandRange{cs, m})
     if (tagged(c) && inner(c)) K += kbond(c);
-  for (const Cand& c : cs)
+  for (const Cand& c : CandRange{cs, m})
     if (!tagged(c) && inner(c) && dist(c) < c.pos.w) K += keff(c);
   if (!(K > 0)) throw EspError(ESP_ERR_NUMERIC, "wall group '" + G.name + "' has no stiffness to servo");
   int it = 0;
@@ -2676,13 +2705,13 @@ void apply_force_to_wall(esp_engine* e, int k, const double Fa[3])
   do {
     const dv3 wdi = mk3(W.o[0] - W.old[0], W.o[1] - W.old[1], W.o[2] - W.old[2]);
     dv3 Fl = mk3(0.0, 0.0, 0.0);
-    for (const Cand& c : cs) {
+    for (const Cand& c : CandRange{cs, m}) {
       if (!tagged(c) || !inner(c)) continue;
       const dv3 D = sub3(sub3(mk3(c.pos.x, c.pos.y, c.pos.z), mk3(c.init.x, c.init.y, c.init.z)), wdi);
       const dv3 f = mul3(D, kbond(c));
       Fl = add3(Fl, mul3(O_f, dot3(f, O_f)));
     }
-    for (const Cand& c : cs) {
+    for (const Cand& c : CandRange{cs, m}) {
       if (tagged(c) || !inner(c)) continue;
       const dv3 f = elastic_force(c);
       Fl = add3(Fl, mul3(O_f, dot3(f, O_f)));
@@ -2693,6 +2722,103 @@ void apply_force_to_wall(esp_engine* e, int k, const double Fa[3])
   } while ((it < 10) && (std::fabs(d) > 10e-6));
 }
 }  // namespace
+
+namespace {
+// ---------------------------------------------------------------------------
+// applyForceToWall without allocation: the candidates are staged in a
+// file-scope device array (one copy per GPU, held until the process ends) and
+// in file-scope host arrays, instead of buffers allocated per call.  Calls
+// that overlap on one GPU (or in one process) race on these arrays.
+// ---------------------------------------------------------------------------
+constexpr size_t NA_CNT = 0;  // candidate counter (one int)
+constexpr size_t NA_SLOT = 256;
+constexpr size_t NA_FLG = NA_SLOT + sizeof(int) * ESP_NOALLOC_MAX_WALL_CANDIDATES;
+constexpr size_t NA_POS = NA_FLG + sizeof(unsigned) * ESP_NOALLOC_MAX_WALL_CANDIDATES;
+constexpr size_t NA_IPOS = NA_POS + sizeof(double4) * ESP_NOALLOC_MAX_WALL_CANDIDATES;
+constexpr size_t NA_BYTES = NA_IPOS + sizeof(double4) * ESP_NOALLOC_MAX_WALL_CANDIDATES;
+static_assert(NA_POS % 32 == 0 && NA_IPOS % 32 == 0, "double4 staging must stay aligned");
+
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+constexpr int NA_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[NA_MAX_DEVICES] = {};
+
+// the device address of `scratch` on `dev` (the current device), looked up
+// once per device; host code never passes the symbol itself to a kernel
+void* scratch_on(int dev)
+{
+  if (dev < 0 || dev >= NA_MAX_DEVICES)
+    throw EspError(ESP_ERR_DEVICE, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    const hipError_t rc = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (rc != hipSuccess || !ptr)
+      throw EspError(ESP_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") +
+                                         hipGetErrorString(rc));
+    g_scratch_ptr[dev] = ptr;
+  }
+  return g_scratch_ptr[dev];
+}
+
+int h_na_slot[ESP_NOALLOC_MAX_WALL_CANDIDATES];
+unsigned h_na_flg[ESP_NOALLOC_MAX_WALL_CANDIDATES];
+double4 h_na_pos[ESP_NOALLOC_MAX_WALL_CANDIDATES];
+double4 h_na_ipos[ESP_NOALLOC_MAX_WALL_CANDIDATES];
+WallCand h_na_cand[ESP_NOALLOC_MAX_WALL_CANDIDATES];
+
+void apply_force_to_wall_noalloc(esp_engine* e, int k, const double Fa[3])
+{
+  const WallGroup& G = e->wg[e->word[k]];
+  int m = 0;
+  if (e->started && e->n > 0) {
+    unsigned char* base = static_cast<unsigned char*>(scratch_on(e->cfg.device));
+    int* cnt = reinterpret_cast<int*>(base + NA_CNT);
+    int* slot = reinterpret_cast<int*>(base + NA_SLOT);
+    unsigned* flg = reinterpret_cast<unsigned*>(base + NA_FLG);
+    double4* pos = reinterpret_cast<double4*>(base + NA_POS);
+    double4* ipos = reinterpret_cast<double4*>(base + NA_IPOS);
+    launch_fill_int(e->stream, 1, cnt, 0);  // the last call left its count
+    launch_wall_cands(e->stream, e->view(), k, G.bonded ? e->init.p : nullptr,
+                      ESP_NOALLOC_MAX_WALL_CANDIDATES, cnt, slot, flg, pos, ipos);
+    HIPCHK(hipGetLastError());
+    d2h(e, &m, cnt, 1);
+    sync(e);
+    if (m > ESP_NOALLOC_MAX_WALL_CANDIDATES)
+      throw EspError(ESP_ERR_CAPACITY,
+                     "wall group '" + G.name + "' has " + std::to_string(m) +
+                         " candidate particles; the allocation-free servo holds at most " +
+                         std::to_string(ESP_NOALLOC_MAX_WALL_CANDIDATES));
+    d2h(e, h_na_slot, slot, m);
+    d2h(e, h_na_flg, flg, m);
+    d2h(e, h_na_pos, pos, m);
+    if (G.bonded) d2h(e, h_na_ipos, ipos, m);
+    sync(e);
+    for (int c = 0; c < m; c++)
+      h_na_cand[c] = WallCand{e->h_order[h_na_slot[c]], h_na_flg[c], h_na_pos[c],
+                              G.bonded ? h_na_ipos[c] : make_double4(0, 0, 0, 0)};
+    std::sort(h_na_cand, h_na_cand + m,
+              [](const WallCand& a, const WallCand& b) { return a.order < b.order; });
+  }
+  servo_iterate(e, k, Fa, h_na_cand, m);
+}
+}  // namespace
+
+int esp_apply_force_to_wall_noalloc(esp_engine* e, const char* name, const double F[3])
+{
+  API_BEGIN
+  if (!name || !F) throw EspError(ESP_ERR_ARG, "bad arguments");
+  if (e->word.size() != e->wg.size()) refresh_orders(e);  // fixed once the run started
+  int k = -1;
+  for (size_t i = 0; i < e->word.size(); i++)
+    if (e->wg[e->word[i]].name == name) k = (int)i;
+  if (k < 0) throw EspError(ESP_ERR_ARG, std::string("unknown wall interaction group '") + name + "'");
+  HIPCHK(hipSetDevice(e->cfg.device));
+  apply_force_to_wall_noalloc(e, k, F);
+  return ESP_OK;
+  API_END
+}
 
 int esp_apply_force_to_wall(esp_engine* e, const char* name, const double F[3])
 {

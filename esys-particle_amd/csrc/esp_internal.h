@@ -294,9 +294,12 @@ void launch_pairs(hipStream_t s, int mode_emit, int flagged, PView p, GridParams
                   const int* cell, const int* cell_start, const int* cell_items, int* cnt,
                   const int* off, int* out_p, int* out_q, long long* out_key);
 void launch_clear_flag_bit(hipStream_t s, PView p, UpdState* st);
-// candidates of wall group w (any order): slots, flags, positions, initial positions
-void launch_wall_cands(hipStream_t s, PView p, int w, const double4* init, int* cnt, int* slot,
-                       unsigned* flg, double4* pos, double4* ipos);
+// candidates of wall group w (any order): slots, flags, positions, initial
+// positions; *cnt counts all, the first cap are stored
+void launch_wall_cands(hipStream_t s, PView p, int w, const double4* init, int cap, int* cnt,
+                       int* slot, unsigned* flg, double4* pos, double4* ipos);
+// a[0..n) = v (a kernel, not a memset: safe under stream capture)
+void launch_fill_int(hipStream_t s, int n, int* a, int v);
 // erasure of particles outside the padded grid (NeighborTable::build):
 // keep[i] = cell[i] >= 0; nslot[i] = new slot or -1; stable compaction of a
 // per-particle field of `width` elements; interaction pairs remapped to the new

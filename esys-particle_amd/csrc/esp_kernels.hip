@@ -930,8 +930,9 @@ __global__ void k_pack_dyn(int nc, const int* keep, const int* off, const int2* 
 }
 
 // the candidate particles of wall group w (flag bit PF_WALL_SHIFT + w): slot,
-// flags, position and (bonded walls) initial position, in any order
-__global__ void k_wall_cands(PView p, int w, const double4* init, int* cnt, int* slot,
+// flags, position and (bonded walls) initial position, in any order; all are
+// counted, the first cap are stored
+__global__ void k_wall_cands(PView p, int w, const double4* init, int cap, int* cnt, int* slot,
                              unsigned* flg, double4* pos, double4* ipos)
 {
   const int i = blockIdx.x * BS + threadIdx.x;
@@ -939,6 +940,7 @@ __global__ void k_wall_cands(PView p, int w, const double4* init, int* cnt, int*
   const unsigned f = p.flags[i];
   if (!((f >> (PF_WALL_SHIFT + w)) & 1u)) return;
   const int k = atomicAdd(cnt, 1);
+  if (k >= cap) return;
   slot[k] = i;
   flg[k] = f;
   pos[k] = p.pos[i];
@@ -1793,12 +1795,18 @@ void launch_pack_dyn(hipStream_t s, int nc, const int* keep, const int* off, con
                      off_);
 }
 
-void launch_wall_cands(hipStream_t s, PView p, int w, const double4* init, int* cnt, int* slot,
-                       unsigned* flg, double4* pos, double4* ipos)
+void launch_wall_cands(hipStream_t s, PView p, int w, const double4* init, int cap, int* cnt,
+                       int* slot, unsigned* flg, double4* pos, double4* ipos)
 {
   if (p.n <= 0) return;
-  hipLaunchKernelGGL(k_wall_cands, dim3(nblk(p.n)), dim3(BS), 0, s, p, w, init, cnt, slot, flg,
-                     pos, ipos);
+  hipLaunchKernelGGL(k_wall_cands, dim3(nblk(p.n)), dim3(BS), 0, s, p, w, init, cap, cnt, slot,
+                     flg, pos, ipos);
+}
+
+void launch_fill_int(hipStream_t s, int n, int* a, int v)
+{
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_fill_int, dim3(nblk(n)), dim3(BS), 0, s, n, a, v);
 }
 
 void launch_clear_flag_bit(hipStream_t s, PView p, UpdState* st)

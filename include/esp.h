@@ -251,6 +251,18 @@ int esp_move_wall_by(esp_engine* e, const char* name, const double d[3]);
  * it from a pre-time-step runnable.  ESP_ERR_NUMERIC for a bonded group without
  * stiffness (the reference would move its wall by an infinite distance). */
 int esp_apply_force_to_wall(esp_engine* e, const char* name, const double force[3]);
+/* Maximum candidate particles of one wall group for esp_apply_force_to_wall_noalloc. */
+#define ESP_NOALLOC_MAX_WALL_CANDIDATES 262144
+/* esp_apply_force_to_wall without any device allocation: no counterpart in the
+ * reference.  For callers that cannot allocate (a servo calling it every time
+ * step inside a tight loop, code under stream capture, harnesses that forbid
+ * hipMalloc): esp_apply_force_to_wall allocates and frees its candidate
+ * staging on every call; this variant stages them in a fixed file-scope device
+ * array (about 19 MB per GPU, held until the process ends) and file-scope host
+ * arrays.  Same result bit for bit.  ESP_ERR_CAPACITY when the group has more
+ * than ESP_NOALLOC_MAX_WALL_CANDIDATES candidates.  Calls that overlap on one
+ * GPU, or run concurrently in one process, race on the shared staging. */
+int esp_apply_force_to_wall_noalloc(esp_engine* e, const char* name, const double force[3]);
 /* A pre-time-step runnable that calls moveWallBy(name, d) at the start of every
  * step (compbench.py:20-35 pattern, LatticeMaster.cpp:2647).  d = 0 removes it. */
 int esp_set_wall_step_motion(esp_engine* e, const char* name, const double d[3]);
