@@ -23,6 +23,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 HBM_PEAK_GBS = 8000.0
 
@@ -87,11 +89,12 @@ def cpu_baseline(pkg, spm, spec, budget_s):
     sim = orc.oracle_lsm_class(pkg)()
     spm.apply(sim, spec)
     sim.runTimeSteps(1)  # first step: initial neighbour search, excluded
+    chunk = 10 if sim.getNumParticles() <= 200000 else 1
     steps, psteps, t0 = 0, 0, time.perf_counter()
     while time.perf_counter() - t0 < budget_s:
-        sim.runTimeSteps(10)
-        steps += 10
-        psteps += 10 * sim.getNumParticles()
+        sim.runTimeSteps(chunk)
+        steps += chunk
+        psteps += chunk * sim.getNumParticles()
     el = time.perf_counter() - t0
     return {"value": psteps / el, "unit": "particle-steps/s", "cores": 1, "kind": "port",
             "sample": "steps 2-%d of the same specimen, %d particles at the end (%.1f s)" % (
@@ -179,10 +182,13 @@ def main():
                     help="npz file: load the specimen from it if present, else build and save it")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device 10 times (counter calibration)")
-    ap.add_argument("--workload", default="compression", choices=["compression", "settling"],
+    ap.add_argument("--workload", default="compression",
+                    choices=["compression", "settling", "shearcell"],
                     help="compression: the bonded 100k-particle compression (default); "
                          "settling: 10,000 NRotSphere settling under gravity on an elastic "
-                         "floor (NRotElastic k=1, dt=1e-4, the reference's CPU-sized case)")
+                         "floor (NRotElastic k=1, dt=1e-4, the reference's CPU-sized case); "
+                         "shearcell: 2,000,000 RotSphere frictional shear cell, periodic in "
+                         "x, rough walls sliding at +-0.1, servo-held confining stress 1.0")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the engine holds one subdomain on one device "
@@ -203,9 +209,16 @@ def main():
     spm = importlib.import_module("esys_particle_amd.specimens")
     t0 = time.perf_counter()
     settling = args.workload == "settling"
+    shear = args.workload == "shearcell"
     spec = load_specimen(args.specimen) if args.specimen and os.path.exists(args.specimen) else None
     if spec is None:
-        if settling:
+        if shear:
+            # 2,000,000 spheres, radii U[0.25, 0.75]: the solid volume (1.3e6) exceeds the
+            # stated 200x40x100 box, so the box keeps its aspect at solid fraction 0.58
+            # (grow-and-relax packing, periodic in x)
+            spec = spm.shear_cell(n=args.particles if args.particles != 100000 else 2000000,
+                                  seed=11, phi=0.58, packing="dense")
+        elif settling:
             # the soft floor (k=1) lets the lowest particles sink through it and
             # out of the neighbour table, which erases them, as the reference does
             spec = spm.gravity_settling(n=10000, seed=1, box=(20.0, 40.0, 20.0), dt=1e-4)
@@ -219,7 +232,7 @@ def main():
     sim = pkg.LsmMpi(device=0, stepsPerBatch=args.batch)
     spm.apply(sim, spec)
     eng = sim.engine
-    cond = condition(sim, eng, args.condition, settling)
+    cond = condition(sim, eng, args.condition, settling or shear)
     t0 = time.perf_counter()
     sim.runTimeSteps(args.warmup)
     log("warmup %d steps: %.2f s, stats %s" % (args.warmup, time.perf_counter() - t0, eng.stats()))
@@ -240,6 +253,7 @@ def main():
         touch = None
         if not settling:
             _, fl = eng.friction_state("friction")
+            fl = np.asarray(fl)
             touch = int(((fl >> 1) & 1).sum())
         return st, touch
     s1b, t_a = units()
@@ -282,6 +296,15 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": ({
+            "workload": "frictional shear cell, %d RotSphere, radii U[0.25, 0.75], periodic x, "
+                        "RotFriction kn=ks=1e4 mu=0.5, rough bonded walls sliding at +-0.1, "
+                        "servo confining stress 1.0, dt=5e-5" % n,
+            "particles": n,
+            "friction_pairs": int(s1["num_dynamic"]),
+            "touching_pairs": None if ntouch is None else int(ntouch),
+            "box": [round(b, 3) for b in spec["box"]],
+            "steps_per_sync": 1,
+        } if shear else {
             "workload": "gravity settling, %d NRotSphere, radii U[0.2, 0.5], density 1, "
                         "NRotElastic k=1, elastic floor k=1, g=9.81, dt=1e-4, box "
                         "20x40x20" % n,

@@ -139,7 +139,49 @@ struct DStep {
   double g[3];
   double dt;
   int step;
+  const double* wo;  // wall origins per step [step][wall][3] (forward), or wall[].o
 };
+
+__device__ __forceinline__ dv3 wall_origin(const DStep& sp, int k, int step)
+{
+  if (sp.wo) {
+    const double* o = sp.wo + ((size_t)step * sp.nw + k) * 3;
+    return mk3(o[0], o[1], o[2]);
+  }
+  return mk3(sp.wall[k].o[0], sp.wall[k].o[1], sp.wall[k].o[2]);
+}
+
+// Device-driven stepping: the kernels of a forward step read the step they
+// belong to from `step`; a step whose displacement test asks for a neighbour
+// search records itself in `halt`, and every later step of the queue (or of a
+// replayed graph) returns at once; steps past `end` return too.
+struct DRun {
+  int step, halt, end, pad;
+};
+
+__device__ __forceinline__ bool run_skip(const DRun* R, int& step)
+{
+  step = R->step;
+  const int h = R->halt;
+  return step > R->end || (h != 0 && h < step);
+}
+
+__global__ void k_set_run(DRun* R, int step, int halt, int end)
+{
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    R->step = step;
+    R->halt = halt;
+    R->end = end;
+  }
+}
+
+// the last launch of a step
+__global__ void k_advance(DRun* R)
+{
+  int step;
+  if (blockIdx.x != 0 || threadIdx.x != 0 || run_skip(R, step)) return;
+  R->step = step + 1;
+}
 
 struct DBonds {
   int nb;
@@ -181,8 +223,10 @@ __device__ __forceinline__ tquat<esp_dual> ddq(double4 a, double4 t)
 }
 
 __global__ void k_bond_tan(DState S, DTan T, DBonds B, const double4* theta, const double4* dtheta,
-                           int step, double* bt)
+                           const DRun* run, double* bt)
 {
+  int step;
+  if (run_skip(run, step)) return;
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= B.nb) return;
   double* o = bt + 9 * (size_t)b;
@@ -210,11 +254,17 @@ __global__ void k_bond_tan(DState S, DTan T, DBonds B, const double4* theta, con
 }
 
 __global__ void k_fric_tan(DState S, DTan T, DCands C, const int* bstep, const double4* theta,
-                           const double4* dtheta, const double4* hin, const double4* thin,
-                           double4* thout, int step, double dt, double* ct)
+                           const double4* dtheta, const double4* H0, const double4* H1,
+                           double4* TH0, double4* TH1, const DRun* run, double dt, double* ct)
 {
+  int step;
+  if (run_skip(run, step)) return;
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= C.nc) return;
+  // step s reads the histories of buffer s & 1 and writes the other
+  const double4* hin = (step & 1) ? H1 : H0;
+  const double4* thin = (step & 1) ? TH1 : TH0;
+  double4* thout = (step & 1) ? TH0 : TH1;
   double* o = ct + 9 * (size_t)c;
   const int bb = C.bond[c];
   if (bb >= 0 && bstep[bb] >= step) {
@@ -244,8 +294,10 @@ __global__ void k_fric_tan(DState S, DTan T, DCands C, const int* bstep, const d
 // ---------------------------------------------------------------------------
 // forward kernels
 // ---------------------------------------------------------------------------
-__global__ void k_bond_fwd(DState S, DBonds B, const double4* theta, int step, double* bout)
+__global__ void k_bond_fwd(DState S, DBonds B, const double4* theta, const DRun* run, double* bout)
 {
+  int step;
+  if (run_skip(run, step)) return;
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= B.nb) return;
   double* o = bout + 9 * (size_t)b;
@@ -284,11 +336,17 @@ __global__ void k_bond_fwd(DState S, DBonds B, const double4* theta, int step, d
 }
 
 __global__ void k_fric_fwd(DState S, DCands C, const int* bstep, const double4* theta,
-                           const double4* hin, double4* hout, int step, double dt, double* cout,
-                           unsigned long long* dig)
+                           double4* H0, double4* H1, const DRun* run, double dt, double* cout,
+                           unsigned long long* dig_base)
 {
+  int step;
+  if (run_skip(run, step)) return;
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= C.nc) return;
+  // step s reads the histories of buffer s & 1 and writes the other
+  const double4* hin = (step & 1) ? H1 : H0;
+  double4* hout = (step & 1) ? H0 : H1;
+  unsigned long long* dig = dig_base ? dig_base + step : nullptr;
   double* o = cout + 9 * (size_t)c;
   const int bb = C.bond[c];
   if (bb >= 0 && bstep[bb] >= step) {  // excluded while the bond lives
@@ -323,12 +381,18 @@ __global__ void k_fric_fwd(DState S, DCands C, const int* bstep, const double4* 
 // the total force and moment (integrate = 0 writes fm)
 __global__ void k_gather_fwd(DState S, const double4* mass_old, const int* inc_off,
                              const int* inc_ent, int ncand, const double* cout, const double* bout,
-                             DStep sp, double* wfn, unsigned long long* md,
-                             unsigned long long thresh, unsigned long long* dig, DTan T,
-                             const double* ct, const double* bt, double* wft)
+                             DStep sp, int every, double* wfn_s, DRun* run,
+                             unsigned long long thresh, unsigned long long* dig_base, DTan T,
+                             const double* ct, const double* bt, double* wft_s)
 {
+  int step;
+  if (run_skip(run, step)) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= S.n) return;
+  const bool sample = every > 0 && step % every == 0;
+  double* wfn = sample ? wfn_s : nullptr;
+  double* wft = sample ? wft_s : nullptr;
+  unsigned long long* dig = dig_base ? dig_base + step : nullptr;
   const double4 P = S.x[i];
   const dv3 x0 = mk3(P.x, P.y, P.z);
   const double r = P.w;
@@ -338,7 +402,7 @@ __global__ void k_gather_fwd(DState S, const double4* mass_old, const int* inc_o
   for (int k = 0; k < sp.nw; k++) {  // CElasticWallInteraction::calcForces
     const DWall& W = sp.wall[k];
     const dv3 n = mk3(W.n[0], W.n[1], W.n[2]);
-    const double dist = dot3(sub3(x0, mk3(W.o[0], W.o[1], W.o[2])), n);
+    const double dist = dot3(sub3(x0, wall_origin(sp, k, step)), n);
     if (dig) wave_digest(dig, ((unsigned long long)(i & ~63) << 3) | (unsigned long long)(k + 1),
                          __ballot(dist < r), 0ull);
     if (dist < r) {
@@ -428,13 +492,18 @@ __global__ void k_gather_fwd(DState S, const double4* mass_old, const int* inc_o
   S.q[i] = make_double4(qq.s, qq.v.x, qq.v.y, qq.v.z);
   const dv3 d = sub3(x, mk3(Om.x, Om.y, Om.z));
   const unsigned long long m = (unsigned long long)__double_as_longlong(dot3(d, d));
-  if (m > thresh) atomicMax(md, m);
+  if (m > thresh) atomicMax(&run->halt, step);  // a neighbour search before the next step
 }
 
-// per specimen: fixed-order sum of its particles' measured wall forces
-__global__ void k_spec_sum(const double* val, const int* spec_start, double* out)
+// per specimen: fixed-order sum of its particles' measured wall forces, at the
+// sample steps (the sum of sample k goes to out_base[k * nspec + s])
+__global__ void k_spec_sum(const DRun* run, int every, const double* val, const int* spec_start,
+                           int nspec, double* out_base)
 {
   __shared__ double red[1024];
+  int step;
+  if (run_skip(run, step) || every <= 0 || step % every != 0) return;
+  double* out = out_base + (size_t)(step / every - 1) * nspec;
   const int s = blockIdx.x;
   const int a = spec_start[s], b = spec_start[s + 1];
   double acc = 0.0;
@@ -811,7 +880,14 @@ struct esp_diff {
   std::vector<std::unique_ptr<Epoch>> epochs;
   std::vector<std::unique_ptr<Checkpoint>> ckpts;
   std::vector<int> rebuild_steps;
-  DevBuf<double4> hA, hB;  // shear histories (read / write)
+  DevBuf<double4> H[2];  // shear histories: step s reads H[s & 1], writes the other
+  DevBuf<DRun> run;
+  DevBuf<double> wo;     // wall origins per step of the run [step][wall][3]
+  // the forward steps of one epoch captured as a graph (device-driven steps)
+  hipGraphExec_t gexec = nullptr;
+  int g_epoch = -1, g_len = 0;
+  bool use_graph = true;
+  int graph_steps = 16;
   DevBuf<double> bout, cout, wfn, stress_dev;
   DevBuf<double4> theta;
   std::vector<double> h_theta, h_target, h_stress;
@@ -819,7 +895,7 @@ struct esp_diff {
   DevBuf<unsigned long long> digest;  // per step: branch digest of the last forward run
   // tangent-linear run (forward with a parameter direction)
   bool tan_on = false;
-  DevBuf<double4> tx, tv, tw, tq, thA, thB, dtheta;
+  DevBuf<double4> tx, tv, tw, tq, TH[2], dtheta;
   DevBuf<double> bt, ct, wft, stress_t;
   std::vector<double> h_stress_t;
   DTan tan() { return DTan{tx.p, tv.p, tw.p, tq.p}; }
@@ -939,32 +1015,32 @@ void rebuild(esp_diff* d, int step)
                      E->inc_ent.p);
   // shear histories carried by key (PIS_ED::update: persistent pairs keep theirs;
   // a pair that is not is apart and holds zero anyway)
-  DevBuf<double4> hn, hn2;
+  // (the step reads buffer step & 1)
+  const int cur = step & 1;
+  DevBuf<double4> hn;
   hn.reserve(c1);
-  hn2.reserve(c1);
   if (nc > 0) {
     hipLaunchKernelGGL(k_carry_hist, dim3(nblocks(nc, bs)), dim3(bs), 0, d->stream, nc, E->src.p,
-                       P ? d->hA.p : nullptr, hn.p);
+                       P ? d->H[cur].p : nullptr, hn.p);
     if (!P) DCHK(hipMemsetAsync(hn.p, 0, sizeof(double4) * nc, d->stream));
   }
-  DevBuf<double4> tn, tn2;
+  DevBuf<double4> tn;
   if (d->tan_on) {
     tn.reserve(c1);
-    tn2.reserve(c1);
     if (nc > 0) {
       hipLaunchKernelGGL(k_carry_hist, dim3(nblocks(nc, bs)), dim3(bs), 0, d->stream, nc, E->src.p,
-                         P ? d->thA.p : nullptr, tn.p);
+                         P ? d->TH[cur].p : nullptr, tn.p);
       if (!P) DCHK(hipMemsetAsync(tn.p, 0, sizeof(double4) * nc, d->stream));
     }
     d->ct.reserve(c1 * 9);
   }
   DCHK(hipGetLastError());
   dsync(d);
-  d->hA.swap(hn);
-  d->hB.swap(hn2);
+  d->H[cur].swap(hn);
+  d->H[cur ^ 1].reserve(c1);
   if (d->tan_on) {
-    d->thA.swap(tn);
-    d->thB.swap(tn2);
+    d->TH[cur].swap(tn);
+    d->TH[cur ^ 1].reserve(c1);
   }
   d->epochs.push_back(std::move(E));
   d->rebuild_steps.push_back(step);
@@ -990,7 +1066,7 @@ void save_checkpoint(esp_diff* d, int step)
   d2d(d, C->q.p, d->q.p, d->n); d2d(d, C->o.p, d->o.p, d->n);
   const int nc = d->epochs.back()->nc;
   C->hist.reserve(std::max(nc, 1));
-  d2d(d, C->hist.p, d->hA.p, nc);
+  d2d(d, C->hist.p, d->H[step & 1].p, nc);
   C->bstep.reserve(std::max(d->nb, 1));
   d2d(d, C->bstep.p, d->bstep.p, d->nb);
   d->ckpts.push_back(std::move(C));
@@ -1001,9 +1077,9 @@ void restore_checkpoint(esp_diff* d, const Checkpoint& C)
   d2d(d, d->x.p, C.x.p, d->n); d2d(d, d->v.p, C.v.p, d->n); d2d(d, d->w.p, C.w.p, d->n);
   d2d(d, d->q.p, C.q.p, d->n); d2d(d, d->o.p, C.o.p, d->n);
   const int nc = d->epochs[C.epoch]->nc;
-  d->hA.reserve(std::max(nc, 1));
-  d->hB.reserve(std::max(nc, 1));
-  d2d(d, d->hA.p, C.hist.p, nc);
+  d->H[0].reserve(std::max(nc, 1));
+  d->H[1].reserve(std::max(nc, 1));
+  d2d(d, d->H[C.step & 1].p, C.hist.p, nc);
   d2d(d, d->bstep.p, C.bstep.p, d->nb);
 }
 
@@ -1035,48 +1111,100 @@ void upload_initial(esp_diff* d)
 }
 
 // one forward step (no search): forces, update, sample
-// dig: the step's branch-digest word (nullptr: not recorded)
-void fwd_step(esp_diff* d, const Epoch& E, int step, bool sample, int sample_idx,
-              unsigned long long* dig)
+// The launches of one forward step (no search): tangents (tangent runs),
+// forces, update and sampling; the step index comes from d->run on the device
+// (k_advance ends the step), so the same launches serve every step and can be
+// captured in a graph.  record: branch digests into d->digest (forward runs);
+// halting: the displacement test may stop the queue (not while recomputing a
+// recorded segment).
+void fwd_step(esp_diff* d, const Epoch& E, bool record, bool halting)
 {
   const int bs = d->bs;
   DState S = d->state();
   DBonds B{d->nb, d->bij.p, d->bD0.p, d->brel.p, d->bstep.p, d->tkb.p};
   DCands C{E.nc, E.ij.p, E.bond.p};
+  DRun* R = d->run.p;
   const DTan T = d->tan_on ? d->tan() : DTan{nullptr, nullptr, nullptr, nullptr};
   if (d->tan_on) {  // tangents from the state before the step (before the forward kernels)
     if (d->nb > 0)
       hipLaunchKernelGGL(k_bond_tan, dim3(nblocks(d->nb, bs)), dim3(bs), 0, d->stream, S, T, B,
-                         d->theta.p, d->dtheta.p, step, d->bt.p);
+                         d->theta.p, d->dtheta.p, R, d->bt.p);
     if (E.nc > 0)
       hipLaunchKernelGGL(k_fric_tan, dim3(nblocks(E.nc, bs)), dim3(bs), 0, d->stream, S, T, C,
-                         d->bstep.p, d->theta.p, d->dtheta.p, d->hA.p, d->thA.p, d->thB.p, step,
-                         d->cfg.dt, d->ct.p);
+                         d->bstep.p, d->theta.p, d->dtheta.p, d->H[0].p, d->H[1].p, d->TH[0].p,
+                         d->TH[1].p, R, d->cfg.dt, d->ct.p);
   }
   if (d->nb > 0)
     hipLaunchKernelGGL(k_bond_fwd, dim3(nblocks(d->nb, bs)), dim3(bs), 0, d->stream, S, B,
-                       d->theta.p, step, d->bout.p);
+                       d->theta.p, R, d->bout.p);
   if (E.nc > 0)
     hipLaunchKernelGGL(k_fric_fwd, dim3(nblocks(E.nc, bs)), dim3(bs), 0, d->stream, S, C, d->bstep.p,
-                       d->theta.p, d->hA.p, d->hB.p, step, d->cfg.dt, d->cout.p, dig);
-  const double a = 0.5 * d->cfg.verlet_dist;
-  const double sr = a * a;
-  unsigned long long thresh;
-  memcpy(&thresh, &sr, 8);
-  hipLaunchKernelGGL(k_gather_fwd, dim3(nblocks(d->n, bs)), dim3(bs), 0, d->stream, S, d->o.p,
-                     E.inc_off.p, E.inc_ent.p, E.nc, d->cout.p, d->bout.p, d->step_params(step),
-                     sample ? d->wfn.p : nullptr, d->md.p, thresh, dig, T, d->ct.p, d->bt.p,
-                     (sample && d->tan_on) ? d->wft.p : nullptr);
-  if (sample) {
-    hipLaunchKernelGGL(k_spec_sum, dim3(d->nspec), dim3(256), 0, d->stream, d->wfn.p,
-                       d->spec_start.p, d->stress_dev.p + (size_t)sample_idx * d->nspec);
-    if (d->tan_on)
-      hipLaunchKernelGGL(k_spec_sum, dim3(d->nspec), dim3(256), 0, d->stream, d->wft.p,
-                         d->spec_start.p, d->stress_t.p + (size_t)sample_idx * d->nspec);
+                       d->theta.p, d->H[0].p, d->H[1].p, R, d->cfg.dt, d->cout.p,
+                       record ? d->digest.p : nullptr);
+  unsigned long long thresh = ~0ull;
+  if (halting) {
+    const double a = 0.5 * d->cfg.verlet_dist;
+    const double sr = a * a;
+    memcpy(&thresh, &sr, 8);
   }
+  DStep sp = d->step_params(0);
+  sp.wo = d->wo.p;
+  hipLaunchKernelGGL(k_gather_fwd, dim3(nblocks(d->n, bs)), dim3(bs), 0, d->stream, S, d->o.p,
+                     E.inc_off.p, E.inc_ent.p, E.nc, d->cout.p, d->bout.p, sp, d->every, d->wfn.p,
+                     R, thresh, record ? d->digest.p : nullptr, T, d->ct.p, d->bt.p,
+                     d->tan_on ? d->wft.p : nullptr);
+  if (record) {
+    hipLaunchKernelGGL(k_spec_sum, dim3(d->nspec), dim3(256), 0, d->stream, R, d->every, d->wfn.p,
+                       d->spec_start.p, d->nspec, d->stress_dev.p);
+    if (d->tan_on)
+      hipLaunchKernelGGL(k_spec_sum, dim3(d->nspec), dim3(256), 0, d->stream, R, d->every,
+                         d->wft.p, d->spec_start.p, d->nspec, d->stress_t.p);
+  }
+  hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, d->stream, R);
   DCHK(hipGetLastError());
-  d->hA.swap(d->hB);
-  if (d->tan_on) d->thA.swap(d->thB);
+}
+
+void set_run(esp_diff* d, int step, int end)
+{
+  hipLaunchKernelGGL(k_set_run, dim3(1), dim3(64), 0, d->stream, d->run.p, step, 0, end);
+  DCHK(hipGetLastError());
+}
+
+void drop_graph(esp_diff* d)
+{
+  if (d->gexec) {
+    (void)hipStreamSynchronize(d->stream);
+    (void)hipGraphExecDestroy(d->gexec);
+  }
+  d->gexec = nullptr;
+  d->g_epoch = -1;
+}
+
+// forward steps d->run.step .. end on the current epoch (the run record is
+// set): graph_steps forward steps captured once per epoch and replayed, or
+// plain launches
+void run_chunk(esp_diff* d, int from, int end)
+{
+  const Epoch& E = *d->epochs.back();
+  const int G = std::max(1, d->graph_steps);
+  if (!d->use_graph) {
+    for (int s = from; s <= end; s++) fwd_step(d, E, true, true);
+    return;
+  }
+  const int ep = (int)d->epochs.size() - 1;
+  if (d->g_epoch != ep || d->g_len != G) {
+    drop_graph(d);
+    hipGraph_t g = nullptr;
+    DCHK(hipStreamBeginCapture(d->stream, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < G; k++) fwd_step(d, E, true, true);
+    DCHK(hipStreamEndCapture(d->stream, &g));
+    const hipError_t rc = hipGraphInstantiate(&d->gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    DCHK(rc);
+    d->g_epoch = ep;
+    d->g_len = G;
+  }
+  for (int s = from; s <= end; s += G) DCHK(hipGraphLaunch(d->gexec, d->stream));
 }
 
 int nsamples(const esp_diff* d) { return d->nsteps / d->every; }
@@ -1124,19 +1252,54 @@ void forward(esp_diff* d, const double* theta, const double* target, double* str
                         d->stream));
   }
   upload_initial(d);
+  // wall origins of every step: moved by their step before every step (the
+  // pre-time-step moveWallBy of the main engine's setWallStepMotion)
+  {
+    const int nw = std::max(d->nw, 1);
+    std::vector<double> wo((size_t)(d->nsteps + 2) * nw * 3, 0.0);
+    double o[DIFF_MAX_WALLS][3];
+    for (int k = 0; k < d->nw; k++)
+      for (int c = 0; c < 3; c++) o[k][c] = d->wall0[k].o[c];
+    for (int t = 0; t <= d->nsteps + 1; t++) {
+      for (int k = 0; k < d->nw; k++)
+        for (int c = 0; c < 3; c++) wo[((size_t)t * nw + k) * 3 + c] = o[k][c];
+      for (int k = 0; k < d->nw; k++)
+        for (int c = 0; c < 3; c++) o[k][c] = o[k][c] + d->wstep[k][c];
+    }
+    d->wo.reserve(wo.size());
+    DCHK(hipMemcpyAsync(d->wo.p, wo.data(), sizeof(double) * wo.size(), hipMemcpyHostToDevice,
+                        d->stream));
+  }
+  d->run.reserve(1);
+  d->wfn.reserve(std::max(d->n, 1));
+  drop_graph(d);
+  // chunks of steps between checkpoints queued without host decisions; a step
+  // whose displacement asks for a neighbour search halts the queue, the host
+  // rebuilds the table before the next step and continues
   const int K = std::max(1, d->cfg.checkpoint_every);
   bool need = true;
-  for (int step = 1; step <= d->nsteps; step++) {
-    if (need) rebuild(d, step);
+  int step = 1;
+  while (step <= d->nsteps) {
+    if (need) {
+      drop_graph(d);
+      rebuild(d, step);
+      need = false;
+    }
     if ((step - 1) % K == 0) save_checkpoint(d, step);
-    const bool sample = step % d->every == 0;
-    DCHK(hipMemsetAsync(d->md.p, 0, sizeof(unsigned long long), d->stream));
-    fwd_step(d, *d->epochs.back(), step, sample, step / d->every - 1, d->digest.p + step);
-    unsigned long long m = 0;
-    DCHK(hipMemcpyAsync(&m, d->md.p, sizeof m, hipMemcpyDeviceToHost, d->stream));
+    const int end = std::min(d->nsteps, ((step - 1) / K + 1) * K);
+    set_run(d, step, end);
+    run_chunk(d, step, end);
+    DRun r;
+    DCHK(hipMemcpyAsync(&r, d->run.p, sizeof r, hipMemcpyDeviceToHost, d->stream));
     dsync(d);
-    need = m != 0;  // a displacement beyond verletDist/2: rebuild before the next step
+    if (r.halt != 0) {
+      need = true;
+      step = r.halt + 1;
+    } else {
+      step = end + 1;
+    }
   }
+  drop_graph(d);
   d->h_stress.assign((size_t)ns * d->nspec, 0.0);
   DCHK(hipMemcpyAsync(d->h_stress.data(), d->stress_dev.p, sizeof(double) * ns * d->nspec,
                       hipMemcpyDeviceToHost, d->stream));
@@ -1223,18 +1386,18 @@ void backward(esp_diff* d, double* grad)
     // recompute the segment forward, recording the states and histories
     if ((int)htape.size() < K) htape.resize(K);
     int cur = C.epoch;
+    set_run(d, s0, s1 - 1);
     for (int step = s0; step < s1; step++) {
       const int e = epoch_of(step);
       if (e != cur) {  // the rebuild of this step: histories carried by key
         const Epoch& E = *d->epochs[e];
-        DevBuf<double4> hn, hn2;
+        DevBuf<double4> hn;
         hn.reserve(std::max(E.nc, 1));
-        hn2.reserve(std::max(E.nc, 1));
         if (E.nc > 0)
           hipLaunchKernelGGL(k_carry_hist, dim3(nblocks(E.nc, bs)), dim3(bs), 0, d->stream, E.nc,
-                             E.src.p, d->hA.p, hn.p);
-        d->hA.swap(hn);
-        d->hB.swap(hn2);
+                             E.src.p, d->H[step & 1].p, hn.p);
+        d->H[step & 1].swap(hn);
+        d->H[(step + 1) & 1].reserve(std::max(E.nc, 1));
         d->cout.reserve((size_t)std::max(E.nc, 1) * 9);
         cur = e;
       }
@@ -1245,8 +1408,8 @@ void backward(esp_diff* d, double* grad)
       const int nc = d->epochs[cur]->nc;
       if (!htape[t]) htape[t] = std::make_unique<DevBuf<double4>>();
       htape[t]->reserve(std::max(nc, 1));
-      d2d(d, htape[t]->p, d->hA.p, nc);
-      fwd_step(d, *d->epochs[cur], step, false, 0, nullptr);
+      d2d(d, htape[t]->p, d->H[step & 1].p, nc);
+      fwd_step(d, *d->epochs[cur], false, false);
     }
     {
       double4* T = tape.p + (size_t)(s1 - s0) * 4 * c;
@@ -1421,6 +1584,7 @@ void esp_diff_destroy(esp_diff* d)
 {
   if (!d) return;
   if (d->stream) (void)hipStreamSynchronize(d->stream);
+  drop_graph(d);
   d->epochs.clear();
   d->ckpts.clear();
   hipStream_t s = d->stream;

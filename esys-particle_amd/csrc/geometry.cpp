@@ -61,8 +61,10 @@ extern "C" {
 // Returns the final maximum overlap relative to the smallest radius.
 // Each sweep computes every particle's displacement from its own overlaps (a
 // gather, so the sweep is deterministic under any thread count).
-double esp_geom_pack(int n, const double* rad, const double* box, unsigned long long seed,
-                     int max_sweeps, double* pos)
+// periodic_x: x is circular (overlaps across the x ends count through the
+// nearest image; positions wrap instead of being clamped)
+double esp_geom_pack_px(int n, const double* rad, const double* box, int periodic_x,
+                        unsigned long long seed, int max_sweeps, double* pos)
 {
   SplitMix rng(seed);
   std::vector<double> x(3 * (size_t)n), d(3 * (size_t)n);
@@ -86,17 +88,34 @@ double esp_geom_pack(int n, const double* rad, const double* box, unsigned long 
       const int cy = std::min(g.ny - 1, std::max(0, (int)(x[3 * i + 1] / g.cs)));
       const int cz = std::min(g.nz - 1, std::max(0, (int)(x[3 * i + 2] / g.cs)));
       double ddx = 0, ddy = 0, ddz = 0;
-      for (int ox = -1; ox <= 1; ox++)
+      // x cells to visit: the stencil, wrapped when x is circular (every cell
+      // once when there are fewer than three)
+      int xs[3], nxs = 0;
+      if (periodic_x && g.nx < 3) {
+        for (int c = 0; c < g.nx; c++) xs[nxs++] = c;
+      } else {
+        for (int ox = -1; ox <= 1; ox++) {
+          int x2 = cx + ox;
+          if (periodic_x) x2 = (x2 + g.nx) % g.nx;
+          else if (x2 < 0 || x2 >= g.nx) continue;
+          xs[nxs++] = x2;
+        }
+      }
+      for (int ix = 0; ix < nxs; ix++)
         for (int oy = -1; oy <= 1; oy++)
           for (int oz = -1; oz <= 1; oz++) {
-            const int x2 = cx + ox, y2 = cy + oy, z2 = cz + oz;
-            if (x2 < 0 || y2 < 0 || z2 < 0 || x2 >= g.nx || y2 >= g.ny || z2 >= g.nz) continue;
+            const int x2 = xs[ix], y2 = cy + oy, z2 = cz + oz;
+            if (y2 < 0 || z2 < 0 || y2 >= g.ny || z2 >= g.nz) continue;
             const int c2 = (x2 * g.ny + y2) * g.nz + z2;
             for (int b = g.start[c2]; b < g.start[c2 + 1]; b++) {
               const int j = g.items[b];
               if (j == i) continue;
-              const double dx = x[3 * j] - x[3 * i], dy = x[3 * j + 1] - x[3 * i + 1],
-                           dz = x[3 * j + 2] - x[3 * i + 2];
+              double dx = x[3 * j] - x[3 * i];
+              if (periodic_x) {
+                if (dx > 0.5 * box[0]) dx -= box[0];
+                else if (dx < -0.5 * box[0]) dx += box[0];
+              }
+              const double dy = x[3 * j + 1] - x[3 * i + 1], dz = x[3 * j + 2] - x[3 * i + 2];
               const double dist2 = dx * dx + dy * dy + dz * dz;
               const double rs = (rad[i] + rad[j]) * s;
               if (dist2 >= rs * rs) continue;
@@ -120,7 +139,12 @@ double esp_geom_pack(int n, const double* rad, const double* box, unsigned long 
       const double r = rad[i] * s;
       for (int c = 0; c < 3; c++) {
         double v = x[3 * i + c] + 0.7 * d[3 * i + c];
-        v = std::min(std::max(v, r), box[c] - r);
+        if (c == 0 && periodic_x) {
+          if (v < 0) v += box[0];
+          else if (v >= box[0]) v -= box[0];
+        } else {
+          v = std::min(std::max(v, r), box[c] - r);
+        }
         x[3 * i + c] = v;
       }
     }
@@ -131,13 +155,20 @@ double esp_geom_pack(int n, const double* rad, const double* box, unsigned long 
   return maxov;
 }
 
+double esp_geom_pack(int n, const double* rad, const double* box, unsigned long long seed,
+                     int max_sweeps, double* pos)
+{
+  return esp_geom_pack_px(n, rad, box, 0, seed, max_sweeps, pos);
+}
+
 // Moves centres until every listed (bonded) pair is at exact contact,
 // |x_i - x_j| = r_i + r_j, and no other pair overlaps, by Jacobi sweeps (each
 // particle gathers half of every residual it takes part in).  The bond network
 // of a packing has fewer constraints than degrees of freedom, so the sweeps
 // converge.  Returns the final maximum residual relative to r_i + r_j.
-double esp_geom_snap(int n, double* pos, const double* rad, int nb, const int* bonds,
-                     const double* box, int max_sweeps, double tol)
+// periodic_x: pairs across the x ends through the nearest image; x wraps
+double esp_geom_snap_px(int n, double* pos, const double* rad, int nb, const int* bonds,
+                        const double* box, int periodic_x, int max_sweeps, double tol)
 {
   std::vector<int> off(n + 1, 0), adj(2 * (size_t)nb);
   for (int b = 0; b < nb; b++) {
@@ -166,8 +197,12 @@ double esp_geom_snap(int n, double* pos, const double* rad, int nb, const int* b
       double ddx = 0, ddy = 0, ddz = 0;
       int active = 0;
       auto push = [&](int j, double target, bool bonded) {
-        const double dx = x[3 * j] - x[3 * i], dy = x[3 * j + 1] - x[3 * i + 1],
-                     dz = x[3 * j + 2] - x[3 * i + 2];
+        double dx = x[3 * j] - x[3 * i];
+        if (periodic_x) {
+          if (dx > 0.5 * box[0]) dx -= box[0];
+          else if (dx < -0.5 * box[0]) dx += box[0];
+        }
+        const double dy = x[3 * j + 1] - x[3 * i + 1], dz = x[3 * j + 2] - x[3 * i + 2];
         const double dist = std::sqrt(dx * dx + dy * dy + dz * dz);
         double res = dist - target;  // > 0: too far
         if (!bonded && res >= 0) return;
@@ -181,11 +216,22 @@ double esp_geom_snap(int n, double* pos, const double* rad, int nb, const int* b
       const int cx = std::min(g.nx - 1, std::max(0, (int)(x[3 * i] / g.cs)));
       const int cy = std::min(g.ny - 1, std::max(0, (int)(x[3 * i + 1] / g.cs)));
       const int cz = std::min(g.nz - 1, std::max(0, (int)(x[3 * i + 2] / g.cs)));
-      for (int ox = -1; ox <= 1; ox++)
+      int xs[3], nxs = 0;
+      if (periodic_x && g.nx < 3) {
+        for (int c = 0; c < g.nx; c++) xs[nxs++] = c;
+      } else {
+        for (int ox = -1; ox <= 1; ox++) {
+          int x2 = cx + ox;
+          if (periodic_x) x2 = (x2 + g.nx) % g.nx;
+          else if (x2 < 0 || x2 >= g.nx) continue;
+          xs[nxs++] = x2;
+        }
+      }
+      for (int ix = 0; ix < nxs; ix++)
         for (int oy = -1; oy <= 1; oy++)
           for (int oz = -1; oz <= 1; oz++) {
-            const int x2 = cx + ox, y2 = cy + oy, z2 = cz + oz;
-            if (x2 < 0 || y2 < 0 || z2 < 0 || x2 >= g.nx || y2 >= g.ny || z2 >= g.nz) continue;
+            const int x2 = xs[ix], y2 = cy + oy, z2 = cz + oz;
+            if (y2 < 0 || z2 < 0 || y2 >= g.ny || z2 >= g.nz) continue;
             const int c2 = (x2 * g.ny + y2) * g.nz + z2;
             for (int b = g.start[c2]; b < g.start[c2 + 1]; b++) {
               const int j = g.items[b];
@@ -203,11 +249,23 @@ double esp_geom_snap(int n, double* pos, const double* rad, int nb, const int* b
     for (int i = 0; i < n; i++)
       for (int c = 0; c < 3; c++) {
         double v = x[3 * i + c] + 1.5 * d[3 * i + c];
-        x[3 * i + c] = std::min(std::max(v, rad[i]), box[c] - rad[i]);
+        if (c == 0 && periodic_x) {
+          if (v < 0) v += box[0];
+          else if (v >= box[0]) v -= box[0];
+          x[3 * i] = v;
+        } else {
+          x[3 * i + c] = std::min(std::max(v, rad[i]), box[c] - rad[i]);
+        }
       }
   }
   memcpy(pos, x.data(), sizeof(double) * 3 * (size_t)n);
   return maxres;
+}
+
+double esp_geom_snap(int n, double* pos, const double* rad, int nb, const int* bonds,
+                     const double* box, int max_sweeps, double tol)
+{
+  return esp_geom_snap_px(n, pos, rad, nb, bonds, box, 0, max_sweeps, tol);
 }
 
 // Random sequential insertion: sphere k (radii in the given order) is placed

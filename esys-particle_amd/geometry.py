@@ -18,9 +18,16 @@ def _load():
         pd = ctypes.POINTER(ctypes.c_double)
         _lib.esp_geom_pack.restype = ctypes.c_double
         _lib.esp_geom_pack.argtypes = [ctypes.c_int, pd, pd, ctypes.c_ulonglong, ctypes.c_int, pd]
+        _lib.esp_geom_pack_px.restype = ctypes.c_double
+        _lib.esp_geom_pack_px.argtypes = [ctypes.c_int, pd, pd, ctypes.c_int, ctypes.c_ulonglong,
+                                          ctypes.c_int, pd]
         _lib.esp_geom_rsa.restype = ctypes.c_int
         _lib.esp_geom_rsa.argtypes = [ctypes.c_int, pd, pd, ctypes.c_int, ctypes.c_ulonglong,
                                       ctypes.c_longlong, pd]
+        _lib.esp_geom_snap_px.restype = ctypes.c_double
+        _lib.esp_geom_snap_px.argtypes = [ctypes.c_int, pd, pd, ctypes.c_int,
+                                          ctypes.POINTER(ctypes.c_int), pd, ctypes.c_int,
+                                          ctypes.c_int, ctypes.c_double]
         _lib.esp_geom_snap.restype = ctypes.c_double
         _lib.esp_geom_snap.argtypes = [ctypes.c_int, pd, pd, ctypes.c_int,
                                        ctypes.POINTER(ctypes.c_int), pd, ctypes.c_int,
@@ -28,16 +35,18 @@ def _load():
     return _lib
 
 
-def dense_packing(rad, box, seed, max_sweeps=6000):
+def dense_packing(rad, box, seed, max_sweeps=6000, periodic_x=False):
     """Positions of spheres with radii ``rad`` packed by grow-and-relax in
-    [0, box]; returns (pos, max relative overlap)."""
+    [0, box] (x circular with ``periodic_x``); returns (pos, max relative
+    overlap)."""
     lib = _load()
     rad = np.ascontiguousarray(rad, np.float64)
     box = np.ascontiguousarray(box, np.float64)
     pos = np.zeros((len(rad), 3))
     pd = ctypes.POINTER(ctypes.c_double)
-    ov = lib.esp_geom_pack(len(rad), rad.ctypes.data_as(pd), box.ctypes.data_as(pd),
-                           ctypes.c_ulonglong(seed), int(max_sweeps), pos.ctypes.data_as(pd))
+    ov = lib.esp_geom_pack_px(len(rad), rad.ctypes.data_as(pd), box.ctypes.data_as(pd),
+                              int(bool(periodic_x)), ctypes.c_ulonglong(seed), int(max_sweeps),
+                              pos.ctypes.data_as(pd))
     return pos, ov
 
 
@@ -58,18 +67,20 @@ def rsa_packing(rad, box, seed, periodic_x=False, max_tries=100000):
     return pos
 
 
-def snap_bonds(pos, rad, bonds, box, max_sweeps=20000, tol=1e-10):
+def snap_bonds(pos, rad, bonds, box, max_sweeps=20000, tol=1e-10, periodic_x=False):
     """Move centres so that every bonded pair is at exact contact and no other
-    pair overlaps; returns (pos, max relative residual)."""
+    pair overlaps (x circular with ``periodic_x``); returns (pos, max relative
+    residual)."""
     lib = _load()
     pd = ctypes.POINTER(ctypes.c_double)
     pos = np.ascontiguousarray(pos, np.float64).copy()
     rad = np.ascontiguousarray(rad, np.float64)
     b = np.ascontiguousarray(bonds, np.int32)
     box = np.ascontiguousarray(box, np.float64)
-    res = lib.esp_geom_snap(len(rad), pos.ctypes.data_as(pd), rad.ctypes.data_as(pd), len(b),
-                            b.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
-                            box.ctypes.data_as(pd), int(max_sweeps), float(tol))
+    res = lib.esp_geom_snap_px(len(rad), pos.ctypes.data_as(pd), rad.ctypes.data_as(pd), len(b),
+                               b.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                               box.ctypes.data_as(pd), int(bool(periodic_x)), int(max_sweeps),
+                               float(tol))
     return pos, res
 
 

@@ -225,13 +225,15 @@ class ServoWallLoader(lsm.Runnable):
 
 def shear_cell(n=2000000, seed=11, rmin=0.25, rmax=0.75, aspect=(200.0, 40.0, 100.0), phi=0.35,
                box=None, dt=5e-5, kn=1e4, ks=1e4, mu=0.5, wall_speed=0.1, stress=1.0,
-               wall_k=1e4, alpha=0.2, density=1.0):
+               wall_k=1e4, alpha=0.2, density=1.0, packing="rsa", sweeps=400):
     """Frictional shear cell, periodic in x: RotSphere, radii U[rmin, rmax],
-    random sequential packing at solid fraction ``phi`` in a box of the given
-    aspect (or ``box``); RotFriction kn, ks, mu between all particles; the
-    particles within 2 rmax of the bottom (tag 2) and top (tag 3) are bonded to
-    rough walls (NRotBondedWallPrms) that slide at -/+wall_speed in x; the top
-    wall is servo-loaded to the normal stress ``stress`` over its area."""
+    packed at solid fraction ``phi`` in a box of the given aspect (or ``box``):
+    random sequential insertion (``packing="rsa"``, loose) or grow-and-relax
+    (``"dense"``, contacts at small overlaps, ``sweeps`` relaxation sweeps);
+    RotFriction kn, ks, mu between all particles; the particles within 2 rmax
+    of the bottom (tag 2) and top (tag 3) are bonded to rough walls
+    (NRotBondedWallPrms) that slide at -/+wall_speed in x; the top wall is
+    servo-loaded to the normal stress ``stress`` over its area."""
     from . import geometry
     rng = np.random.default_rng(seed)
     rad = rng.uniform(rmin, rmax, n)
@@ -240,7 +242,13 @@ def shear_cell(n=2000000, seed=11, rmin=0.25, rmax=0.75, aspect=(200.0, 40.0, 10
         vs = float(np.sum(4.0 / 3.0 * math.pi * rad ** 3))
         s = (vs / phi / (aspect[0] * aspect[1] * aspect[2])) ** (1.0 / 3.0)
         box = (aspect[0] * s, aspect[1] * s, aspect[2] * s)
-    pos = geometry.rsa_packing(rad, box, seed + 1, periodic_x=True)
+    if packing == "dense":
+        pos, _ = geometry.dense_packing(rad, box, seed + 1, sweeps, periodic_x=True)
+        # relax the remaining overlaps: contacts at (near) exact touching
+        pos, _ = geometry.snap_bonds(pos, rad, np.zeros((0, 2), np.int32), box, sweeps, 1e-9,
+                                     periodic_x=True)
+    else:
+        pos = geometry.rsa_packing(rad, box, seed + 1, periodic_x=True)
     # the ids follow a random permutation of the insertion order
     perm = rng.permutation(n)
     pos, rad = pos[perm], rad[perm]
@@ -311,3 +319,55 @@ def apply(sim, spec, steps_per_batch=None):
     if spec.get("servo") is not None:
         sim.addPreTimeStepRunnable(ServoWallLoader(sim, *spec["servo"]))
     return sim
+
+
+def _calibration_member(args):
+    """One specimen of calibration_batch (a worker-process task)."""
+    n, seed, height, sweeps, coh = args
+    return dense_bonded_compression(n=n, seed=seed, height=height, max_sweeps=sweeps[0],
+                                    snap_sweeps=sweeps[1], coh_mean=coh, coh_sd=0.1 * coh)
+
+
+def calibration_batch(nspec=256, seed=21, nmin=2000, nmax=30000, height=20.0,
+                      sweeps=(400, 300), workers=None):
+    """The batched calibration sweep: nspec bonded compression specimens with
+    particle counts log-uniform in [nmin, nmax], each with its own seeded
+    packing (dense, square prisms of one height so that the batch shares the
+    walls) and its own parameters theta = (k_n in [5e4, 2e5], k_s = 0.3 k_n,
+    cohesion mean in [20, 80], mu in [0.3, 0.8]).  Returns (specs, theta
+    [nspec, 4]).  Packings are built in worker processes (one OpenMP thread
+    each), `workers` of them (default: the CPU share, at most 16)."""
+    import concurrent.futures
+    import multiprocessing
+    import os
+    rng = np.random.default_rng(seed)
+    ns = np.exp(rng.uniform(math.log(nmin), math.log(nmax), nspec)).astype(int)
+    kn = rng.uniform(5e4, 2e5, nspec)
+    coh = rng.uniform(20.0, 80.0, nspec)
+    mu = rng.uniform(0.3, 0.8, nspec)
+    theta = np.stack([kn, 0.3 * kn, coh, mu], 1)
+    tasks = [(int(ns[s]), seed * 1000 + s, height, sweeps, float(coh[s])) for s in range(nspec)]
+    if workers is None:
+        try:
+            workers = len(os.sched_getaffinity(0))
+        except AttributeError:
+            workers = os.cpu_count() or 1
+        workers = max(1, min(workers, 16))
+    if workers == 1:
+        return [_calibration_member(t) for t in tasks], theta
+    old = os.environ.get("OMP_NUM_THREADS")
+    os.environ["OMP_NUM_THREADS"] = "1"
+    try:
+        import runpy
+        ctx = multiprocessing.get_context("spawn")
+        boot = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_bootstrap.py")
+        with concurrent.futures.ProcessPoolExecutor(workers, mp_context=ctx,
+                                                    initializer=runpy.run_path,
+                                                    initargs=(boot,)) as ex:
+            specs = list(ex.map(_calibration_member, tasks, chunksize=1))
+    finally:
+        if old is None:
+            os.environ.pop("OMP_NUM_THREADS", None)
+        else:
+            os.environ["OMP_NUM_THREADS"] = old
+    return specs, theta
