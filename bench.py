@@ -161,6 +161,104 @@ def condition(sim, eng, steps, settling):
     return done
 
 
+def calibration_bench(args):
+    """The batched calibration sweep: every specimen's loss (mean squared
+    difference of its sampled axial stress to a target curve) and gradient with
+    respect to (k_n, k_s, cohesion, mu) from one batched forward run and one
+    adjoint sweep; value = specimens / wall time of both (gradient evaluations
+    per second).  Baseline: central finite differences on the plain engine,
+    2 runs per parameter, each specimen alone, timed on a sample of specimens
+    across the size range and summed over the batch by interpolation in N."""
+    pkg = load_pkg()
+    spm = importlib.import_module("esys_particle_amd.specimens")
+    cal = importlib.import_module("esys_particle_amd.calibration")
+    t0 = time.perf_counter()
+    specs, theta = spm.calibration_batch(nspec=args.specimens, workers=1, progress=log)
+    ns = np.array([len(sp["rad"]) for sp in specs])
+    log("specimens: %d, %d particles, %d bonds (%.1f s)" % (
+        len(specs), ns.sum(), sum(len(sp["bonds"]) for sp in specs), time.perf_counter() - t0))
+    every = 20
+    tune = None
+    if not args.no_autotune:
+        t0 = time.perf_counter()
+        tune = cal.autotune(specs, theta, trial_steps=60, sample_every=every, log=log)
+        log("autotune: %.1f s -> block %d alpha %.3g grid %.4g" % (
+            time.perf_counter() - t0, tune["block"], tune["alpha"], tune["grid"]))
+    kw = {} if tune is None else {"block": tune["block"], "alpha": tune["alpha"],
+                                  "grid": tune["grid"], "gap": tune["gap"]}
+    steps = args.steps
+    # warm-up: code loading, allocations
+    mw = cal.DiffModel.from_specimens(specs, max(args.warmup, 20), every, **kw)
+    mw.forward(theta, np.zeros((len(specs), mw.nsamples)))
+    mw.backward()
+    mw.close()
+    m = cal.DiffModel.from_specimens(specs, steps, every, checkpoint_every=100, **kw)
+    target = np.zeros((len(specs), m.nsamples))
+    t0 = time.perf_counter()
+    _, loss = m.forward(theta, target)
+    tf = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    grad = m.backward()
+    tb = time.perf_counter() - t1
+    fwd_ms, bwd_ms = m.times_ms()
+    value = len(specs) / (tf + tb)
+    log("forward %.2f s (device %.0f ms), adjoint %.2f s (device %.0f ms), %d breaks, "
+        "rebuilds %s" % (tf, fwd_ms, tb, bwd_ms, len(m.bond_breaks()), m.rebuild_steps().tolist()))
+    m.close()
+    # plain engine, one specimen at a time, forward runs of the same length
+    order = np.argsort(ns)
+    pick = order[np.linspace(0, len(order) - 1, min(6, len(order))).astype(int)]
+    pts = []
+    for s in pick:
+        mm = cal.DiffModel.from_specimens([specs[s]], steps, every)
+        sim = pkg.LsmMpi(stepsPerBatch=128)
+        cal.plain_setup(sim, mm, theta[s])
+        sim.runTimeSteps(1)
+        t0 = time.perf_counter()
+        sim.runTimeSteps(steps - 1)
+        el = time.perf_counter() - t0
+        pts.append((int(ns[s]), el * steps / max(steps - 1, 1)))
+        mm.close()
+        log("plain engine: %d particles, %d steps: %.3f s" % (pts[-1][0], steps, pts[-1][1]))
+    px, py = np.array([p[0] for p in pts], float), np.array([p[1] for p in pts], float)
+    t_plain = float(np.interp(ns, px, py).sum())
+    nparam = 4
+    fd_value = len(specs) / (2 * nparam * t_plain)
+    out = {
+        "metric": "gradient evaluations/s",
+        "value": value,
+        "unit": "gradients/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": (tf + tb) / steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "batched calibration sweep: %d bonded compression specimens, "
+                        "N log-uniform [2000, 30000], per-specimen k_n in [5e4, 2e5], "
+                        "k_s = 0.3 k_n, cohesion in [20, 80], mu in [0.3, 0.8]; forward "
+                        "%d steps + adjoint, stress sampled every %d steps, checkpoints "
+                        "every 100 steps" % (len(specs), steps, every),
+            "specimens": len(specs),
+            "particles": int(ns.sum()),
+            "autotune": None if tune is None else {k: tune[k] for k in ("block", "alpha", "grid")},
+        },
+        "forward_s": tf,
+        "adjoint_s": tb,
+        "fd_baseline": {"value": fd_value, "unit": "gradients/s",
+                        "runs_per_gradient": 2 * nparam,
+                        "plain_forward_s": pts,
+                        "ratio": value / fd_value},
+        "loss_mean": float(np.mean(loss)),
+        "grad_finite": bool(np.all(np.isfinite(grad))),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -183,12 +281,18 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device 10 times (counter calibration)")
     ap.add_argument("--workload", default="compression",
-                    choices=["compression", "settling", "shearcell"],
+                    choices=["compression", "settling", "shearcell", "calibration"],
                     help="compression: the bonded 100k-particle compression (default); "
                          "settling: 10,000 NRotSphere settling under gravity on an elastic "
                          "floor (NRotElastic k=1, dt=1e-4, the reference's CPU-sized case); "
                          "shearcell: 2,000,000 RotSphere frictional shear cell, periodic in "
-                         "x, rough walls sliding at +-0.1, servo-held confining stress 1.0")
+                         "x, rough walls sliding at +-0.1, servo-held confining stress 1.0; "
+                         "calibration: the batched forward + adjoint sweep over --specimens "
+                         "bonded specimens of 2,000-30,000 particles, --steps steps each")
+    ap.add_argument("--specimens", type=int, default=256,
+                    help="calibration: specimens in the batch")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="calibration: default launch / search settings")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the engine holds one subdomain on one device "
@@ -196,6 +300,9 @@ def main():
         sys.exit(2)
     if args.copy_probe:
         copy_probe(args.copy_probe)
+        return
+    if args.workload == "calibration":
+        calibration_bench(args)
         return
     if args.cpu_worker:
         pkg = load_pkg()

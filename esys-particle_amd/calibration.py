@@ -230,7 +230,7 @@ class DiffModel:
     # ---- specimens ----
     @staticmethod
     def from_specimens(specs, nsteps, sample_every, gap=None, checkpoint_every=100, device=0,
-                       block=256, wall_k=None, x_start=0.0):
+                       block=256, wall_k=None, x_start=0.0, grid=None, alpha=None):
         """A model of bonded compression specimens (specimens.bonded_compression
         dicts: RotSphere, one RotBond group with per-bond strengths, RotFriction,
         top and bottom walls moving at their steps).  Specimen s is shifted by
@@ -240,8 +240,10 @@ class DiffModel:
         specimen runs bitwise as it would alone only at the same x_s."""
         specs = list(specs)
         s0 = specs[0]
-        grid, alpha, dt = s0["grid"], s0["alpha"], s0["dt"]
-        gap = 2.0 * grid if gap is None else gap
+        dt = s0["dt"]
+        alpha = s0["alpha"] if alpha is None else float(alpha)
+        grid = s0["grid"] if grid is None else float(grid)
+        gap = 2.0 * max(grid, s0["grid"]) if gap is None else gap
         pos, rad, mass, sid, bi, bj, rel = [], [], [], [], [], [], []
         kt, kb, area = [], [], []
         x = float(x_start)
@@ -297,6 +299,62 @@ class DiffModel:
                       "kt": np.array(kt, float), "kb": np.array(kb, float), "walls": walls,
                       "area": np.array(area, float)}
         return model
+
+
+def autotune(specs, theta, trial_steps=60, sample_every=20, blocks=(128, 256, 512),
+             cell_factors=(1.0, 1.3), alpha_factors=(0.5, 1.0, 2.0), device=0, log=None):
+    """Launch and neighbour-search settings for a batch of specimens, chosen by
+    timing short forward runs: the block size of every kernel, the Verlet skin
+    alpha (the rebuild interval) and the neighbour-table cell size (a factor on
+    2 r_max + alpha).  None of them may change a result: every trial's break
+    events and final state must equal the first trial's bit for bit, else the
+    setting is rejected.  Coordinate descent: block, then alpha, then cell.
+    Returns {"block", "alpha", "grid", "gap" (the specimens' spacing all trials
+    used; the model must use it too), "trials": [(block, alpha, grid, ms, ok)]}."""
+    specs = list(specs)
+    rmax = max(float(np.max(sp["rad"])) for sp in specs)
+    a0 = float(specs[0]["alpha"])
+    th = np.asarray(theta, np.float64).reshape(len(specs), 4)
+    # one placement of the specimens for every trial (coordinates round alike)
+    gap = 2.0 * max(cell_factors) * (2.0 * rmax + max(alpha_factors) * a0)
+    ref = {}
+    trials = []
+
+    def trial(block, alpha, cell):
+        grid = cell * (2.0 * rmax + alpha)
+        m = DiffModel.from_specimens(specs, trial_steps, sample_every, device=device,
+                                     block=block, grid=grid, alpha=alpha, gap=gap)
+        tgt = np.zeros((m.nspec, m.nsamples))
+        m.forward(th, tgt)  # first run: allocations, code loading
+        st, _ = m.forward(th, tgt)
+        ms = m.times_ms()[0]
+        state = (m.bond_breaks().tobytes(), m.particle_data("pos").tobytes(),
+                 m.particle_data("quat").tobytes(), st.tobytes())
+        m.close()
+        if not ref:
+            ref["state"] = state
+        ok = state == ref["state"]
+        trials.append((block, alpha, grid, ms, ok))
+        if log:
+            log("autotune: block %d alpha %.3g grid %.4g: %.2f ms%s"
+                % (block, alpha, grid, ms, "" if ok else " (results differ: rejected)"))
+        return ms if ok else float("inf")
+
+    best = {"block": 256, "alpha": a0, "cell": 1.0}
+    cost = trial(best["block"], best["alpha"], best["cell"])
+    for key, values in (("block", blocks), ("alpha", [f * a0 for f in alpha_factors]),
+                        ("cell", cell_factors)):
+        for v in values:
+            if v == best[key]:
+                continue
+            cand = dict(best)
+            cand[key] = v
+            c = trial(cand["block"], cand["alpha"], cand["cell"])
+            if c < cost:
+                best, cost = cand, c
+    return {"block": best["block"], "alpha": best["alpha"],
+            "grid": best["cell"] * (2.0 * rmax + best["alpha"]), "gap": gap, "trial_ms": cost,
+            "trials": trials}
 
 
 def plain_setup(sim, model, theta, spec_index=0):

@@ -658,16 +658,28 @@ __global__ void k_adj_particle(DState Sn, DState Sn1, double4* lam, double* lfm,
 #define NB_TAN 16
 #define NF_TAN 24
 
-// bond law VJP: c[k] = lambda_out . d out / d in_k over inputs
-// (x1, x2, q1, q2, theta_kn, theta_ks)
+// bond law VJP: c[k] = lambda_out . d out / d in_k over the inputs
+// (x1, x2, q1, q2, theta_kn, theta_ks), one thread per (bond, input): 13
+// dual passes per bond, since the outputs depend on the positions only through
+// x2 - x1, so the x1 column is the x2 column negated (bit for bit: the seeds'
+// tangents enter through that difference alone)
+#define NB_DIRS 13
+__device__ __forceinline__ int bond_dir_input(int j)
+{
+  return j < 3 ? 3 + j : (j < 11 ? 6 + (j - 3) : 14 + (j - 11));
+}
+
 __global__ void k_adj_bond(DState S, DBonds B, const double4* theta, int step, const double* lfm,
                            double* ab)
 {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= B.nb) return;
+  const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= (long long)B.nb * NB_DIRS) return;
+  const int b = (int)(tid / NB_DIRS), j = (int)(tid - (long long)b * NB_DIRS);
+  const int k = bond_dir_input(j);
   double* c = ab + NB_TAN * (size_t)b;
   if (B.bstep[b] < step) {
-    for (int k = 0; k < NB_TAN; k++) c[k] = 0.0;
+    c[k] = 0.0;
+    if (j < 3) c[j] = 0.0;
     return;
   }
   const int2 e = B.ij[b];
@@ -680,42 +692,60 @@ __global__ void k_adj_bond(DState S, DBonds B, const double4* theta, int step, c
   const double* l2 = lfm + 6 * (size_t)e.y;
   const double in[NB_TAN] = {P1.x, P1.y, P1.z, P2.x, P2.y, P2.z, Q1.x, Q1.y,
                              Q1.z, Q1.w, Q2.x, Q2.y, Q2.z, Q2.w, th.x, th.y};
-  for (int k = 0; k < NB_TAN; k++) {
-    esp_dual u[NB_TAN];
-    for (int t = 0; t < NB_TAN; t++) u[t] = esp_dual(in[t], t == k ? 1.0 : 0.0);
-    esp_dual kr, ks;
-    double kt, kb;
-    tbond_k<esp_dual>(u[14], u[15], tk.x, tk.y, D.w, P1.w, P2.w, 1, 1, 0, kr, ks, kt, kb);
-    TBondResult<esp_dual> r;
-    tbond_law<esp_dual>(tv3<esp_dual>{u[0], u[1], u[2]}, tquat<esp_dual>{u[6], {u[7], u[8], u[9]}},
-                        P1.w, tv3<esp_dual>{u[3], u[4], u[5]},
-                        tquat<esp_dual>{u[10], {u[11], u[12], u[13]}}, P2.w, kr, ks, esp_dual(kt),
-                        esp_dual(kb), tv3<esp_dual>{esp_dual(D.x), esp_dual(D.y), esp_dual(D.z)}, r);
-    // f2 = -f1
-    c[k] = (l1[0] - l2[0]) * r.f1.x.d + (l1[1] - l2[1]) * r.f1.y.d + (l1[2] - l2[2]) * r.f1.z.d +
-           l1[3] * r.m1.x.d + l1[4] * r.m1.y.d + l1[5] * r.m1.z.d + l2[3] * r.m2.x.d +
-           l2[4] * r.m2.y.d + l2[5] * r.m2.z.d;
-  }
+  esp_dual u[NB_TAN];
+  for (int t = 0; t < NB_TAN; t++) u[t] = esp_dual(in[t], t == k ? 1.0 : 0.0);
+  esp_dual kr, ks;
+  double kt, kb;
+  tbond_k<esp_dual>(u[14], u[15], tk.x, tk.y, D.w, P1.w, P2.w, 1, 1, 0, kr, ks, kt, kb);
+  TBondResult<esp_dual> r;
+  tbond_law<esp_dual>(tv3<esp_dual>{u[0], u[1], u[2]}, tquat<esp_dual>{u[6], {u[7], u[8], u[9]}},
+                      P1.w, tv3<esp_dual>{u[3], u[4], u[5]},
+                      tquat<esp_dual>{u[10], {u[11], u[12], u[13]}}, P2.w, kr, ks, esp_dual(kt),
+                      esp_dual(kb), tv3<esp_dual>{esp_dual(D.x), esp_dual(D.y), esp_dual(D.z)}, r);
+  // f2 = -f1
+  const double v = (l1[0] - l2[0]) * r.f1.x.d + (l1[1] - l2[1]) * r.f1.y.d +
+                   (l1[2] - l2[2]) * r.f1.z.d + l1[3] * r.m1.x.d + l1[4] * r.m1.y.d +
+                   l1[5] * r.m1.z.d + l2[3] * r.m2.x.d + l2[4] * r.m2.y.d + l2[5] * r.m2.z.d;
+  c[k] = v;
+  if (j < 3) c[j] = -v;
 }
 
-// friction law VJP over inputs (x1, x2, v1, v2, w1, w2, history, theta_kn,
-// theta_ks, theta_mu); lh: adjoint of the history this step wrote; lh_prev:
-// adjoint of the history it read
+// friction law VJP over the inputs (x1, x2, v1, v2, w1, w2, history,
+// theta_kn, theta_ks, theta_mu), one thread per (pair, input): 18 dual passes
+// per pair (the law depends on the positions through x1 - x2 and on the
+// velocities through v2 - v1, so the x1 and v1 columns are the x2 and v2 ones
+// negated); lh: adjoint of the history this step wrote; lh_prev: adjoint of
+// the history it read
+#define NF_DIRS 18
+__device__ __forceinline__ int fric_dir_input(int j)
+{
+  // x2 (3..5), v2 (9..11), w1 (12..14), w2 (15..17), history (18..20), theta (21..23)
+  return j < 3 ? 3 + j : 6 + j;
+}
+
 __global__ void k_adj_fric(DState S, DCands C, const int* bstep, const double4* theta,
                            const double4* hin, int step, double dt, const double* lfm,
                            const double4* lh, double* ac, double4* lh_prev)
 {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= C.nc) return;
+  const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= (long long)C.nc * NF_DIRS) return;
+  const int c = (int)(tid / NF_DIRS), j = (int)(tid - (long long)c * NF_DIRS);
+  const int k = fric_dir_input(j);
   double* o = ac + NF_TAN * (size_t)c;
+  double* lp = reinterpret_cast<double*>(lh_prev + c);
   const int bb = C.bond[c];
-  if (bb >= 0 && bstep[bb] >= step) {
-    for (int k = 0; k < NF_TAN; k++) o[k] = 0.0;
-    lh_prev[c] = make_double4(0.0, 0.0, 0.0, 0.0);
-    return;
-  }
   const int2 e = C.ij[c];
   const double4 P1 = S.x[e.x], P2 = S.x[e.y];
+  // excluded while bonded, or apart (no force, history cleared): zero Jacobian
+  const double dxx = P1.x - P2.x, dyy = P1.y - P2.y, dzz = P1.z - P2.z;
+  const double eq = P1.w + P2.w;
+  if ((bb >= 0 && bstep[bb] >= step) || !((dxx * dxx + dyy * dyy + dzz * dzz) < eq * eq)) {
+    o[k] = 0.0;
+    if (j < 6) o[k - 3] = 0.0;  // the x1 / v1 columns
+    if (j >= 12 && j < 15) lp[j - 12] = 0.0;
+    if (j == 15) lp[3] = 0.0;
+    return;
+  }
   const double4 V1 = S.v[e.x], V2 = S.v[e.y], W1 = S.w[e.x], W2 = S.w[e.y];
   const double4 th = theta[S.spec[e.x]];
   const double4 hh = hin[c];
@@ -724,31 +754,25 @@ __global__ void k_adj_fric(DState S, DCands C, const int* bstep, const double4* 
   const double* l2 = lfm + 6 * (size_t)e.y;
   const double in[NF_TAN] = {P1.x, P1.y, P1.z, P2.x, P2.y, P2.z, V1.x, V1.y, V1.z, V2.x, V2.y, V2.z,
                              W1.x, W1.y, W1.z, W2.x, W2.y, W2.z, hh.x, hh.y, hh.z, th.x, th.y, th.w};
-  // quick exit for pairs apart (no force, history cleared: zero Jacobian)
-  const double dxx = P1.x - P2.x, dyy = P1.y - P2.y, dzz = P1.z - P2.z;
-  const double eq = P1.w + P2.w;
-  if (!((dxx * dxx + dyy * dyy + dzz * dzz) < eq * eq)) {
-    for (int k = 0; k < NF_TAN; k++) o[k] = 0.0;
-    lh_prev[c] = make_double4(0.0, 0.0, 0.0, 0.0);
-    return;
-  }
-  for (int k = 0; k < NF_TAN; k++) {
-    esp_dual u[NF_TAN];
-    for (int t = 0; t < NF_TAN; t++) u[t] = esp_dual(in[t], t == k ? 1.0 : 0.0);
-    esp_dual kk, kks;
-    tfriction_k<esp_dual>(u[21], u[22], 1, 1, P1.w, P2.w, 0, kk, kks);
-    tv3<esp_dual> ffr{u[18], u[19], u[20]}, f1, m1, m2;
-    bool slipping = (((int)hh.w) & 1) != 0;
-    tfriction_law<esp_dual>(tv3<esp_dual>{u[0], u[1], u[2]}, tv3<esp_dual>{u[6], u[7], u[8]},
-                            tv3<esp_dual>{u[12], u[13], u[14]}, P1.w,
-                            tv3<esp_dual>{u[3], u[4], u[5]}, tv3<esp_dual>{u[9], u[10], u[11]},
-                            tv3<esp_dual>{u[15], u[16], u[17]}, P2.w, kk, kks, u[23], u[23], dt,
-                            ffr, slipping, f1, m1, m2);
-    o[k] = (l1[0] - l2[0]) * f1.x.d + (l1[1] - l2[1]) * f1.y.d + (l1[2] - l2[2]) * f1.z.d +
-           l1[3] * m1.x.d + l1[4] * m1.y.d + l1[5] * m1.z.d + l2[3] * m2.x.d + l2[4] * m2.y.d +
-           l2[5] * m2.z.d + lo.x * ffr.x.d + lo.y * ffr.y.d + lo.z * ffr.z.d;
-  }
-  lh_prev[c] = make_double4(o[18], o[19], o[20], 0.0);
+  esp_dual u[NF_TAN];
+  for (int t = 0; t < NF_TAN; t++) u[t] = esp_dual(in[t], t == k ? 1.0 : 0.0);
+  esp_dual kk, kks;
+  tfriction_k<esp_dual>(u[21], u[22], 1, 1, P1.w, P2.w, 0, kk, kks);
+  tv3<esp_dual> ffr{u[18], u[19], u[20]}, f1, m1, m2;
+  bool slipping = (((int)hh.w) & 1) != 0;
+  tfriction_law<esp_dual>(tv3<esp_dual>{u[0], u[1], u[2]}, tv3<esp_dual>{u[6], u[7], u[8]},
+                          tv3<esp_dual>{u[12], u[13], u[14]}, P1.w,
+                          tv3<esp_dual>{u[3], u[4], u[5]}, tv3<esp_dual>{u[9], u[10], u[11]},
+                          tv3<esp_dual>{u[15], u[16], u[17]}, P2.w, kk, kks, u[23], u[23], dt, ffr,
+                          slipping, f1, m1, m2);
+  const double v = (l1[0] - l2[0]) * f1.x.d + (l1[1] - l2[1]) * f1.y.d +
+                   (l1[2] - l2[2]) * f1.z.d + l1[3] * m1.x.d + l1[4] * m1.y.d + l1[5] * m1.z.d +
+                   l2[3] * m2.x.d + l2[4] * m2.y.d + l2[5] * m2.z.d + lo.x * ffr.x.d +
+                   lo.y * ffr.y.d + lo.z * ffr.z.d;
+  o[k] = v;
+  if (j < 6) o[k - 3] = -v;  // x1 = -x2 column, v1 = -v2 column
+  if (j >= 12 && j < 15) lp[j - 12] = v;
+  if (j == 15) lp[3] = 0.0;
 }
 
 // per particle: add the interactions' adjoints (CSR order) to the state
@@ -1447,11 +1471,12 @@ void backward(esp_diff* d, double* grad)
       hipLaunchKernelGGL(k_adj_particle, dim3(nblocks(n, bs)), dim3(bs), 0, d->stream, Sn, Sn1,
                          lam.p, lfm.p, d->step_params(step), cf);
       if (d->nb > 0)
-        hipLaunchKernelGGL(k_adj_bond, dim3(nblocks(d->nb, bs)), dim3(bs), 0, d->stream, Sn, B,
-                           d->theta.p, step, lfm.p, ab.p);
+        hipLaunchKernelGGL(k_adj_bond, dim3(nblocks((long long)d->nb * NB_DIRS, bs)), dim3(bs), 0,
+                           d->stream, Sn, B, d->theta.p, step, lfm.p, ab.p);
       DCands Cd{E.nc, E.ij.p, E.bond.p};
       if (E.nc > 0)
-        hipLaunchKernelGGL(k_adj_fric, dim3(nblocks(E.nc, bs)), dim3(bs), 0, d->stream, Sn, Cd,
+        hipLaunchKernelGGL(k_adj_fric, dim3(nblocks((long long)E.nc * NF_DIRS, bs)), dim3(bs), 0,
+                           d->stream, Sn, Cd,
                            d->bstep.p, d->theta.p, htape[t]->p, step, d->cfg.dt, lfm.p, lh.p, ac.p,
                            lh2.p);
       hipLaunchKernelGGL(k_adj_gather, dim3(nblocks(n, bs)), dim3(bs), 0, d->stream, n,

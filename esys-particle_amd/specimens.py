@@ -329,14 +329,16 @@ def _calibration_member(args):
 
 
 def calibration_batch(nspec=256, seed=21, nmin=2000, nmax=30000, height=20.0,
-                      sweeps=(400, 300), workers=None):
+                      sweeps=(400, 300), workers=None, progress=None):
     """The batched calibration sweep: nspec bonded compression specimens with
     particle counts log-uniform in [nmin, nmax], each with its own seeded
     packing (dense, square prisms of one height so that the batch shares the
     walls) and its own parameters theta = (k_n in [5e4, 2e5], k_s = 0.3 k_n,
     cohesion mean in [20, 80], mu in [0.3, 0.8]).  Returns (specs, theta
     [nspec, 4]).  Packings are built in worker processes (one OpenMP thread
-    each), `workers` of them (default: the CPU share, at most 16)."""
+    each), `workers` of them (default: the CPU share, at most 16); with one
+    worker they are built here (OpenMP threads inside each packing) and
+    `progress` is called every 16 specimens."""
     import concurrent.futures
     import multiprocessing
     import os
@@ -354,7 +356,12 @@ def calibration_batch(nspec=256, seed=21, nmin=2000, nmax=30000, height=20.0,
             workers = os.cpu_count() or 1
         workers = max(1, min(workers, 16))
     if workers == 1:
-        return [_calibration_member(t) for t in tasks], theta
+        specs = []
+        for k, t in enumerate(tasks):
+            specs.append(_calibration_member(t))
+            if progress and (k + 1) % 16 == 0:
+                progress("%d of %d specimens built" % (k + 1, nspec))
+        return specs, theta
     old = os.environ.get("OMP_NUM_THREADS")
     os.environ["OMP_NUM_THREADS"] = "1"
     try:
