@@ -324,7 +324,8 @@ def main():
             # stated 200x40x100 box, so the box keeps its aspect at solid fraction 0.58
             # (grow-and-relax packing, periodic in x)
             spec = spm.shear_cell(n=args.particles if args.particles != 100000 else 2000000,
-                                  seed=11, phi=0.58, packing="dense")
+                                  seed=11, phi=0.58, packing="dense", sweeps=600,
+                                  progress=log)
         elif settling:
             # the soft floor (k=1) lets the lowest particles sink through it and
             # out of the neighbour table, which erases them, as the reference does

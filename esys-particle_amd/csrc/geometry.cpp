@@ -62,20 +62,30 @@ extern "C" {
 // Each sweep computes every particle's displacement from its own overlaps (a
 // gather, so the sweep is deterministic under any thread count).
 // periodic_x: x is circular (overlaps across the x ends count through the
-// nearest image; positions wrap instead of being clamped)
-double esp_geom_pack_px(int n, const double* rad, const double* box, int periodic_x,
-                        unsigned long long seed, int max_sweeps, double* pos)
+// nearest image; positions wrap instead of being clamped).  state[0] is the
+// radius scale of the growth: <= 0 starts from seeded random positions (scale
+// 0.2), otherwise the sweeps continue from pos at that scale; on return it
+// holds the scale reached, state[1] the last maximum overlap, and state[2] 1
+// once converged.  So a long packing can run as several calls.
+double esp_geom_pack_steps(int n, const double* rad, const double* box, int periodic_x,
+                           unsigned long long seed, int max_sweeps, double* pos, double* state)
 {
-  SplitMix rng(seed);
   std::vector<double> x(3 * (size_t)n), d(3 * (size_t)n);
   double rmax = 0, rmin = 1e300;
   for (int i = 0; i < n; i++) {
     rmax = std::max(rmax, rad[i]);
     rmin = std::min(rmin, rad[i]);
   }
-  for (int i = 0; i < n; i++)
-    for (int c = 0; c < 3; c++) x[3 * i + c] = rng.uniform() * box[c];
-  double s = 0.2;
+  double s = state[0];
+  if (s <= 0) {
+    SplitMix rng(seed);
+    for (int i = 0; i < n; i++)
+      for (int c = 0; c < 3; c++) x[3 * i + c] = rng.uniform() * box[c];
+    s = 0.2;
+  } else {
+    memcpy(x.data(), pos, sizeof(double) * 3 * (size_t)n);
+  }
+  state[2] = 0;
   Grid g;
   std::vector<int> cellof(n);
   double maxov = 0;
@@ -148,11 +158,23 @@ double esp_geom_pack_px(int n, const double* rad, const double* box, int periodi
         x[3 * i + c] = v;
       }
     }
-    if (s >= 1.0 && maxov < 2e-3) break;
+    if (s >= 1.0 && maxov < 2e-3) {
+      state[2] = 1;
+      break;
+    }
     if (maxov < 0.05) s = std::min(1.0, s * 1.01);
   }
   memcpy(pos, x.data(), sizeof(double) * 3 * (size_t)n);
+  state[0] = s;
+  state[1] = maxov;
   return maxov;
+}
+
+double esp_geom_pack_px(int n, const double* rad, const double* box, int periodic_x,
+                        unsigned long long seed, int max_sweeps, double* pos)
+{
+  double state[3] = {0, 0, 0};
+  return esp_geom_pack_steps(n, rad, box, periodic_x, seed, max_sweeps, pos, state);
 }
 
 double esp_geom_pack(int n, const double* rad, const double* box, unsigned long long seed,

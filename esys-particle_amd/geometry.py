@@ -18,6 +18,9 @@ def _load():
         pd = ctypes.POINTER(ctypes.c_double)
         _lib.esp_geom_pack.restype = ctypes.c_double
         _lib.esp_geom_pack.argtypes = [ctypes.c_int, pd, pd, ctypes.c_ulonglong, ctypes.c_int, pd]
+        _lib.esp_geom_pack_steps.restype = ctypes.c_double
+        _lib.esp_geom_pack_steps.argtypes = [ctypes.c_int, pd, pd, ctypes.c_int,
+                                             ctypes.c_ulonglong, ctypes.c_int, pd, pd]
         _lib.esp_geom_pack_px.restype = ctypes.c_double
         _lib.esp_geom_pack_px.argtypes = [ctypes.c_int, pd, pd, ctypes.c_int, ctypes.c_ulonglong,
                                           ctypes.c_int, pd]
@@ -35,18 +38,29 @@ def _load():
     return _lib
 
 
-def dense_packing(rad, box, seed, max_sweeps=6000, periodic_x=False):
+def dense_packing(rad, box, seed, max_sweeps=6000, periodic_x=False, progress=None):
     """Positions of spheres with radii ``rad`` packed by grow-and-relax in
     [0, box] (x circular with ``periodic_x``); returns (pos, max relative
-    overlap)."""
+    overlap).  With ``progress`` the sweeps run in rounds of 100, each reported
+    (the result is the same)."""
     lib = _load()
     rad = np.ascontiguousarray(rad, np.float64)
     box = np.ascontiguousarray(box, np.float64)
     pos = np.zeros((len(rad), 3))
     pd = ctypes.POINTER(ctypes.c_double)
-    ov = lib.esp_geom_pack_px(len(rad), rad.ctypes.data_as(pd), box.ctypes.data_as(pd),
-                              int(bool(periodic_x)), ctypes.c_ulonglong(seed), int(max_sweeps),
-                              pos.ctypes.data_as(pd))
+    state = np.zeros(3)
+    done, ov = 0, 0.0
+    chunk = 100 if progress else int(max_sweeps)
+    while done < max_sweeps:
+        k = min(chunk, int(max_sweeps) - done)
+        ov = lib.esp_geom_pack_steps(len(rad), rad.ctypes.data_as(pd), box.ctypes.data_as(pd),
+                                     int(bool(periodic_x)), ctypes.c_ulonglong(seed), k,
+                                     pos.ctypes.data_as(pd), state.ctypes.data_as(pd))
+        done += k
+        if progress:
+            progress("packing: %d sweeps, scale %.3f, overlap %.3g" % (done, state[0], ov))
+        if state[2]:
+            break
     return pos, ov
 
 
@@ -67,10 +81,22 @@ def rsa_packing(rad, box, seed, periodic_x=False, max_tries=100000):
     return pos
 
 
-def snap_bonds(pos, rad, bonds, box, max_sweeps=20000, tol=1e-10, periodic_x=False):
+def snap_bonds(pos, rad, bonds, box, max_sweeps=20000, tol=1e-10, periodic_x=False,
+               progress=None):
     """Move centres so that every bonded pair is at exact contact and no other
     pair overlaps (x circular with ``periodic_x``); returns (pos, max relative
-    residual)."""
+    residual).  With ``progress`` the sweeps run in rounds of 100, each
+    reported (the result is the same)."""
+    if progress:
+        done, res = 0, 0.0
+        while done < max_sweeps:
+            k = min(100, int(max_sweeps) - done)
+            pos, res = snap_bonds(pos, rad, bonds, box, k, tol, periodic_x)
+            done += k
+            progress("relaxation: %d sweeps, residual %.3g" % (done, res))
+            if res < tol:
+                break
+        return pos, res
     lib = _load()
     pd = ctypes.POINTER(ctypes.c_double)
     pos = np.ascontiguousarray(pos, np.float64).copy()

@@ -225,7 +225,7 @@ class ServoWallLoader(lsm.Runnable):
 
 def shear_cell(n=2000000, seed=11, rmin=0.25, rmax=0.75, aspect=(200.0, 40.0, 100.0), phi=0.35,
                box=None, dt=5e-5, kn=1e4, ks=1e4, mu=0.5, wall_speed=0.1, stress=1.0,
-               wall_k=1e4, alpha=0.2, density=1.0, packing="rsa", sweeps=400):
+               wall_k=1e4, alpha=0.2, density=1.0, packing="rsa", sweeps=400, progress=None):
     """Frictional shear cell, periodic in x: RotSphere, radii U[rmin, rmax],
     packed at solid fraction ``phi`` in a box of the given aspect (or ``box``):
     random sequential insertion (``packing="rsa"``, loose) or grow-and-relax
@@ -243,10 +243,11 @@ def shear_cell(n=2000000, seed=11, rmin=0.25, rmax=0.75, aspect=(200.0, 40.0, 10
         s = (vs / phi / (aspect[0] * aspect[1] * aspect[2])) ** (1.0 / 3.0)
         box = (aspect[0] * s, aspect[1] * s, aspect[2] * s)
     if packing == "dense":
-        pos, _ = geometry.dense_packing(rad, box, seed + 1, sweeps, periodic_x=True)
+        pos, _ = geometry.dense_packing(rad, box, seed + 1, sweeps, periodic_x=True,
+                                        progress=progress)
         # relax the remaining overlaps: contacts at (near) exact touching
-        pos, _ = geometry.snap_bonds(pos, rad, np.zeros((0, 2), np.int32), box, sweeps, 1e-9,
-                                     periodic_x=True)
+        pos, _ = geometry.snap_bonds(pos, rad, np.zeros((0, 2), np.int32), box, sweeps // 2,
+                                     1e-9, periodic_x=True, progress=progress)
     else:
         pos = geometry.rsa_packing(rad, box, seed + 1, periodic_x=True)
     # the ids follow a random permutation of the insertion order
