@@ -36,6 +36,15 @@ B_FRICTION = 64       # touching contact: i, j + shear spring (3) + flags, read 
 B_FRICTION_IDLE = 12  # stored pair apart: i, j + the flag word saying its history is zero
 B_ELASTIC = 8         # i, j
 
+# FP64 work per unit, from the SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 counters of the
+# default workload (64 lanes x (ADD + MUL + 2 FMA + TRANS) per dispatch over its units)
+FLOP_BOND = 1391            # bond law + break test (k_forces)
+FLOP_PARTICLE_GATHER = 189  # rotational particle: slot sums, walls, integration
+FP64_PEAK_GFLOPS = 78600.0
+# HBM bytes per launch measured with FETCH_SIZE / WRITE_SIZE (copy-probe scaled) for
+# the default workload, in profiles/step_kernels_bonded100k.md
+MEASURED_TRAFFIC = {("compression", 100000): {"forces": 63.6e6, "gather_integrate": 57.8e6}}
+
 
 def _load(name, path, pkgdir=None):
     if name in sys.modules:
@@ -388,6 +397,8 @@ def main():
     dom = max((k for k in per_launch if kt[k][1] > 0), key=lambda k: kt[k][0])
     avg_s = kt[dom][0] / kt[dom][1] * 1e-3
     achieved = per_launch[dom] / avg_s / 1e9
+    flops = {"forces": nb * FLOP_BOND, "gather_integrate": n * FLOP_PARTICLE_GATHER}
+    traffic = MEASURED_TRAFFIC.get((args.workload, n), {}).get(dom)
     step_bytes = n * b_part + nb * B_BOND + dyn_bytes
     ms_per_step = el / args.steps * 1e3
     out = {
@@ -440,10 +451,20 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": None if traffic is None else
+            "profiles/step_kernels_bonded100k.md (FETCH_SIZE, WRITE_SIZE of the same command)",
             "algorithmic_bytes_per_launch": per_launch[dom],
             "avg_launch_us": avg_s * 1e6,
         },
+        "fp64_roofline": {
+            "kernel": dom,
+            "flop_per_launch": flops[dom],
+            "achieved": flops[dom] / avg_s / 1e9,
+            "peak": FP64_PEAK_GFLOPS,
+            "unit": "GFLOP/s",
+            "frac": flops[dom] / avg_s / 1e9 / FP64_PEAK_GFLOPS,
+        } if flops[dom] > 0 else None,
         "step_roofline": {
             "algorithmic_bytes_per_step": step_bytes,
             "achieved_GBs": step_bytes / (ms_per_step * 1e-3) / 1e9,
