@@ -143,6 +143,7 @@ KERNEL_CLASSES = ("forces", "gather_integrate")
 # entry points of the HIP library only (the oracle has no counterpart)
 HIP_ONLY_SIGNATURES = {
     "apply_force_to_wall_noalloc": (_i, [_vp, _s, _pd]),
+    "check_state": (_i, [_vp]),
 }
 
 
@@ -361,6 +362,10 @@ class Engine:
     def move_wall_by(self, name, d):
         d = np.asarray(d, np.float64)
         self._check(self.lib.move_wall_by(self.h, _b(name), _dp(d)), "moveWallBy")
+
+    def check_state(self):
+        """Raises if the device bookkeeping is inconsistent (esp_check_state)."""
+        self._check(self.lib.check_state(self.h), "checkState")
 
     def apply_force_to_wall_noalloc(self, name, f):
         f = np.ascontiguousarray(f, np.float64)

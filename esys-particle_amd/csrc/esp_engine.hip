@@ -3177,6 +3177,115 @@ int esp_get_bond_breaks(esp_engine* e, int* step, int* id1, int* id2)
   API_END
 }
 
+namespace {
+// Consistency of the device bookkeeping that queued steps index with: every
+// per-particle appendix count, flagged-list entry, event count, pair, key and
+// force-record slot inside its buffer (and a record's CSR slots inside its
+// particles' runs).  Returns an empty string when consistent.
+std::string check_state(esp_engine* e)
+{
+  if (!e->started) return "";
+  HIPCHK(hipSetDevice(e->cfg.device));
+  sync(e);
+  const int n = e->n;
+  std::ostringstream err;
+  auto fail = [&](const std::string& m) {
+    if (err.tellp() < 2000) err << m << "; ";
+  };
+  std::vector<int> xcnt(std::max(n, 1)), off(n + 1);
+  UpdState st;
+  d2h(e, xcnt.data(), e->xcnt.p, n);
+  d2h(e, off.data(), e->inc_off.p, n + 1);
+  d2h(e, &st, e->upd.p, 1);
+  std::vector<int2> so(std::max(e->nrec, 1));
+  d2h(e, so.data(), e->slot_of.p, e->nrec);
+  sync(e);
+  for (int i = 0; i < n; i++)
+    if (xcnt[i] < 0 || xcnt[i] > ESP_XSLOTS) {
+      fail("appendix slot count " + std::to_string(xcnt[i]) + " at slot " + std::to_string(i));
+      break;
+    }
+  for (int i = 0; i < n; i++)
+    if (off[i] > off[i + 1]) {
+      fail("CSR offsets decrease at slot " + std::to_string(i));
+      break;
+    }
+  const long long nslot = (long long)e->xbase + (long long)n * ESP_XSLOTS;
+  if (off[n] != e->xbase) fail("CSR total differs from the appendix base");
+  if (st.fcount < 0 || st.fcount > n) fail("flagged count " + std::to_string(st.fcount));
+  if (st.evt_count < 0 || st.evt_count > e->broken_cap)
+    fail("event count " + std::to_string(st.evt_count));
+  if (st.fcount > 0 && st.fcount <= n) {
+    std::vector<int> fl(st.fcount);
+    d2h(e, fl.data(), e->flist.p, st.fcount);
+    sync(e);
+    for (int v : fl)
+      if (v < 0 || v >= n) {
+        fail("flagged particle slot " + std::to_string(v));
+        break;
+      }
+  }
+  auto in_run = [&](int slot, int p) { return slot >= off[p] && slot < off[p + 1]; };
+  for (int gi : e->bord) {
+    BondGroup& G = e->bg[gi];
+    std::vector<int2> ij(std::max(G.n, 1));
+    d2h(e, ij.data(), G.ij.p, G.n);
+    sync(e);
+    for (int b = 0; b < G.n; b++) {
+      const int2 q = ij[b];
+      if (q.x == -1 && q.y == -1) continue;  // dead until the next compaction
+      const int2 sl = so[G.rec_off + b];
+      if (q.x < 0 || q.x >= n || q.y < 0 || q.y >= n || q.x == q.y || !in_run(sl.x, q.x) ||
+          !in_run(sl.y, q.y)) {
+        fail("bond " + std::to_string(b) + " of '" + G.name + "'");
+        break;
+      }
+    }
+  }
+  for (int gi : e->dord) {
+    DynGroup& G = e->dg[gi];
+    int na[2] = {0, 0};
+    d2h(e, na, G.napp.p, 2);
+    sync(e);
+    if (na[0] < 0 || na[0] > ESP_APP_CAP || na[1] < 0 || na[1] > na[0]) {
+      fail("appendix counts of '" + G.name + "'");
+      continue;
+    }
+    const int tot = G.n + na[0];
+    std::vector<int2> ij(std::max(tot, 1));
+    std::vector<long long> key(std::max(tot, 1));
+    d2h(e, ij.data(), G.ij.p, tot);
+    d2h(e, key.data(), G.key.p, tot);
+    sync(e);
+    for (int c = 0; c < tot; c++) {
+      const int2 q = ij[c];
+      const bool app = c >= G.n;
+      const int2 sl = app ? so[G.app_rec_off + (c - G.n)] : so[G.rec_off + c];
+      bool ok = q.x >= 0 && q.x < n && q.y >= 0 && q.y < n && q.x != q.y && key[c] == pkey(q.x, q.y);
+      if (ok && !app) ok = in_run(sl.x, q.x) && in_run(sl.y, q.y);
+      if (ok && app)
+        ok = sl.x >= e->xbase + ESP_XSLOTS * q.x && sl.x < e->xbase + ESP_XSLOTS * (q.x + 1) &&
+             sl.y >= e->xbase + ESP_XSLOTS * q.y && sl.y < e->xbase + ESP_XSLOTS * (q.y + 1) &&
+             sl.y < nslot;
+      if (!ok) {
+        fail("pair " + std::to_string(c) + " of '" + G.name + "'" + (app ? " (appendix)" : ""));
+        break;
+      }
+    }
+  }
+  return err.str();
+}
+}  // namespace
+
+int esp_check_state(esp_engine* e)
+{
+  API_BEGIN
+  const std::string m = check_state(e);
+  if (!m.empty()) throw EspError(ESP_ERR_STATE, "device bookkeeping inconsistent: " + m);
+  return ESP_OK;
+  API_END
+}
+
 int esp_get_rebuild_steps(esp_engine* e, int* steps)
 {
   API_BEGIN

@@ -251,6 +251,14 @@ int esp_move_wall_by(esp_engine* e, const char* name, const double d[3]);
  * it from a pre-time-step runnable.  ESP_ERR_NUMERIC for a bonded group without
  * stiffness (the reference would move its wall by an infinite distance). */
 int esp_apply_force_to_wall(esp_engine* e, const char* name, const double force[3]);
+/* Checks the device bookkeeping the queued steps index with: per-particle
+ * appendix slot counts, the flagged-particle list, the event count, every
+ * stored pair and bond (slots in range, keys matching, force-record slots
+ * inside their particles' summation runs), appendix counts.  ESP_OK, or
+ * ESP_ERR_STATE with the first inconsistencies in esp_last_error.  No
+ * counterpart in the reference: a consistency check for tests and debugging
+ * (it synchronises and copies the bookkeeping to the host). */
+int esp_check_state(esp_engine* e);
 /* Maximum candidate particles of one wall group for esp_apply_force_to_wall_noalloc. */
 #define ESP_NOALLOC_MAX_WALL_CANDIDATES 262144
 /* esp_apply_force_to_wall without any device allocation: no counterpart in the
