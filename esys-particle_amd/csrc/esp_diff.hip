@@ -21,7 +21,8 @@
 // then swept backwards:
 //   k_adj_particle  the integrator's adjoint (symplectic Euler, quaternion
 //                   step and normalisation), the walls' and the loss's terms
-//   k_adj_bond      vector-Jacobian product of the bond law (16 inputs)
+//   k_adj_bond      vector-Jacobian product of the bond law (16 inputs), by
+//                   its hand-written reverse mode
 //   k_adj_fric      vector-Jacobian product of the friction law (24 inputs,
 //                   the history among them, on the recorded slip branch)
 //   k_adj_gather    per particle, the interactions' adjoints in a fixed order,
@@ -659,27 +660,18 @@ __global__ void k_adj_particle(DState Sn, DState Sn1, double4* lam, double* lfm,
 #define NF_TAN 24
 
 // bond law VJP: c[k] = lambda_out . d out / d in_k over the inputs
-// (x1, x2, q1, q2, theta_kn, theta_ks), one thread per (bond, input): 13
-// dual passes per bond, since the outputs depend on the positions only through
-// x2 - x1, so the x1 column is the x2 column negated (bit for bit: the seeds'
-// tangents enter through that difference alone)
-#define NB_DIRS 13
-__device__ __forceinline__ int bond_dir_input(int j)
+// (x1, x2, q1, q2, theta_kn, theta_ks), one thread per bond, by the law's
+// hand-written reverse mode (tbond_law_vjp) on the branch the forward step took
+#define ADJ_BOND_BS 256
+__global__ __launch_bounds__(ADJ_BOND_BS) void k_adj_bond(DState S, DBonds B,
+                                                           const double4* theta, int step,
+                                                           const double* lfm, double* ab)
 {
-  return j < 3 ? 3 + j : (j < 11 ? 6 + (j - 3) : 14 + (j - 11));
-}
-
-__global__ void k_adj_bond(DState S, DBonds B, const double4* theta, int step, const double* lfm,
-                           double* ab)
-{
-  const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (tid >= (long long)B.nb * NB_DIRS) return;
-  const int b = (int)(tid / NB_DIRS), j = (int)(tid - (long long)b * NB_DIRS);
-  const int k = bond_dir_input(j);
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B.nb) return;
   double* c = ab + NB_TAN * (size_t)b;
   if (B.bstep[b] < step) {
-    c[k] = 0.0;
-    if (j < 3) c[j] = 0.0;
+    for (int k = 0; k < NB_TAN; k++) c[k] = 0.0;
     return;
   }
   const int2 e = B.ij[b];
@@ -690,24 +682,25 @@ __global__ void k_adj_bond(DState S, DBonds B, const double4* theta, int step, c
   const double2 tk = B.tkb[s];
   const double* l1 = lfm + 6 * (size_t)e.x;
   const double* l2 = lfm + 6 * (size_t)e.y;
-  const double in[NB_TAN] = {P1.x, P1.y, P1.z, P2.x, P2.y, P2.z, Q1.x, Q1.y,
-                             Q1.z, Q1.w, Q2.x, Q2.y, Q2.z, Q2.w, th.x, th.y};
-  esp_dual u[NB_TAN];
-  for (int t = 0; t < NB_TAN; t++) u[t] = esp_dual(in[t], t == k ? 1.0 : 0.0);
-  esp_dual kr, ks;
-  double kt, kb;
-  tbond_k<esp_dual>(u[14], u[15], tk.x, tk.y, D.w, P1.w, P2.w, 1, 1, 0, kr, ks, kt, kb);
-  TBondResult<esp_dual> r;
-  tbond_law<esp_dual>(tv3<esp_dual>{u[0], u[1], u[2]}, tquat<esp_dual>{u[6], {u[7], u[8], u[9]}},
-                      P1.w, tv3<esp_dual>{u[3], u[4], u[5]},
-                      tquat<esp_dual>{u[10], {u[11], u[12], u[13]}}, P2.w, kr, ks, esp_dual(kt),
-                      esp_dual(kb), tv3<esp_dual>{esp_dual(D.x), esp_dual(D.y), esp_dual(D.z)}, r);
-  // f2 = -f1
-  const double v = (l1[0] - l2[0]) * r.f1.x.d + (l1[1] - l2[1]) * r.f1.y.d +
-                   (l1[2] - l2[2]) * r.f1.z.d + l1[3] * r.m1.x.d + l1[4] * r.m1.y.d +
-                   l1[5] * r.m1.z.d + l2[3] * r.m2.x.d + l2[4] * r.m2.y.d + l2[5] * r.m2.z.d;
-  c[k] = v;
-  if (j < 3) c[j] = -v;
+  double kr, ks, kt, kb, dkr, dks, dkt, dkb;
+  tbond_k<double>(th.x, th.y, tk.x, tk.y, D.w, P1.w, P2.w, 1, 1, 0, kr, ks, kt, kb);
+  // d kr / d theta_kn, d ks / d theta_ks (the constants are linear in them)
+  tbond_k<double>(1.0, 1.0, tk.x, tk.y, D.w, P1.w, P2.w, 1, 1, 0, dkr, dks, dkt, dkb);
+  tv3<double> g1, g2;
+  tquat<double> gq1, gq2;
+  double gkr, gks;
+  // f2 = -f1: particle 2's force adjoint enters with the opposite sign
+  tbond_law_vjp(dx3(P1), tquat<double>{Q1.x, {Q1.y, Q1.z, Q1.w}}, P1.w, dx3(P2),
+                tquat<double>{Q2.x, {Q2.y, Q2.z, Q2.w}}, P2.w, kr, ks, kt, kb,
+                tv3<double>{D.x, D.y, D.z}, tv3<double>{l1[0] - l2[0], l1[1] - l2[1], l1[2] - l2[2]},
+                tv3<double>{l1[3], l1[4], l1[5]}, tv3<double>{l2[3], l2[4], l2[5]}, g1, g2, gq1, gq2,
+                gkr, gks);
+  c[0] = g1.x; c[1] = g1.y; c[2] = g1.z;
+  c[3] = g2.x; c[4] = g2.y; c[5] = g2.z;
+  c[6] = gq1.s; c[7] = gq1.v.x; c[8] = gq1.v.y; c[9] = gq1.v.z;
+  c[10] = gq2.s; c[11] = gq2.v.x; c[12] = gq2.v.y; c[13] = gq2.v.z;
+  c[14] = gkr * dkr;
+  c[15] = gks * dks;
 }
 
 // friction law VJP over the inputs (x1, x2, v1, v2, w1, w2, history,
@@ -1471,7 +1464,7 @@ void backward(esp_diff* d, double* grad)
       hipLaunchKernelGGL(k_adj_particle, dim3(nblocks(n, bs)), dim3(bs), 0, d->stream, Sn, Sn1,
                          lam.p, lfm.p, d->step_params(step), cf);
       if (d->nb > 0)
-        hipLaunchKernelGGL(k_adj_bond, dim3(nblocks((long long)d->nb * NB_DIRS, bs)), dim3(bs), 0,
+        hipLaunchKernelGGL(k_adj_bond, dim3(nblocks(d->nb, ADJ_BOND_BS)), dim3(ADJ_BOND_BS), 0,
                            d->stream, Sn, B, d->theta.p, step, lfm.p, ab.p);
       DCands Cd{E.nc, E.ij.p, E.bond.p};
       if (E.nc > 0)

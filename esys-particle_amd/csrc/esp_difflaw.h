@@ -374,3 +374,302 @@ EDL_HD bool tfriction_law(tv3<T> pos1, tv3<T> vel1, tv3<T> w1, double r1, tv3<T>
   m2 = ffric;
   return false;
 }
+
+// ---------------------------------------------------------------------------
+// Reverse mode of tbond_law: given the adjoints lf1, lm1, lm2 of (f1, m1, m2),
+// the adjoints of the inputs (x1, x2, q1, q2, kr, ks), on the branch the
+// forward evaluation takes (the same branches as tbond_law<double>; at a
+// square root or arc cosine of zero slope the derivative is taken as zero, as
+// the dual numbers do).  One forward pass keeping its intermediates, then the
+// chain rule backwards through it: a few forward passes' work instead of one
+// dual pass per input.
+// ---------------------------------------------------------------------------
+namespace edl_rev {
+typedef tv3<double> V;
+typedef tquat<double> Q;
+typedef tmat3<double> M;
+
+EDL_HD V vz() { return V{0.0, 0.0, 0.0}; }
+EDL_HD void vacc(V& a, V b) { a = tadd(a, b); }
+// out = m^T v: v's adjoint m lo, m's adjoint v (x) lo
+EDL_HD V mt_adj(const M& m, V v, V lo, M& lm)
+{
+  lm.m[0][0] += v.x * lo.x; lm.m[0][1] += v.x * lo.y; lm.m[0][2] += v.x * lo.z;
+  lm.m[1][0] += v.y * lo.x; lm.m[1][1] += v.y * lo.y; lm.m[1][2] += v.y * lo.z;
+  lm.m[2][0] += v.z * lo.x; lm.m[2][1] += v.z * lo.y; lm.m[2][2] += v.z * lo.z;
+  return tmmul(m, lo);
+}
+// out = m v: v's adjoint m^T lo, m's adjoint lo (x) v
+EDL_HD V m_adj(const M& m, V v, V lo, M& lm)
+{
+  lm.m[0][0] += lo.x * v.x; lm.m[0][1] += lo.x * v.y; lm.m[0][2] += lo.x * v.z;
+  lm.m[1][0] += lo.y * v.x; lm.m[1][1] += lo.y * v.y; lm.m[1][2] += lo.y * v.z;
+  lm.m[2][0] += lo.z * v.x; lm.m[2][1] += lo.z * v.y; lm.m[2][2] += lo.z * v.z;
+  return tmtmul(m, lo);
+}
+// adjoint of tqmat(q) for the matrix adjoint lm
+EDL_HD Q qmat_adj(Q q, const M& L)
+{
+  const double s = q.s, x = q.v.x, y = q.v.y, z = q.v.z;
+  const double (*l)[3] = L.m;
+  Q g;
+  g.s = 2.0 * (s * l[0][0] + z * l[0][1] - y * l[0][2] - z * l[1][0] + s * l[1][1] + x * l[1][2] +
+               y * l[2][0] - x * l[2][1] + s * l[2][2]);
+  g.v.x = 2.0 * (x * l[0][0] + y * l[0][1] + z * l[0][2] + y * l[1][0] - x * l[1][1] + s * l[1][2] +
+                 z * l[2][0] - s * l[2][1] - x * l[2][2]);
+  g.v.y = 2.0 * (-y * l[0][0] + x * l[0][1] - s * l[0][2] + x * l[1][0] + y * l[1][1] + z * l[1][2] +
+                 s * l[2][0] + z * l[2][1] - y * l[2][2]);
+  g.v.z = 2.0 * (-z * l[0][0] + s * l[0][1] + x * l[0][2] - s * l[1][0] - z * l[1][1] + y * l[1][2] +
+                 x * l[2][0] + y * l[2][1] + z * l[2][2]);
+  return g;
+}
+// c = a b: adjoints of a and b for the adjoint lc of c
+EDL_HD void qmul_adj(Q a, Q b, Q lc, Q& la, Q& lb)
+{
+  la.s += lc.s * b.s + tdot(lc.v, b.v);
+  la.v = tadd(la.v, tadd(tsub(tmul(lc.v, b.s), tmul(b.v, lc.s)), tcross(b.v, lc.v)));
+  lb.s += lc.s * a.s + tdot(lc.v, a.v);
+  lb.v = tadd(lb.v, tadd(tsub(tmul(lc.v, a.s), tmul(a.v, lc.s)), tcross(lc.v, a.v)));
+}
+EDL_HD double dsqrt_half(double s) { return s > 0.0 ? 0.5 / s : 0.0; }   // d sqrt(u)/du
+EDL_HD double dacos(double c)
+{
+  const double s = sqrt(1.0 - c * c);
+  return s > 0.0 ? -1.0 / s : 0.0;
+}
+}  // namespace edl_rev
+
+EDL_HD void tbond_law_vjp(tv3<double> pos1, tquat<double> q1, double r1, tv3<double> pos2,
+                          tquat<double> q2, double r2, double kr, double ks, double kt, double kb,
+                          tv3<double> D0, tv3<double> lf1, tv3<double> lm1, tv3<double> lm2,
+                          tv3<double>& g_pos1, tv3<double>& g_pos2, tquat<double>& g_q1,
+                          tquat<double>& g_q2, double& g_kr, double& g_ks)
+{
+  using namespace edl_rev;
+  typedef double T;
+  const double HALF_SQRT_2 = 0.5 * sqrt(2.0);
+  // ---- forward (tbond_law's expressions), intermediates kept ----
+  const M mat0 = tqmat(q1);
+  const V rbp = tsub(pos2, pos1);
+  const T rbpNorm = tnorm(rbp);
+  const V rb = tmmul(mat0, rbp);
+  const T rbNorm = tnorm(rb);
+  const double r0 = r1 + r2;
+  const T sdr = rbNorm - r0;
+  const V delta_r = tdiv(tmul(rb, sdr), rbNorm);
+  const V Fr = tmul(delta_r, kr);
+  const T D0n = tnorm(D0);
+  const T gden = rbNorm * D0n;
+  const T gama_cos = tdot(rb, D0) / gden;
+  const bool g_in = gama_cos < 1.0 && gama_cos > -1.0;
+  const T gama = g_in ? acos(gama_cos) : 0.0;
+  const V rb_0 = tcross(rb, D0);
+  const V rb_b0 = tcross(rb, rb_0);
+  const T B = tnorm(rb_b0);
+  const T gmul = ks * r0 * gama;
+  V Fst = vz();
+  if (B != 0.0) Fst = tdiv(tmul(rb_b0, gmul), B);
+  const T R0 = tnorm(rb_0);
+  const T FstN = tnorm(Fst);
+  V Mst = vz();
+  if (R0 != 0.0) Mst = tdiv(tmul(rb_0, -FstN), R0);
+  const T a0 = (rbNorm + rb.z) / rbNorm, a1 = (rbNorm - rb.z) / rbNorm;
+  const T sa0 = sqrt(a0), sa1 = sqrt(a1);
+  const T m0 = HALF_SQRT_2 * sa0;
+  const T m1a = -HALF_SQRT_2 * sa1;
+  T m1 = m1a, m2 = -m1a;
+  const T xy2 = rb.x * rb.x + rb.y * rb.y;
+  T dS = 0.0;
+  if (xy2 != 0.0) {
+    dS = sqrt(xy2);
+    m1 = m1a * rb.y / dS;
+    m2 = (-m1a) * rb.x / dS;
+  }
+  const Q qm = {m0, {m1, m2, 0.0}};
+  const M mat = tqmat(qm);
+  const Q iq1 = tqinv(q1), iqm = tqinv(qm);
+  const Q rp = tqmul(iq1, q2);
+  const Q u = tqmul(iqm, rp);
+  const Q r = tqmul(u, qm);
+  const T temp0 = r.s * r.s + r.v.z * r.v.z;
+  T pasi = 0.0, sq0 = 0.0, c_p = 0.0, dang = 0.0;
+  if (temp0 != 0.0) {
+    sq0 = sqrt(temp0);
+    c_p = r.s / sq0;
+    const T s_p = r.v.z / sq0;
+    const bool out = c_p >= 1.0 || c_p <= -1.0;
+    if (s_p > 0.0) {
+      pasi = 2.0 * (out ? 0.0 : acos(c_p));
+      dang = out ? 0.0 : dacos(c_p);
+    } else if (s_p < 0.0) {
+      pasi = 2.0 * (out ? 0.0 : -acos(c_p));
+      dang = out ? 0.0 : -dacos(c_p);
+    }
+  }
+  const V mt = tmk<T>(0.0, 0.0, kt * pasi);
+  const V Mtr = tmtmul(mat, mt);
+  const T temp2 = r.v.x * r.v.x + r.v.y * r.v.y;
+  const T temp3 = r.v.x * r.v.z + r.s * r.v.y;
+  const T temp4 = r.v.y * r.v.z - r.s * r.v.x;
+  V Mbr = vz(), Fsr = vz(), Msr = vz(), mb = vz(), fs = vz(), ms = vz();
+  T sita = 0.0, c_fai = 0.0, s_fai = 0.0, sD = 0.0, temp1 = 0.0;
+  const bool bend = temp2 != 0.0;
+  if (bend) {
+    temp1 = temp0 - temp2;
+    if (!(temp1 >= 1.0 || temp1 <= -1.0)) sita = acos(temp1);
+    if (temp0 == 0.0) {
+      sD = sqrt(temp2);
+      c_fai = r.v.y / sD;
+      s_fai = -r.v.x / sD;
+    } else {
+      sD = sqrt(temp0 * temp2);
+      c_fai = temp3 / sD;
+      s_fai = temp4 / sD;
+    }
+    mb = tmk<T>(-0.5 * kb * sita * s_fai, 0.5 * kb * sita * c_fai, 0.0);
+    fs = tmk<T>(-0.5 * ks * c_fai * sita * r0, -0.5 * ks * s_fai * sita * r0, 0.0);
+    ms = tmk<T>(0.5 * ks * s_fai * sita * r0, -0.5 * ks * c_fai * sita * r0, 0.0);
+    Mbr = tmtmul(mat, mb);
+    Fsr = tmtmul(mat, fs);
+    Msr = tmtmul(mat, ms);
+  }
+  const double eq1 = r1 / (r1 + r2), eq2 = r2 / (r1 + r2);
+  const V A = tadd(tadd(Fr, Fst), Fsr);
+  const V Bv = tadd(tadd(tadd(Mbr, Mtr), tmul(tmul(Msr, eq1), rbpNorm)), tmul(tmul(Mst, eq1), rbpNorm));
+  const V C = tsub(tsub(tadd(tmul(tmul(Msr, eq2), rbpNorm), tmul(tmul(Mst, eq2), rbpNorm)), Mbr), Mtr);
+
+  // ---- reverse ----
+  M lmat0 = {}, lmat = {};
+  const V lA = mt_adj(mat0, A, lf1, lmat0);
+  const V lB = mt_adj(mat0, Bv, lm1, lmat0);
+  const V lC = mt_adj(mat0, C, lm2, lmat0);
+  const V lFr = lA, lFst0 = lA, lFsr = lA;
+  const V lMbr = tsub(lB, lC), lMtr = tsub(lB, lC);
+  const V lMsr = tadd(tmul(lB, eq1 * rbpNorm), tmul(lC, eq2 * rbpNorm));
+  const V lMst = lMsr;
+  const T lrbpNorm = (tdot(Msr, lB) + tdot(Mst, lB)) * eq1 + (tdot(Msr, lC) + tdot(Mst, lC)) * eq2;
+  // twist
+  const V lmt = mt_adj(mat, mt, lMtr, lmat);
+  const T lpasi = kt * lmt.z;
+  // bending
+  Q lr = {0.0, vz()};
+  T lks = 0.0, ltemp0 = 0.0, ltemp2 = 0.0, ltemp3 = 0.0, ltemp4 = 0.0;
+  if (bend) {
+    const V lmb = mt_adj(mat, mb, lMbr, lmat);
+    const V lfs = mt_adj(mat, fs, lFsr, lmat);
+    const V lms = mt_adj(mat, ms, lMsr, lmat);
+    const T lsita = -0.5 * kb * s_fai * lmb.x + 0.5 * kb * c_fai * lmb.y -
+                    0.5 * ks * c_fai * r0 * lfs.x - 0.5 * ks * s_fai * r0 * lfs.y +
+                    0.5 * ks * s_fai * r0 * lms.x - 0.5 * ks * c_fai * r0 * lms.y;
+    const T ls_fai = -0.5 * kb * sita * lmb.x - 0.5 * ks * sita * r0 * lfs.y +
+                     0.5 * ks * sita * r0 * lms.x;
+    const T lc_fai = 0.5 * kb * sita * lmb.y - 0.5 * ks * sita * r0 * lfs.x -
+                     0.5 * ks * sita * r0 * lms.y;
+    lks += -0.5 * c_fai * sita * r0 * lfs.x - 0.5 * s_fai * sita * r0 * lfs.y +
+           0.5 * s_fai * sita * r0 * lms.x - 0.5 * c_fai * sita * r0 * lms.y;
+    if (!(temp1 >= 1.0 || temp1 <= -1.0)) {
+      const T ltemp1 = lsita * dacos(temp1);
+      ltemp0 += ltemp1;
+      ltemp2 -= ltemp1;
+    }
+    const T lsD = -(lc_fai * c_fai + ls_fai * s_fai) / sD;
+    if (temp0 == 0.0) {
+      lr.v.y += lc_fai / sD;
+      lr.v.x -= ls_fai / sD;
+      ltemp2 += lsD * dsqrt_half(sD);
+    } else {
+      ltemp3 += lc_fai / sD;
+      ltemp4 += ls_fai / sD;
+      const T lP = lsD * dsqrt_half(sD);
+      ltemp0 += lP * temp2;
+      ltemp2 += lP * temp0;
+    }
+  }
+  lr.v.x += ltemp3 * r.v.z - ltemp4 * r.s + 2.0 * r.v.x * ltemp2;
+  lr.v.z += ltemp3 * r.v.x + ltemp4 * r.v.y;
+  lr.s += ltemp3 * r.v.y - ltemp4 * r.v.x;
+  lr.v.y += ltemp3 * r.s + ltemp4 * r.v.z + 2.0 * r.v.y * ltemp2;
+  if (temp0 != 0.0) {
+    const T lc_p = 2.0 * dang * lpasi;
+    lr.s += lc_p / sq0;
+    const T lsq0 = -lc_p * c_p / sq0;
+    ltemp0 += lsq0 * dsqrt_half(sq0);
+  }
+  lr.s += 2.0 * r.s * ltemp0;
+  lr.v.z += 2.0 * r.v.z * ltemp0;
+  // r = u qm, u = qm^-1 rp, rp = q1^-1 q2
+  Q lu = {0.0, vz()}, lqm = {0.0, vz()}, liqm = {0.0, vz()}, lrp = {0.0, vz()}, liq1 = {0.0, vz()};
+  Q lq2 = {0.0, vz()};
+  qmul_adj(u, qm, lr, lu, lqm);
+  qmul_adj(iqm, rp, lu, liqm, lrp);
+  qmul_adj(iq1, q2, lrp, liq1, lq2);
+  lqm.s += liqm.s;
+  lqm.v = tsub(lqm.v, liqm.v);
+  const Q lqm_m = qmat_adj(qm, lmat);
+  lqm.s += lqm_m.s;
+  lqm.v = tadd(lqm.v, lqm_m.v);
+  // qm = (m0, m1, m2, 0) from rb
+  V lrb = vz();
+  T lN = 0.0;
+  T lm1a = 0.0;
+  const T lqm0 = lqm.s, lqm1 = lqm.v.x, lqm2 = lqm.v.y;
+  if (xy2 != 0.0) {
+    lm1a += lqm1 * rb.y / dS - lqm2 * rb.x / dS;
+    lrb.y += lqm1 * m1a / dS;
+    lrb.x += lqm2 * (-m1a) / dS;
+    const T ldS = -(lqm1 * m1 + lqm2 * m2) / dS;
+    const T lxy2 = ldS * dsqrt_half(dS);
+    lrb.x += 2.0 * rb.x * lxy2;
+    lrb.y += 2.0 * rb.y * lxy2;
+  } else {
+    lm1a += lqm1 - lqm2;
+  }
+  const T la0 = lqm0 * HALF_SQRT_2 * dsqrt_half(sa0);
+  const T la1 = lm1a * (-HALF_SQRT_2) * dsqrt_half(sa1);
+  lN += la0 * (1.0 - a0) / rbNorm + la1 * (1.0 - a1) / rbNorm;
+  lrb.z += la0 / rbNorm - la1 / rbNorm;
+  // Mst = rb_0 (-|Fst|) / |rb_0|
+  V lFst = lFst0, lrb_0 = vz(), lrb_b0 = vz();
+  if (R0 != 0.0) {
+    lrb_0 = tadd(lrb_0, tmul(lMst, -FstN / R0));
+    const T lFstN = -tdot(lMst, rb_0) / R0;
+    const T lR0 = -tdot(lMst, Mst) / R0;
+    lrb_0 = tadd(lrb_0, tmul(rb_0, lR0 / R0));
+    if (FstN > 0.0) lFst = tadd(lFst, tmul(Fst, lFstN / FstN));
+  }
+  // Fst = rb_b0 (ks r0 gama) / |rb_b0|
+  T lgama = 0.0;
+  if (B != 0.0) {
+    lrb_b0 = tadd(lrb_b0, tmul(lFst, gmul / B));
+    const T lgmul = tdot(lFst, rb_b0) / B;
+    const T lBn = -tdot(lFst, Fst) / B;
+    lrb_b0 = tadd(lrb_b0, tmul(rb_b0, lBn / B));
+    lks += lgmul * r0 * gama;
+    lgama = lgmul * ks * r0;
+  }
+  // rb_b0 = rb x rb_0, rb_0 = rb x D0
+  lrb = tadd(lrb, tcross(rb_0, lrb_b0));
+  lrb_0 = tadd(lrb_0, tcross(lrb_b0, rb));
+  lrb = tadd(lrb, tcross(D0, lrb_0));
+  // gama = acos(rb . D0 / (|rb| |D0|))
+  if (g_in) {
+    const T lgc = lgama * dacos(gama_cos);
+    lrb = tadd(lrb, tmul(D0, lgc / gden));
+    lN += -lgc * gama_cos / gden * D0n;
+  }
+  // Fr = kr (rb (|rb| - r0)) / |rb|
+  const V ldr = tmul(lFr, kr);
+  g_kr = tdot(lFr, delta_r);
+  lrb = tadd(lrb, tmul(ldr, sdr / rbNorm));
+  lN += tdot(ldr, rb) / rbNorm - tdot(ldr, delta_r) / rbNorm;
+  lrb = tadd(lrb, tmul(rb, lN / rbNorm));
+  // rb = mat0 rbp
+  V lrbp = m_adj(mat0, rbp, lrb, lmat0);
+  if (rbpNorm > 0.0) lrbp = tadd(lrbp, tmul(rbp, lrbpNorm / rbpNorm));
+  const Q lq1m = qmat_adj(q1, lmat0);
+  g_q1 = tquat<double>{lq1m.s + liq1.s, tsub(lq1m.v, liq1.v)};
+  g_q2 = lq2;
+  g_pos2 = lrbp;
+  g_pos1 = tneg(lrbp);
+  g_ks = lks;
+}

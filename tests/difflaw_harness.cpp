@@ -73,6 +73,22 @@ void dl_bond_tangent(const double* in, const double* prm, int k, double* out)
 {
   bond_eval<esp_dual>(in, prm, k, out, true);
 }
+// reverse mode: grad[16] = lam[9] . d(f1, m1, m2) / d(x1, x2, q1, q2, kr, ks)
+void dl_bond_vjp(const double* in, const double* prm, const double* lam, double* grad)
+{
+  tv3<double> g1, g2;
+  tquat<double> gq1, gq2;
+  double gkr, gks;
+  tbond_law_vjp(tv3<double>{in[0], in[1], in[2]}, tquat<double>{in[6], {in[7], in[8], in[9]}},
+                prm[0], tv3<double>{in[3], in[4], in[5]},
+                tquat<double>{in[10], {in[11], in[12], in[13]}}, prm[1], prm[2], prm[3], prm[4],
+                prm[5], tv3<double>{prm[6], prm[7], prm[8]}, tv3<double>{lam[0], lam[1], lam[2]},
+                tv3<double>{lam[3], lam[4], lam[5]}, tv3<double>{lam[6], lam[7], lam[8]}, g1, g2,
+                gq1, gq2, gkr, gks);
+  const double g[16] = {g1.x, g1.y, g1.z, g2.x, g2.y, g2.z, gq1.s, gq1.v.x,
+                        gq1.v.y, gq1.v.z, gq2.s, gq2.v.x, gq2.v.y, gq2.v.z, gkr, gks};
+  for (int t = 0; t < 16; t++) grad[t] = g[t];
+}
 int dl_fric(const double* in, const double* prm, int slip, double* out)
 {
   return fric_eval<double>(in, prm, slip, -1, out, false);

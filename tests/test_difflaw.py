@@ -1,7 +1,9 @@
 """The laws of the differentiable engine (esys-particle_amd/csrc/esp_difflaw.h),
 compiled for the host: on doubles they equal the oracle's laws bit for bit
 (so the closed forms that pin the oracle pin them too); on dual numbers their
-tangents equal central finite differences of the double evaluation."""
+tangents equal central finite differences of the double evaluation; the
+hand-written reverse mode of the bond law equals the dual-number Jacobian's
+transpose applied to random output adjoints, on every branch of the law."""
 import ctypes
 import math
 import os
@@ -30,6 +32,7 @@ def dl(tmp_path_factory):
     lib = ctypes.CDLL(so)
     lib.dl_bond.argtypes = [pd, pd, pd]
     lib.dl_bond_tangent.argtypes = [pd, pd, ctypes.c_int, pd]
+    lib.dl_bond_vjp.argtypes = [pd, pd, pd, pd]
     lib.dl_fric.argtypes = [pd, pd, ctypes.c_int, pd]
     lib.dl_fric.restype = ctypes.c_int
     lib.dl_fric_tangent.argtypes = [pd, pd, ctypes.c_int, ctypes.c_int, pd]
@@ -148,3 +151,72 @@ def test_friction_tangents_equal_finite_differences(dl):
             _fd_check(lambda i, p, o: dl.dl_fric(P(i), P(p), slip, P(o)),
                       lambda i, p, k, o: dl.dl_fric_tangent(P(i), P(p), slip, k, P(o)),
                       inp, prm, 24, 12, lambda t, fd: max(np.abs(fd).max(), 1.0))
+
+
+def _vjp_cases(rng):
+    """Random bonds, and the special configurations whose branches random ones
+    miss: both particles unrotated (no relative rotation: no twist, no
+    bending), a bond along z (the bond frame's xy branch), a bond parallel to
+    its creation vector (no lateral shear), pure twist (no bending) and pure
+    bending about the bond frame's axis."""
+    for _ in range(200):
+        yield bond_case(rng)
+    for _ in range(20):
+        inp, prm = bond_case(rng)
+        inp[6:14] = [1, 0, 0, 0, 1, 0, 0, 0]
+        yield inp, prm
+    for _ in range(20):
+        inp, prm = bond_case(rng)
+        inp[3:6] = inp[0:3] + np.array([0.0, 0.0, prm[0] + prm[1]]) * rng.uniform(0.99, 1.01)
+        prm[6:9] = [0.0, 0.0, prm[0] + prm[1]]
+        yield inp, prm
+    for _ in range(20):
+        inp, prm = bond_case(rng)
+        d = prm[6:9] / np.linalg.norm(prm[6:9])
+        inp[3:6] = inp[0:3] + d * (prm[0] + prm[1]) * rng.uniform(0.99, 1.01)
+        inp[6:10] = [1, 0, 0, 0]
+        yield inp, prm
+    for _ in range(20):
+        inp, prm = bond_case(rng)
+        inp[6:10] = [1, 0, 0, 0]
+        d = (inp[3:6] - inp[0:3]) / np.linalg.norm(inp[3:6] - inp[0:3])
+        t = rng.uniform(-0.3, 0.3)
+        inp[10:14] = [math.cos(t / 2), *(math.sin(t / 2) * d)]
+        yield inp, prm
+
+
+def test_bond_reverse_mode_equals_dual_jacobian(dl):
+    rng = np.random.default_rng(11)
+    worst = 0.0
+    for inp, prm in _vjp_cases(rng):
+        lam = rng.normal(size=9)
+        cols = np.zeros((16, 13))
+        for k in range(16):
+            dl.dl_bond_tangent(P(inp), P(prm), k, P(cols[k]))
+        want = cols[:, :9] @ lam
+        got = np.zeros(16)
+        dl.dl_bond_vjp(P(inp), P(prm), P(lam), P(got))
+        # each input's error against the magnitude of the terms its sum adds
+        scale = np.abs(cols[:, :9]) @ np.abs(lam) + 1e-300
+        err = np.max(np.abs(got - want) / scale)
+        worst = max(worst, err)
+        assert err <= 1e-9, (err, inp, prm)
+    print("worst relative difference %.3g" % worst)
+
+
+def test_bond_reverse_mode_planted_mistake_fails(dl):
+    """The comparison above catches a plausible reverse-mode slip: dropping
+    the kr column's contribution (g_kr) or negating the x1 column."""
+    rng = np.random.default_rng(12)
+    inp, prm = bond_case(rng)
+    lam = rng.normal(size=9)
+    cols = np.zeros((16, 13))
+    for k in range(16):
+        dl.dl_bond_tangent(P(inp), P(prm), k, P(cols[k]))
+    want = cols[:, :9] @ lam
+    got = np.zeros(16)
+    dl.dl_bond_vjp(P(inp), P(prm), P(lam), P(got))
+    scale = np.abs(cols[:, :9]) @ np.abs(lam) + 1e-300
+    assert np.max(np.abs(got - want) / scale) <= 1e-9
+    for bad in (lambda g: np.r_[g[:14], 0.0, g[15]], lambda g: np.r_[-g[:3], g[3:]]):
+        assert np.max(np.abs(bad(got) - want) / scale) > 1e-3
