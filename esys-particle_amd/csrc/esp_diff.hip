@@ -24,7 +24,8 @@
 //   k_adj_bond      vector-Jacobian product of the bond law (16 inputs), by
 //                   its hand-written reverse mode
 //   k_adj_fric      vector-Jacobian product of the friction law (24 inputs,
-//                   the history among them, on the recorded slip branch)
+//                   the history among them, on the recorded slip branch), by
+//                   its hand-written reverse mode
 //   k_adj_gather    per particle, the interactions' adjoints in a fixed order,
 //                   and the parameter gradients credited to the first particle
 // The laws' Jacobian columns come from their dual-number evaluation
@@ -704,28 +705,17 @@ __global__ __launch_bounds__(ADJ_BOND_BS) void k_adj_bond(DState S, DBonds B,
 }
 
 // friction law VJP over the inputs (x1, x2, v1, v2, w1, w2, history,
-// theta_kn, theta_ks, theta_mu), one thread per (pair, input): 18 dual passes
-// per pair (the law depends on the positions through x1 - x2 and on the
-// velocities through v2 - v1, so the x1 and v1 columns are the x2 and v2 ones
-// negated); lh: adjoint of the history this step wrote; lh_prev: adjoint of
-// the history it read
-#define NF_DIRS 18
-__device__ __forceinline__ int fric_dir_input(int j)
-{
-  // x2 (3..5), v2 (9..11), w1 (12..14), w2 (15..17), history (18..20), theta (21..23)
-  return j < 3 ? 3 + j : 6 + j;
-}
-
+// theta_kn, theta_ks, theta_mu), one thread per pair, by the law's hand-written
+// reverse mode (tfriction_law_vjp) on the slip branch the forward step took;
+// lh: adjoint of the history this step wrote; lh_prev: adjoint of the history
+// it read
 __global__ void k_adj_fric(DState S, DCands C, const int* bstep, const double4* theta,
                            const double4* hin, int step, double dt, const double* lfm,
                            const double4* lh, double* ac, double4* lh_prev)
 {
-  const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (tid >= (long long)C.nc * NF_DIRS) return;
-  const int c = (int)(tid / NF_DIRS), j = (int)(tid - (long long)c * NF_DIRS);
-  const int k = fric_dir_input(j);
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= C.nc) return;
   double* o = ac + NF_TAN * (size_t)c;
-  double* lp = reinterpret_cast<double*>(lh_prev + c);
   const int bb = C.bond[c];
   const int2 e = C.ij[c];
   const double4 P1 = S.x[e.x], P2 = S.x[e.y];
@@ -733,10 +723,8 @@ __global__ void k_adj_fric(DState S, DCands C, const int* bstep, const double4* 
   const double dxx = P1.x - P2.x, dyy = P1.y - P2.y, dzz = P1.z - P2.z;
   const double eq = P1.w + P2.w;
   if ((bb >= 0 && bstep[bb] >= step) || !((dxx * dxx + dyy * dyy + dzz * dzz) < eq * eq)) {
-    o[k] = 0.0;
-    if (j < 6) o[k - 3] = 0.0;  // the x1 / v1 columns
-    if (j >= 12 && j < 15) lp[j - 12] = 0.0;
-    if (j == 15) lp[3] = 0.0;
+    for (int k = 0; k < NF_TAN; k++) o[k] = 0.0;
+    lh_prev[c] = make_double4(0.0, 0.0, 0.0, 0.0);
     return;
   }
   const double4 V1 = S.v[e.x], V2 = S.v[e.y], W1 = S.w[e.x], W2 = S.w[e.y];
@@ -745,27 +733,26 @@ __global__ void k_adj_fric(DState S, DCands C, const int* bstep, const double4* 
   const double4 lo = lh[c];
   const double* l1 = lfm + 6 * (size_t)e.x;
   const double* l2 = lfm + 6 * (size_t)e.y;
-  const double in[NF_TAN] = {P1.x, P1.y, P1.z, P2.x, P2.y, P2.z, V1.x, V1.y, V1.z, V2.x, V2.y, V2.z,
-                             W1.x, W1.y, W1.z, W2.x, W2.y, W2.z, hh.x, hh.y, hh.z, th.x, th.y, th.w};
-  esp_dual u[NF_TAN];
-  for (int t = 0; t < NF_TAN; t++) u[t] = esp_dual(in[t], t == k ? 1.0 : 0.0);
-  esp_dual kk, kks;
-  tfriction_k<esp_dual>(u[21], u[22], 1, 1, P1.w, P2.w, 0, kk, kks);
-  tv3<esp_dual> ffr{u[18], u[19], u[20]}, f1, m1, m2;
-  bool slipping = (((int)hh.w) & 1) != 0;
-  tfriction_law<esp_dual>(tv3<esp_dual>{u[0], u[1], u[2]}, tv3<esp_dual>{u[6], u[7], u[8]},
-                          tv3<esp_dual>{u[12], u[13], u[14]}, P1.w,
-                          tv3<esp_dual>{u[3], u[4], u[5]}, tv3<esp_dual>{u[9], u[10], u[11]},
-                          tv3<esp_dual>{u[15], u[16], u[17]}, P2.w, kk, kks, u[23], u[23], dt, ffr,
-                          slipping, f1, m1, m2);
-  const double v = (l1[0] - l2[0]) * f1.x.d + (l1[1] - l2[1]) * f1.y.d +
-                   (l1[2] - l2[2]) * f1.z.d + l1[3] * m1.x.d + l1[4] * m1.y.d + l1[5] * m1.z.d +
-                   l2[3] * m2.x.d + l2[4] * m2.y.d + l2[5] * m2.z.d + lo.x * ffr.x.d +
-                   lo.y * ffr.y.d + lo.z * ffr.z.d;
-  o[k] = v;
-  if (j < 6) o[k - 3] = -v;  // x1 = -x2 column, v1 = -v2 column
-  if (j >= 12 && j < 15) lp[j - 12] = v;
-  if (j == 15) lp[3] = 0.0;
+  double kk, kks, dk, dks;
+  tfriction_k<double>(th.x, th.y, 1, 1, P1.w, P2.w, 0, kk, kks);
+  // d k / d theta_kn, d ks / d theta_ks (linear in them)
+  tfriction_k<double>(1.0, 1.0, 1, 1, P1.w, P2.w, 0, dk, dks);
+  tv3<double> g[8];
+  // f2 = -f1: particle 2's force adjoint enters with the opposite sign
+  tfriction_law_vjp(dx3(P1), dx3(V1), dx3(W1), P1.w, dx3(P2), dx3(V2), dx3(W2), P2.w, kk, kks, th.w,
+                    dt, tv3<double>{hh.x, hh.y, hh.z}, (((int)hh.w) & 1) != 0,
+                    tv3<double>{l1[0] - l2[0], l1[1] - l2[1], l1[2] - l2[2]},
+                    tv3<double>{l1[3], l1[4], l1[5]}, tv3<double>{l2[3], l2[4], l2[5]},
+                    tv3<double>{lo.x, lo.y, lo.z}, g);
+  for (int t = 0; t < 7; t++) {
+    o[3 * t] = g[t].x;
+    o[3 * t + 1] = g[t].y;
+    o[3 * t + 2] = g[t].z;
+  }
+  o[21] = g[7].x * dk;
+  o[22] = g[7].y * dks;
+  o[23] = g[7].z;
+  lh_prev[c] = make_double4(g[6].x, g[6].y, g[6].z, 0.0);
 }
 
 // per particle: add the interactions' adjoints (CSR order) to the state
@@ -1468,8 +1455,7 @@ void backward(esp_diff* d, double* grad)
                            d->stream, Sn, B, d->theta.p, step, lfm.p, ab.p);
       DCands Cd{E.nc, E.ij.p, E.bond.p};
       if (E.nc > 0)
-        hipLaunchKernelGGL(k_adj_fric, dim3(nblocks((long long)E.nc * NF_DIRS, bs)), dim3(bs), 0,
-                           d->stream, Sn, Cd,
+        hipLaunchKernelGGL(k_adj_fric, dim3(nblocks(E.nc, bs)), dim3(bs), 0, d->stream, Sn, Cd,
                            d->bstep.p, d->theta.p, htape[t]->p, step, d->cfg.dt, lfm.p, lh.p, ac.p,
                            lh2.p);
       hipLaunchKernelGGL(k_adj_gather, dim3(nblocks(n, bs)), dim3(bs), 0, d->stream, n,

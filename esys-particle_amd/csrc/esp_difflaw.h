@@ -673,3 +673,98 @@ EDL_HD void tbond_law_vjp(tv3<double> pos1, tquat<double> q1, double r1, tv3<dou
   g_pos1 = tneg(lrbp);
   g_ks = lks;
 }
+
+// Reverse mode of tfriction_law for a touching pair (the caller handles pairs
+// apart: their outputs are constants): given the adjoints of (f1, m1, m2) and
+// of the history written (lh), the adjoints of (x1, x2, v1, v2, w1, w2, the
+// history read, k, ks, mu) on the slip branch the forward evaluation takes
+// (mu_d = mu_s = mu).
+EDL_HD void tfriction_law_vjp(tv3<double> pos1, tv3<double> vel1, tv3<double> w1, double r1,
+                              tv3<double> pos2, tv3<double> vel2, tv3<double> w2, double r2,
+                              double k, double ks, double mu, double dt, tv3<double> ffric,
+                              bool slipping, tv3<double> lf1, tv3<double> lm1, tv3<double> lm2,
+                              tv3<double> lh, tv3<double>* g)
+{
+  using namespace edl_rev;
+  typedef double T;
+  (void)slipping;  // mu_d = mu_s: the slip flag of the last step selects the same mu
+  // ---- forward (tfriction_law's expressions) ----
+  const V D = tsub(pos1, pos2);
+  const T dist2 = tdot(D, D);
+  const double eq = r1 + r2;
+  const T dist = sqrt(dist2);
+  const T sc = k * (dist - eq) / dist;
+  const V force = tmul(D, sc);
+  const double cr = r2 / eq;
+  const V pos = tadd(pos2, tmul(D, cr));
+  const V a1 = tsub(pos, pos1), a2 = tsub(pos, pos2);
+  const V vp1 = tadd(vel1, tcross(w1, a1));
+  const V vp2 = tadd(vel2, tcross(w2, a2));
+  const V ds0 = tmul(tsub(vp2, vp1), dt);
+  const T p = tdot(ds0, D), q = tdot(D, D);
+  const T t = p / q;
+  const V ds = tsub(ds0, tmul(D, t));
+  const V ffr1 = tadd(ffric, tmul(ds, ks));
+  const T Fn = tnorm(ffr1);
+  const T Nn = tnorm(force);
+  const bool slip = Fn > Nn * mu;
+  T sc2 = 1.0;
+  V ffr2 = ffr1;
+  if (slip) {
+    sc2 = (mu * Nn) / Fn;
+    ffr2 = tmul(ffr1, sc2);
+  }
+  // ---- reverse ----
+  // m1 = a1 x ffr2, m2 = a2 x (-ffr2), f1 = -force + ffr2, history out = ffr2
+  V lffr2 = tadd(tadd(lh, lf1), tsub(tcross(lm1, a1), tcross(lm2, a2)));
+  V la1 = tcross(ffr2, lm1);
+  V la2 = tcross(tneg(ffr2), lm2);
+  V lforce = tneg(lf1);
+  V lffr1 = lffr2;
+  T lmu = 0.0;
+  if (slip) {
+    lffr1 = tmul(lffr2, sc2);
+    const T lsc2 = tdot(lffr2, ffr1);
+    lmu += lsc2 * Nn / Fn;
+    const T lNn = lsc2 * mu / Fn;
+    const T lFn = -lsc2 * sc2 / Fn;
+    if (Fn > 0.0) lffr1 = tadd(lffr1, tmul(ffr1, lFn / Fn));
+    if (Nn > 0.0) lforce = tadd(lforce, tmul(force, lNn / Nn));
+  }
+  // ffr1 = ffric + ds ks
+  const V lffric = lffr1;
+  const V lds = tmul(lffr1, ks);
+  const T lks = tdot(lffr1, ds);
+  // ds = ds0 - D (ds0 . D) / (D . D)
+  V lD = tmul(lds, -t);
+  const T lt = -tdot(lds, D);
+  const T lp = lt / q, lq = -lt * t / q;
+  const V lds0 = tadd(lds, tmul(D, lp));
+  lD = tadd(lD, tadd(tmul(ds0, lp), tmul(D, 2.0 * lq)));
+  // ds0 = (vp2 - vp1) dt; vp = v + w x a
+  const V lvp2 = tmul(lds0, dt), lvp1 = tmul(lds0, -dt);
+  const V lw1 = tcross(a1, lvp1), lw2 = tcross(a2, lvp2);
+  la1 = tadd(la1, tcross(lvp1, w1));
+  la2 = tadd(la2, tcross(lvp2, w2));
+  // a1 = pos - x1, a2 = pos - x2, pos = x2 + D r2 / eq
+  const V lpos = tadd(la1, la2);
+  V lx1 = tneg(la1), lx2 = tsub(lpos, la2);
+  lD = tadd(lD, tmul(lpos, cr));
+  // force = D k (|D| - eq) / |D|
+  lD = tadd(lD, tmul(lforce, sc));
+  const T lsc = tdot(lforce, D);
+  const T lk = lsc * (dist - eq) / dist;
+  const T ldist = lsc * (k - sc) / dist;
+  lD = tadd(lD, tmul(D, 2.0 * ldist * dsqrt_half(dist)));
+  // D = x1 - x2
+  lx1 = tadd(lx1, lD);
+  lx2 = tsub(lx2, lD);
+  g[0] = lx1;
+  g[1] = lx2;
+  g[2] = lvp1;  // v1
+  g[3] = lvp2;  // v2
+  g[4] = lw1;
+  g[5] = lw2;
+  g[6] = lffric;
+  g[7] = tv3<double>{lk, lks, lmu};
+}

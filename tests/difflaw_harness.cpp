@@ -89,6 +89,27 @@ void dl_bond_vjp(const double* in, const double* prm, const double* lam, double*
                         gq1.v.y, gq1.v.z, gq2.s, gq2.v.x, gq2.v.y, gq2.v.z, gkr, gks};
   for (int t = 0; t < 16; t++) grad[t] = g[t];
 }
+// reverse mode of a touching pair: grad[24] = lam[12] . d(f1, m1, m2, ffric out) /
+// d(x1, x2, v1, v2, w1, w2, ffric in, k, ks, mu); returns 0 for a pair apart
+int dl_fric_vjp(const double* in, const double* prm, int slip, const double* lam, double* grad)
+{
+  const tv3<double> x1{in[0], in[1], in[2]}, x2{in[3], in[4], in[5]};
+  const tv3<double> d = tsub(x1, x2);
+  if (!(tdot(d, d) < (prm[0] + prm[1]) * (prm[0] + prm[1]))) return 0;
+  tv3<double> g[8];
+  tfriction_law_vjp(x1, tv3<double>{in[6], in[7], in[8]}, tv3<double>{in[12], in[13], in[14]},
+                    prm[0], x2, tv3<double>{in[9], in[10], in[11]},
+                    tv3<double>{in[15], in[16], in[17]}, prm[1], prm[2], prm[3], prm[4], prm[5],
+                    tv3<double>{in[18], in[19], in[20]}, slip != 0,
+                    tv3<double>{lam[0], lam[1], lam[2]}, tv3<double>{lam[3], lam[4], lam[5]},
+                    tv3<double>{lam[6], lam[7], lam[8]}, tv3<double>{lam[9], lam[10], lam[11]}, g);
+  for (int t = 0; t < 8; t++) {
+    grad[3 * t] = g[t].x;
+    grad[3 * t + 1] = g[t].y;
+    grad[3 * t + 2] = g[t].z;
+  }
+  return 1;
+}
 int dl_fric(const double* in, const double* prm, int slip, double* out)
 {
   return fric_eval<double>(in, prm, slip, -1, out, false);

@@ -33,6 +33,8 @@ def dl(tmp_path_factory):
     lib.dl_bond.argtypes = [pd, pd, pd]
     lib.dl_bond_tangent.argtypes = [pd, pd, ctypes.c_int, pd]
     lib.dl_bond_vjp.argtypes = [pd, pd, pd, pd]
+    lib.dl_fric_vjp.argtypes = [pd, pd, ctypes.c_int, pd, pd]
+    lib.dl_fric_vjp.restype = ctypes.c_int
     lib.dl_fric.argtypes = [pd, pd, ctypes.c_int, pd]
     lib.dl_fric.restype = ctypes.c_int
     lib.dl_fric_tangent.argtypes = [pd, pd, ctypes.c_int, ctypes.c_int, pd]
@@ -220,3 +222,30 @@ def test_bond_reverse_mode_planted_mistake_fails(dl):
     assert np.max(np.abs(got - want) / scale) <= 1e-9
     for bad in (lambda g: np.r_[g[:14], 0.0, g[15]], lambda g: np.r_[-g[:3], g[3:]]):
         assert np.max(np.abs(bad(got) - want) / scale) > 1e-3
+
+
+def test_friction_reverse_mode_equals_dual_jacobian(dl):
+    """The reverse mode of the friction law against the dual-number Jacobian
+    (outputs f1, m1, m2 and the history written), on both slip branches; the
+    dual inputs' order is (x1, x2, v1, v2, w1, w2, history, k, ks, mu)."""
+    rng = np.random.default_rng(13)
+    done = {0: 0, 1: 0}
+    for it in range(400):
+        inp, prm = fric_case(rng)
+        if it % 2:  # a shear spring beyond the Coulomb limit: the slip branch
+            inp[18:21] *= rng.uniform(1e2, 1e4)
+        for slip in (0, 1):
+            cols = np.zeros((24, 12))
+            for k in range(24):
+                f = dl.dl_fric_tangent(P(inp), P(prm), slip, k, P(cols[k]))
+            if not f & 1:
+                continue
+            lam = rng.normal(size=12)
+            want = cols @ lam
+            got = np.zeros(24)
+            assert dl.dl_fric_vjp(P(inp), P(prm), slip, P(lam), P(got)) == 1
+            scale = np.abs(cols) @ np.abs(lam) + 1e-300
+            err = np.max(np.abs(got - want) / scale)
+            assert err <= 1e-9, (err, slip, inp, prm)
+            done[(f >> 1) & 1] += 1
+    assert done[0] > 20 and done[1] > 20  # both slip outcomes covered
