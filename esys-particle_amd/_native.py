@@ -131,6 +131,8 @@ SIGNATURES = {
     "load_checkpoint": (_i, [_vp, _s]),
     "get_bond_breaks": (_i, [_vp, _pi, _pi, _pi]),
     "get_rebuild_steps": (_i, [_vp, _pi]),
+    "track_pair_events": (_i, [_vp, _s]),
+    "get_pair_events": (_i, [_vp, _s, _pi, _pi, _pi, _pi]),
     "track_bond_criteria": (_i, [_vp]),
     "get_bond_criteria": (_i, [_vp, _s, _pi, _pi, _pd]),
     "set_kernel_timing": (_i, [_vp, _i]),
@@ -490,3 +492,16 @@ class Engine:
         s = np.zeros(max(n, 1), np.int32)
         self._check(self.lib.get_rebuild_steps(self.h, _ip(s)), "getRebuildSteps")
         return s[:n]
+
+    def track_pair_events(self, name):
+        self._check(self.lib.track_pair_events(self.h, _b(name)), "trackPairEvents")
+
+    def pair_events(self, name):
+        """(step, id1, id2, kind) arrays of a tracked group's pair events, kind
+        +1 for a created interaction and -1 for a removed one."""
+        n = self._check(self.lib.get_pair_events(self.h, _b(name), None, None, None, None),
+                        "getPairEvents")
+        v = [np.zeros(max(n, 1), np.int32) for _ in range(4)]
+        self._check(self.lib.get_pair_events(self.h, _b(name), *[_ip(a) for a in v]),
+                    "getPairEvents")
+        return tuple(a[:n] for a in v)

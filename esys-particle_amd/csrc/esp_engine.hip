@@ -104,6 +104,15 @@ struct DynGroup {
   // first step (PIS_ED::loadCheckPointData, pi_storage_ed.hpp:225-255)
   std::vector<int> pre_id1, pre_id2;
   std::vector<double4> pre_ff;
+  // pair-event log (esp_track_pair_events): the stored pairs as sorted id keys
+  // at the last logged update, appendix pairs logged since then (not yet in
+  // ev_prev), the appendix entries already logged, each appendix entry's step
+  // of creation, and the events (step, id1, id2, kind)
+  int track = 0;
+  std::vector<long long> ev_prev, ev_pend;
+  int app_logged = 0;
+  DBuf<int> app_step;
+  std::vector<int4> ev;
 };
 
 struct BondGroup {
@@ -674,6 +683,15 @@ void upload(esp_engine* e)
     }
     G.n = np;
     rebuild_presence(e, G);
+    if (G.track) {
+      G.app_step.reserve(ESP_APP_CAP);
+      G.ev_prev.clear();
+      for (int k = 0; k < np; k++) {
+        const int s = pk[k].second;
+        G.ev_prev.push_back(pkey(G.pre_id1[s], G.pre_id2[s]));  // as dyn_view reads it
+      }
+      std::sort(G.ev_prev.begin(), G.ev_prev.end());
+    }
   }
   e->broken_cap = std::max(total_bonds, 1);
   if (e->track_crit) {
@@ -729,6 +747,8 @@ void upload(esp_engine* e)
 // with one copy of the keep flags.  Called on the host bookkeeping path after
 // fold() (no dead bonds, no appendices), with e->cell holding -1 for the
 // particles to erase.
+int* broken_list_of(esp_engine* e, long long step);
+
 template <class T>
 void compact_buf(esp_engine* e, DBuf<T>& b, int n, int nk, int width)
 {
@@ -765,6 +785,28 @@ void erase_outside(esp_engine* e)
   launch_new_slot(e->stream, n, e->keep.p, e->keep_off.p, nslot.p);
   d2h(e, hslot.data(), nslot.p, n);
   sync(e);
+  // Bonds whose break criterion held in the last step: the next step's update
+  // breaks them by global bond index (bond_off + index in the group), so the
+  // list is re-indexed below; bonds the erasure drops leave it (no break event
+  // for an interaction that no longer exists).
+  std::vector<int> pend;
+  const size_t ring_i = (size_t)(e->t % e->ring_size);
+  if (!e->first_time && e->t >= 1 && !e->bg.empty()) {
+    StepStatus pst;
+    d2h(e, &pst, e->ring.p + ring_i, 1);
+    sync(e);
+    if (pst.nbroken > 0) {
+      pend.resize(std::min(pst.nbroken, e->broken_cap));
+      d2h(e, pend.data(), broken_list_of(e, e->t), pend.size());
+      sync(e);
+    }
+  }
+  std::vector<int> old_off(e->bg.size()), old_n(e->bg.size());
+  std::vector<std::vector<int>> bkeep(e->bg.size());
+  for (size_t g = 0; g < e->bg.size(); g++) {
+    old_off[g] = e->bg[g].bond_off;
+    old_n[g] = e->bg[g].n;
+  }
   // interactions first (they read the old slots through nslot)
   DBuf<int> pkeep, poff;
   DBuf<int2> rij;
@@ -775,7 +817,8 @@ void erase_outside(esp_engine* e)
     rij.reserve(G.n);
     launch_remap_pairs(e->stream, G.n, G.ij.p, nslot.p, rij.p, pkeep.p);
     const int nb2 = scan_total(e, pkeep.p, poff.p, G.n);
-    std::vector<int> bk(G.n);
+    std::vector<int>& bk = bkeep[&G - e->bg.data()];
+    bk.resize(G.n);
     d2h(e, bk.data(), pkeep.p, G.n);
     const size_t c2 = std::max(nb2, 1);
     e->tmp_ij.reserve(c2); e->tmp_a.reserve(c2); e->tmp_b.reserve(c2); e->tmp_c.reserve(c2);
@@ -792,6 +835,37 @@ void erase_outside(esp_engine* e)
     G.hid1.resize(w);
     G.hid2.resize(w);
     G.n = nb2;
+  }
+  {
+    int off = 0;
+    for (int gi : e->bord) {
+      e->bg[gi].bond_off = off;
+      off += e->bg[gi].n;
+    }
+  }
+  if (!pend.empty()) {
+    // new index of a kept bond = its group's new offset + kept bonds before it
+    std::vector<std::vector<int>> rank(e->bg.size());
+    for (size_t g = 0; g < e->bg.size(); g++) {
+      rank[g].resize(bkeep[g].size());
+      int r = 0;
+      for (size_t b = 0; b < bkeep[g].size(); b++) {
+        rank[g][b] = bkeep[g][b] ? r : -1;
+        r += bkeep[g][b];
+      }
+    }
+    std::vector<int> kept;
+    for (int x : pend)
+      for (size_t g = 0; g < e->bg.size(); g++) {
+        if (x < old_off[g] || x >= old_off[g] + old_n[g]) continue;
+        const int r = rank[g][x - old_off[g]];
+        if (r >= 0) kept.push_back(e->bg[g].bond_off + r);
+        break;
+      }
+    if (!kept.empty()) h2d(e, broken_list_of(e, e->t), kept.data(), kept.size());
+    const int nk2 = (int)kept.size();
+    h2d(e, &e->ring.p[ring_i].nbroken, &nk2, 1);
+    sync(e);
   }
   for (auto& G : e->dg) {
     if (G.n <= 0) continue;
@@ -918,6 +992,70 @@ void new_pair_list(esp_engine* e)
   e->nnew = total;
 }
 
+int dyn_appended(esp_engine* e, DynGroup& G);
+long long id_key(esp_engine* e, long long skey);
+void dyn_view(esp_engine* e, DynGroup& G, std::vector<long long>& key, std::vector<double4>* ff);
+
+// Pair events of a tracked group, part 1: appendix pairs that queued steps
+// created on the device since the last harvest, logged with the step of their
+// creation (the device writes it beside each appendix entry).  Reads only, so
+// queries may call it at any time; the slots must be the ones the appendix
+// was written with (no erasure in between).
+void harvest_appendix(esp_engine* e, DynGroup& G)
+{
+  if (!G.track || !e->started) return;
+  const int na = dyn_appended(e, G);
+  const int m = na - G.app_logged;
+  if (m <= 0) return;
+  std::vector<long long> k(m);
+  std::vector<int> st(m);
+  d2h(e, k.data(), G.key.p + G.n + G.app_logged, m);
+  d2h(e, st.data(), G.app_step.p + G.app_logged, m);
+  sync(e);
+  for (int i = 0; i < m; i++) {
+    const long long ik = id_key(e, k[i]);
+    G.ev.push_back(make_int4(st[i], (int)(ik >> 32), (int)(ik & 0xffffffffLL), 1));
+    G.ev_pend.push_back(ik);
+  }
+  G.app_logged = na;
+}
+
+// Pair events of a tracked group, part 2: after a host-side interaction update
+// (table rebuild with erasure, flagged-list update) at `step`, the stored pairs
+// as sorted id keys against those of the last logged update plus the appendix
+// pairs logged since: a key whose count differs gives one event per unit of
+// difference, kind +1 (created) or -1 (removed).
+void log_pair_diff(esp_engine* e, DynGroup& G, long long step)
+{
+  std::vector<long long> cur;
+  dyn_view(e, G, cur, nullptr);
+  std::vector<long long>& prev = G.ev_prev;
+  if (!G.ev_pend.empty()) {
+    prev.insert(prev.end(), G.ev_pend.begin(), G.ev_pend.end());
+    std::sort(prev.begin(), prev.end());
+    G.ev_pend.clear();
+  }
+  const size_t np = prev.size(), nc = cur.size();
+  size_t a = 0, b = 0;
+  while (a < np || b < nc) {
+    long long k;
+    int kind;
+    if (b == nc || (a < np && prev[a] < cur[b])) {
+      k = prev[a++];
+      kind = -1;
+    } else if (a == np || cur[b] < prev[a]) {
+      k = cur[b++];
+      kind = 1;
+    } else {
+      a++;
+      b++;
+      continue;
+    }
+    G.ev.push_back(make_int4((int)step, (int)(k >> 32), (int)(k & 0xffffffffLL), kind));
+  }
+  prev.swap(cur);
+}
+
 // Break events logged on the device move to the host log.  Touches nothing the
 // steps read, so queries may call it at any time.
 void fold_events(esp_engine* e)
@@ -943,12 +1081,11 @@ void fold_events(esp_engine* e)
 // each dynamic group's appendix merged into its key-sorted main list.  Record
 // indices change, so this runs only in host bookkeeping, right before the
 // force-record layout and the CSR are rebuilt (rebuild_csr).
+void fold_appendices(esp_engine* e);
+
 void fold(esp_engine* e)
 {
   fold_events(e);
-  std::vector<int> napp(e->dg.size(), 0);
-  for (size_t g = 0; g < e->dg.size(); g++) d2h(e, &napp[g], e->dg[g].napp.p, 1);
-  sync(e);
   if (e->dead_pending > 0) {
     e->dead_pending = 0;
     // compact dead bonds
@@ -986,6 +1123,17 @@ void fold(esp_engine* e)
       off += G.n;
     }
   }
+  fold_appendices(e);
+}
+
+// Each dynamic group's appendix merged into its key-sorted main list (record
+// indices of the group change; bond indices do not).
+void fold_appendices(esp_engine* e)
+{
+  for (auto& G : e->dg) harvest_appendix(e, G);
+  std::vector<int> napp(e->dg.size(), 0);
+  for (size_t g = 0; g < e->dg.size(); g++) d2h(e, &napp[g], e->dg[g].napp.p, 1);
+  sync(e);
   for (size_t g = 0; g < e->dg.size(); g++) {
     DynGroup& G = e->dg[g];
     const int na = napp[g];
@@ -1030,6 +1178,7 @@ void fold(esp_engine* e)
     if (hist) G.ff.swap(e->tmp_a);
     G.n = tot;
     zero(e, G.napp.p, 2);
+    G.app_logged = 0;
     reserve_group(G, G.n, hist);  // keeps content when the capacity suffices
     rebuild_presence(e, G);       // room for the next appendix
   }
@@ -1317,6 +1466,7 @@ void launch_step(esp_engine* e, const StepCtl& ctl, long long step, bool flip, b
       da.key = G.key.p;
       da.ff = G.kind == 3 ? (flip ? G.ff2.p : G.ff.p) : nullptr;
       da.n_app = G.napp.p;
+      da.app_step = G.track ? G.app_step.p : nullptr;
       da.n_main = G.n;
       da.rec_off = G.rec_off;
       da.app_rec_off = G.app_rec_off;
@@ -1450,23 +1600,25 @@ void host_step(esp_engine* e, int reason)
     d2h(e, &prev, e->ring.p + ((step - 1) % e->ring_size), 1);
     sync(e);
   }
+  // forced search of the first runOneStep, or displacement past the threshold;
+  // after a checkpoint's search displacements count from there
+  const bool search = e->first_time || (!e->forced_search && prev.maxdisp2 > thresh_bits(e));
   // PIS_EB::update for the previous step's breaks (already applied on the
-  // device when the queue stopped in the incremental update)
+  // device when the queue stopped in the incremental update).  The reference
+  // searches first: particles that left the grid are erased with their bonds
+  // before the bonds are tested, so with a search the break test skips bonds
+  // of particles outside the grid (positions as the search sees them: wrapped
+  // into the circular x range).
   if (prev.nbroken > 0 && reason != 2 && !e->bg.empty()) {
+    if (search && e->grid.circ_x) launch_wrap_x(e->stream, e->view(), e->grid.x0, e->grid.period, e->init.p);
     StepCtl c0 = make_ctl(e, step, 1, 0);
     launch_break_apply(e->stream, c0, e->view(), bond_set(e), broken_list_of(e, step - 1),
-                       e->upd.p, e->flist.p, e->evt.p, step);
+                       e->upd.p, e->flist.p, e->evt.p, step, search ? &e->grid : nullptr);
     HIPCHK(hipGetLastError());
   }
   fold(e);
-  if (e->first_time) {
-    rebuild(e, step);  // forced search of the first runOneStep
-    e->first_time = false;
-  } else if (e->forced_search) {
-    // a checkpoint searched after the last step; displacements count from there
-  } else if (prev.maxdisp2 > thresh_bits(e)) {
-    rebuild(e, step);
-  }
+  if (search) rebuild(e, step);
+  e->first_time = false;
   e->forced_search = false;
   e->t = step;
   apply_wall_motion(e);
@@ -1475,6 +1627,8 @@ void host_step(esp_engine* e, int reason)
   if (e->timestamp > e->last_ns || need_update) {
     bool new_list_taken = false;
     for (int gi : e->dord) dyn_update(e, e->dg[gi], new_list_taken);
+    for (auto& G : e->dg)
+      if (G.track) log_pair_diff(e, G, step);
     if (!e->wg.empty()) {
       WallParam w[ESP_MAX_WALL_GROUPS];
       wall_params(e, w);
@@ -3105,7 +3259,13 @@ int esp_save_checkpoint(esp_engine* e, const char* prefix, int num_time_steps, i
   if (e->started) {
     HIPCHK(hipSetDevice(e->cfg.device));
     // the reference searches neighbours before every checkpoint
-    // (LatticeMaster.cpp:2697-2700); the next step then updates every group
+    // (LatticeMaster.cpp:2697-2700); the next step then updates every group.
+    // The search may erase particles, whose remap of the dynamic groups needs
+    // key-sorted lists without appendices.  Dead bonds stay (the erasure drops
+    // them), so bond indices change only where the erasure re-indexes the last
+    // step's pending breaks with them.
+    fold_events(e);
+    fold_appendices(e);
     rebuild(e, e->t);
     e->forced_search = true;
     e->pending_host = true;
@@ -3162,6 +3322,36 @@ int esp_get_neighbour_pairs(esp_engine* e, int* id1, int* id2)
   API_END
 }
 
+int esp_track_pair_events(esp_engine* e, const char* name)
+{
+  API_BEGIN
+  require_setup(e, "trackPairEvents");
+  const int d = find_dg(e, name ? name : "");
+  if (d < 0) throw EspError(ESP_ERR_ARG, std::string("no dynamic interaction group '") + (name ? name : "") + "'");
+  e->dg[d].track = 1;
+  return ESP_OK;
+  API_END
+}
+
+int esp_get_pair_events(esp_engine* e, const char* name, int* step, int* id1, int* id2, int* kind)
+{
+  API_BEGIN
+  const int d = find_dg(e, name ? name : "");
+  if (d < 0) throw EspError(ESP_ERR_ARG, std::string("no dynamic interaction group '") + (name ? name : "") + "'");
+  DynGroup& G = e->dg[d];
+  if (!G.track) throw EspError(ESP_ERR_STATE, std::string("pair events of '") + name + "' are not tracked");
+  harvest_appendix(e, G);
+  const int n = (int)G.ev.size();
+  if (step && id1 && id2 && kind) {
+    std::vector<std::array<int, 4>> v(n);
+    for (int i = 0; i < n; i++) v[i] = {G.ev[i].x, G.ev[i].y, G.ev[i].z, G.ev[i].w};
+    std::sort(v.begin(), v.end());
+    for (int i = 0; i < n; i++) { step[i] = v[i][0]; id1[i] = v[i][1]; id2[i] = v[i][2]; kind[i] = v[i][3]; }
+  }
+  return n;
+  API_END
+}
+
 int esp_get_bond_breaks(esp_engine* e, int* step, int* id1, int* id2)
 {
   API_BEGIN
@@ -3177,17 +3367,23 @@ int esp_get_bond_breaks(esp_engine* e, int* step, int* id1, int* id2)
   API_END
 }
 
+}  // extern "C"
+
 namespace {
 // Consistency of the device bookkeeping that queued steps index with: every
 // per-particle appendix count, flagged-list entry, event count, pair, key and
 // force-record slot inside its buffer (and a record's CSR slots inside its
-// particles' runs).  Returns an empty string when consistent.
+// particles' runs).  When the next step is a host bookkeeping step (a halted
+// queue, a checkpoint's search), that step lays the records and the CSR out
+// anew from the lists: then only the lists are checked.  Returns an empty
+// string when consistent.
 std::string check_state(esp_engine* e)
 {
   if (!e->started) return "";
   HIPCHK(hipSetDevice(e->cfg.device));
   sync(e);
   const int n = e->n;
+  const bool csr = !e->pending_host;  // records and CSR the next step indexes with
   std::ostringstream err;
   auto fail = [&](const std::string& m) {
     if (err.tellp() < 2000) err << m << "; ";
@@ -3205,13 +3401,13 @@ std::string check_state(esp_engine* e)
       fail("appendix slot count " + std::to_string(xcnt[i]) + " at slot " + std::to_string(i));
       break;
     }
-  for (int i = 0; i < n; i++)
+  for (int i = 0; i < n && csr; i++)
     if (off[i] > off[i + 1]) {
       fail("CSR offsets decrease at slot " + std::to_string(i));
       break;
     }
   const long long nslot = (long long)e->xbase + (long long)n * ESP_XSLOTS;
-  if (off[n] != e->xbase) fail("CSR total differs from the appendix base");
+  if (csr && off[n] != e->xbase) fail("CSR total differs from the appendix base");
   if (st.fcount < 0 || st.fcount > n) fail("flagged count " + std::to_string(st.fcount));
   if (st.evt_count < 0 || st.evt_count > e->broken_cap)
     fail("event count " + std::to_string(st.evt_count));
@@ -3234,9 +3430,9 @@ std::string check_state(esp_engine* e)
     for (int b = 0; b < G.n; b++) {
       const int2 q = ij[b];
       if (q.x == -1 && q.y == -1) continue;  // dead until the next compaction
-      const int2 sl = so[G.rec_off + b];
-      if (q.x < 0 || q.x >= n || q.y < 0 || q.y >= n || q.x == q.y || !in_run(sl.x, q.x) ||
-          !in_run(sl.y, q.y)) {
+      const int2 sl = csr ? so[G.rec_off + b] : make_int2(0, 0);
+      if (q.x < 0 || q.x >= n || q.y < 0 || q.y >= n || q.x == q.y ||
+          (csr && (!in_run(sl.x, q.x) || !in_run(sl.y, q.y)))) {
         fail("bond " + std::to_string(b) + " of '" + G.name + "'");
         break;
       }
@@ -3260,10 +3456,11 @@ std::string check_state(esp_engine* e)
     for (int c = 0; c < tot; c++) {
       const int2 q = ij[c];
       const bool app = c >= G.n;
-      const int2 sl = app ? so[G.app_rec_off + (c - G.n)] : so[G.rec_off + c];
+      const int2 sl = !csr ? make_int2(0, 0)
+                           : app ? so[G.app_rec_off + (c - G.n)] : so[G.rec_off + c];
       bool ok = q.x >= 0 && q.x < n && q.y >= 0 && q.y < n && q.x != q.y && key[c] == pkey(q.x, q.y);
-      if (ok && !app) ok = in_run(sl.x, q.x) && in_run(sl.y, q.y);
-      if (ok && app)
+      if (ok && !app && csr) ok = in_run(sl.x, q.x) && in_run(sl.y, q.y);
+      if (ok && app && csr)
         ok = sl.x >= e->xbase + ESP_XSLOTS * q.x && sl.x < e->xbase + ESP_XSLOTS * (q.x + 1) &&
              sl.y >= e->xbase + ESP_XSLOTS * q.y && sl.y < e->xbase + ESP_XSLOTS * (q.y + 1) &&
              sl.y < nslot;
@@ -3276,6 +3473,8 @@ std::string check_state(esp_engine* e)
   return err.str();
 }
 }  // namespace
+
+extern "C" {
 
 int esp_check_state(esp_engine* e)
 {

@@ -178,6 +178,7 @@ struct DynAppend {
   long long* key;
   double4* ff;       // shear history read by this step (history groups) or nullptr
   int* n_app;        // device counters: [0] pairs appended, [1] pairs the force blocks cover
+  int* app_step;     // step each appendix pair was created in (pair-event log), or nullptr
   int n_main;
   int rec_off;       // records of the main list
   int app_rec_off;   // records of the appendix
@@ -266,9 +267,12 @@ struct ScLaunch {
 };
 void launch_sc_break(hipStream_t s, StepCtl ctl, PView p, const ScLaunch& sl);
 // PIS_EB::update on the device: bonds of the previous step's break list die,
-// their particles are flagged, events are logged
+// their particles are flagged, events are logged.  erase_grid: a table build
+// follows in this step; bonds with a particle outside that grid are skipped
+// (the build erases them).
 void launch_break_apply(hipStream_t s, StepCtl ctl, PView p, BondSet bs, const int* broken_list,
-                        UpdState* st, int* flist, int4* evt, long long step);
+                        UpdState* st, int* flist, int4* evt, long long step,
+                        const GridParams* erase_grid = nullptr);
 void launch_hash_clear(hipStream_t s, HashRef h);
 void launch_hash_insert(hipStream_t s, HashRef h, const long long* keys, int n);
 void launch_bond_keys(hipStream_t s, int nb, const int2* ij, const int* id, long long* keys);

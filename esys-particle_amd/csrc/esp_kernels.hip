@@ -1127,8 +1127,22 @@ __global__ void k_wall_update(PView p, GridParams g, int nwg, WallParam w0, Wall
 // the previous step: the bond leaves its group (marked dead, record zeroed, key
 // erased from the exclusion set), both particles are flagged
 // (RotBondedInteraction.cpp:211-213) and the event is logged.
+// outside the table's padded grid: erased by the next table build (the
+// predicate of k_cells_count)
+__device__ __forceinline__ bool outside_grid(const GridParams& g, const double4& P)
+{
+  int ix, iy, iz;
+  return !(cell_coords(g, P.x, P.y, P.z, ix, iy, iz) && ix >= 0 && ix < g.nx && iy >= 0 &&
+           iy < g.ny && iz >= 0 && iz < g.nz);
+}
+
+// erase_out: this step's table build follows (host bookkeeping), and the
+// reference erases the particles outside the grid, with their bonds, before it
+// tests the bonds (NeighborTable::build then PIS_EB::update): a bond with
+// such a particle is left to the erasure, no break event.
 __device__ __forceinline__ void break_one(PView p, const BondSet& bs, int gb, UpdState* st,
-                                          int* flist, int4* evt, long long step)
+                                          int* flist, int4* evt, long long step, bool erase_out,
+                                          const GridParams& g_erase)
 {
   int g = 0;
   while (g + 1 < bs.count && gb >= bs.off[g + 1]) g++;
@@ -1136,6 +1150,7 @@ __device__ __forceinline__ void break_one(PView p, const BondSet& bs, int gb, Up
   if (b < 0 || b >= bs.n[g]) return;
   const int2 e = bs.ij[g][b];
   if (e.x < 0) return;
+  if (erase_out && (outside_grid(g_erase, p.pos[e.x]) || outside_grid(g_erase, p.pos[e.y]))) return;
   // independent memory operations first, so their latencies overlap
   const int id1 = p.id[e.x], id2 = p.id[e.y];
   const int2 so = bs.slot_of[bs.rec_off[g] + b];
@@ -1163,12 +1178,13 @@ __device__ __forceinline__ void break_one(PView p, const BondSet& bs, int gb, Up
 
 // the previous step's breaks on the host bookkeeping path
 __global__ void k_break_apply(StepCtl ctl, PView p, BondSet bs, const int* broken_list,
-                              UpdState* st, int* flist, int4* evt, long long step)
+                              UpdState* st, int* flist, int4* evt, long long step, int erase_out,
+                              GridParams g)
 {
   if (step_skip(ctl)) return;
   const int nb = ctl.prev->nbroken;
   for (int t = blockIdx.x * BS + threadIdx.x; t < nb; t += gridDim.x * BS)
-    break_one(p, bs, broken_list[t], st, flist, evt, step);
+    break_one(p, bs, broken_list[t], st, flist, evt, step, erase_out != 0, g);
 }
 
 // TSubLattice::updateInteractions after bond breaks, for a queued step, run by
@@ -1205,7 +1221,7 @@ __device__ void incremental_update(const StepCtl& ctl, const PView& p, const Upd
   if (nbk <= 0) return;
   const int tid = threadIdx.x;
   for (int t = tid; t < nbk; t += BS)
-    break_one(p, up.bs, up.broken_prev[t], up.st, up.flist, up.evt, up.step);
+    break_one(p, up.bs, up.broken_prev[t], up.st, up.flist, up.evt, up.step, false, up.g);
   if (!up.has_group) return;
   const DynAppend& da = up.da;
   const GridParams& g = up.g;
@@ -1349,6 +1365,7 @@ __device__ void incremental_update(const StepCtl& ctl, const PView& p, const Upd
     da.ij[w] = make_int2(a, b);
     da.key[w] = key;
     if (da.ff) da.ff[w] = make_double4(0.0, 0.0, 0.0, 0.0);
+    if (da.app_step) da.app_step[napp + t] = (int)up.step;
     if (da.is_hist) hs_insert(da.present, key);
     // appendix slots start at zero (cleared at the last rebuild, used once)
     da.slot_of[da.app_rec_off + napp + t] =
@@ -1610,11 +1627,15 @@ bool launch_forces(hipStream_t s, StepCtl ctl, PView p, const ForceLaunch& fl,
 }
 
 void launch_break_apply(hipStream_t s, StepCtl ctl, PView p, BondSet bs, const int* broken_list,
-                        UpdState* st, int* flist, int4* evt, long long step)
+                        UpdState* st, int* flist, int4* evt, long long step,
+                        const GridParams* erase_grid)
 {
   if (bs.count <= 0) return;
+  GridParams g;
+  memset(&g, 0, sizeof g);
+  if (erase_grid) g = *erase_grid;
   hipLaunchKernelGGL(k_break_apply, dim3(16), dim3(BS), 0, s, ctl, p, bs, broken_list, st, flist,
-                     evt, step);
+                     evt, step, erase_grid ? 1 : 0, g);
 }
 
 void launch_hash_clear(hipStream_t s, HashRef h)

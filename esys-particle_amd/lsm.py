@@ -828,6 +828,17 @@ class LsmMpi:
     def getBondBreaks(self):
         return self._e().bond_breaks()
 
+    def trackPairEvents(self, name):
+        """Log the interactions the dynamic group `name` creates and removes,
+        from the first time step on (call before it)."""
+        self._e().track_pair_events(name)
+
+    def getPairEvents(self, name):
+        """(step, id1, id2, kind) arrays in (step, id1, id2) order: kind +1 for
+        an interaction the group created in that step's update, -1 for one it
+        removed (rebuilt list, erased particle)."""
+        return self._e().pair_events(name)
+
     def getParticleStress(self):
         """Per-particle stress sums of the last step (CParticle::m_sigma,
         Particle.cpp:391-400), shape (n, 3, 3), ascending id order; needs a

@@ -338,6 +338,19 @@ int esp_track_bond_criteria(esp_engine* e);
 int esp_get_bond_criteria(esp_engine* e, const char* name, int* id1, int* id2, double* crit);
 /* Steps at which the neighbour table was rebuilt; returns the count when NULL. */
 int esp_get_rebuild_steps(esp_engine* e, int* steps);
+/* Log of the pairs a dynamic interaction group creates and removes
+ * (PIS_NE::update / PIS_ED::update, pi_storage_ne.hpp:49-102,
+ * pi_storage_ed.hpp:84-161, with the erasure of interactions whose particle
+ * left the table, pi_storage_e.hpp:111-155).  Enabled per group before the
+ * first time step.  From then on, after each step's interaction update, a pair
+ * (id1, id2) whose number of stored interactions changed gives one event per
+ * unit of change: kind +1 (created) or -1 (removed), step = that time step.
+ * A pair dropped and created again in one update (a RotFriction pair that
+ * stopped touching but is still a neighbour) is no event. */
+int esp_track_pair_events(esp_engine* e, const char* name);
+/* The events in (step, id1, id2) order, kind per event; returns the count when
+ * the pointers are NULL.  ESP_ERR_STATE unless the group is tracked. */
+int esp_get_pair_events(esp_engine* e, const char* name, int* step, int* id1, int* id2, int* kind);
 
 /* ---- measurement (no counterpart in the reference) ----------------------- */
 /* Kernel classes timed by esp_get_kernel_times. */

@@ -165,6 +165,13 @@ typedef struct {
   /* createInteractionGroupTagged: tags sorted so that tag1 <= tag2
    * (pi_storage_ed_t.hpp:15-30, pi_storage_ne_t.hpp:15-30) */
   int tagged, tag1, mask1, tag2, mask2;
+  /* pair-event log (eso_track_pair_events): the stored pairs as sorted id
+   * keys after the last step, and the events, 4 ints each (step, id1, id2, kind) */
+  int track;
+  long long* evprev;
+  int nevprev;
+  int* ev;
+  int nev, capev;
 } ogroup;
 
 typedef struct {
@@ -1072,6 +1079,41 @@ static void dg_update(eso_engine* e, ogroup* G)
   G->update_ts = e->timestamp;
 }
 
+/* Pair-event log of a tracked group, after a step's interaction update: a
+ * pair key whose stored count differs from the previous step's gives one event
+ * per unit of difference, kind +1 (an interaction created) or -1 (removed).
+ * Keys compare as (id1, id2), lower id first, so events come out in that order. */
+static void log_pair_events(eso_engine* e, ogroup* G)
+{
+  long long* cur = malloc(sizeof(long long) * (size_t)(G->n > 0 ? G->n : 1));
+  for (int i = 0; i < G->n; i++) cur[i] = pkey(G->v[i].id1, G->v[i].id2);
+  qsort(cur, (size_t)G->n, sizeof(long long), cmp_ll);
+  int a = 0, b = 0;
+  while (a < G->nevprev || b < G->n) {
+    long long k;
+    int kind;
+    if (b == G->n || (a < G->nevprev && G->evprev[a] < cur[b])) {
+      k = G->evprev[a++];
+      kind = -1;
+    } else if (a == G->nevprev || cur[b] < G->evprev[a]) {
+      k = cur[b++];
+      kind = 1;
+    } else {
+      a++;
+      b++;
+      continue;
+    }
+    GROW(G->ev, G->nev + 3, G->capev);
+    G->ev[G->nev++] = (int)e->t;
+    G->ev[G->nev++] = (int)(k >> 32);
+    G->ev[G->nev++] = (int)(k & 0xffffffffLL);
+    G->ev[G->nev++] = kind;
+  }
+  free(G->evprev);
+  G->evprev = cur;
+  G->nevprev = G->n;
+}
+
 /* CEWallInteractionGroup::Update (EWallInteractionGroup.hpp:57-82) with
  * getParticlesAtPlane (ntable.hpp:674-690) and the CElasticWallInteraction
  * constructor (EWallInteraction.hpp:26-36). */
@@ -1424,6 +1466,8 @@ static void run_one_step(eso_engine* e)
     if (e->wl[w].has_motion) e->wl[w].origin = vadd(e->wl[w].origin, e->wl[w].step_motion);
   if (check_neighbours(e)) search_neighbours(e, e->t);
   update_interactions(e);
+  for (int g = 0; g < e->ndg; g++)
+    if (e->dg[g].track) log_pair_events(e, &e->dg[g]);
   calc_and_integrate(e);
 }
 
@@ -1515,7 +1559,7 @@ int eso_create(const esp_config* cfg, eso_engine** out)
 void eso_destroy(eso_engine* e)
 {
   if (!e) return;
-  for (int i = 0; i < e->ndg; i++) free(e->dg[i].v);
+  for (int i = 0; i < e->ndg; i++) { free(e->dg[i].v); free(e->dg[i].evprev); free(e->dg[i].ev); }
   for (int i = 0; i < e->nbg; i++) { free(e->bg[i].v); free(e->bg[i].keys); }
   for (int i = 0; i < e->nwl; i++) free(e->wl[i].hist);
   for (int i = 0; i < e->nwg; i++) { free(e->wg[i].pidx); free(e->wg[i].inner); free(e->wg[i].keff); }
@@ -2254,6 +2298,33 @@ int eso_get_bond_breaks(eso_engine* e, int* step, int* id1, int* id2)
     for (int i = 0; i < n; i++) { step[i] = v[i].s; id1[i] = v[i].a; id2[i] = v[i].b; }
     free(v);
   }
+  return n;
+}
+
+int eso_track_pair_events(eso_engine* e, const char* name)
+{
+  int g = find_dg(e, name ? name : "");
+  if (g < 0) return fail(e, ESP_ERR_ARG, "no dynamic interaction group '%s'", name ? name : "");
+  if (!e->first_time) return fail(e, ESP_ERR_STATE, "pair events are tracked from the first time step on");
+  e->dg[g].track = 1;
+  return ESP_OK;
+}
+
+int eso_get_pair_events(eso_engine* e, const char* name, int* step, int* id1, int* id2, int* kind)
+{
+  int g = find_dg(e, name ? name : "");
+  if (g < 0) return fail(e, ESP_ERR_ARG, "no dynamic interaction group '%s'", name ? name : "");
+  ogroup* G = &e->dg[g];
+  if (!G->track) return fail(e, ESP_ERR_STATE, "pair events of '%s' are not tracked", name);
+  int n = G->nev / 4;
+  /* logged in step order and, within a step, in (id1, id2) order */
+  if (step && id1 && id2 && kind)
+    for (int i = 0; i < n; i++) {
+      step[i] = G->ev[4 * i];
+      id1[i] = G->ev[4 * i + 1];
+      id2[i] = G->ev[4 * i + 2];
+      kind[i] = G->ev[4 * i + 3];
+    }
   return n;
 }
 

@@ -71,6 +71,8 @@ int eso_get_friction_state(eso_engine* e, const char* name, double* ffric, int* 
 int eso_get_neighbour_pairs(eso_engine* e, int* id1, int* id2);
 int eso_get_bond_breaks(eso_engine* e, int* step, int* id1, int* id2);
 int eso_get_rebuild_steps(eso_engine* e, int* steps);
+int eso_track_pair_events(eso_engine* e, const char* name);
+int eso_get_pair_events(eso_engine* e, const char* name, int* step, int* id1, int* id2, int* kind);
 int eso_set_kernel_timing(eso_engine* e, int enable);
 int eso_get_kernel_times(eso_engine* e, double* ms, long long* launches);
 

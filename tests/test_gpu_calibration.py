@@ -175,7 +175,7 @@ def test_autograd_function(cal, spec2k):
     import torch
     m = model_of(cal, [spec2k])
     target = torch.full((1, m.nsamples), 4.0, dtype=torch.float64)
-    theta = torch.tensor([THETA0], dtype=torch.float64, requires_grad=True)
+    theta = torch.tensor(np.array([THETA0]), dtype=torch.float64, requires_grad=True)
     loss = cal.CalibrationLoss.apply(theta, m, target)
     loss.sum().backward()
     g = m.backward()
