@@ -87,6 +87,7 @@ struct DynGroup {
   std::string name;
   int kind;  // 1 NRotElastic, 2 RotElastic, 3 RotFriction
   double k = 0, ks = 0, mu_d = 0, mu_s = 0;
+  int rigid = 0;  // RotFriction rigid=True
   int scaling = 1, meanR = 1;
   int n = 0;  // main list sorted by slot key; the appendix (device count napp) follows it
   DBuf<int2> ij;
@@ -936,7 +937,11 @@ void erase_outside(esp_engine* e)
   sync(e);
 }
 
-void rebuild(esp_engine* e, long long step)
+// First part of the table build: positions wrapped into the circular x range,
+// each particle's cell (-1 outside the padded grid), the displacements reset;
+// returns the number of particles outside.  Reads and writes no interaction
+// storage, so the host step runs it before the bond-break update and folds.
+int rebuild_count(esp_engine* e)
 {
   PView p = e->view();
   // circular x: particles that crossed an x end take their image position
@@ -948,6 +953,14 @@ void rebuild(esp_engine* e, long long step)
   int nout = 0;
   d2h(e, &nout, e->dev_int.p, 1);
   sync(e);
+  return nout;
+}
+
+// The rest of the table build after rebuild_count (nout particles outside),
+// with nothing moved in between: erasure, cell lists, the full candidate list.
+void rebuild_rest(esp_engine* e, long long step, int nout)
+{
+  PView p = e->view();
   if (nout > 0) {
     erase_outside(e);
     p = e->view();
@@ -974,6 +987,8 @@ void rebuild(esp_engine* e, long long step)
   e->timestamp++;
   e->rebuild_steps.push_back((int)step);
 }
+
+void rebuild(esp_engine* e, long long step) { rebuild_rest(e, step, rebuild_count(e)); }
 
 // flagged pair list (getNewList) and clearing of all flags
 void new_pair_list(esp_engine* e)
@@ -1402,6 +1417,7 @@ ForceLaunch force_launch(esp_engine* e, long long step, bool flip)
     s.ks = G.ks;
     s.mu_d = G.mu_d;
     s.mu_s = G.mu_s;
+    s.rigid = G.rigid;
     s.scaling = G.scaling;
     s.meanR = G.meanR;
     blk += force_blocks(G.n + ESP_APP_CAP);
@@ -1466,7 +1482,6 @@ void launch_step(esp_engine* e, const StepCtl& ctl, long long step, bool flip, b
       da.key = G.key.p;
       da.ff = G.kind == 3 ? (flip ? G.ff2.p : G.ff.p) : nullptr;
       da.n_app = G.napp.p;
-      da.app_step = G.track ? G.app_step.p : nullptr;
       da.n_main = G.n;
       da.rec_off = G.rec_off;
       da.app_rec_off = G.app_rec_off;
@@ -1481,8 +1496,14 @@ void launch_step(esp_engine* e, const StepCtl& ctl, long long step, bool flip, b
     }
   }
   {
+    // the pair-event log stamps the step of each pair the device appends
+    ForceExt ext;
+    ext.app_step = nullptr;
+    if (dev_update && !e->dg.empty() && e->dg[e->dord[0]].track)
+      ext.app_step = e->dg[e->dord[0]].app_step.p;
     const LaunchEvents t = launch_events(e);
-    note_launch(e, t, launch_forces(e->stream, ctl, p, fl, dev_update ? &up : nullptr, t.a, t.b),
+    note_launch(e, t,
+                launch_forces(e->stream, ctl, p, fl, dev_update ? &up : nullptr, t.a, t.b, &ext),
                 ESP_KCLASS_FORCES, ctl.step_id);
   }
   GatherParams gp;
@@ -1603,21 +1624,21 @@ void host_step(esp_engine* e, int reason)
   // forced search of the first runOneStep, or displacement past the threshold;
   // after a checkpoint's search displacements count from there
   const bool search = e->first_time || (!e->forced_search && prev.maxdisp2 > thresh_bits(e));
+  // The reference searches first: particles that left the grid are erased with
+  // their bonds before the bonds are tested.  The search's cell pass runs
+  // first, so the break update knows whether any particle is outside.
+  const int nout = search ? rebuild_count(e) : 0;
   // PIS_EB::update for the previous step's breaks (already applied on the
-  // device when the queue stopped in the incremental update).  The reference
-  // searches first: particles that left the grid are erased with their bonds
-  // before the bonds are tested, so with a search the break test skips bonds
-  // of particles outside the grid (positions as the search sees them: wrapped
-  // into the circular x range).
+  // device when the queue stopped in the incremental update); bonds of
+  // particles the search erases are skipped
   if (prev.nbroken > 0 && reason != 2 && !e->bg.empty()) {
-    if (search && e->grid.circ_x) launch_wrap_x(e->stream, e->view(), e->grid.x0, e->grid.period, e->init.p);
     StepCtl c0 = make_ctl(e, step, 1, 0);
     launch_break_apply(e->stream, c0, e->view(), bond_set(e), broken_list_of(e, step - 1),
-                       e->upd.p, e->flist.p, e->evt.p, step, search ? &e->grid : nullptr);
+                       e->upd.p, e->flist.p, e->evt.p, step, nout > 0 ? &e->grid : nullptr);
     HIPCHK(hipGetLastError());
   }
   fold(e);
-  if (search) rebuild(e, step);
+  if (search) rebuild_rest(e, step, nout);
   e->first_time = false;
   e->forced_search = false;
   e->t = step;
@@ -2596,11 +2617,9 @@ int esp_add_rot_friction_group(esp_engine* e, const char* name, const esp_rot_fr
     e->err = "null parameters";
     return ESP_ERR_ARG;
   }
-  if (p->rigid) {
-    e->err = "RotFriction rigid=True is not supported";
-    return ESP_ERR_ARG;
-  }
-  return add_dyn(e, name, 3, p->k_n, p->k_s, p->mu_d, p->mu_s, p->scaling, p->meanR_scaling);
+  const int rc = add_dyn(e, name, 3, p->k_n, p->k_s, p->mu_d, p->mu_s, p->scaling, p->meanR_scaling);
+  if (rc == ESP_OK) e->dg.back().rigid = p->rigid != 0;
+  return rc;
 }
 
 int esp_add_exclusion(esp_engine* e, const char* excluding, const char* excluded)

@@ -178,7 +178,6 @@ struct DynAppend {
   long long* key;
   double4* ff;       // shear history read by this step (history groups) or nullptr
   int* n_app;        // device counters: [0] pairs appended, [1] pairs the force blocks cover
-  int* app_step;     // step each appendix pair was created in (pair-event log), or nullptr
   int n_main;
   int rec_off;       // records of the main list
   int app_rec_off;   // records of the appendix
@@ -203,6 +202,7 @@ struct ForceSeg {
   int bond_off;     // global index of the first bond (break list entries)
   int scaling, meanR;
   int sc;            // bonded group broken by the stress criterion (no force test)
+  int rigid;         // RotFriction rigid=True (calcRigidBodyForces)
   const int* n_app;  // appended pairs the force blocks cover, device count (dynamic groups)
   const int2* ij;
   double4* ff;       // shear history read by this step (RotFriction)
@@ -246,12 +246,20 @@ struct UpdParams {
   int has_group;
   int useg;
 };
+// Inputs only the extended force launch reads (k_forces_ext: rigid=True
+// RotFriction groups, the pair-event log); the common launch takes none of
+// them, so its code is the same with or without these features.
+struct ForceExt {
+  int* app_step;  // step each appendix pair is created in (tracked group), or nullptr
+};
 // blocks a segment of n interactions takes (dynamic groups: n + ESP_APP_CAP)
 int force_blocks(int n);
 // up: the incremental update of a queued step (fl.ublk = 1), or nullptr;
-// ev_a/ev_b: events the launch stamps at the kernel's start and end
+// ev_a/ev_b: events the launch stamps at the kernel's start and end; ext:
+// the extended launch's inputs (nullptr: none)
 bool launch_forces(hipStream_t s, StepCtl ctl, PView p, const ForceLaunch& fl,
-                   const UpdParams* up, hipEvent_t ev_a = nullptr, hipEvent_t ev_b = nullptr);
+                   const UpdParams* up, hipEvent_t ev_a = nullptr, hipEvent_t ev_b = nullptr,
+                   const ForceExt* ext = nullptr);
 // BrittleBeamSC groups: the stress test after the step's integration, one
 // segment of blocks per group
 struct ScLaunch {

@@ -208,6 +208,9 @@ __device__ __forceinline__ int record_of(int c, int nc, int rec_off, int app_rec
 }
 
 // CRotFrictionInteraction::calcForces (RotFricInteraction.cpp:103-231)
+// RIGID: the launch may hold rigid=True groups (their law is only compiled into
+// the kernel that handles them, so the common kernel keeps its registers)
+template <bool RIGID>
 __device__ __forceinline__ void rot_friction_item(const PView& p, int c, const ForceSeg& g,
                                                   const ForceLaunch& fl)
 {
@@ -246,8 +249,13 @@ __device__ __forceinline__ void rot_friction_item(const PView& p, int c, const F
   dv3 ffr = mk3(st.x, st.y, st.z);
   bool slipping = (((int)st.w) & 1) != 0;
   dv3 f1, m1, m2;
-  esp_rot_friction_law(xyz(P1), xyz(V1), xyz(W1), P1.w, xyz(P2), xyz(V2), xyz(W2), P2.w, kk, kks,
-                       g.mu_d, g.mu_s, fl.dt, ffr, slipping, f1, m1, m2);
+  if (RIGID && g.rigid)
+    esp_rot_friction_rigid_law(xyz(P1), xyz(V1), xyz(W1), P1.w, xyz(P2), xyz(V2), xyz(W2), P2.w,
+                               kk, kks, g.mu_d, g.mu_s, fl.dt, (((int)st.w) & 2) != 0, ffr,
+                               slipping, f1, m1, m2);
+  else
+    esp_rot_friction_law(xyz(P1), xyz(V1), xyz(W1), P1.w, xyz(P2), xyz(V2), xyz(W2), P2.w, kk,
+                         kks, g.mu_d, g.mu_s, fl.dt, ffr, slipping, f1, m1, m2);
   g.ff_out[c] = make_double4(ffr.x, ffr.y, ffr.z, (double)((slipping ? 1 : 0) | 2));
   put3(s0, f1);
   put3(s0 + 3, m1);
@@ -1127,22 +1135,8 @@ __global__ void k_wall_update(PView p, GridParams g, int nwg, WallParam w0, Wall
 // the previous step: the bond leaves its group (marked dead, record zeroed, key
 // erased from the exclusion set), both particles are flagged
 // (RotBondedInteraction.cpp:211-213) and the event is logged.
-// outside the table's padded grid: erased by the next table build (the
-// predicate of k_cells_count)
-__device__ __forceinline__ bool outside_grid(const GridParams& g, const double4& P)
-{
-  int ix, iy, iz;
-  return !(cell_coords(g, P.x, P.y, P.z, ix, iy, iz) && ix >= 0 && ix < g.nx && iy >= 0 &&
-           iy < g.ny && iz >= 0 && iz < g.nz);
-}
-
-// erase_out: this step's table build follows (host bookkeeping), and the
-// reference erases the particles outside the grid, with their bonds, before it
-// tests the bonds (NeighborTable::build then PIS_EB::update): a bond with
-// such a particle is left to the erasure, no break event.
 __device__ __forceinline__ void break_one(PView p, const BondSet& bs, int gb, UpdState* st,
-                                          int* flist, int4* evt, long long step, bool erase_out,
-                                          const GridParams& g_erase)
+                                          int* flist, int4* evt, long long step)
 {
   int g = 0;
   while (g + 1 < bs.count && gb >= bs.off[g + 1]) g++;
@@ -1150,7 +1144,6 @@ __device__ __forceinline__ void break_one(PView p, const BondSet& bs, int gb, Up
   if (b < 0 || b >= bs.n[g]) return;
   const int2 e = bs.ij[g][b];
   if (e.x < 0) return;
-  if (erase_out && (outside_grid(g_erase, p.pos[e.x]) || outside_grid(g_erase, p.pos[e.y]))) return;
   // independent memory operations first, so their latencies overlap
   const int id1 = p.id[e.x], id2 = p.id[e.y];
   const int2 so = bs.slot_of[bs.rec_off[g] + b];
@@ -1178,13 +1171,43 @@ __device__ __forceinline__ void break_one(PView p, const BondSet& bs, int gb, Up
 
 // the previous step's breaks on the host bookkeeping path
 __global__ void k_break_apply(StepCtl ctl, PView p, BondSet bs, const int* broken_list,
-                              UpdState* st, int* flist, int4* evt, long long step, int erase_out,
-                              GridParams g)
+                              UpdState* st, int* flist, int4* evt, long long step)
 {
   if (step_skip(ctl)) return;
   const int nb = ctl.prev->nbroken;
   for (int t = blockIdx.x * BS + threadIdx.x; t < nb; t += gridDim.x * BS)
-    break_one(p, bs, broken_list[t], st, flist, evt, step, erase_out != 0, g);
+    break_one(p, bs, broken_list[t], st, flist, evt, step);
+}
+
+// outside the table's padded grid: erased by the next table build (the
+// predicate of k_cells_count)
+__device__ __forceinline__ bool outside_grid(const GridParams& g, const double4& P)
+{
+  int ix, iy, iz;
+  return !(cell_coords(g, P.x, P.y, P.z, ix, iy, iz) && ix >= 0 && ix < g.nx && iy >= 0 &&
+           iy < g.ny && iz >= 0 && iz < g.nz);
+}
+
+// k_break_apply before a table build that erases particles: the reference
+// erases the particles outside the grid, with their bonds, before it tests the
+// bonds (NeighborTable::build, then PIS_EB::update), so a bond with such a
+// particle is left to the erasure: no break event, no flags.
+__global__ void k_break_apply_erase(StepCtl ctl, PView p, BondSet bs, const int* broken_list,
+                                    UpdState* st, int* flist, int4* evt, long long step,
+                                    GridParams g)
+{
+  if (step_skip(ctl)) return;
+  const int nb = ctl.prev->nbroken;
+  for (int t = blockIdx.x * BS + threadIdx.x; t < nb; t += gridDim.x * BS) {
+    const int gb = broken_list[t];
+    int k = 0;
+    while (k + 1 < bs.count && gb >= bs.off[k + 1]) k++;
+    const int b = gb - bs.off[k];
+    if (b < 0 || b >= bs.n[k]) continue;
+    const int2 e = bs.ij[k][b];
+    if (e.x >= 0 && (outside_grid(g, p.pos[e.x]) || outside_grid(g, p.pos[e.y]))) continue;
+    break_one(p, bs, gb, st, flist, evt, step);
+  }
 }
 
 // TSubLattice::updateInteractions after bond breaks, for a queued step, run by
@@ -1201,19 +1224,23 @@ __global__ void k_break_apply(StepCtl ctl, PView p, BondSet bs, const int* broke
 // waiting for this block.  When the pairs do not fit the appendix, the queue
 // halts and the host redoes the step (its input history buffer is intact).
 #define NEW_SENTINEL 0x7fffffffffffffffll
+template <bool RIGID>
 __device__ __forceinline__ void dyn_item(const PView& p, int c, const ForceSeg& g,
                                          const ForceLaunch& fl)
 {
   switch (g.kind) {
-    case ESP_SEG_ROT_FRICTION: rot_friction_item(p, c, g, fl); break;
+    case ESP_SEG_ROT_FRICTION: rot_friction_item<RIGID>(p, c, g, fl); break;
     case ESP_SEG_NROT_ELASTIC: nrot_elastic_item(p, c, g, fl); break;
     case ESP_SEG_ROT_ELASTIC: rot_elastic_item(p, c, g, fl); break;
     default: break;
   }
 }
 
+// RIGID, TRACK: the extended launch (rigid=True RotFriction groups; app_step,
+// the pair-event log's step of each appended pair)
+template <bool RIGID, bool TRACK>
 __device__ void incremental_update(const StepCtl& ctl, const PView& p, const UpdParams& up,
-                                   const ForceLaunch& fl)
+                                   const ForceLaunch& fl, int* app_step)
 {
   __shared__ long long s_key[ESP_NEW_CAP];
   __shared__ int s_cnt, s_over, s_keep;
@@ -1221,7 +1248,7 @@ __device__ void incremental_update(const StepCtl& ctl, const PView& p, const Upd
   if (nbk <= 0) return;
   const int tid = threadIdx.x;
   for (int t = tid; t < nbk; t += BS)
-    break_one(p, up.bs, up.broken_prev[t], up.st, up.flist, up.evt, up.step, false, up.g);
+    break_one(p, up.bs, up.broken_prev[t], up.st, up.flist, up.evt, up.step);
   if (!up.has_group) return;
   const DynAppend& da = up.da;
   const GridParams& g = up.g;
@@ -1365,7 +1392,7 @@ __device__ void incremental_update(const StepCtl& ctl, const PView& p, const Upd
     da.ij[w] = make_int2(a, b);
     da.key[w] = key;
     if (da.ff) da.ff[w] = make_double4(0.0, 0.0, 0.0, 0.0);
-    if (da.app_step) da.app_step[napp + t] = (int)up.step;
+    if (TRACK && app_step) app_step[napp + t] = (int)up.step;
     if (da.is_hist) hs_insert(da.present, key);
     // appendix slots start at zero (cleared at the last rebuild, used once)
     da.slot_of[da.app_rec_off + napp + t] =
@@ -1380,7 +1407,7 @@ __device__ void incremental_update(const StepCtl& ctl, const PView& p, const Upd
   // this step's forces of the new pairs (each entry was written by the same thread)
   const ForceSeg& us = fl.seg[up.useg];
   const int c0 = da.n_main + napp;
-  for (int t = tid; t < kept; t += BS) dyn_item(p, c0 + t, us, fl);
+  for (int t = tid; t < kept; t += BS) dyn_item<RIGID>(p, c0 + t, us, fl);
   if (tid == 0) da.n_app[0] = napp + kept;
   for (int t = tid; t < nf; t += BS) p.flags[up.flist[t]] &= ~PF_FLAG;
   if (tid == 0) up.st->fcount = 0;
@@ -1390,11 +1417,14 @@ __device__ void incremental_update(const StepCtl& ctl, const PView& p, const Upd
 // blocks covers one group (bonded groups first, their law is the longest), so
 // the groups' memory latencies overlap.  Every interaction writes its slots.
 // In a queued step block 0 runs the incremental update (fl.ublk).
-__global__ __launch_bounds__(BS, 4) void k_forces(StepCtl ctl, PView p, ForceLaunch fl, UpdParams up)
+template <bool RIGID, bool TRACK>
+__device__ __forceinline__ void forces_body(const StepCtl& ctl, const PView& p,
+                                            const ForceLaunch& fl, const UpdParams& up,
+                                            int* app_step)
 {
   if (step_skip(ctl)) return;
   if (fl.ublk && blockIdx.x == 0) {
-    incremental_update(ctl, p, up, fl);
+    incremental_update<RIGID, TRACK>(ctl, p, up, fl, app_step);
     return;
   }
   const int vb = (int)blockIdx.x - fl.ublk;
@@ -1414,8 +1444,43 @@ __global__ __launch_bounds__(BS, 4) void k_forces(StepCtl ctl, PView p, ForceLau
   if (g.kind == ESP_SEG_ROT_BOND) {
     rot_bond_item(ctl, p, c, g, fl);
   } else if (c < g.n + *g.n_app) {
-    dyn_item(p, c, g, fl);
+    dyn_item<RIGID>(p, c, g, fl);
   }
+}
+
+__global__ __launch_bounds__(BS, 4) void k_forces(StepCtl ctl, PView p, ForceLaunch fl, UpdParams up)
+{
+  if (step_skip(ctl)) return;
+  if (fl.ublk && blockIdx.x == 0) {
+    incremental_update<false, false>(ctl, p, up, fl, nullptr);
+    return;
+  }
+  const int vb = (int)blockIdx.x - fl.ublk;
+  int s = 0;
+  while (s + 1 < fl.count && vb >= fl.seg[s + 1].blk0) s++;
+  const ForceSeg& g = fl.seg[s];
+  // blocks go round-robin to the 8 XCDs: within a segment each XCD takes a
+  // contiguous range of blocks, so the state of particles shared by
+  // neighbouring interactions is fetched into one L2 (a bijection on the
+  // segment's blocks)
+  const int nseg = (s + 1 < fl.count ? fl.seg[s + 1].blk0 : fl.nblocks) - g.blk0;
+  const int b0 = g.blk0 + fl.ublk;  // launch index of the segment's first block
+  const int d = ((int)blockIdx.x - b0) & 7;
+  const int q8 = nseg >> 3, r8 = nseg & 7;
+  const int lbk = d * q8 + min(d, r8) + ((int)blockIdx.x - b0 - d) / 8;
+  const int c = lbk * BS + (int)threadIdx.x;
+  if (g.kind == ESP_SEG_ROT_BOND) {
+    rot_bond_item(ctl, p, c, g, fl);
+  } else if (c < g.n + *g.n_app) {
+    dyn_item<false>(p, c, g, fl);
+  }
+}
+
+// the same launch with a rigid=True RotFriction group or a tracked group
+__global__ __launch_bounds__(BS, 4) void k_forces_ext(StepCtl ctl, PView p, ForceLaunch fl,
+                                                      UpdParams up, ForceExt ext)
+{
+  forces_body<true, true>(ctl, p, fl, up, ext.app_step);
 }
 
 __global__ void k_hash_clear(HashRef h)
@@ -1607,7 +1672,7 @@ int force_blocks(int n) { return nblk(n); }
 // the kernel's start and end (hipExtLaunchKernelGGL), so their elapsed time is
 // the kernel's own execution time; false when nothing was launched.
 bool launch_forces(hipStream_t s, StepCtl ctl, PView p, const ForceLaunch& fl,
-                   const UpdParams* up, hipEvent_t ev_a, hipEvent_t ev_b)
+                   const UpdParams* up, hipEvent_t ev_a, hipEvent_t ev_b, const ForceExt* ext)
 {
   UpdParams u;
   if (up) {
@@ -1618,11 +1683,24 @@ bool launch_forces(hipStream_t s, StepCtl ctl, PView p, const ForceLaunch& fl,
   ForceLaunch f = fl;
   f.ublk = up ? 1 : 0;
   if (f.nblocks + f.ublk <= 0) return false;
-  if (ev_a && ev_b)
-    hipExtLaunchKernelGGL(k_forces, dim3(f.nblocks + f.ublk), dim3(BS), 0, s, ev_a, ev_b, 0, ctl, p,
-                          f, u);
-  else
+  bool extended = ext && ext->app_step;
+  for (int k = 0; k < f.count; k++)
+    extended |= f.seg[k].kind == ESP_SEG_ROT_FRICTION && f.seg[k].rigid;
+  ForceExt x;
+  memset(&x, 0, sizeof x);
+  if (ext) x = *ext;
+  if (ev_a && ev_b) {
+    if (extended)
+      hipExtLaunchKernelGGL(k_forces_ext, dim3(f.nblocks + f.ublk), dim3(BS), 0, s, ev_a, ev_b, 0,
+                            ctl, p, f, u, x);
+    else
+      hipExtLaunchKernelGGL(k_forces, dim3(f.nblocks + f.ublk), dim3(BS), 0, s, ev_a, ev_b, 0, ctl,
+                            p, f, u);
+  } else if (extended) {
+    hipLaunchKernelGGL(k_forces_ext, dim3(f.nblocks + f.ublk), dim3(BS), 0, s, ctl, p, f, u, x);
+  } else {
     hipLaunchKernelGGL(k_forces, dim3(f.nblocks + f.ublk), dim3(BS), 0, s, ctl, p, f, u);
+  }
   return true;
 }
 
@@ -1631,11 +1709,12 @@ void launch_break_apply(hipStream_t s, StepCtl ctl, PView p, BondSet bs, const i
                         const GridParams* erase_grid)
 {
   if (bs.count <= 0) return;
-  GridParams g;
-  memset(&g, 0, sizeof g);
-  if (erase_grid) g = *erase_grid;
-  hipLaunchKernelGGL(k_break_apply, dim3(16), dim3(BS), 0, s, ctl, p, bs, broken_list, st, flist,
-                     evt, step, erase_grid ? 1 : 0, g);
+  if (erase_grid)
+    hipLaunchKernelGGL(k_break_apply_erase, dim3(16), dim3(BS), 0, s, ctl, p, bs, broken_list, st,
+                       flist, evt, step, *erase_grid);
+  else
+    hipLaunchKernelGGL(k_break_apply, dim3(16), dim3(BS), 0, s, ctl, p, bs, broken_list, st, flist,
+                       evt, step);
 }
 
 void launch_hash_clear(hipStream_t s, HashRef h)

@@ -340,6 +340,65 @@ ESP_HD bool esp_rot_friction_law(dv3 pos1, dv3 vel1, dv3 w1, double r1, dv3 pos2
   return false;
 }
 
+// CRotFrictionInteraction::calcRigidBodyForces (RotFricInteraction.cpp:237-343),
+// the rigid=True form.  The slip increment is the tangential part of the
+// translational velocity difference plus the difference of the rotational
+// velocities at the contact point; a sticking pair that touched in its last
+// step (touching) first turns its stored shear force with the pair's rigid-body
+// rotation (spin about the axis plus the axis' own rotation).  A slipping pair
+// keeps slipping while the increment exceeds 1e-8 and otherwise sticks with
+// the force it had.  Outputs as esp_rot_friction_law.
+ESP_HD bool esp_rot_friction_rigid_law(dv3 pos1, dv3 vel1, dv3 w1, double r1, dv3 pos2, dv3 vel2,
+                                       dv3 w2, double r2, double k, double ks, double mu_d,
+                                       double mu_s, double dt, bool touching, dv3& ffric,
+                                       bool& slipping, dv3& f1, dv3& m1, dv3& m2)
+{
+  const dv3 D = sub3(pos1, pos2);
+  double dist = dot3(D, D);
+  const double eq_dist = r1 + r2;
+  if (dist < (eq_dist * eq_dist)) {
+    dist = sqrt(dist);
+    const dv3 force = mul3(D, k * (dist - eq_dist) / dist);
+    const dv3 pos = add3(pos2, mul3(D, r2 / eq_dist));
+    const dv3 vp1_rot = cross3(w1, sub3(pos, pos1));
+    const dv3 vp2_rot = cross3(w2, sub3(pos, pos2));
+    const dv3 dv_trs = sub3(vel2, vel1);
+    const dv3 dv_trs_s = sub3(dv_trs, mul3(D, dot3(dv_trs, D) / dot3(D, D)));
+    const dv3 ds = mul3(add3(sub3(vp2_rot, vp1_rot), dv_trs_s), dt);
+    const double forceNorm = norm3(force);
+    if (!slipping) {
+      if (!touching) {
+        ffric = mk3(0.0, 0.0, 0.0);
+      } else {
+        const dv3 rbp = sub3(pos2, pos1);
+        const dv3 vbp = sub3(vel2, vel1);
+        const double rbp0 = norm3(rbp);
+        const dv3 om_s = mul3(add3(w1, w2), 0.5);
+        const dv3 om_spin = div3(mul3(rbp, dot3(om_s, rbp)), rbp0 * rbp0);
+        const dv3 om_m = div3(cross3(rbp, vbp), rbp0 * rbp0);
+        ffric = add3(ffric, mul3(cross3(add3(om_spin, om_m), ffric), dt));
+      }
+      if (norm3(add3(ffric, mul3(ds, ks))) > forceNorm * mu_s) {
+        slipping = true;
+        ffric = div3(mul3(ds, mu_d * forceNorm), norm3(ds));
+      } else {
+        ffric = add3(ffric, mul3(ds, ks));
+      }
+    } else if (norm3(ds) > 1.0e-8) {
+      ffric = div3(mul3(ds, mu_d * forceNorm), norm3(ds));
+    } else {
+      slipping = false;
+    }
+    m1 = cross3(sub3(pos, pos1), ffric);
+    m2 = cross3(sub3(pos, pos2), neg3(ffric));
+    f1 = add3(mul3(force, -1.0), ffric);
+    return true;
+  }
+  ffric = mk3(0.0, 0.0, 0.0);
+  slipping = false;
+  return false;
+}
+
 // 1 / particle volume (CParticle::m_div_vol, Particle.cpp:52)
 ESP_HD double esp_div_vol(double r) { return 1.0 / (4.0 / 3.0 * M_PI * r * r * r); }
 

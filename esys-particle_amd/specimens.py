@@ -302,7 +302,8 @@ def apply(sim, spec, steps_per_batch=None):
         sim.createInteractionGroup(lsm.NRotElasticPrms(name, k))
     if "friction" in spec:
         name, kn, mud, mus, kss = spec["friction"]
-        sim.createInteractionGroup(lsm.RotFrictionPrms(name, kn, mud, mus, kss))
+        sim.createInteractionGroup(lsm.RotFrictionPrms(name, kn, mud, mus, kss,
+                                                       rigid=spec.get("friction_rigid", False)))
     if "exclusion" in spec:
         sim.createExclusion(*spec["exclusion"])
     if spec.get("gravity") is not None:

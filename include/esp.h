@@ -102,7 +102,7 @@ typedef struct esp_rot_friction_prms {
   double mu_s;       /* staticMu */
   double k_s;        /* shearK */
   int scaling;       /* default 1 */
-  int rigid;         /* default 0; 1 is rejected (rigid-body variant not built) */
+  int rigid;         /* default 0; 1: calcRigidBodyForces (RotFricInteraction.cpp:237-343) */
   int meanR_scaling; /* default 1 */
 } esp_rot_friction_prms;
 

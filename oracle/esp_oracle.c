@@ -165,6 +165,7 @@ typedef struct {
   /* createInteractionGroupTagged: tags sorted so that tag1 <= tag2
    * (pi_storage_ed_t.hpp:15-30, pi_storage_ne_t.hpp:15-30) */
   int tagged, tag1, mask1, tag2, mask2;
+  int rigid; /* RotFriction rigid=True: calcRigidBodyForces */
   /* pair-event log (eso_track_pair_events): the stored pairs as sorted id
    * keys after the last step, and the events, 4 ints each (step, id1, id2, kind) */
   int track;
@@ -857,6 +858,77 @@ static void rot_friction_law(v3 pos1, v3 vel1, v3 w1, double r1, v3 pos2, v3 vel
   }
 }
 
+/* CRotFrictionInteraction::calcRigidBodyForces (RotFricInteraction.cpp:237-343),
+ * the rigid=True form: the slip increment takes the tangential part of the
+ * translational velocity difference plus the rotational one, and a stored
+ * shear force of a touching, sticking pair first turns with the pair as a
+ * rigid body.  touching is the interaction's flag from its last step.  Outputs
+ * as rot_friction_law. */
+static void rot_friction_rigid_law(v3 pos1, v3 vel1, v3 w1, double r1, v3 pos2, v3 vel2, v3 w2,
+                                   double r2, double k, double ks, double mu_d, double mu_s,
+                                   double dt, int touching, v3* Ffric, int* slipping,
+                                   double* ediss, v3* fn, fric_out* o)
+{
+  const v3 D = vsub(pos1, pos2);
+  double dist = vdot(D, D);
+  const double eq_dist = r1 + r2;
+  if (dist < (eq_dist * eq_dist)) {
+    dist = sqrt(dist);
+    const v3 force = vmul(D, k * (dist - eq_dist) / dist);
+    const v3 pos = vadd(pos2, vmul(D, r2 / eq_dist));
+    *fn = force;
+    const v3 vp1_rot = vcross(w1, vsub(pos, pos1));
+    const v3 vp2_rot = vcross(w2, vsub(pos, pos2));
+    const v3 dv_trs = vsub(vel2, vel1);
+    const v3 dv_trs_s = vsub(dv_trs, vmul(D, vdot(dv_trs, D) / vnorm2(D)));
+    const v3 ds = vmul(vadd(vsub(vp2_rot, vp1_rot), dv_trs_s), dt);
+    const double forceNorm = vnorm(force);
+    if (!*slipping) {
+      if (!touching) {
+        *Ffric = V(0.0, 0.0, 0.0); /* first touch */
+      } else {
+        /* the pair's rigid-body rotation: spin about the axis plus the
+         * rotation of the axis itself */
+        const v3 rbp = vsub(pos2, pos1);
+        const v3 vbp = vsub(vel2, vel1);
+        const double rbp0 = vnorm(rbp);
+        const v3 omiga_s = vmul(vadd(w1, w2), 0.5);
+        const v3 omiga_spin = vdiv(vmul(rbp, vdot(omiga_s, rbp)), rbp0 * rbp0);
+        const v3 omiga_m = vdiv(vcross(rbp, vbp), rbp0 * rbp0);
+        const v3 d_Ffric = vmul(vcross(vadd(omiga_spin, omiga_m), *Ffric), dt);
+        *Ffric = vadd(*Ffric, d_Ffric);
+      }
+      if (vnorm(vadd(*Ffric, vmul(ds, ks))) > forceNorm * mu_s) {
+        *slipping = 1;
+        *Ffric = vdiv(vmul(ds, mu_d * forceNorm), vnorm(ds));
+        *ediss = fabs(vdot(*Ffric, ds));
+      } else {
+        *Ffric = vadd(*Ffric, vmul(ds, ks));
+        *ediss = 0.0;
+      }
+    } else {
+      if (vnorm(ds) > 1.0e-8) {
+        *Ffric = vdiv(vmul(ds, mu_d * forceNorm), vnorm(ds));
+        *ediss = fabs(vdot(*Ffric, ds));
+      } else {
+        *slipping = 0;
+        *ediss = 0.0;
+      }
+    }
+    o->m1 = vcross(vsub(pos, pos1), *Ffric);
+    o->m2 = vcross(vsub(pos, pos2), vneg(*Ffric));
+    o->f1 = *Ffric;
+    o->f2 = vmul(*Ffric, -1.0);
+    o->cpos = pos;
+    o->touching = 1;
+  } else {
+    *Ffric = V(0.0, 0.0, 0.0);
+    *slipping = 0;
+    *ediss = 0.0;
+    o->touching = 0;
+  }
+}
+
 /* Scaled stiffnesses of the pair laws (constructors). */
 static double elastic_k(double k, int scaling, double r1, double r2, int do2d)
 { /* ElasticInteraction.cpp:25-46 */
@@ -1371,9 +1443,17 @@ static void calc_and_integrate(eso_engine* e)
       } else {
         fric_out o;
         v3 fn = V(0, 0, 0);
-        rot_friction_law(p1->pos, p1->vel, p1->angvel, p1->rad, x2, p2->vel, p2->angvel,
-                         p2->rad, d->k, d->ks, G->mu_d, G->mu_s, e->dt, &d->ffric, &d->slipping,
-                         &d->ediss, &fn, &o);
+        if (G->rigid)
+          rot_friction_rigid_law(p1->pos, p1->vel, p1->angvel, p1->rad, x2, p2->vel, p2->angvel,
+                                 p2->rad, d->k, d->ks, G->mu_d, G->mu_s, e->dt, d->touching,
+                                 &d->ffric, &d->slipping, &d->ediss, &fn, &o);
+        else
+          rot_friction_law(p1->pos, p1->vel, p1->angvel, p1->rad, x2, p2->vel, p2->angvel,
+                           p2->rad, d->k, d->ks, G->mu_d, G->mu_s, e->dt, &d->ffric, &d->slipping,
+                           &d->ediss, &fn, &o);
+        /* per particle the rigid form adds the friction force's moment before
+         * the explicit moment (RotFricInteraction.cpp:324-327); both are
+         * cross(pos - x, Ffric), so the sums agree with this order */
         if (o.touching) {
           apply_force_at(p2, fn, o.cpos, x2);
           apply_force(p1, vmul(fn, -1.0), o.cpos);
@@ -1817,9 +1897,10 @@ int eso_add_rot_friction_group(eso_engine* e, const char* name, const esp_rot_fr
 {
   if (!e->rot) return fail(e, ESP_ERR_STATE, "RotFriction needs particle type RotSphere");
   if (!p) return fail(e, ESP_ERR_ARG, "null parameters");
-  if (p->rigid) return fail(e, ESP_ERR_ARG, "RotFriction rigid=True is not supported");
-  return add_dg(e, name, G_ED_ROTFRICTION, p->k_n, p->k_s, p->mu_d, p->mu_s, p->scaling,
-                p->meanR_scaling);
+  int rc = add_dg(e, name, G_ED_ROTFRICTION, p->k_n, p->k_s, p->mu_d, p->mu_s, p->scaling,
+                  p->meanR_scaling);
+  if (rc == ESP_OK) e->dg[e->ndg - 1].rigid = p->rigid != 0;
+  return rc;
 }
 
 int eso_add_exclusion(eso_engine* e, const char* excluding, const char* excluded)
@@ -2387,6 +2468,31 @@ void eso_law_rot_friction(const double* p, const double* prm, double* state, dou
   for (int i = 0; i < 13; i++) out[i] = 0.0;
   if (o.touching) {
     /* particle 1 receives -Fn then +Ffric; particle 2 receives +Fn then -Ffric */
+    v3 f1 = vadd(vmul(fn, -1.0), o.f1);
+    v3 f2 = vadd(fn, o.f2);
+    out[0] = f1.x; out[1] = f1.y; out[2] = f1.z;
+    out[3] = o.m1.x; out[4] = o.m1.y; out[5] = o.m1.z;
+    out[6] = f2.x; out[7] = f2.y; out[8] = f2.z;
+    out[9] = o.m2.x; out[10] = o.m2.y; out[11] = o.m2.z;
+    out[12] = 1.0;
+  }
+}
+
+/* The rigid form; state[4] is the touching flag of the last step. */
+void eso_law_rot_friction_rigid(const double* p, const double* prm, double* state, double* out)
+{
+  v3 ff = V(state[0], state[1], state[2]);
+  int slipping = state[3] != 0.0;
+  double ediss = 0;
+  v3 fn;
+  fric_out o;
+  rot_friction_rigid_law(V(p[0], p[1], p[2]), V(p[3], p[4], p[5]), V(p[6], p[7], p[8]), p[9],
+                         V(p[10], p[11], p[12]), V(p[13], p[14], p[15]), V(p[16], p[17], p[18]),
+                         p[19], prm[0], prm[1], prm[2], prm[3], prm[4], state[4] != 0.0, &ff,
+                         &slipping, &ediss, &fn, &o);
+  state[0] = ff.x; state[1] = ff.y; state[2] = ff.z; state[3] = slipping; state[4] = o.touching;
+  for (int i = 0; i < 13; i++) out[i] = 0.0;
+  if (o.touching) {
     v3 f1 = vadd(vmul(fn, -1.0), o.f1);
     v3 f2 = vadd(fn, o.f2);
     out[0] = f1.x; out[1] = f1.y; out[2] = f1.z;

@@ -105,6 +105,7 @@ void eso_law_rot_bond(const double* p, const double* b, double* out);
  * prm = {k, ks, mu_d, mu_s, dt} (already scaled), state = {Ffric[3], slipping}
  * (updated in place), out = {F1[3], M1[3], F2[3], M2[3], touching} (13) */
 void eso_law_rot_friction(const double* p, const double* prm, double* state, double* out);
+void eso_law_rot_friction_rigid(const double* p, const double* prm, double* state, double* out);
 /* CElasticInteraction::calcForces (ElasticInteraction.cpp:54-71):
  * p = {pos1[3], rad1, pos2[3], rad2}, k scaled; out = {F1[3], F2[3], touching} */
 void eso_law_nrot_elastic(const double* p, double k, double* out);

@@ -20,6 +20,7 @@ ORACLE_EXTRA = {
     "get_force_scale": (ctypes.c_int, [_vp, _pd]),
     "law_rot_bond": (None, [_pd, _pd, _pd]),
     "law_rot_friction": (None, [_pd, _pd, _pd, _pd]),
+    "law_rot_friction_rigid": (None, [_pd, _pd, _pd, _pd]),
     "law_nrot_elastic": (None, [_pd, ctypes.c_double, _pd]),
     "bond_constants": (None, [_pd, _pd, _pd]),
     "math_quat_mul": (None, [_pd, _pd, _pd]),

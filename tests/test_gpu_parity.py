@@ -102,6 +102,33 @@ def test_bonded_trajectory_and_events(pkg, orc, spm):
         assert rel_field(hg, ho) <= TRAJ_TOL
 
 
+def test_rigid_friction_block(pkg, orc, spm):
+    """RotFrictionPrms(rigid=True) (calcRigidBodyForces,
+    RotFricInteraction.cpp:237-343) on the breaking block: forces of the first
+    steps within 1e-12, events, lists and slip/touch flags bit-exact, trajectories
+    within 1e-9 after 200 steps; the rigid form changes the run."""
+    spec = lattice_spec(spm)
+    spec["friction_rigid"] = True
+    g, o = run_pair(pkg, orc, spm, spec, 2)
+    assert rel_particle(g, o, "force") <= FORCE_TOL
+    assert rel_particle(g, o, "moment") <= FORCE_TOL
+    g, o = run_pair(pkg, orc, spm, spec, 200)
+    assert len(o.getBondBreaks()) > 100
+    assert_events_equal(g, o, ["bonded", "friction"])
+    r = float(np.mean(spec["rad"]))
+    assert np.max(np.abs(g.getParticleData("pos") - o.getParticleData("pos"))) <= TRAJ_TOL * r
+    assert rel_field(g.getParticleData("angvel"), o.getParticleData("angvel")) <= TRAJ_TOL
+    gf, gfl = g.engine.friction_state("friction")
+    of, ofl = o.engine.friction_state("friction")
+    np.testing.assert_array_equal(gfl, ofl)
+    assert rel_field(gf, of) <= TRAJ_TOL
+    spec["friction_rigid"] = False
+    s = pkg.LsmMpi(stepsPerBatch=32)
+    spm.apply(s, spec)
+    s.runTimeSteps(200)
+    assert np.max(np.abs(s.getParticleData("pos") - g.getParticleData("pos"))) > 1e-12
+
+
 def test_gravity_settling_with_rebuilds(pkg, orc, spm):
     spec = spm.gravity_settling(n=600, seed=5, box=(8.0, 16.0, 8.0), dt=2e-3)
     g, o = run_pair(pkg, orc, spm, spec, 200)

@@ -211,6 +211,109 @@ def test_friction_rolling_contact_point(olib):
     np.testing.assert_allclose(st[:3], [0, -50 * 0.49 * w * 1e-3, 0], rtol=1e-12)
 
 
+def fric_rigid(olib, p1, v1, w1, r1, p2, v2, w2, r2, k, ks, mud, mus, dt, state):
+    """The rigid=True law; state = (Ffric, slipping, touching)."""
+    inp = np.array([*p1, *v1, *w1, r1, *p2, *v2, *w2, r2], float)
+    prm = np.array([k, ks, mud, mus, dt], float)
+    out = np.zeros(13)
+    olib.law_rot_friction_rigid(P(inp), P(prm), P(state), P(out))
+    return out
+
+
+def _tangent(D, rng):
+    t = np.cross(D, rng.normal(size=3))
+    return t / np.linalg.norm(t)
+
+
+def test_rigid_friction_turns_with_the_pair(olib):
+    """A touching, sticking pair moving as a rigid body (v_i = W x (x_i - c),
+    w_i = W) has no slip at the contact point, and the rigid form turns the
+    stored shear force with the pair: Ffric + dt W x Ffric
+    (RotFricInteraction.cpp:283-297: spin plus axis rotation add up to W).
+    The simple form leaves the force where it was."""
+    rng = np.random.default_rng(4)
+    k, ks, mud, mus, dt = 100.0, 50.0, 0.4, 0.6, 1e-3
+    for W in ([0, 0, 2.0], [1.5, 0, 0], [0.3, -0.7, 1.1]):
+        W = np.array(W)
+        n = rng.normal(size=3)
+        n /= np.linalg.norm(n)
+        p1 = rng.normal(size=3)
+        p2 = p1 + 0.95 * n  # overlap 0.05: |Fn| = 5, mu_s |Fn| = 3
+        c = 0.5 * (p1 + p2)
+        v1, v2 = np.cross(W, p1 - c), np.cross(W, p2 - c)
+        F0 = 0.8 * _tangent(p1 - p2, rng)
+        st = np.array([*F0, 0.0, 1.0])
+        fric_rigid(olib, p1, v1, W, 0.5, p2, v2, W, 0.5, k, ks, mud, mus, dt, st)
+        assert st[3] == 0 and st[4] == 1
+        np.testing.assert_allclose(st[:3], F0 + dt * np.cross(W, F0), rtol=0, atol=1e-13)
+        assert np.linalg.norm(dt * np.cross(W, F0)) > 1e-4  # the turn is resolved
+        ss = np.array([*F0, 0.0])
+        fric(olib, p1, v1, W, 0.5, p2, v2, W, 0.5, k, ks, mud, mus, dt, ss)
+        np.testing.assert_allclose(ss[:3], F0, rtol=0, atol=1e-13)
+
+
+def test_rigid_friction_without_rotation_matches_simple_law(olib):
+    """Without particle rotation and from a first touch (any stored force is
+    dropped, RotFricInteraction.cpp:277-281) both forms take the tangential
+    relative displacement: the same spring, the same slip force."""
+    rng = np.random.default_rng(5)
+    k, ks, mud, mus, dt = 100.0, 50.0, 0.4, 0.6, 1e-3
+    for trial in range(20):
+        n = rng.normal(size=3)
+        n /= np.linalg.norm(n)
+        p1 = rng.normal(size=3)
+        p2 = p1 + 0.95 * n
+        v1, v2 = rng.normal(size=3), rng.normal(size=3) * (1 + 200 * (trial % 2))
+        st = np.array([*rng.normal(size=3), 0.0, 0.0])
+        o_r = fric_rigid(olib, p1, v1, [0, 0, 0], 0.5, p2, v2, [0, 0, 0], 0.5, k, ks, mud, mus,
+                         dt, st)
+        ss = np.zeros(4)
+        o_s = fric(olib, p1, v1, [0, 0, 0], 0.5, p2, v2, [0, 0, 0], 0.5, k, ks, mud, mus, dt, ss)
+        assert st[3] == ss[3] and st[4] == 1
+        np.testing.assert_allclose(st[:3], ss[:3], rtol=1e-12, atol=1e-15)
+        np.testing.assert_allclose(o_r, o_s, rtol=1e-12, atol=1e-14)
+
+
+def test_rigid_friction_slipping_branches(olib):
+    """A slipping pair: with a slip increment above 1e-8 the force is
+    mu_d |Fn| along it; without one the pair sticks and keeps its force
+    (RotFricInteraction.cpp:310-321)."""
+    k, ks, mud, mus, dt = 100.0, 50.0, 0.4, 0.6, 1e-3
+    p1, p2 = [0, 0, 0], [0.95, 0, 0]
+    st = np.array([0.0, 1.0, 0.0, 1.0, 1.0])
+    fric_rigid(olib, p1, [0, 0, 0], [0, 0, 0], 0.5, p2, [0, 0, 3.0], [0, 0, 0], 0.5, k, ks, mud,
+               mus, dt, st)
+    assert st[3] == 1
+    np.testing.assert_allclose(st[:3], [0, 0, mud * k * 0.05], rtol=1e-14)
+    held = np.array([0.3, -0.2, 0.1, 1.0, 1.0])
+    st = held.copy()
+    fric_rigid(olib, p1, [0, 0, 0], [0, 0, 0], 0.5, p2, [0, 0, 0], [0, 0, 0], 0.5, k, ks, mud, mus,
+               dt, st)
+    assert st[3] == 0 and st[4] == 1
+    np.testing.assert_array_equal(st[:3], held[:3])
+
+
+def test_rigid_friction_no_contact_and_balance(olib):
+    st = np.array([1.0, 2.0, 3.0, 1.0, 1.0])
+    out = fric_rigid(olib, [0, 0, 0], [0, 0, 0], [0, 0, 0], 0.5, [1.2, 0, 0], [0, 0, 0],
+                     [0, 0, 0], 0.5, 100, 50, 0.4, 0.6, 1e-3, st)
+    np.testing.assert_array_equal(st, 0)
+    assert out[12] == 0
+    # Newton's third law and the angular momentum balance about the origin
+    rng = np.random.default_rng(6)
+    for _ in range(20):
+        n = rng.normal(size=3)
+        n /= np.linalg.norm(n)
+        p1 = rng.normal(size=3)
+        p2 = p1 + 0.93 * n
+        st = np.array([*(0.5 * _tangent(p1 - p2, rng)), float(rng.integers(2)), 1.0])
+        out = fric_rigid(olib, p1, rng.normal(size=3), rng.normal(size=3), 0.5, p2,
+                         rng.normal(size=3), rng.normal(size=3), 0.5, 100, 50, 0.4, 0.6, 1e-3, st)
+        np.testing.assert_allclose(out[0:3], -out[6:9], rtol=0, atol=1e-14)
+        L = out[3:6] + out[9:12] + np.cross(p1, out[0:3]) + np.cross(p2, out[6:9])
+        assert np.linalg.norm(L) <= 1e-12 * (1 + np.linalg.norm(out[0:3]))
+
+
 def test_nrot_elastic_hooke(olib):
     out = np.zeros(7)
     p = np.array([0, 0, 0, 0.5, 0.9, 0, 0, 0.5])
