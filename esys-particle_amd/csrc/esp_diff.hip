@@ -168,6 +168,11 @@ __device__ __forceinline__ bool run_skip(const DRun* R, int& step)
   return step > R->end || (h != 0 && h < step);
 }
 
+// Largest block of the per-step kernels.  The law kernels declare it as their
+// launch bound, which gives them 256 VGPRs a lane (the default bound of 1024
+// threads caps them at 128, and the bond law then spills to scratch memory).
+#define DIFF_MAX_BS 512
+
 __global__ void k_set_run(DRun* R, int step, int halt, int end)
 {
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -224,7 +229,7 @@ __device__ __forceinline__ tquat<esp_dual> ddq(double4 a, double4 t)
   return tquat<esp_dual>{esp_dual(a.x, t.x), {esp_dual(a.y, t.y), esp_dual(a.z, t.z), esp_dual(a.w, t.w)}};
 }
 
-__global__ void k_bond_tan(DState S, DTan T, DBonds B, const double4* theta, const double4* dtheta,
+__global__ __launch_bounds__(DIFF_MAX_BS) void k_bond_tan(DState S, DTan T, DBonds B, const double4* theta, const double4* dtheta,
                            const DRun* run, double* bt)
 {
   int step;
@@ -255,7 +260,7 @@ __global__ void k_bond_tan(DState S, DTan T, DBonds B, const double4* theta, con
   o[6] = r.m2.x.d; o[7] = r.m2.y.d; o[8] = r.m2.z.d;
 }
 
-__global__ void k_fric_tan(DState S, DTan T, DCands C, const int* bstep, const double4* theta,
+__global__ __launch_bounds__(DIFF_MAX_BS) void k_fric_tan(DState S, DTan T, DCands C, const int* bstep, const double4* theta,
                            const double4* dtheta, const double4* H0, const double4* H1,
                            double4* TH0, double4* TH1, const DRun* run, double dt, double* ct)
 {
@@ -296,7 +301,7 @@ __global__ void k_fric_tan(DState S, DTan T, DCands C, const int* bstep, const d
 // ---------------------------------------------------------------------------
 // forward kernels
 // ---------------------------------------------------------------------------
-__global__ void k_bond_fwd(DState S, DBonds B, const double4* theta, const DRun* run, double* bout)
+__global__ __launch_bounds__(DIFF_MAX_BS) void k_bond_fwd(DState S, DBonds B, const double4* theta, const DRun* run, double* bout)
 {
   int step;
   if (run_skip(run, step)) return;
@@ -1523,8 +1528,8 @@ int esp_diff_create(const esp_diff_config* cfg, esp_diff** out)
     return ESP_ERR_ARG;
   }
   const int bsz = cfg->block > 0 ? cfg->block : 256;
-  if (bsz < 64 || bsz > 1024 || bsz % 64) {
-    g_diff_create_err = "block must be a multiple of 64 in [64, 1024]";
+  if (bsz < 64 || bsz > DIFF_MAX_BS || bsz % 64) {
+    g_diff_create_err = "block must be a multiple of 64 in [64, 512]";
     return ESP_ERR_ARG;
   }
   // the neighbour table of TSubLattice::initNeighborTable (SubLattice.hpp:455-478)

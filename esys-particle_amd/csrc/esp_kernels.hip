@@ -174,13 +174,16 @@ __device__ __forceinline__ void rot_bond_item(const StepCtl& ctl, const PView& p
   if (e.x < 0) return;  // broken earlier: removed from the group, slots left at zero
   const double4 P1 = p.pos[e.x], P2 = x_image(P1, p.pos[e.y], fl.period);
   const double4 Q1 = p.quat[e.x], Q2 = p.quat[e.y];
-  const double4 K = g.k4[b], D = g.D4[b], M = g.mx4[b];
+  const double4 K = g.k4[b], D = g.D4[b];
   // broken in an earlier step (D.w holds that step): the incremental update
   // that runs beside this launch removes the bond and clears its slots
   if (D.w != 0.0 && D.w < fl.step) return;
   BondResult o;
   esp_rot_bond_law(xyz(P1), dquat{Q1.x, mk3(Q1.y, Q1.z, Q1.w)}, P1.w, xyz(P2),
                    dquat{Q2.x, mk3(Q2.y, Q2.z, Q2.w)}, P2.w, K.x, K.y, K.z, K.w, xyz(D), o);
+  // the strengths are needed only by the break test after the law: loaded
+  // here, their latency overlaps the slot stores
+  const double4 M = g.mx4[b];
   double* s0 = fl.slots + (size_t)fl.ss * so.x;
   double* s1 = fl.slots + (size_t)fl.ss * so.y;
   put3(s0, o.f1);
@@ -1118,14 +1121,16 @@ __global__ void k_compact_bonds(int nb, const int* keep, const int* off, const i
 // CEWallInteractionGroup::Update (EWallInteractionGroup.hpp:57-82): particles
 // with |(x - o).n| < gridSpacing become candidates; isInInner (ntable.hpp:207-222)
 // decides whether their force counts towards the wall force.
-__global__ void k_wall_update(PView p, GridParams g, int nwg, WallParam w0, WallParam w1,
-                              WallParam w2, WallParam w3, WallParam w4, WallParam w5,
-                              WallParam w6, WallParam w7)
+// The walls travel as one by-value array in the kernel arguments and are read
+// in place (a local copy of the array would live in scratch memory).
+struct WallSet {
+  WallParam w[ESP_MAX_WALL_GROUPS];
+};
+__global__ void k_wall_update(PView p, GridParams g, int nwg, WallSet ws)
 {
   const int i = blockIdx.x * BS + threadIdx.x;
   if (i >= p.n) return;
-  const WallParam W[ESP_MAX_WALL_GROUPS] = {w0, w1, w2, w3, w4, w5, w6, w7};
-  p.flags[i] = wall_membership(g, nwg, W, p.pos[i], p.flags[i]);
+  p.flags[i] = wall_membership(g, nwg, ws.w, p.pos[i], p.flags[i]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1476,8 +1481,10 @@ __global__ __launch_bounds__(BS, 4) void k_forces(StepCtl ctl, PView p, ForceLau
   }
 }
 
-// the same launch with a rigid=True RotFriction group or a tracked group
-__global__ __launch_bounds__(BS, 4) void k_forces_ext(StepCtl ctl, PView p, ForceLaunch fl,
+// the same launch with a rigid=True RotFriction group or a tracked group (the
+// rigid law needs more registers than the common kernel's budget of 128, so
+// this kernel has no minimum occupancy: it runs without register spills)
+__global__ __launch_bounds__(BS) void k_forces_ext(StepCtl ctl, PView p, ForceLaunch fl,
                                                       UpdParams up, ForceExt ext)
 {
   forces_body<true, true>(ctl, p, fl, up, ext.app_step);
@@ -2007,8 +2014,9 @@ void launch_compact_bonds(hipStream_t s, int nb, const int* keep, const int* off
 void launch_wall_update(hipStream_t s, PView p, GridParams g, int nwg, const WallParam* w)
 {
   if (p.n <= 0) return;
-  hipLaunchKernelGGL(k_wall_update, dim3(nblk(p.n)), dim3(BS), 0, s, p, g, nwg, w[0], w[1], w[2],
-                     w[3], w[4], w[5], w[6], w[7]);
+  WallSet ws;
+  for (int k = 0; k < ESP_MAX_WALL_GROUPS; k++) ws.w[k] = w[k];
+  hipLaunchKernelGGL(k_wall_update, dim3(nblk(p.n)), dim3(BS), 0, s, p, g, nwg, ws);
 }
 
 void launch_inc_count(hipStream_t s, int nc, const int2* ij, int* inc_cnt)

@@ -54,7 +54,7 @@ typedef struct esp_diff_config {
   double dt;
   int device;
   int checkpoint_every;  /* steps per checkpointed segment of the adjoint (>= 1) */
-  int block;             /* threads per block of the per-step kernels (64..1024, multiple of 64) */
+  int block;             /* threads per block of the per-step kernels (64..512, multiple of 64) */
 } esp_diff_config;
 
 typedef struct esp_diff esp_diff;
