@@ -161,6 +161,22 @@ __device__ __forceinline__ void put_rel(double* a, double* b, double4 P1, double
   put3(b + 6, sub3(c, xyz(P2)));
 }
 
+// A bond's particle pair read with one 8-byte load.  The incremental update
+// that runs beside the force blocks (block 0 of a queued step) marks the bonds
+// broken in the previous step dead by storing (-1, -1) over their pairs.  A
+// plain int2 read may be compiled into two 4-byte loads, the second issued
+// only after the first index passed its test, and could then return a live
+// first index with the dead second one (and the force law would gather
+// position -1).  One relaxed atomic load returns either the whole old pair or
+// (-1, -1).
+__device__ __forceinline__ int2 load_pair_once(const int2* a)
+{
+  const unsigned long long v =
+      __hip_atomic_load(reinterpret_cast<unsigned long long*>(const_cast<int2*>(a)),
+                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  return make_int2((int)(unsigned)(v & 0xffffffffull), (int)(unsigned)(v >> 32));
+}
+
 // ARotBondedInteraction::calcForces + CRotBondedInteraction::broken
 // (ARotBondedInteraction.cpp:307-448, RotBondedInteraction.cpp:175-219)
 __device__ __forceinline__ void rot_bond_item(const StepCtl& ctl, const PView& p, int b,
@@ -169,9 +185,9 @@ __device__ __forceinline__ void rot_bond_item(const StepCtl& ctl, const PView& p
   const int nb = g.n;
   const int2* __restrict__ ij = g.ij;
   if (b >= nb) return;
-  const int2 e = ij[b];
+  const int2 e = load_pair_once(ij + b);
   const int2 so = fl.slot_of[g.rec_off + b];
-  if (e.x < 0) return;  // broken earlier: removed from the group, slots left at zero
+  if (e.x < 0 || e.y < 0) return;  // broken earlier: removed from the group, slots left at zero
   const double4 P1 = p.pos[e.x], P2 = x_image(P1, p.pos[e.y], fl.period);
   const double4 Q1 = p.quat[e.x], Q2 = p.quat[e.y];
   const double4 K = g.k4[b], D = g.D4[b];
