@@ -64,6 +64,7 @@ _SIGS = {
     "esp_diff_get_times": (ctypes.c_int, [ctypes.c_void_p, _pd, _pd]),
     "esp_diff_get_branch_digest": (ctypes.c_int, [ctypes.c_void_p,
                                                   ctypes.POINTER(ctypes.c_uint64)]),
+    "esp_diff_track_branches": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
 }
 
 _LIB = None
@@ -212,6 +213,11 @@ class DiffModel:
         s = np.zeros(max(m, 1), np.int32)
         self._check(self.lib.esp_diff_get_rebuild_steps(self.h, _ip(s)), "rebuild_steps")
         return s[:m]
+
+    def track_branches(self, enable=True):
+        """Compute the per-step branch digests of later forward runs (off by
+        default: it costs forward time)."""
+        self._check(self.lib.esp_diff_track_branches(self.h, int(bool(enable))), "track_branches")
 
     def branch_digest(self):
         """Per-step branch digests of the last forward run (uint64 [nsteps]):

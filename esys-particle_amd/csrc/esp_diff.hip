@@ -309,10 +309,9 @@ __global__ __launch_bounds__(DIFF_MAX_BS) void k_bond_fwd(DState S, DBonds B, co
   if (b >= B.nb) return;
   double* o = bout + 9 * (size_t)b;
   const int bs = B.bstep[b];
-  if (bs < step) {  // removed after the step whose criterion broke it
-    for (int k = 0; k < 9; k++) o[k] = 0.0;
-    return;
-  }
+  // removed after the step whose criterion broke it: no record (the gathers
+  // skip a bond whose break step is before theirs)
+  if (bs < step) return;
   const int2 e = B.ij[b];
   const double4 P1 = S.x[e.x], P2 = S.x[e.y], Q1 = S.q[e.x], Q2 = S.q[e.y];
   const double4 D = B.D0[b];
@@ -342,45 +341,71 @@ __global__ __launch_bounds__(DIFF_MAX_BS) void k_bond_fwd(DState S, DBonds B, co
     B.bstep[b] = step;
 }
 
+// History flag bits (w of a history): 1 slipping, 2 touching, 4 zero in both
+// buffers (the pair has been apart for at least the last step, so neither
+// buffer needs writing until it touches).  act[c]: the record of pair c holds
+// this step's (nonzero) force; the gathers read only such records.
 __global__ void k_fric_fwd(DState S, DCands C, const int* bstep, const double4* theta,
                            double4* H0, double4* H1, const DRun* run, double dt, double* cout,
-                           unsigned long long* dig_base)
+                           unsigned char* act, unsigned long long* dig_base)
 {
   int step;
   if (run_skip(run, step)) return;
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= C.nc) return;
   // step s reads the histories of buffer s & 1 and writes the other
-  const double4* hin = (step & 1) ? H1 : H0;
+  double4* hin = (step & 1) ? H1 : H0;
   double4* hout = (step & 1) ? H0 : H1;
   unsigned long long* dig = dig_base ? dig_base + step : nullptr;
-  double* o = cout + 9 * (size_t)c;
   const int bb = C.bond[c];
-  if (bb >= 0 && bstep[bb] >= step) {  // excluded while the bond lives
-    for (int k = 0; k < 9; k++) o[k] = 0.0;
-    hout[c] = make_double4(0.0, 0.0, 0.0, 0.0);
+  if (bb >= 0 && bstep[bb] >= step) {
+    // excluded while the bond lives: no record, and no history is read until
+    // the step after its break, which reads the zero history written here at
+    // the break step
+    act[c] = 0;
+    if (bstep[bb] == step) hout[c] = make_double4(0.0, 0.0, 0.0, 0.0);
     return;
   }
   const int2 e = C.ij[c];
   const double4 P1 = S.x[e.x], P2 = S.x[e.y];
-  const double4 V1 = S.v[e.x], V2 = S.v[e.y], W1 = S.w[e.x], W2 = S.w[e.y];
-  const double4 th = theta[S.spec[e.x]];
-  double kk, kks;
-  tfriction_k<double>(th.x, th.y, 1, 1, P1.w, P2.w, 0, kk, kks);
   const double4 h = hin[c];
+  // the law's contact test, before the velocities are loaded
+  const tv3<double> Dq = tsub(dx3(P1), dx3(P2));
+  const double eq = P1.w + P2.w;
+  const bool touching = tdot(Dq, Dq) < (eq * eq);
   tv3<double> ffr{h.x, h.y, h.z}, f1, m1, m2;
-  bool slipping = (((int)h.w) & 1) != 0;
-  const bool touching = tfriction_law<double>(dx3(P1), dx3(V1), dx3(W1), P1.w, dx3(P2), dx3(V2),
-                                              dx3(W2), P2.w, kk, kks, th.w, th.w, dt, ffr,
-                                              slipping, f1, m1, m2);
-  hout[c] = make_double4(ffr.x, ffr.y, ffr.z, (double)((slipping ? 1 : 0) | (touching ? 2 : 0)));
+  bool slipping = false;
+  if (touching) {
+    const double4 V1 = S.v[e.x], V2 = S.v[e.y], W1 = S.w[e.x], W2 = S.w[e.y];
+    const double4 th = theta[S.spec[e.x]];
+    double kk, kks;
+    tfriction_k<double>(th.x, th.y, 1, 1, P1.w, P2.w, 0, kk, kks);
+    slipping = (((int)h.w) & 1) != 0;
+    tfriction_law<double>(dx3(P1), dx3(V1), dx3(W1), P1.w, dx3(P2), dx3(V2), dx3(W2), P2.w, kk, kks,
+                          th.w, th.w, dt, ffr, slipping, f1, m1, m2);
+  }
   if (dig) {
     const unsigned long long mt = __ballot(touching), ms = __ballot(touching && slipping);
     wave_digest(dig, (unsigned long long)(c & ~63), mt, ms);
   }
+  if (!touching) {
+    // apart: history cleared, no force
+    act[c] = 0;
+    const int fw = (int)h.w;
+    if ((fw & 3) != 0 || h.x != 0.0 || h.y != 0.0 || h.z != 0.0) {
+      hout[c] = make_double4(0.0, 0.0, 0.0, 0.0);
+    } else if (!(fw & 4)) {
+      hin[c] = make_double4(0.0, 0.0, 0.0, 4.0);
+      hout[c] = make_double4(0.0, 0.0, 0.0, 4.0);
+    }
+    return;
+  }
+  double* o = cout + 9 * (size_t)c;
+  hout[c] = make_double4(ffr.x, ffr.y, ffr.z, (double)((slipping ? 1 : 0) | 2));
   o[0] = f1.x; o[1] = f1.y; o[2] = f1.z;
   o[3] = m1.x; o[4] = m1.y; o[5] = m1.z;
   o[6] = m2.x; o[7] = m2.y; o[8] = m2.z;
+  act[c] = 1;
 }
 
 // per particle: the measured wall's normal force on it (sample steps), walls,
@@ -388,9 +413,10 @@ __global__ void k_fric_fwd(DState S, DCands C, const int* bstep, const double4* 
 // the total force and moment (integrate = 0 writes fm)
 __global__ void k_gather_fwd(DState S, const double4* mass_old, const int* inc_off,
                              const int* inc_ent, int ncand, const double* cout, const double* bout,
-                             DStep sp, int every, double* wfn_s, DRun* run,
-                             unsigned long long thresh, unsigned long long* dig_base, DTan T,
-                             const double* ct, const double* bt, double* wft_s)
+                             const unsigned char* cact, const int* bstep, DStep sp, int every,
+                             double* wfn_s, DRun* run, unsigned long long thresh,
+                             unsigned long long* dig_base, DTan T, const double* ct,
+                             const double* bt, double* wft_s)
 {
   int step;
   if (run_skip(run, step)) return;
@@ -430,26 +456,52 @@ __global__ void k_gather_fwd(DState S, const double4* mass_old, const int* inc_o
   if (wft) wft[i] = twn;
   const double4 Om = mass_old[i];
   if (sp.has_g) F = add3(F, mul3(mk3(sp.g[0], sp.g[1], sp.g[2]), Om.w));
+  // The interactions in CSR order, GATHER_FWD_CH entries at a time with their
+  // loads in flight together; the sums stay sequential.  A record with no
+  // force this step (a pair apart or excluded by its bond, a bond broken in an
+  // earlier step) is skipped: it would add zeros, and a sum that starts at +0
+  // is not changed by adding a zero.
+  constexpr int GATHER_FWD_CH = 4;
   const int e0 = inc_off[i], e1 = inc_off[i + 1];
-  for (int t = e0; t < e1; t++) {
-    const int ent = inc_ent[t];
-    const int rec = ent >> 1, side = ent & 1;
-    const double* q = rec < ncand ? cout + 9 * (size_t)rec : bout + 9 * (size_t)(rec - ncand);
-    if (side == 0) {
-      F = add3(F, mk3(q[0], q[1], q[2]));
-      M = add3(M, mk3(q[3], q[4], q[5]));
-    } else {
-      F = add3(F, mk3(-q[0], -q[1], -q[2]));
-      M = add3(M, mk3(q[6], q[7], q[8]));
+  for (int t0 = e0; t0 < e1; t0 += GATHER_FWD_CH) {
+    int ent[GATHER_FWD_CH];
+#pragma unroll
+    for (int k = 0; k < GATHER_FWD_CH; k++) ent[k] = t0 + k < e1 ? inc_ent[t0 + k] : -1;
+    bool on[GATHER_FWD_CH];
+#pragma unroll
+    for (int k = 0; k < GATHER_FWD_CH; k++) {
+      const int rec = ent[k] >> 1;
+      on[k] = ent[k] >= 0 && (rec < ncand ? cact[rec] != 0 : bstep[rec - ncand] >= step);
+    }
+    dv3 f[GATHER_FWD_CH], m[GATHER_FWD_CH];
+#pragma unroll
+    for (int k = 0; k < GATHER_FWD_CH; k++) {
+      if (!on[k]) continue;
+      const int rec = ent[k] >> 1, side = ent[k] & 1;
+      const double* q = rec < ncand ? cout + 9 * (size_t)rec : bout + 9 * (size_t)(rec - ncand);
+      f[k] = mk3(q[0], q[1], q[2]);
+      m[k] = side == 0 ? mk3(q[3], q[4], q[5]) : mk3(q[6], q[7], q[8]);
+    }
+#pragma unroll
+    for (int k = 0; k < GATHER_FWD_CH; k++) {
+      if (!on[k]) continue;
+      if ((ent[k] & 1) == 0) F = add3(F, f[k]);
+      else F = add3(F, mk3(-f[k].x, -f[k].y, -f[k].z));
+      M = add3(M, m[k]);
     }
     if (T.x) {
-      const double* u = rec < ncand ? ct + 9 * (size_t)rec : bt + 9 * (size_t)(rec - ncand);
-      if (side == 0) {
-        tF = add3(tF, mk3(u[0], u[1], u[2]));
-        tM = add3(tM, mk3(u[3], u[4], u[5]));
-      } else {
-        tF = sub3(tF, mk3(u[0], u[1], u[2]));
-        tM = add3(tM, mk3(u[6], u[7], u[8]));
+#pragma unroll
+      for (int k = 0; k < GATHER_FWD_CH; k++) {
+        if (!on[k]) continue;
+        const int rec = ent[k] >> 1, side = ent[k] & 1;
+        const double* u = rec < ncand ? ct + 9 * (size_t)rec : bt + 9 * (size_t)(rec - ncand);
+        if (side == 0) {
+          tF = add3(tF, mk3(u[0], u[1], u[2]));
+          tM = add3(tM, mk3(u[3], u[4], u[5]));
+        } else {
+          tF = sub3(tF, mk3(u[0], u[1], u[2]));
+          tM = add3(tM, mk3(u[6], u[7], u[8]));
+        }
       }
     }
   }
@@ -676,10 +728,7 @@ __global__ __launch_bounds__(ADJ_BOND_BS) void k_adj_bond(DState S, DBonds B,
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= B.nb) return;
   double* c = ab + NB_TAN * (size_t)b;
-  if (B.bstep[b] < step) {
-    for (int k = 0; k < NB_TAN; k++) c[k] = 0.0;
-    return;
-  }
+  if (B.bstep[b] < step) return;  // broken earlier: no record (k_adj_gather skips it)
   const int2 e = B.ij[b];
   const double4 P1 = S.x[e.x], P2 = S.x[e.y], Q1 = S.q[e.x], Q2 = S.q[e.y];
   const double4 D = B.D0[b];
@@ -714,24 +763,34 @@ __global__ __launch_bounds__(ADJ_BOND_BS) void k_adj_bond(DState S, DBonds B,
 // reverse mode (tfriction_law_vjp) on the slip branch the forward step took;
 // lh: adjoint of the history this step wrote; lh_prev: adjoint of the history
 // it read
+// act[c]: the record of pair c holds a (nonzero) Jacobian this step; the
+// gather reads only such records.
 __global__ void k_adj_fric(DState S, DCands C, const int* bstep, const double4* theta,
                            const double4* hin, int step, double dt, const double* lfm,
-                           const double4* lh, double* ac, double4* lh_prev)
+                           const double4* lh, double* ac, unsigned char* act, double4* lh_prev)
 {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= C.nc) return;
   double* o = ac + NF_TAN * (size_t)c;
   const int bb = C.bond[c];
+  if (bb >= 0 && bstep[bb] >= step) {
+    // excluded while bonded: zero Jacobian.  The history adjoint it would pass
+    // back is not written: the pair was excluded in the step before too (a
+    // bond that lives now lived then), so no step reads it.
+    act[c] = 0;
+    return;
+  }
   const int2 e = C.ij[c];
   const double4 P1 = S.x[e.x], P2 = S.x[e.y];
-  // excluded while bonded, or apart (no force, history cleared): zero Jacobian
+  // apart (no force, history cleared): zero Jacobian
   const double dxx = P1.x - P2.x, dyy = P1.y - P2.y, dzz = P1.z - P2.z;
   const double eq = P1.w + P2.w;
-  if ((bb >= 0 && bstep[bb] >= step) || !((dxx * dxx + dyy * dyy + dzz * dzz) < eq * eq)) {
-    for (int k = 0; k < NF_TAN; k++) o[k] = 0.0;
+  if (!((dxx * dxx + dyy * dyy + dzz * dzz) < eq * eq)) {
+    act[c] = 0;
     lh_prev[c] = make_double4(0.0, 0.0, 0.0, 0.0);
     return;
   }
+  act[c] = 1;
   const double4 V1 = S.v[e.x], V2 = S.v[e.y], W1 = S.w[e.x], W2 = S.w[e.y];
   const double4 th = theta[S.spec[e.x]];
   const double4 hh = hin[c];
@@ -762,37 +821,74 @@ __global__ void k_adj_fric(DState S, DCands C, const int* bstep, const double4* 
 
 // per particle: add the interactions' adjoints (CSR order) to the state
 // adjoint; parameter gradients of the interactions whose first particle this
-// is go to gth (kn, ks, -, mu)
+// is go to gth (kn, ks, -, mu).  Entries go ADJ_GATHER_CH at a time with their
+// loads in flight together; records without a Jacobian this step (pairs apart
+// or excluded, bonds broken earlier) are skipped: they would add zeros.
 __global__ void k_adj_gather(int n, const int* inc_off, const int* inc_ent, int ncand,
-                             const double* ac, const double* ab, double4* lam, double4* gth)
+                             const double* ac, const double* ab, const unsigned char* cact,
+                             const int* bstep, int step, double4* lam, double4* gth)
 {
+  constexpr int ADJ_GATHER_CH = 2;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double4* L = lam + 4 * (size_t)i;
   double4 lx = L[0], lv = L[1], lw = L[2], lq = L[3];
   double4 g = gth[i];
-  for (int t = inc_off[i]; t < inc_off[i + 1]; t++) {
-    const int ent = inc_ent[t];
-    const int rec = ent >> 1, side = ent & 1;
-    if (rec < ncand) {
-      const double* c = ac + NF_TAN * (size_t)rec;
-      const int xo = side ? 3 : 0, vo = side ? 9 : 6, wo = side ? 15 : 12;
-      lx.x += c[xo]; lx.y += c[xo + 1]; lx.z += c[xo + 2];
-      lv.x += c[vo]; lv.y += c[vo + 1]; lv.z += c[vo + 2];
-      lw.x += c[wo]; lw.y += c[wo + 1]; lw.z += c[wo + 2];
-      if (!side) {
-        g.x += c[21];
-        g.y += c[22];
-        g.w += c[23];
+  const int e0 = inc_off[i], e1 = inc_off[i + 1];
+  for (int t0 = e0; t0 < e1; t0 += ADJ_GATHER_CH) {
+    int ent[ADJ_GATHER_CH];
+#pragma unroll
+    for (int k = 0; k < ADJ_GATHER_CH; k++) ent[k] = t0 + k < e1 ? inc_ent[t0 + k] : -1;
+    bool on[ADJ_GATHER_CH];
+#pragma unroll
+    for (int k = 0; k < ADJ_GATHER_CH; k++) {
+      const int rec = ent[k] >> 1;
+      on[k] = ent[k] >= 0 && (rec < ncand ? cact[rec] != 0 : bstep[rec - ncand] >= step);
+    }
+    // per entry: x (3), v or q (4), w (3), parameter terms (3)
+    double a[ADJ_GATHER_CH][13];
+#pragma unroll
+    for (int k = 0; k < ADJ_GATHER_CH; k++) {
+      if (!on[k]) continue;
+      const int rec = ent[k] >> 1, side = ent[k] & 1;
+      if (rec < ncand) {
+        const double* c = ac + NF_TAN * (size_t)rec;
+        const int xo = side ? 3 : 0, vo = side ? 9 : 6, wo = side ? 15 : 12;
+        a[k][0] = c[xo]; a[k][1] = c[xo + 1]; a[k][2] = c[xo + 2];
+        a[k][3] = c[vo]; a[k][4] = c[vo + 1]; a[k][5] = c[vo + 2]; a[k][6] = 0.0;
+        a[k][7] = c[wo]; a[k][8] = c[wo + 1]; a[k][9] = c[wo + 2];
+        if (!side) {
+          a[k][10] = c[21]; a[k][11] = c[22]; a[k][12] = c[23];
+        }
+      } else {
+        const double* c = ab + NB_TAN * (size_t)(rec - ncand);
+        const int xo = side ? 3 : 0, qo = side ? 10 : 6;
+        a[k][0] = c[xo]; a[k][1] = c[xo + 1]; a[k][2] = c[xo + 2];
+        a[k][3] = c[qo]; a[k][4] = c[qo + 1]; a[k][5] = c[qo + 2]; a[k][6] = c[qo + 3];
+        if (!side) {
+          a[k][10] = c[14]; a[k][11] = c[15];
+        }
       }
-    } else {
-      const double* c = ab + NB_TAN * (size_t)(rec - ncand);
-      const int xo = side ? 3 : 0, qo = side ? 10 : 6;
-      lx.x += c[xo]; lx.y += c[xo + 1]; lx.z += c[xo + 2];
-      lq.x += c[qo]; lq.y += c[qo + 1]; lq.z += c[qo + 2]; lq.w += c[qo + 3];
-      if (!side) {
-        g.x += c[14];
-        g.y += c[15];
+    }
+#pragma unroll
+    for (int k = 0; k < ADJ_GATHER_CH; k++) {
+      if (!on[k]) continue;
+      const int rec = ent[k] >> 1, side = ent[k] & 1;
+      lx.x += a[k][0]; lx.y += a[k][1]; lx.z += a[k][2];
+      if (rec < ncand) {
+        lv.x += a[k][3]; lv.y += a[k][4]; lv.z += a[k][5];
+        lw.x += a[k][7]; lw.y += a[k][8]; lw.z += a[k][9];
+        if (!side) {
+          g.x += a[k][10];
+          g.y += a[k][11];
+          g.w += a[k][12];
+        }
+      } else {
+        lq.x += a[k][3]; lq.y += a[k][4]; lq.z += a[k][5]; lq.w += a[k][6];
+        if (!side) {
+          g.x += a[k][10];
+          g.y += a[k][11];
+        }
       }
     }
   }
@@ -898,10 +994,14 @@ struct esp_diff {
   bool use_graph = true;
   int graph_steps = 16;
   DevBuf<double> bout, cout, wfn, stress_dev;
+  DevBuf<unsigned char> cact;  // per pair of the epoch: its record holds this step's force
   DevBuf<double4> theta;
   std::vector<double> h_theta, h_target, h_stress;
   DevBuf<unsigned long long> md;
   DevBuf<unsigned long long> digest;  // per step: branch digest of the last forward run
+  // branch digests are computed only when asked for (esp_diff_track_branches):
+  // every wave of the contact and gather kernels adds to one counter per step
+  bool track_digest = false;
   // tangent-linear run (forward with a parameter direction)
   bool tan_on = false;
   DevBuf<double4> tx, tv, tw, tq, TH[2], dtheta;
@@ -1047,6 +1147,10 @@ void rebuild(esp_diff* d, int step)
   dsync(d);
   d->H[cur].swap(hn);
   d->H[cur ^ 1].reserve(c1);
+  // both buffers equal: a pair flagged "zero in both buffers" writes neither
+  if (nc > 0)
+    DCHK(hipMemcpyAsync(d->H[cur ^ 1].p, d->H[cur].p, sizeof(double4) * nc,
+                        hipMemcpyDeviceToDevice, d->stream));
   if (d->tan_on) {
     d->TH[cur].swap(tn);
     d->TH[cur ^ 1].reserve(c1);
@@ -1055,6 +1159,7 @@ void rebuild(esp_diff* d, int step)
   d->rebuild_steps.push_back(step);
   d->bout.reserve((size_t)std::max(d->nb, 1) * 9);
   d->cout.reserve(c1 * 9);
+  d->cact.reserve(c1);
   if (d->tan_on) d->bt.reserve((size_t)std::max(d->nb, 1) * 9);
 }
 
@@ -1088,7 +1193,10 @@ void restore_checkpoint(esp_diff* d, const Checkpoint& C)
   const int nc = d->epochs[C.epoch]->nc;
   d->H[0].reserve(std::max(nc, 1));
   d->H[1].reserve(std::max(nc, 1));
-  d2d(d, d->H[C.step & 1].p, C.hist.p, nc);
+  // the history the checkpoint's step reads, in both buffers (a pair flagged
+  // "zero in both buffers" writes neither)
+  d2d(d, d->H[0].p, C.hist.p, nc);
+  d2d(d, d->H[1].p, C.hist.p, nc);
   d2d(d, d->bstep.p, C.bstep.p, d->nb);
 }
 
@@ -1148,8 +1256,8 @@ void fwd_step(esp_diff* d, const Epoch& E, bool record, bool halting)
                        d->theta.p, R, d->bout.p);
   if (E.nc > 0)
     hipLaunchKernelGGL(k_fric_fwd, dim3(nblocks(E.nc, bs)), dim3(bs), 0, d->stream, S, C, d->bstep.p,
-                       d->theta.p, d->H[0].p, d->H[1].p, R, d->cfg.dt, d->cout.p,
-                       record ? d->digest.p : nullptr);
+                       d->theta.p, d->H[0].p, d->H[1].p, R, d->cfg.dt, d->cout.p, d->cact.p,
+                       record && d->track_digest ? d->digest.p : nullptr);
   unsigned long long thresh = ~0ull;
   if (halting) {
     const double a = 0.5 * d->cfg.verlet_dist;
@@ -1159,8 +1267,10 @@ void fwd_step(esp_diff* d, const Epoch& E, bool record, bool halting)
   DStep sp = d->step_params(0);
   sp.wo = d->wo.p;
   hipLaunchKernelGGL(k_gather_fwd, dim3(nblocks(d->n, bs)), dim3(bs), 0, d->stream, S, d->o.p,
-                     E.inc_off.p, E.inc_ent.p, E.nc, d->cout.p, d->bout.p, sp, d->every, d->wfn.p,
-                     R, thresh, record ? d->digest.p : nullptr, T, d->ct.p, d->bt.p,
+                     E.inc_off.p, E.inc_ent.p, E.nc, d->cout.p, d->bout.p, d->cact.p, d->bstep.p,
+                     sp, d->every, d->wfn.p,
+                     R, thresh, record && d->track_digest ? d->digest.p : nullptr, T, d->ct.p,
+                     d->bt.p,
                      d->tan_on ? d->wft.p : nullptr);
   if (record) {
     hipLaunchKernelGGL(k_spec_sum, dim3(d->nspec), dim3(256), 0, d->stream, R, d->every, d->wfn.p,
@@ -1259,6 +1369,7 @@ void forward(esp_diff* d, const double* theta, const double* target, double* str
       dt4[s] = make_double4(dtheta[4 * s], dtheta[4 * s + 1], dtheta[4 * s + 2], dtheta[4 * s + 3]);
     DCHK(hipMemcpyAsync(d->dtheta.p, dt4.data(), sizeof(double4) * d->nspec, hipMemcpyHostToDevice,
                         d->stream));
+    dsync(d);  // the copy reads dt4, which ends with this block
   }
   upload_initial(d);
   // wall origins of every step: moved by their step before every step (the
@@ -1278,6 +1389,7 @@ void forward(esp_diff* d, const double* theta, const double* target, double* str
     d->wo.reserve(wo.size());
     DCHK(hipMemcpyAsync(d->wo.p, wo.data(), sizeof(double) * wo.size(), hipMemcpyHostToDevice,
                         d->stream));
+    dsync(d);  // the copy reads wo, which ends with this block
   }
   d->run.reserve(1);
   d->wfn.reserve(std::max(d->n, 1));
@@ -1371,6 +1483,7 @@ void backward(esp_diff* d, double* grad)
                       d->stream));
   DevBuf<double4> lam, gth, lh, lh2;
   DevBuf<double> lfm, ab, ac;
+  DevBuf<unsigned char> aact;  // per pair: its record holds a Jacobian this step
   lam.reserve(4 * c);
   gth.reserve(c);
   lfm.reserve(6 * c);
@@ -1407,7 +1520,9 @@ void backward(esp_diff* d, double* grad)
                              E.src.p, d->H[step & 1].p, hn.p);
         d->H[step & 1].swap(hn);
         d->H[(step + 1) & 1].reserve(std::max(E.nc, 1));
+        d2d(d, d->H[(step + 1) & 1].p, d->H[step & 1].p, E.nc);
         d->cout.reserve((size_t)std::max(E.nc, 1) * 9);
+        d->cact.reserve((size_t)std::max(E.nc, 1));
         cur = e;
       }
       const int t = step - s0;
@@ -1445,6 +1560,7 @@ void backward(esp_diff* d, double* grad)
         lh_epoch = e;
         lh2.reserve(std::max(E.nc, 1));
         ac.reserve((size_t)std::max(E.nc, 1) * NF_TAN);
+        aact.reserve((size_t)std::max(E.nc, 1));
       }
       const int t = step - s0;
       double4* T0 = tape.p + (size_t)t * 4 * c;
@@ -1462,9 +1578,10 @@ void backward(esp_diff* d, double* grad)
       if (E.nc > 0)
         hipLaunchKernelGGL(k_adj_fric, dim3(nblocks(E.nc, bs)), dim3(bs), 0, d->stream, Sn, Cd,
                            d->bstep.p, d->theta.p, htape[t]->p, step, d->cfg.dt, lfm.p, lh.p, ac.p,
-                           lh2.p);
+                           aact.p, lh2.p);
       hipLaunchKernelGGL(k_adj_gather, dim3(nblocks(n, bs)), dim3(bs), 0, d->stream, n,
-                         E.inc_off.p, E.inc_ent.p, E.nc, ac.p, ab.p, lam.p, gth.p);
+                         E.inc_off.p, E.inc_ent.p, E.nc, ac.p, ab.p, aact.p, d->bstep.p, step,
+                         lam.p, gth.p);
       DCHK(hipGetLastError());
       lh.swap(lh2);
     }
@@ -1823,9 +1940,19 @@ int esp_diff_get_rebuild_steps(esp_diff* d, int* steps)
   DAPI_END
 }
 
+int esp_diff_track_branches(esp_diff* d, int enable)
+{
+  DAPI_BEGIN
+  d->track_digest = enable != 0;
+  return ESP_OK;
+  DAPI_END
+}
+
 int esp_diff_get_branch_digest(esp_diff* d, unsigned long long* out)
 {
   DAPI_BEGIN
+  if (!d->track_digest)
+    throw DiffError(ESP_ERR_STATE, "branch digests are not tracked (esp_diff_track_branches)");
   if (!d->have_forward) throw DiffError(ESP_ERR_STATE, "no forward run");
   if (out) {
     DCHK(hipMemcpyAsync(out, d->digest.p + 1, sizeof(unsigned long long) * d->nsteps,

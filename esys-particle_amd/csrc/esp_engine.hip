@@ -587,6 +587,7 @@ void upload(esp_engine* e)
       I0[i] = make_double4(e->h_init[3 * i], e->h_init[3 * i + 1], e->h_init[3 * i + 2], 0.0);
     e->init.reserve(cap);
     h2d(e, e->init.p, I0.data(), n);
+    sync(e);  // the copy reads I0, which ends with this block
   }
   h2d(e, e->flags.p, e->h_flags.data(), n);
   h2d(e, e->id.p, e->h_id.data(), n);
@@ -606,6 +607,7 @@ void upload(esp_engine* e)
     }
     e->inert.reserve(cap);
     h2d(e, e->inert.p, I.data(), n);
+    sync(e);  // the copy reads I, which ends with this block
   }
 
   int total_bonds = 0;
@@ -681,6 +683,7 @@ void upload(esp_engine* e)
       h2d(e, G.ij.p, ij.data(), np);
       h2d(e, G.key.p, key.data(), np);
       if (G.kind == 3) h2d(e, G.ff.p, ff.data(), np);
+      sync(e);  // the copies read ij, key and ff, which end with this block
     }
     G.n = np;
     rebuild_presence(e, G);

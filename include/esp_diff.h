@@ -107,12 +107,18 @@ int esp_diff_get_rebuild_steps(esp_diff* d, int* steps);
  * adjoint (grad . dtheta); no reference counterpart. */
 int esp_diff_forward_tangent(esp_diff* d, const double* theta, const double* dtheta,
                              const double* target, double* stress, double* loss, double* dloss);
+/* Compute (1) or skip (0, the default) the branch digests of later forward
+ * runs.  Every wave of the contact and gather kernels adds its hash to one
+ * counter per step, so tracking costs forward time.  No reference
+ * counterpart: it serves the finite-difference checks of the adjoint. */
+int esp_diff_track_branches(esp_diff* d, int enable);
 /* Branch digest of each step of the last forward run (steps 1..nsteps into
  * out[0..nsteps-1]): a hash sum of every contact's touching and slipping state
  * and every particle's wall contacts.  Runs whose digests and bond breaks are
  * equal took the same branches, so their loss is differentiable between them.
- * Returns nsteps (out NULL: count only).  No reference counterpart: it serves
- * the finite-difference checks of the adjoint. */
+ * Returns nsteps (out NULL: count only); ESP_ERR_STATE unless digests are
+ * tracked (esp_diff_track_branches).  No reference counterpart: it serves the
+ * finite-difference checks of the adjoint. */
 int esp_diff_get_branch_digest(esp_diff* d, unsigned long long* out);
 /* Device time (ms) of the last forward and backward runs. */
 int esp_diff_get_times(esp_diff* d, double* forward_ms, double* backward_ms);

@@ -84,6 +84,7 @@ def _perturbed(m, cls, theta, k, rel, target, ref_digest, ref_breaks, cal):
 
 def test_gradient_matches_oracle_finite_differences(pkg, orc, cal, spec2k):
     m = model_of(cal, [spec2k])
+    m.track_branches()
     true = THETA0 * np.array([1.08, 0.93, 1.0, 0.85])
     target, _ = m.forward(true, np.zeros((1, m.nsamples)))
     _, loss = m.forward(THETA0, target)
