@@ -271,8 +271,10 @@ def calibration_bench(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20000)
-    ap.add_argument("--warmup", type=int, default=500)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default: 20000; calibration: 1000, the sweep's run length)")
+    ap.add_argument("--warmup", type=int, default=None,
+                    help="untimed steps before them (default: 500; calibration: 20)")
     ap.add_argument("--particles", type=int, default=100000)
     ap.add_argument("--batch", type=int, default=128, help="steps queued per host sync")
     ap.add_argument("--condition", type=int, default=-1,
@@ -303,6 +305,11 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="calibration: default launch / search settings")
     args = ap.parse_args()
+    calib = args.workload == "calibration"
+    if args.steps is None:
+        args.steps = 1000 if calib else 20000
+    if args.warmup is None:
+        args.warmup = 20 if calib else 500
     if args.gpus != 1:
         print("bench.py measures one GPU: the engine holds one subdomain on one device "
               "(runs over several GPUs are not supported)", file=sys.stderr)
