@@ -1002,6 +1002,10 @@ struct esp_diff {
   // branch digests are computed only when asked for (esp_diff_track_branches):
   // every wave of the contact and gather kernels adds to one counter per step
   bool track_digest = false;
+  // the state and break steps at the end of the last forward run: the backward
+  // sweep recomputes earlier states in the same arrays and puts these back
+  DevBuf<double4> fx, fv, fw, fq;
+  DevBuf<int> fbstep;
   // tangent-linear run (forward with a parameter direction)
   bool tan_on = false;
   DevBuf<double4> tx, tv, tw, tq, TH[2], dtheta;
@@ -1421,6 +1425,14 @@ void forward(esp_diff* d, const double* theta, const double* target, double* str
     }
   }
   drop_graph(d);
+  {
+    const size_t c = std::max(d->n, 1);
+    d->fx.reserve(c); d->fv.reserve(c); d->fw.reserve(c); d->fq.reserve(c);
+    d->fbstep.reserve(std::max(d->nb, 1));
+    d2d(d, d->fx.p, d->x.p, d->n); d2d(d, d->fv.p, d->v.p, d->n);
+    d2d(d, d->fw.p, d->w.p, d->n); d2d(d, d->fq.p, d->q.p, d->n);
+    d2d(d, d->fbstep.p, d->bstep.p, d->nb);
+  }
   d->h_stress.assign((size_t)ns * d->nspec, 0.0);
   DCHK(hipMemcpyAsync(d->h_stress.data(), d->stress_dev.p, sizeof(double) * ns * d->nspec,
                       hipMemcpyDeviceToHost, d->stream));
@@ -1587,6 +1599,10 @@ void backward(esp_diff* d, double* grad)
     }
     dsync(d);
   }
+  // the forward run's final state and break steps, for later queries
+  d2d(d, d->x.p, d->fx.p, n); d2d(d, d->v.p, d->fv.p, n);
+  d2d(d, d->w.p, d->fw.p, n); d2d(d, d->q.p, d->fq.p, n);
+  d2d(d, d->bstep.p, d->fbstep.p, d->nb);
   DevBuf<double4> gs;
   gs.reserve(std::max(d->nspec, 1));
   hipLaunchKernelGGL(k_spec_sum4, dim3(d->nspec), dim3(256), 0, d->stream, gth.p, d->spec_start.p,
