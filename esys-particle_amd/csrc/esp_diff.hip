@@ -456,51 +456,59 @@ __global__ void k_gather_fwd(DState S, const double4* mass_old, const int* inc_o
   if (wft) wft[i] = twn;
   const double4 Om = mass_old[i];
   if (sp.has_g) F = add3(F, mul3(mk3(sp.g[0], sp.g[1], sp.g[2]), Om.w));
-  // The interactions in CSR order, GATHER_FWD_CH entries at a time with their
-  // loads in flight together; the sums stay sequential.  A record with no
-  // force this step (a pair apart or excluded by its bond, a bond broken in an
-  // earlier step) is skipped: it would add zeros, and a sum that starts at +0
-  // is not changed by adding a zero.
-  constexpr int GATHER_FWD_CH = 4;
+  // The interactions in CSR order.  Up to GATHER_FWD_RUN entries of the run
+  // load their record references together, then their activity together, then
+  // the records of each chunk of GATHER_FWD_CH entries that holds an active one
+  // (bonds, which are nearly all active, sit at the end of a particle's run);
+  // the sums stay sequential.  A record with no force this step (a pair apart
+  // or excluded by its bond, a bond broken in an earlier step) is skipped: it
+  // would add zeros, and a sum that starts at +0 is not changed by a zero.
+  constexpr int GATHER_FWD_RUN = 8, GATHER_FWD_CH = 4;
   const int e0 = inc_off[i], e1 = inc_off[i + 1];
-  for (int t0 = e0; t0 < e1; t0 += GATHER_FWD_CH) {
-    int ent[GATHER_FWD_CH];
+  for (int r0 = e0; r0 < e1; r0 += GATHER_FWD_RUN) {
+    int ent[GATHER_FWD_RUN];
 #pragma unroll
-    for (int k = 0; k < GATHER_FWD_CH; k++) ent[k] = t0 + k < e1 ? inc_ent[t0 + k] : -1;
-    bool on[GATHER_FWD_CH];
+    for (int k = 0; k < GATHER_FWD_RUN; k++) ent[k] = r0 + k < e1 ? inc_ent[r0 + k] : -1;
+    unsigned act = 0;
 #pragma unroll
-    for (int k = 0; k < GATHER_FWD_CH; k++) {
+    for (int k = 0; k < GATHER_FWD_RUN; k++) {
       const int rec = ent[k] >> 1;
-      on[k] = ent[k] >= 0 && (rec < ncand ? cact[rec] != 0 : bstep[rec - ncand] >= step);
-    }
-    dv3 f[GATHER_FWD_CH], m[GATHER_FWD_CH];
-#pragma unroll
-    for (int k = 0; k < GATHER_FWD_CH; k++) {
-      if (!on[k]) continue;
-      const int rec = ent[k] >> 1, side = ent[k] & 1;
-      const double* q = rec < ncand ? cout + 9 * (size_t)rec : bout + 9 * (size_t)(rec - ncand);
-      f[k] = mk3(q[0], q[1], q[2]);
-      m[k] = side == 0 ? mk3(q[3], q[4], q[5]) : mk3(q[6], q[7], q[8]);
+      const bool on =
+          ent[k] >= 0 && (rec < ncand ? cact[rec] != 0 : bstep[rec - ncand] >= step);
+      act |= (on ? 1u : 0u) << k;
     }
 #pragma unroll
-    for (int k = 0; k < GATHER_FWD_CH; k++) {
-      if (!on[k]) continue;
-      if ((ent[k] & 1) == 0) F = add3(F, f[k]);
-      else F = add3(F, mk3(-f[k].x, -f[k].y, -f[k].z));
-      M = add3(M, m[k]);
-    }
-    if (T.x) {
+    for (int c0 = 0; c0 < GATHER_FWD_RUN; c0 += GATHER_FWD_CH) {
+      if (!((act >> c0) & ((1u << GATHER_FWD_CH) - 1u))) continue;
+      dv3 f[GATHER_FWD_CH], m[GATHER_FWD_CH];
 #pragma unroll
       for (int k = 0; k < GATHER_FWD_CH; k++) {
-        if (!on[k]) continue;
-        const int rec = ent[k] >> 1, side = ent[k] & 1;
-        const double* u = rec < ncand ? ct + 9 * (size_t)rec : bt + 9 * (size_t)(rec - ncand);
-        if (side == 0) {
-          tF = add3(tF, mk3(u[0], u[1], u[2]));
-          tM = add3(tM, mk3(u[3], u[4], u[5]));
-        } else {
-          tF = sub3(tF, mk3(u[0], u[1], u[2]));
-          tM = add3(tM, mk3(u[6], u[7], u[8]));
+        if (!((act >> (c0 + k)) & 1u)) continue;
+        const int rec = ent[c0 + k] >> 1, side = ent[c0 + k] & 1;
+        const double* q = rec < ncand ? cout + 9 * (size_t)rec : bout + 9 * (size_t)(rec - ncand);
+        f[k] = mk3(q[0], q[1], q[2]);
+        m[k] = side == 0 ? mk3(q[3], q[4], q[5]) : mk3(q[6], q[7], q[8]);
+      }
+#pragma unroll
+      for (int k = 0; k < GATHER_FWD_CH; k++) {
+        if (!((act >> (c0 + k)) & 1u)) continue;
+        if ((ent[c0 + k] & 1) == 0) F = add3(F, f[k]);
+        else F = add3(F, mk3(-f[k].x, -f[k].y, -f[k].z));
+        M = add3(M, m[k]);
+      }
+      if (T.x) {
+#pragma unroll
+        for (int k = 0; k < GATHER_FWD_CH; k++) {
+          if (!((act >> (c0 + k)) & 1u)) continue;
+          const int rec = ent[c0 + k] >> 1, side = ent[c0 + k] & 1;
+          const double* u = rec < ncand ? ct + 9 * (size_t)rec : bt + 9 * (size_t)(rec - ncand);
+          if (side == 0) {
+            tF = add3(tF, mk3(u[0], u[1], u[2]));
+            tM = add3(tM, mk3(u[3], u[4], u[5]));
+          } else {
+            tF = sub3(tF, mk3(u[0], u[1], u[2]));
+            tM = add3(tM, mk3(u[6], u[7], u[8]));
+          }
         }
       }
     }
@@ -828,66 +836,72 @@ __global__ void k_adj_gather(int n, const int* inc_off, const int* inc_ent, int 
                              const double* ac, const double* ab, const unsigned char* cact,
                              const int* bstep, int step, double4* lam, double4* gth)
 {
-  constexpr int ADJ_GATHER_CH = 2;
+  constexpr int ADJ_GATHER_RUN = 12, ADJ_GATHER_CH = 2;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double4* L = lam + 4 * (size_t)i;
   double4 lx = L[0], lv = L[1], lw = L[2], lq = L[3];
   double4 g = gth[i];
   const int e0 = inc_off[i], e1 = inc_off[i + 1];
-  for (int t0 = e0; t0 < e1; t0 += ADJ_GATHER_CH) {
-    int ent[ADJ_GATHER_CH];
+  for (int r0 = e0; r0 < e1; r0 += ADJ_GATHER_RUN) {
+    int ent[ADJ_GATHER_RUN];
 #pragma unroll
-    for (int k = 0; k < ADJ_GATHER_CH; k++) ent[k] = t0 + k < e1 ? inc_ent[t0 + k] : -1;
-    bool on[ADJ_GATHER_CH];
+    for (int k = 0; k < ADJ_GATHER_RUN; k++) ent[k] = r0 + k < e1 ? inc_ent[r0 + k] : -1;
+    unsigned act = 0;
 #pragma unroll
-    for (int k = 0; k < ADJ_GATHER_CH; k++) {
+    for (int k = 0; k < ADJ_GATHER_RUN; k++) {
       const int rec = ent[k] >> 1;
-      on[k] = ent[k] >= 0 && (rec < ncand ? cact[rec] != 0 : bstep[rec - ncand] >= step);
+      const bool on =
+          ent[k] >= 0 && (rec < ncand ? cact[rec] != 0 : bstep[rec - ncand] >= step);
+      act |= (on ? 1u : 0u) << k;
     }
-    // per entry: x (3), v or q (4), w (3), parameter terms (3)
-    double a[ADJ_GATHER_CH][13];
 #pragma unroll
-    for (int k = 0; k < ADJ_GATHER_CH; k++) {
-      if (!on[k]) continue;
-      const int rec = ent[k] >> 1, side = ent[k] & 1;
-      if (rec < ncand) {
-        const double* c = ac + NF_TAN * (size_t)rec;
-        const int xo = side ? 3 : 0, vo = side ? 9 : 6, wo = side ? 15 : 12;
-        a[k][0] = c[xo]; a[k][1] = c[xo + 1]; a[k][2] = c[xo + 2];
-        a[k][3] = c[vo]; a[k][4] = c[vo + 1]; a[k][5] = c[vo + 2]; a[k][6] = 0.0;
-        a[k][7] = c[wo]; a[k][8] = c[wo + 1]; a[k][9] = c[wo + 2];
-        if (!side) {
-          a[k][10] = c[21]; a[k][11] = c[22]; a[k][12] = c[23];
-        }
-      } else {
-        const double* c = ab + NB_TAN * (size_t)(rec - ncand);
-        const int xo = side ? 3 : 0, qo = side ? 10 : 6;
-        a[k][0] = c[xo]; a[k][1] = c[xo + 1]; a[k][2] = c[xo + 2];
-        a[k][3] = c[qo]; a[k][4] = c[qo + 1]; a[k][5] = c[qo + 2]; a[k][6] = c[qo + 3];
-        if (!side) {
-          a[k][10] = c[14]; a[k][11] = c[15];
+    for (int c0 = 0; c0 < ADJ_GATHER_RUN; c0 += ADJ_GATHER_CH) {
+      if (!((act >> c0) & ((1u << ADJ_GATHER_CH) - 1u))) continue;
+      // per entry: x (3), v or q (4), w (3), parameter terms (3)
+      double a[ADJ_GATHER_CH][13];
+#pragma unroll
+      for (int k = 0; k < ADJ_GATHER_CH; k++) {
+        if (!((act >> (c0 + k)) & 1u)) continue;
+        const int rec = ent[c0 + k] >> 1, side = ent[c0 + k] & 1;
+        if (rec < ncand) {
+          const double* c = ac + NF_TAN * (size_t)rec;
+          const int xo = side ? 3 : 0, vo = side ? 9 : 6, wo = side ? 15 : 12;
+          a[k][0] = c[xo]; a[k][1] = c[xo + 1]; a[k][2] = c[xo + 2];
+          a[k][3] = c[vo]; a[k][4] = c[vo + 1]; a[k][5] = c[vo + 2]; a[k][6] = 0.0;
+          a[k][7] = c[wo]; a[k][8] = c[wo + 1]; a[k][9] = c[wo + 2];
+          if (!side) {
+            a[k][10] = c[21]; a[k][11] = c[22]; a[k][12] = c[23];
+          }
+        } else {
+          const double* c = ab + NB_TAN * (size_t)(rec - ncand);
+          const int xo = side ? 3 : 0, qo = side ? 10 : 6;
+          a[k][0] = c[xo]; a[k][1] = c[xo + 1]; a[k][2] = c[xo + 2];
+          a[k][3] = c[qo]; a[k][4] = c[qo + 1]; a[k][5] = c[qo + 2]; a[k][6] = c[qo + 3];
+          if (!side) {
+            a[k][10] = c[14]; a[k][11] = c[15];
+          }
         }
       }
-    }
 #pragma unroll
-    for (int k = 0; k < ADJ_GATHER_CH; k++) {
-      if (!on[k]) continue;
-      const int rec = ent[k] >> 1, side = ent[k] & 1;
-      lx.x += a[k][0]; lx.y += a[k][1]; lx.z += a[k][2];
-      if (rec < ncand) {
-        lv.x += a[k][3]; lv.y += a[k][4]; lv.z += a[k][5];
-        lw.x += a[k][7]; lw.y += a[k][8]; lw.z += a[k][9];
-        if (!side) {
-          g.x += a[k][10];
-          g.y += a[k][11];
-          g.w += a[k][12];
-        }
-      } else {
-        lq.x += a[k][3]; lq.y += a[k][4]; lq.z += a[k][5]; lq.w += a[k][6];
-        if (!side) {
-          g.x += a[k][10];
-          g.y += a[k][11];
+      for (int k = 0; k < ADJ_GATHER_CH; k++) {
+        if (!((act >> (c0 + k)) & 1u)) continue;
+        const int rec = ent[c0 + k] >> 1, side = ent[c0 + k] & 1;
+        lx.x += a[k][0]; lx.y += a[k][1]; lx.z += a[k][2];
+        if (rec < ncand) {
+          lv.x += a[k][3]; lv.y += a[k][4]; lv.z += a[k][5];
+          lw.x += a[k][7]; lw.y += a[k][8]; lw.z += a[k][9];
+          if (!side) {
+            g.x += a[k][10];
+            g.y += a[k][11];
+            g.w += a[k][12];
+          }
+        } else {
+          lq.x += a[k][3]; lq.y += a[k][4]; lq.z += a[k][5]; lq.w += a[k][6];
+          if (!side) {
+            g.x += a[k][10];
+            g.y += a[k][11];
+          }
         }
       }
     }
