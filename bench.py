@@ -38,12 +38,12 @@ B_ELASTIC = 8         # i, j
 
 # FP64 work per unit, from the SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 counters of the
 # default workload (64 lanes x (ADD + MUL + 2 FMA + TRANS) per dispatch over its units)
-FLOP_BOND = 1391            # bond law + break test (k_forces)
+FLOP_BOND = 1386            # bond law + break test (k_forces)
 FLOP_PARTICLE_GATHER = 189  # rotational particle: slot sums, walls, integration
 FP64_PEAK_GFLOPS = 78600.0
 # HBM bytes per launch measured with FETCH_SIZE / WRITE_SIZE (copy-probe scaled) for
 # the default workload, in profiles/step_kernels_bonded100k.md
-MEASURED_TRAFFIC = {("compression", 100000): {"forces": 63.6e6, "gather_integrate": 57.8e6}}
+MEASURED_TRAFFIC = {("compression", 100000): {"forces": 60.6e6, "gather_integrate": 57.8e6}}
 
 
 def _load(name, path, pkgdir=None):
