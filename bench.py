@@ -177,7 +177,8 @@ def calibration_bench(args):
     adjoint sweep; value = specimens / wall time of both (gradient evaluations
     per second).  Baseline: central finite differences on the plain engine,
     2 runs per parameter, each specimen alone, timed on a sample of specimens
-    across the size range and summed over the batch by interpolation in N."""
+    across the size range (the faster of two runs each) and summed over the
+    batch by interpolation in N."""
     pkg = load_pkg()
     spm = importlib.import_module("esys_particle_amd.specimens")
     cal = importlib.import_module("esys_particle_amd.calibration")
@@ -217,18 +218,25 @@ def calibration_bench(args):
     # plain engine, one specimen at a time, forward runs of the same length
     order = np.argsort(ns)
     pick = order[np.linspace(0, len(order) - 1, min(6, len(order))).astype(int)]
+    # each sample timed twice, the faster run kept (the baseline's runs are
+    # launch-bound and vary between runs; the faster one is the conservative
+    # baseline for the ratio)
     pts = []
     for s in pick:
-        mm = cal.DiffModel.from_specimens([specs[s]], steps, every)
-        sim = pkg.LsmMpi(stepsPerBatch=128)
-        cal.plain_setup(sim, mm, theta[s])
-        sim.runTimeSteps(1)
-        t0 = time.perf_counter()
-        sim.runTimeSteps(steps - 1)
-        el = time.perf_counter() - t0
-        pts.append((int(ns[s]), el * steps / max(steps - 1, 1)))
-        mm.close()
-        log("plain engine: %d particles, %d steps: %.3f s" % (pts[-1][0], steps, pts[-1][1]))
+        best = None
+        for _ in range(2):
+            mm = cal.DiffModel.from_specimens([specs[s]], steps, every)
+            sim = pkg.LsmMpi(stepsPerBatch=128)
+            cal.plain_setup(sim, mm, theta[s])
+            sim.runTimeSteps(1)
+            t0 = time.perf_counter()
+            sim.runTimeSteps(steps - 1)
+            el = time.perf_counter() - t0
+            mm.close()
+            best = el if best is None else min(best, el)
+        pts.append((int(ns[s]), best * steps / max(steps - 1, 1)))
+        log("plain engine: %d particles, %d steps: %.3f s (faster of 2)" % (pts[-1][0], steps,
+                                                                          pts[-1][1]))
     px, py = np.array([p[0] for p in pts], float), np.array([p[1] for p in pts], float)
     t_plain = float(np.interp(ns, px, py).sum())
     nparam = 4
