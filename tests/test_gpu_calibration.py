@@ -181,3 +181,30 @@ def test_autograd_function(cal, spec2k):
     loss.sum().backward()
     g = m.backward()
     np.testing.assert_array_equal(theta.grad.numpy(), g)
+
+
+def test_queries_after_adjoint_and_digest_opt_in(cal, spm):
+    """The adjoint sweep recomputes earlier states in the model's arrays; the
+    queries after it still see the forward run's final state and break steps.
+    Branch digests exist only when tracked, and tracking changes no result."""
+    spec = spm.dense_bonded_compression(n=1500, seed=9, coh_mean=2.0, coh_sd=0.2, wall_speed=20.0)
+    theta = np.array([[1e5, 3e4, 2.0, 0.3]])
+    m = model_of(cal, [spec], K=50)
+    target = np.zeros((1, m.nsamples))
+    st, loss = m.forward(theta, target)
+    brk, pos, q = m.bond_breaks(), m.particle_data("pos"), m.particle_data("quat")
+    print("case: queries after adjoint, %d breaks" % len(brk), flush=True)
+    assert len(brk) > 500
+    with pytest.raises(RuntimeError):
+        m.branch_digest()
+    g = m.backward()
+    np.testing.assert_array_equal(m.bond_breaks(), brk)
+    np.testing.assert_array_equal(m.particle_data("pos"), pos)
+    np.testing.assert_array_equal(m.particle_data("quat"), q)
+    m.track_branches()
+    st2, loss2 = m.forward(theta, target)
+    dig = m.branch_digest()
+    assert dig.shape == (STEPS,) and np.any(dig != 0)
+    np.testing.assert_array_equal(st2, st)
+    np.testing.assert_array_equal(loss2, loss)
+    np.testing.assert_array_equal(m.backward(), g)
