@@ -197,6 +197,16 @@ struct esp_engine {
   // list (insertion order; erasure keeps the order of the rest)
   std::vector<long long> h_order;
   long long next_order = 0;
+  // applyForceToWall's candidates in particle-list order: the slot of every
+  // creation order (-1: erased), rebuilt after the slots change, and the
+  // staging buffers the candidates are packed into in that order
+  DBuf<int> ord_slot;
+  bool ord_valid = false;
+  DBuf<int> wc_mark, wc_off;
+  DBuf<unsigned> wc_flg;
+  DBuf<double4> wc_pos, wc_ipos;
+  std::vector<unsigned> hw_flg;
+  std::vector<double4> hw_pos, hw_ipos;
   bool forced_search = false;  // a checkpoint forced the neighbour search after the last step
   std::vector<int> h_conn;
   std::unordered_map<int, int> id2slot;
@@ -532,6 +542,7 @@ void spatial_order(esp_engine* e)
   permute(e->h_ang, 3);
   permute(e->h_inert, 1);
   permute(e->h_order, 1);
+  e->ord_valid = false;
   e->id2slot.clear();
   for (int i = 0; i < n; i++) e->id2slot[e->h_id[i]] = i;
 }
@@ -932,6 +943,7 @@ void erase_outside(esp_engine* e)
   compact_host(e->h_ang, hkeep, 3);
   compact_host(e->h_inert, hkeep, 1);
   compact_host(e->h_order, hkeep, 1);
+  e->ord_valid = false;
   e->id2slot.clear();
   for (int i = 0; i < nk; i++) e->id2slot[e->h_id[i]] = i;
   e->n = nk;
@@ -2440,6 +2452,7 @@ int esp_add_particles(esp_engine* e, int n, const int* id, const int* tag, const
     e->h_ang.insert(e->h_ang.end(), {0.0, 0.0, 0.0});
     e->h_inert.push_back(0.0);
     e->h_order.push_back(e->next_order++);
+    e->ord_valid = false;
   }
   return ESP_OK;
   API_END
@@ -2779,35 +2792,42 @@ void apply_force_to_wall(esp_engine* e, int k, const double Fa[3])
   const WallGroup& G = e->wg[e->word[k]];
   std::vector<WallCand> cs;
   if (e->started && e->n > 0) {
-    const int n = e->n;
-    DBuf<int> cnt, slot;
-    DBuf<unsigned> flg;
-    DBuf<double4> pos, ipos;
-    cnt.reserve(1);
-    slot.reserve(n);
-    flg.reserve(n);
-    pos.reserve(n);
-    if (G.bonded) ipos.reserve(n);
-    zero(e, cnt.p, 1);
-    launch_wall_cands(e->stream, e->view(), k, G.bonded ? e->init.p : nullptr, n, cnt.p, slot.p,
-                      flg.p, pos.p, ipos.p);
+    // the candidates (the group's wall-membership bit) in particle-list
+    // (creation) order: marked by order on the device, packed at the scanned
+    // offsets, so the host receives them sorted
+    const int no = (int)e->next_order;
+    if (!e->ord_valid) {
+      std::vector<int> os((size_t)std::max(no, 1), -1);
+      for (int s = 0; s < e->n; s++) os[(size_t)e->h_order[s]] = s;
+      e->ord_slot.reserve((size_t)std::max(no, 1));
+      h2d(e, e->ord_slot.p, os.data(), (size_t)no);
+      sync(e);  // the copy reads os, which ends with this block
+      e->ord_valid = true;
+    }
+    e->wc_mark.reserve((size_t)no + 1);
+    e->wc_off.reserve((size_t)no + 1);
+    launch_wall_mark_ordered(e->stream, no, e->ord_slot.p, e->flags.p, k, e->wc_mark.p);
     HIPCHK(hipGetLastError());
-    int m = 0;
-    d2h(e, &m, cnt.p, 1);
-    sync(e);
-    std::vector<int> hs(m);
-    std::vector<unsigned> hf(m);
-    std::vector<double4> hp(m), hi(G.bonded ? m : 0);
-    d2h(e, hs.data(), slot.p, m);
-    d2h(e, hf.data(), flg.p, m);
-    d2h(e, hp.data(), pos.p, m);
-    if (G.bonded) d2h(e, hi.data(), ipos.p, m);
+    const int m = scan_total(e, e->wc_mark.p, e->wc_off.p, no);
+    const size_t c = (size_t)std::max(m, 1);
+    e->wc_flg.reserve(c);
+    e->wc_pos.reserve(c);
+    if (G.bonded) e->wc_ipos.reserve(c);
+    launch_wall_pack_ordered(e->stream, no, e->ord_slot.p, e->wc_mark.p, e->wc_off.p, e->view(),
+                             G.bonded ? e->init.p : nullptr, e->wc_flg.p, e->wc_pos.p,
+                             G.bonded ? e->wc_ipos.p : nullptr);
+    HIPCHK(hipGetLastError());
+    e->hw_flg.resize(c);
+    e->hw_pos.resize(c);
+    e->hw_ipos.resize(G.bonded ? c : 0);
+    d2h(e, e->hw_flg.data(), e->wc_flg.p, m);
+    d2h(e, e->hw_pos.data(), e->wc_pos.p, m);
+    if (G.bonded) d2h(e, e->hw_ipos.data(), e->wc_ipos.p, m);
     sync(e);
     cs.resize(m);
-    for (int c = 0; c < m; c++)
-      cs[c] = WallCand{e->h_order[hs[c]], hf[c], hp[c], G.bonded ? hi[c] : make_double4(0, 0, 0, 0)};
-    std::sort(cs.begin(), cs.end(),
-              [](const WallCand& a, const WallCand& b) { return a.order < b.order; });
+    for (int i = 0; i < m; i++)
+      cs[i] = WallCand{i, e->hw_flg[i], e->hw_pos[i],
+                       G.bonded ? e->hw_ipos[i] : make_double4(0, 0, 0, 0)};
   }
   servo_iterate(e, k, Fa, cs.data(), (int)cs.size());
 }

@@ -974,6 +974,28 @@ __global__ void k_wall_cands(PView p, int w, const double4* init, int cap, int* 
   if (init) ipos[k] = init[i];
 }
 
+__global__ void k_wall_mark_ordered(int no, const int* ord_slot, const unsigned* flags, int w,
+                                    int* mark)
+{
+  const int o = blockIdx.x * BS + threadIdx.x;
+  if (o >= no) return;
+  const int s = ord_slot[o];
+  mark[o] = (s >= 0 && ((flags[s] >> (PF_WALL_SHIFT + w)) & 1u)) ? 1 : 0;
+}
+
+__global__ void k_wall_pack_ordered(int no, const int* ord_slot, const int* mark, const int* off,
+                                    PView p, const double4* init, unsigned* flg, double4* pos,
+                                    double4* ipos)
+{
+  const int o = blockIdx.x * BS + threadIdx.x;
+  if (o >= no || !mark[o]) return;
+  const int s = ord_slot[o];
+  const int k = off[o];
+  flg[k] = p.flags[s];
+  pos[k] = p.pos[s];
+  if (init) ipos[k] = init[s];
+}
+
 __global__ void k_clear_flag_bit(PView p, UpdState* st)
 {
   const int i = blockIdx.x * BS + threadIdx.x;
@@ -1924,6 +1946,23 @@ void launch_wall_cands(hipStream_t s, PView p, int w, const double4* init, int c
   if (p.n <= 0) return;
   hipLaunchKernelGGL(k_wall_cands, dim3(nblk(p.n)), dim3(BS), 0, s, p, w, init, cap, cnt, slot,
                      flg, pos, ipos);
+}
+
+void launch_wall_mark_ordered(hipStream_t s, int no, const int* ord_slot, const unsigned* flags,
+                              int w, int* mark)
+{
+  if (no <= 0) return;
+  hipLaunchKernelGGL(k_wall_mark_ordered, dim3(nblk(no)), dim3(BS), 0, s, no, ord_slot, flags, w,
+                     mark);
+}
+
+void launch_wall_pack_ordered(hipStream_t s, int no, const int* ord_slot, const int* mark,
+                              const int* off, PView p, const double4* init, unsigned* flg,
+                              double4* pos, double4* ipos)
+{
+  if (no <= 0) return;
+  hipLaunchKernelGGL(k_wall_pack_ordered, dim3(nblk(no)), dim3(BS), 0, s, no, ord_slot, mark, off,
+                     p, init, flg, pos, ipos);
 }
 
 void launch_fill_int(hipStream_t s, int n, int* a, int v)
