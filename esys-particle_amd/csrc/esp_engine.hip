@@ -83,6 +83,34 @@ struct DBuf {
   }
 };
 
+// page-locked host memory (hipHostMalloc): device-to-host copies into it run
+// at full link speed instead of through the runtime's staging buffers
+template <class T>
+struct PinBuf {
+  T* p = nullptr;
+  size_t cap = 0;
+  PinBuf() = default;
+  PinBuf(const PinBuf&) = delete;
+  PinBuf& operator=(const PinBuf&) = delete;
+  ~PinBuf()
+  {
+    if (p) (void)hipHostFree(p);
+  }
+  // capacity for n elements; contents are not kept
+  void reserve(size_t n)
+  {
+    if (n <= cap && p) return;
+    if (p) {
+      HIPCHK(hipDeviceSynchronize());  // a queued copy may still write the old buffer
+      HIPCHK(hipHostFree(p));
+    }
+    p = nullptr;
+    size_t c = std::max<size_t>(n + n / 4, 64);
+    HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&p), c * sizeof(T), hipHostMallocDefault));
+    cap = c;
+  }
+};
+
 struct DynGroup {
   std::string name;
   int kind;  // 1 NRotElastic, 2 RotElastic, 3 RotFriction
@@ -203,10 +231,8 @@ struct esp_engine {
   DBuf<int> ord_slot;
   bool ord_valid = false;
   DBuf<int> wc_mark, wc_off;
-  DBuf<unsigned> wc_flg;
-  DBuf<double4> wc_pos, wc_ipos;
-  std::vector<unsigned> hw_flg;
-  std::vector<double4> hw_pos, hw_ipos;
+  DBuf<WallCand> wc_cand;
+  PinBuf<WallCand> hw_cand;
   bool forced_search = false;  // a checkpoint forced the neighbour search after the last step
   std::vector<int> h_conn;
   std::unordered_map<int, int> id2slot;
@@ -2772,12 +2798,6 @@ namespace {
 // candidates come from the device; the iteration runs here over them in the
 // reference's list order (particle creation order), with its sums from zero.
 // a wall candidate: creation rank, flags, position, initial position
-struct WallCand {
-  long long order;
-  unsigned flags;
-  double4 pos, init;
-};
-
 void servo_iterate(esp_engine* e, int k, const double Fa[3], const WallCand* cs, int m);
 
 struct CandRange {
@@ -2790,11 +2810,11 @@ struct CandRange {
 void apply_force_to_wall(esp_engine* e, int k, const double Fa[3])
 {
   const WallGroup& G = e->wg[e->word[k]];
-  std::vector<WallCand> cs;
+  int m = 0;
   if (e->started && e->n > 0) {
     // the candidates (the group's wall-membership bit) in particle-list
     // (creation) order: marked by order on the device, packed at the scanned
-    // offsets, so the host receives them sorted
+    // offsets, so the host receives them sorted, in one copy
     const int no = (int)e->next_order;
     if (!e->ord_valid) {
       std::vector<int> os((size_t)std::max(no, 1), -1);
@@ -2808,28 +2828,17 @@ void apply_force_to_wall(esp_engine* e, int k, const double Fa[3])
     e->wc_off.reserve((size_t)no + 1);
     launch_wall_mark_ordered(e->stream, no, e->ord_slot.p, e->flags.p, k, e->wc_mark.p);
     HIPCHK(hipGetLastError());
-    const int m = scan_total(e, e->wc_mark.p, e->wc_off.p, no);
+    m = scan_total(e, e->wc_mark.p, e->wc_off.p, no);
     const size_t c = (size_t)std::max(m, 1);
-    e->wc_flg.reserve(c);
-    e->wc_pos.reserve(c);
-    if (G.bonded) e->wc_ipos.reserve(c);
+    e->wc_cand.reserve(c);
+    e->hw_cand.reserve(c);
     launch_wall_pack_ordered(e->stream, no, e->ord_slot.p, e->wc_mark.p, e->wc_off.p, e->view(),
-                             G.bonded ? e->init.p : nullptr, e->wc_flg.p, e->wc_pos.p,
-                             G.bonded ? e->wc_ipos.p : nullptr);
+                             G.bonded ? e->init.p : nullptr, e->wc_cand.p);
     HIPCHK(hipGetLastError());
-    e->hw_flg.resize(c);
-    e->hw_pos.resize(c);
-    e->hw_ipos.resize(G.bonded ? c : 0);
-    d2h(e, e->hw_flg.data(), e->wc_flg.p, m);
-    d2h(e, e->hw_pos.data(), e->wc_pos.p, m);
-    if (G.bonded) d2h(e, e->hw_ipos.data(), e->wc_ipos.p, m);
+    d2h(e, e->hw_cand.p, e->wc_cand.p, m);
     sync(e);
-    cs.resize(m);
-    for (int i = 0; i < m; i++)
-      cs[i] = WallCand{i, e->hw_flg[i], e->hw_pos[i],
-                       G.bonded ? e->hw_ipos[i] : make_double4(0, 0, 0, 0)};
   }
-  servo_iterate(e, k, Fa, cs.data(), (int)cs.size());
+  servo_iterate(e, k, Fa, e->hw_cand.p, m);
 }
 
 // the servo's iteration over the candidates in list order

@@ -310,15 +310,20 @@ void launch_clear_flag_bit(hipStream_t s, PView p, UpdState* st);
 // positions; *cnt counts all, the first cap are stored
 void launch_wall_cands(hipStream_t s, PView p, int w, const double4* init, int cap, int* cnt,
                        int* slot, unsigned* flg, double4* pos, double4* ipos);
+// one candidate particle of a wall servo, as the host iterates it
+struct WallCand {
+  long long order;  // particle-list (creation) order
+  unsigned flags;
+  double4 pos, init;  // position and radius; initial position (bonded walls, else 0)
+};
 // the candidates of wall group w in particle-list order: mark[o] = 1 when the
 // particle of creation order o (slot ord_slot[o], -1 erased) has the group's
-// membership bit; pack writes the marked ones' flags, positions and (bonded
-// walls) initial positions at off[o] (the exclusive scan of mark)
+// membership bit; pack writes the marked ones at out[off[o]] (off: the
+// exclusive scan of mark); init null: no initial positions
 void launch_wall_mark_ordered(hipStream_t s, int no, const int* ord_slot, const unsigned* flags,
                               int w, int* mark);
 void launch_wall_pack_ordered(hipStream_t s, int no, const int* ord_slot, const int* mark,
-                              const int* off, PView p, const double4* init, unsigned* flg,
-                              double4* pos, double4* ipos);
+                              const int* off, PView p, const double4* init, WallCand* out);
 // a[0..n) = v (a kernel, not a memset: safe under stream capture)
 void launch_fill_int(hipStream_t s, int n, int* a, int v);
 // erasure of particles outside the padded grid (NeighborTable::build):

@@ -984,16 +984,16 @@ __global__ void k_wall_mark_ordered(int no, const int* ord_slot, const unsigned*
 }
 
 __global__ void k_wall_pack_ordered(int no, const int* ord_slot, const int* mark, const int* off,
-                                    PView p, const double4* init, unsigned* flg, double4* pos,
-                                    double4* ipos)
+                                    PView p, const double4* init, WallCand* out)
 {
   const int o = blockIdx.x * BS + threadIdx.x;
   if (o >= no || !mark[o]) return;
   const int s = ord_slot[o];
-  const int k = off[o];
-  flg[k] = p.flags[s];
-  pos[k] = p.pos[s];
-  if (init) ipos[k] = init[s];
+  WallCand& c = out[off[o]];
+  c.order = o;
+  c.flags = p.flags[s];
+  c.pos = p.pos[s];
+  c.init = init ? init[s] : make_double4(0.0, 0.0, 0.0, 0.0);
 }
 
 __global__ void k_clear_flag_bit(PView p, UpdState* st)
@@ -1957,12 +1957,11 @@ void launch_wall_mark_ordered(hipStream_t s, int no, const int* ord_slot, const 
 }
 
 void launch_wall_pack_ordered(hipStream_t s, int no, const int* ord_slot, const int* mark,
-                              const int* off, PView p, const double4* init, unsigned* flg,
-                              double4* pos, double4* ipos)
+                              const int* off, PView p, const double4* init, WallCand* out)
 {
   if (no <= 0) return;
   hipLaunchKernelGGL(k_wall_pack_ordered, dim3(nblk(no)), dim3(BS), 0, s, no, ord_slot, mark, off,
-                     p, init, flg, pos, ipos);
+                     p, init, out);
 }
 
 void launch_fill_int(hipStream_t s, int n, int* a, int v)
