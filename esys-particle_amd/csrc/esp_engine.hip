@@ -2852,18 +2852,31 @@ void servo_iterate(esp_engine* e, int k, const double Fa[3], const WallCand* cs,
     return (G.bonded || G.tagged) && ((c.flags >> (PF_BTAG_SHIFT + k)) & 1u) != 0;
   };
   const dv3 n = mk3(W.n[0], W.n[1], W.n[2]);
-  auto dist = [&](const Cand& c) {
-    return dot3(sub3(mk3(c.pos.x, c.pos.y, c.pos.z), mk3(W.o[0], W.o[1], W.o[2])), n);
-  };
   // CElasticWallInteraction: k pi r (3-D, EWallInteraction.hpp:26-36); the
   // bonded form k pi r with its own constant (BWallInteraction.hpp:20-31)
   auto keff = [&](const Cand& c) { return (e->do2d ? 1.0 : 3.141592654 * c.pos.w) * G.k; };
   auto kbond = [&](const Cand& c) { return G.k * (e->do2d ? 1.0 : 3.1415926536 * c.pos.w); };
-  auto elastic_force = [&](const Cand& c) {  // CElasticWallInteraction::getForce
+  // the iterations visit only the candidates their loops select, in list
+  // order, with what does not change between iterations computed once (the
+  // same operations as in the loops, so the sums are unchanged)
+  struct Elastic {
+    dv3 x;
+    double r, ke;
+    bool in;
+  };
+  struct Bonded {
+    dv3 rel;  // position - initial position
+    double kb;
+  };
+  auto dist = [&](const Elastic& c) { return dot3(sub3(c.x, mk3(W.o[0], W.o[1], W.o[2])), n); };
+  auto elastic_force = [&](const Elastic& c) {  // CElasticWallInteraction::getForce
     dv3 f = mk3(0.0, 0.0, 0.0);
     const double d = dist(c);
-    if (d < c.pos.w) f = mul3(n, keff(c) * (c.pos.w - d));
+    if (d < c.r) f = mul3(n, c.ke * (c.r - d));
     return mul3(f, -1.0);
+  };
+  auto elastic_of = [&](const Cand& c) {
+    return Elastic{mk3(c.pos.x, c.pos.y, c.pos.z), c.pos.w, keff(c), inner(c)};
   };
   const dv3 F = mk3(Fa[0], Fa[1], Fa[2]);
   const dv3 O_f = div3(F, norm3(F));
@@ -2873,17 +2886,23 @@ void servo_iterate(esp_engine* e, int k, const double Fa[3], const WallCand* cs,
     W.o[1] = W.o[1] + v.y;
     W.o[2] = W.o[2] + v.z;
   };
+  std::vector<Elastic> el;
+  el.reserve((size_t)m);
   if (!G.bonded) {
+    // EWallInteractionGroup::applyForce: the stiffness sum over all (tagged)
+    // candidates in contact, the force over the inner ones
+    for (const Cand& c : CandRange{cs, m})
+      if (!G.tagged || tagged(c)) el.push_back(elastic_of(c));
     int it = 0;
     double d, df, ef;
     do {
       double kk = 0.0;
-      for (const Cand& c : CandRange{cs, m})
-        if ((!G.tagged || tagged(c)) && dist(c) < c.pos.w) kk += keff(c);
+      for (const Elastic& c : el)
+        if (dist(c) < c.r) kk += c.ke;
       if (kk > 0) {
         dv3 Fl = mk3(0.0, 0.0, 0.0);
-        for (const Cand& c : CandRange{cs, m}) {
-          if ((G.tagged && !tagged(c)) || !inner(c)) continue;
+        for (const Elastic& c : el) {
+          if (!c.in) continue;
           const dv3 f = elastic_force(c);
           Fl = add3(Fl, mul3(O_f, dot3(f, O_f)));
         }
@@ -2899,25 +2918,35 @@ void servo_iterate(esp_engine* e, int k, const double Fa[3], const WallCand* cs,
     } while ((it < 50) && (ef > 1e-3));
     return;
   }
+  // BWallInteractionGroup::applyForce: bonded candidates, then the unbonded
+  // inner ones in contact, make the stiffness; the iterations sum the bonded
+  // inner forces, then the unbonded inner ones
+  std::vector<Bonded> bd;
+  bd.reserve((size_t)m);
   double K = 0.0;
   for (const Cand& c : CandRange{cs, m})
-    if (tagged(c) && inner(c)) K += kbond(c);
+    if (tagged(c) && inner(c)) {
+      const double kb = kbond(c);
+      K += kb;
+      bd.push_back(Bonded{sub3(mk3(c.pos.x, c.pos.y, c.pos.z), mk3(c.ix, c.iy, c.iz)), kb});
+    }
   for (const Cand& c : CandRange{cs, m})
-    if (!tagged(c) && inner(c) && dist(c) < c.pos.w) K += keff(c);
+    if (!tagged(c) && inner(c)) {
+      el.push_back(elastic_of(c));
+      if (dist(el.back()) < c.pos.w) K += el.back().ke;
+    }
   if (!(K > 0)) throw EspError(ESP_ERR_NUMERIC, "wall group '" + G.name + "' has no stiffness to servo");
   int it = 0;
   double d;
   do {
     const dv3 wdi = mk3(W.o[0] - W.old[0], W.o[1] - W.old[1], W.o[2] - W.old[2]);
     dv3 Fl = mk3(0.0, 0.0, 0.0);
-    for (const Cand& c : CandRange{cs, m}) {
-      if (!tagged(c) || !inner(c)) continue;
-      const dv3 D = sub3(sub3(mk3(c.pos.x, c.pos.y, c.pos.z), mk3(c.init.x, c.init.y, c.init.z)), wdi);
-      const dv3 f = mul3(D, kbond(c));
+    for (const Bonded& c : bd) {
+      const dv3 D = sub3(c.rel, wdi);
+      const dv3 f = mul3(D, c.kb);
       Fl = add3(Fl, mul3(O_f, dot3(f, O_f)));
     }
-    for (const Cand& c : CandRange{cs, m}) {
-      if (tagged(c) || !inner(c)) continue;
+    for (const Elastic& c : el) {
       const dv3 f = elastic_force(c);
       Fl = add3(Fl, mul3(O_f, dot3(f, O_f)));
     }
@@ -2972,6 +3001,7 @@ unsigned h_na_flg[ESP_NOALLOC_MAX_WALL_CANDIDATES];
 double4 h_na_pos[ESP_NOALLOC_MAX_WALL_CANDIDATES];
 double4 h_na_ipos[ESP_NOALLOC_MAX_WALL_CANDIDATES];
 WallCand h_na_cand[ESP_NOALLOC_MAX_WALL_CANDIDATES];
+int h_na_idx[ESP_NOALLOC_MAX_WALL_CANDIDATES];
 
 void apply_force_to_wall_noalloc(esp_engine* e, int k, const double Fa[3])
 {
@@ -3000,11 +3030,16 @@ void apply_force_to_wall_noalloc(esp_engine* e, int k, const double Fa[3])
     d2h(e, h_na_pos, pos, m);
     if (G.bonded) d2h(e, h_na_ipos, ipos, m);
     sync(e);
-    for (int c = 0; c < m; c++)
-      h_na_cand[c] = WallCand{e->h_order[h_na_slot[c]], h_na_flg[c], h_na_pos[c],
-                              G.bonded ? h_na_ipos[c] : make_double4(0, 0, 0, 0)};
-    std::sort(h_na_cand, h_na_cand + m,
-              [](const WallCand& a, const WallCand& b) { return a.order < b.order; });
+    // the candidates arrive in any order: sorted into particle-list order
+    for (int c = 0; c < m; c++) h_na_idx[c] = c;
+    std::sort(h_na_idx, h_na_idx + m, [&](int a, int b) {
+      return e->h_order[h_na_slot[a]] < e->h_order[h_na_slot[b]];
+    });
+    for (int i = 0; i < m; i++) {
+      const int c = h_na_idx[i];
+      const double4 x0 = G.bonded ? h_na_ipos[c] : make_double4(0, 0, 0, 0);
+      h_na_cand[i] = WallCand{h_na_pos[c], x0.x, x0.y, x0.z, h_na_flg[c], 0u};
+    }
   }
   servo_iterate(e, k, Fa, h_na_cand, m);
 }

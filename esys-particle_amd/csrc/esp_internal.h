@@ -312,10 +312,12 @@ void launch_wall_cands(hipStream_t s, PView p, int w, const double4* init, int c
                        int* slot, unsigned* flg, double4* pos, double4* ipos);
 // one candidate particle of a wall servo, as the host iterates it
 struct WallCand {
-  long long order;  // particle-list (creation) order
+  double4 pos;         // position and radius
+  double ix, iy, iz;   // initial position (bonded walls, else 0)
   unsigned flags;
-  double4 pos, init;  // position and radius; initial position (bonded walls, else 0)
+  unsigned pad;
 };
+static_assert(sizeof(WallCand) == 64, "one servo candidate is one 64-byte record");
 // the candidates of wall group w in particle-list order: mark[o] = 1 when the
 // particle of creation order o (slot ord_slot[o], -1 erased) has the group's
 // membership bit; pack writes the marked ones at out[off[o]] (off: the

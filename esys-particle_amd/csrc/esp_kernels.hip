@@ -989,11 +989,14 @@ __global__ void k_wall_pack_ordered(int no, const int* ord_slot, const int* mark
   const int o = blockIdx.x * BS + threadIdx.x;
   if (o >= no || !mark[o]) return;
   const int s = ord_slot[o];
+  const double4 x0 = init ? init[s] : make_double4(0.0, 0.0, 0.0, 0.0);
   WallCand& c = out[off[o]];
-  c.order = o;
-  c.flags = p.flags[s];
   c.pos = p.pos[s];
-  c.init = init ? init[s] : make_double4(0.0, 0.0, 0.0, 0.0);
+  c.ix = x0.x;
+  c.iy = x0.y;
+  c.iz = x0.z;
+  c.flags = p.flags[s];
+  c.pad = 0u;
 }
 
 __global__ void k_clear_flag_bit(PView p, UpdState* st)
