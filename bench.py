@@ -91,16 +91,20 @@ def load_specimen(path):
         return pickle.load(f)
 
 
-def cpu_baseline(pkg, spm, spec, budget_s):
+def cpu_baseline(pkg, spm, spec, budget_s, max_steps=0):
     """The oracle (C restatement of the reference path, one thread) stepping the
-    same specimen from its initial state until ~budget_s seconds have passed."""
+    same specimen from its initial state until ~budget_s seconds have passed or
+    (max_steps > 0) it reached the last step the timed GPU pass ran, so the
+    sample never runs past the trajectory the GPU figure covers."""
     orc = _load("esp_oracle_driver", os.path.join(ROOT, "oracle", "oracle.py"))
     sim = orc.oracle_lsm_class(pkg)()
     spm.apply(sim, spec)
     sim.runTimeSteps(1)  # first step: initial neighbour search, excluded
     chunk = 10 if sim.getNumParticles() <= 200000 else 1
     steps, psteps, t0 = 0, 0, time.perf_counter()
-    while time.perf_counter() - t0 < budget_s:
+    while time.perf_counter() - t0 < budget_s and (max_steps <= 0 or steps + 1 < max_steps):
+        if max_steps > 0:
+            chunk = min(chunk, max_steps - steps - 1)
         sim.runTimeSteps(chunk)
         steps += chunk
         psteps += chunk * sim.getNumParticles()
@@ -121,7 +125,7 @@ def cpu_model():
     return "unknown"
 
 
-def cpu_baseline_all_cores(spec_path, budget_s, workers):
+def cpu_baseline_all_cores(spec_path, budget_s, workers, max_steps=0):
     """`workers` single-thread oracle processes, each stepping its own copy of
     the same specimen for ~budget_s seconds: aggregate particle-steps/s, the
     throughput of the host cores on independent copies (an upper bound on a
@@ -129,7 +133,7 @@ def cpu_baseline_all_cores(spec_path, budget_s, workers):
     processes, so nothing here shares the GPU process's state."""
     import subprocess
     cmd = [sys.executable, os.path.abspath(__file__), "--cpu-worker", spec_path,
-           "--cpu-budget", str(budget_s)]
+           "--cpu-budget", str(budget_s), "--cpu-max-steps", str(max_steps)]
     procs = [subprocess.Popen(cmd, stdout=subprocess.PIPE, text=True) for _ in range(workers)]
     res = []
     for p in procs:
@@ -295,6 +299,7 @@ def main():
                     help="all-cores CPU leg: this many oracle processes (0: the CPU share, "
                          "at most 16)")
     ap.add_argument("--cpu-worker", default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-max-steps", type=int, default=0, help=argparse.SUPPRESS)
     ap.add_argument("--specimen", default=None,
                     help="npz file: load the specimen from it if present, else build and save it")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -331,7 +336,8 @@ def main():
     if args.cpu_worker:
         pkg = load_pkg()
         spm = importlib.import_module("esys_particle_amd.specimens")
-        r, steps = cpu_baseline(pkg, spm, load_specimen(args.cpu_worker), args.cpu_budget)
+        r, steps = cpu_baseline(pkg, spm, load_specimen(args.cpu_worker), args.cpu_budget,
+                                args.cpu_max_steps)
         r["steps"] = steps
         print(json.dumps(r), flush=True)
         return
@@ -491,7 +497,8 @@ def main():
     }
     if not args.no_cpu_baseline:
         t0 = time.perf_counter()
-        out["cpu_baseline"] = cpu_baseline(pkg, spm, spec, args.cpu_budget)[0]
+        last = cond + args.warmup + args.steps  # the last step of the timed pass
+        out["cpu_baseline"] = cpu_baseline(pkg, spm, spec, args.cpu_budget, last)[0]
         log("cpu baseline: %.1f s" % (time.perf_counter() - t0))
         import tempfile
         try:
@@ -503,7 +510,8 @@ def main():
             path = os.path.join(td, "specimen.pkl")
             save_specimen(path, spec)
             t0 = time.perf_counter()
-            out["cpu_baseline_all_cores"] = cpu_baseline_all_cores(path, args.cpu_budget, workers)
+            out["cpu_baseline_all_cores"] = cpu_baseline_all_cores(path, args.cpu_budget, workers,
+                                                                   last)
         log("cpu baseline, %d processes: %.1f s" % (workers, time.perf_counter() - t0))
     print(json.dumps(out), flush=True)
 
