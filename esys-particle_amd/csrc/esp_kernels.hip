@@ -750,30 +750,11 @@ __global__ __launch_bounds__(BS) void k_wall_finalize(const double* part, int nb
   const double* ps = part + (size_t)blockIdx.x * nblocks * nwg * 3;
   for (int w = 0; w < nwg; w++) {
     double a = 0.0, b = 0.0, c = 0.0;
-    // eight partials loaded before they are added, so their loads are in
-    // flight together; each thread still adds its partials in block order
-    constexpr int U = 8;
-    for (int k0 = threadIdx.x; k0 < nblocks; k0 += U * BS) {
-      double qa[U], qb[U], qc[U];
-#pragma unroll
-      for (int j = 0; j < U; j++) {
-        const int k = k0 + j * BS;
-        qa[j] = qb[j] = qc[j] = 0.0;
-        if (k < nblocks) {
-          const double* q = ps + ((size_t)k * nwg + w) * 3;
-          qa[j] = q[0];
-          qb[j] = q[1];
-          qc[j] = q[2];
-        }
-      }
-#pragma unroll
-      for (int j = 0; j < U; j++) {
-        if (k0 + j * BS < nblocks) {
-          a = a + qa[j];
-          b = b + qb[j];
-          c = c + qc[j];
-        }
-      }
+    for (int k = threadIdx.x; k < nblocks; k += BS) {
+      const double* q = ps + ((size_t)k * nwg + w) * 3;
+      a = a + q[0];
+      b = b + q[1];
+      c = c + q[2];
     }
     block_sum3(a, b, c, lds3);
     if (threadIdx.x == 0) {
