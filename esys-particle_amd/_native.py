@@ -117,6 +117,7 @@ SIGNATURES = {
     "run_time_steps": (_i, [_vp, _i]),
     "get_num_particles": (_i, [_vp]),
     "get_stats": (_i, [_vp, ctypes.POINTER(EspStats)]),
+    "get_time_step": (_i, [_vp, ctypes.POINTER(ctypes.c_longlong)]),
     "get_particle_ids": (_i, [_vp, _pi]),
     "get_particle_data": (_i, [_vp, _i, _pd]),
     "get_wall_position": (_i, [_vp, _s, _pd]),
@@ -389,6 +390,11 @@ class Engine:
     # ---- queries ----
     def num_particles(self):
         return self._check(self.lib.get_num_particles(self.h), "getNumParticles")
+
+    def time_step(self):
+        t = ctypes.c_longlong(0)
+        self._check(self.lib.get_time_step(self.h, ctypes.byref(t)), "getTimeStep")
+        return t.value
 
     def stats(self):
         s = EspStats()

@@ -3098,6 +3098,15 @@ int esp_run_time_steps(esp_engine* e, int nsteps)
 
 int esp_get_num_particles(esp_engine* e) { return e->n; }
 
+int esp_get_time_step(esp_engine* e, long long* out)
+{
+  API_BEGIN
+  if (!out) throw EspError(ESP_ERR_ARG, "bad arguments");
+  *out = e->t;
+  return ESP_OK;
+  API_END
+}
+
 int esp_get_stats(esp_engine* e, esp_stats* o)
 {
   API_BEGIN

@@ -512,7 +512,7 @@ class LsmMpi:
         return self._nts
 
     def getTimeStep(self):
-        return self._e().stats()["time_step"]
+        return self._e().time_step()
 
     # ---- particles ----
     def createParticles(self, particles):
@@ -765,7 +765,7 @@ class LsmMpi:
     def _first_step_checkpoint(self):
         # CLatticeMaster::runOneStep saves the fields of step 0 before the first
         # search and the step-0 checkpoint after it (LatticeMaster.cpp:2611-2640)
-        if self._e().stats()["time_step"] == 0 and not self._started_saving:
+        if not self._started_saving and self._e().time_step() == 0:
             self._started_saving = True
             self._save_if_due(0)
 

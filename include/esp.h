@@ -284,6 +284,11 @@ int esp_run_time_steps(esp_engine* e, int nsteps);
 /* ---- queries ------------------------------------------------------------ */
 int esp_get_num_particles(esp_engine* e);
 int esp_get_stats(esp_engine* e, esp_stats* out);
+/* getTimeStep (LsmMpiPy.cpp:466-468 -> CLatticeMaster::getTimeStep,
+ * LatticeMaster.h:226): the steps completed, answered from the host without
+ * touching the device (esp_get_stats also folds the device's event logs and
+ * reads the interaction counts back). */
+int esp_get_time_step(esp_engine* e, long long* out);
 /* Particle ids, ascending. */
 int esp_get_particle_ids(esp_engine* e, int* out);
 /* One ESP_FIELD_* for every particle, ordered by ascending id. */

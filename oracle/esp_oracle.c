@@ -2123,6 +2123,14 @@ int eso_run_time_steps(eso_engine* e, int nsteps)
 
 int eso_get_num_particles(eso_engine* e) { return e->np; }
 
+/* LsmMpiPy::getTimeStep (LsmMpiPy.cpp:466-468): the steps completed */
+int eso_get_time_step(eso_engine* e, long long* out)
+{
+  if (!out) return ESP_ERR_ARG;
+  *out = e->t;
+  return ESP_OK;
+}
+
 int eso_get_stats(eso_engine* e, esp_stats* o)
 {
   memset(o, 0, sizeof *o);

@@ -60,6 +60,7 @@ int eso_set_wall_step_motion(eso_engine* e, const char* name, const double d[3])
 int eso_run_time_steps(eso_engine* e, int nsteps);
 int eso_get_num_particles(eso_engine* e);
 int eso_get_stats(eso_engine* e, esp_stats* out);
+int eso_get_time_step(eso_engine* e, long long* out);
 int eso_get_particle_ids(eso_engine* e, int* out);
 int eso_get_particle_data(eso_engine* e, int field, double* out);
 int eso_get_wall_position(eso_engine* e, const char* name, double out[3]);
